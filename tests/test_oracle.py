@@ -1,0 +1,224 @@
+"""Pins the CPU oracle before anything is compared against it.
+
+* every known-answer vector of the reference's own tests (tests/golden/known_answers.json);
+* exhaustive enumeration of alignments on tiny inputs, for every end-gap combination,
+  the local method, unitary and BLOSUM62 scoring, and empty sequences;
+* invariants that the mathematics fixes (transposition symmetry, closed forms);
+* planted mistakes: a deliberately broken copy of the recurrence must fail these checks.
+"""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pairwise_aligner_amd import tables
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")
+
+
+def oracle_config(c):
+    score = c["score"]
+    kw = dict(mode=O.LOCAL if c["mode"] == "local" else O.GLOBAL,
+              first_column_free=c["first_column_free"], first_row_free=c["first_row_free"],
+              last_column_free=c["last_column_free"], last_row_free=c["last_row_free"],
+              gap_open=c["gap_open"], gap_extension=c["gap_extension"])
+    if score["kind"] == "unitary":
+        return O.OracleConfig(score_kind=O.UNITARY, match=score["match"], mismatch=score["mismatch"], **kw)
+    table = getattr(tables, score["matrix"])()
+    return O.OracleConfig(score_kind=O.MATRIX, symbols="".join(s for s, _ in table).encode(),
+                          matrix=tuple(v for _, row in table for v in row), **kw)
+
+
+def load_golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("vec", load_golden(), ids=lambda v: v["name"])
+def test_known_answers(vec):
+    cfg = oracle_config(vec["config"])
+    assert O.score(vec["seq1"].encode(), vec["seq2"].encode(), cfg) == vec["expected"], vec["source"]
+
+
+def test_golden_count():
+    # 25 scalar vectors of test/unit/affine plus the two configuration smoke tests.
+    assert len(load_golden()) == 27
+
+
+def _random_seq(rng, alphabet, lo, hi):
+    n = int(rng.integers(lo, hi + 1))
+    return rng.choice(np.frombuffer(alphabet.encode(), dtype=np.uint8), size=n).tobytes() if n else b""
+
+
+FLAG_COMBOS = list(itertools.product([False, True], repeat=4))
+
+
+@pytest.mark.parametrize("flags", FLAG_COMBOS, ids=lambda f: "".join("F" if x else "p" for x in f))
+def test_brute_force_global_unitary(flags):
+    rng = np.random.default_rng(hash(flags) & 0xFFFF)
+    fc, fr, lc, lr = flags
+    for trial in range(25):
+        go = int(rng.integers(-6, 1))
+        ge = int(rng.integers(-3, 0))
+        cfg = O.OracleConfig(mode=O.GLOBAL, first_column_free=fc, first_row_free=fr, last_column_free=lc,
+                             last_row_free=lr, gap_open=go, gap_extension=ge, match=int(rng.integers(1, 5)),
+                             mismatch=int(rng.integers(-5, 0)))
+        a = _random_seq(rng, "ACG", 0, 5)
+        b = _random_seq(rng, "ACG", 0, 5)
+        assert O.score(a, b, cfg) == O.brute_force_score(a, b, cfg), (a, b, cfg)
+
+
+def test_brute_force_local_unitary():
+    rng = np.random.default_rng(7)
+    for trial in range(60):
+        cfg = O.OracleConfig(mode=O.LOCAL, gap_open=int(rng.integers(-6, 0)), gap_extension=int(rng.integers(-3, 0)),
+                             match=int(rng.integers(1, 5)), mismatch=int(rng.integers(-5, 0)))
+        a = _random_seq(rng, "ACGT", 0, 4)
+        b = _random_seq(rng, "ACGT", 0, 4)
+        assert O.score(a, b, cfg) == O.brute_force_score(a, b, cfg), (a, b, cfg)
+
+
+@pytest.mark.parametrize("mode", [O.GLOBAL, O.LOCAL])
+def test_brute_force_blosum(mode):
+    table = tables.blosum62_standard()
+    syms = "".join(s for s, _ in table)
+    rng = np.random.default_rng(11 + mode)
+    for trial in range(30):
+        cfg = O.OracleConfig(mode=mode, gap_open=-11, gap_extension=-1, score_kind=O.MATRIX,
+                             symbols=syms.encode(), matrix=tuple(v for _, r in table for v in r),
+                             first_row_free=bool(trial % 2), last_column_free=bool(trial % 3 == 0))
+        a = _random_seq(rng, syms, 0, 4)
+        b = _random_seq(rng, syms, 0, 4)
+        assert O.score(a, b, cfg) == O.brute_force_score(a, b, cfg), (a, b)
+
+
+def test_transposition_symmetry():
+    """Swapping seq1/seq2 together with first_row<->first_column and last_row<->last_column
+    leaves every score unchanged."""
+    rng = np.random.default_rng(3)
+    for trial in range(200):
+        fc, fr, lc, lr = (bool(x) for x in rng.integers(0, 2, 4))
+        kw = dict(gap_open=int(rng.integers(-12, 0)), gap_extension=int(rng.integers(-4, 0)),
+                  match=int(rng.integers(1, 6)), mismatch=int(rng.integers(-6, 0)))
+        mode = O.LOCAL if trial % 5 == 0 else O.GLOBAL
+        c1 = O.OracleConfig(mode=mode, first_column_free=fc, first_row_free=fr, last_column_free=lc, last_row_free=lr, **kw)
+        c2 = O.OracleConfig(mode=mode, first_column_free=fr, first_row_free=fc, last_column_free=lr, last_row_free=lc, **kw)
+        a = _random_seq(rng, "ACGT", 0, 40)
+        b = _random_seq(rng, "ACGT", 0, 40)
+        assert O.score(a, b, c1) == O.score(b, a, c2)
+
+
+def test_closed_forms():
+    cfg = O.OracleConfig(gap_open=-10, gap_extension=-1, match=4, mismatch=-5)
+    rng = np.random.default_rng(5)
+    for n in [0, 1, 2, 17, 300]:
+        s = _random_seq(rng, "ACGT", n, n)
+        assert O.score(s, s, cfg) == 4 * n                       # identical sequences
+        assert O.score(s, b"", cfg) == (0 if n == 0 else -10 - n)  # one empty sequence
+        assert O.score(b"", s, cfg) == (0 if n == 0 else -10 - n)
+    free = O.OracleConfig(gap_open=-10, gap_extension=-1, first_column_free=True)
+    assert O.score(b"ACGT", b"", free) == 0
+    loc = O.OracleConfig(mode=O.LOCAL)
+    assert O.score(b"", b"ACGT", loc) == O.INT32_MIN and O.score(b"", b"", loc) == O.INT32_MIN
+    # no symbol in common, equal length n: k mismatches plus two gaps of n-k is linear in k, so
+    # the optimum is at k = 0 or k = n: max(n*mismatch, 2*(go + ge*n))
+    for n in [1, 3, 10, 40]:
+        assert O.score(b"A" * n, b"T" * n, cfg) == max(-5 * n, 2 * (-10 - n))
+    # local score of non-empty input is never negative, and never below any global variant
+    for trial in range(50):
+        a = _random_seq(rng, "ACGT", 1, 30)
+        b = _random_seq(rng, "ACGT", 1, 30)
+        lo = O.score(a, b, O.OracleConfig(mode=O.LOCAL))
+        assert lo >= 0
+        assert lo >= O.score(a, b, O.OracleConfig(mode=O.GLOBAL, first_column_free=True, first_row_free=True,
+                                                  last_column_free=True, last_row_free=True))
+
+
+def test_batch_threads_agree():
+    rng = np.random.default_rng(9)
+    s1 = [_random_seq(rng, "ACGT", 0, 60) for _ in range(300)]
+    s2 = [_random_seq(rng, "ACGT", 0, 60) for _ in range(300)]
+    cfg = O.OracleConfig()
+    single = O.score_pairs(s1, s2, cfg, threads=1)
+    multi = O.score_pairs(s1, s2, cfg, threads=4)
+    assert np.array_equal(single, multi)
+    assert all(single[i] == O.score(s1[i], s2[i], cfg) for i in range(0, 300, 37))
+
+
+def test_invalid_matrix_symbol_raises():
+    table = tables.blosum62_standard()
+    cfg = O.OracleConfig(score_kind=O.MATRIX, symbols="".join(s for s, _ in table).encode(),
+                         matrix=tuple(v for _, r in table for v in r))
+    with pytest.raises(ValueError):
+        O.score(b"ACDZ", b"ACD", cfg)
+
+
+# ---------------------------------------------------------------------------------------------
+# Planted mistakes: a pure-Python copy of the recurrence with one switchable bug each.  The
+# checks above (known answers + brute force) must reject every variant, which shows they can
+# tell a plausible mistake from the right recurrence.
+# ---------------------------------------------------------------------------------------------
+
+def _py_dp(a, b, cfg, bug=None):
+    go, ge = cfg.gap_open, cfg.gap_extension
+    m, n = len(a), len(b)
+    local = cfg.mode == O.LOCAL
+    fc = local or cfg.first_column_free
+    fr = local or cfg.first_row_free
+    NEG = -10 ** 9
+    H = [[0] * (n + 1) for _ in range(m + 1)]
+    V = [[NEG] * (n + 1) for _ in range(m + 1)]
+    W = [[NEG] * (n + 1) for _ in range(m + 1)]
+    for i in range(1, m + 1):
+        H[i][0] = 0 if fc else go + ge * i
+    for j in range(1, n + 1):
+        H[0][j] = 0 if fr else go + (ge * j if bug != "init_drops_extension" else 0)
+    best = O.INT32_MIN
+    for i in range(1, m + 1):
+        for j in range(1, n + 1):
+            s = cfg.match if a[i - 1] == b[j - 1] else cfg.mismatch
+            if bug == "transposed_index":
+                s = cfg.match if a[min(j, m) - 1] == b[min(i, n) - 1] else cfg.mismatch
+            d = H[i - 1][j - 1] + s
+            if local:
+                d = max(d, 0)
+            ext = ge if bug != "extension_sign" else -ge
+            V[i][j] = max(V[i - 1][j] + ext, H[i - 1][j] + go + ge)
+            opn = go if bug != "open_without_extension" else 0
+            W[i][j] = max(W[i][j - 1] + ge, H[i][j - 1] + opn + ge)
+            H[i][j] = max(d, V[i][j], W[i][j])
+            best = max(best, H[i][j])
+    if local:
+        return best
+    sc = H[m][n]
+    if cfg.last_row_free:
+        sc = max([sc] + H[m][(0 if bug != "row_max_skips_first" else 1):])
+    if cfg.last_column_free:
+        sc = max([sc] + [H[i][n] for i in range(m + 1)])
+    return sc
+
+
+@pytest.mark.parametrize("bug", [None, "init_drops_extension", "transposed_index", "extension_sign",
+                                 "open_without_extension", "row_max_skips_first"])
+def test_planted_mistakes_are_caught(bug):
+    golden = [v for v in load_golden() if v["config"]["score"]["kind"] == "unitary"]
+    rng = np.random.default_rng(21)
+    caught = False
+    for v in golden:
+        cfg = oracle_config(v["config"])
+        if _py_dp(v["seq1"].encode(), v["seq2"].encode(), cfg, bug) != v["expected"]:
+            caught = True
+    for trial in range(300):
+        fc, fr, lc, lr = (bool(x) for x in rng.integers(0, 2, 4))
+        cfg = O.OracleConfig(first_column_free=fc, first_row_free=fr, last_column_free=lc, last_row_free=lr,
+                             gap_open=int(rng.integers(-6, 0)), gap_extension=int(rng.integers(-3, 0)),
+                             match=2, mismatch=-3)
+        a = _random_seq(rng, "ACG", 0, 4)
+        b = _random_seq(rng, "ACG", 0, 4)
+        if _py_dp(a, b, cfg, bug) != O.brute_force_score(a, b, cfg):
+            caught = True
+            break
+    assert caught == (bug is not None)
