@@ -1,0 +1,199 @@
+"""Benchmark: GCUPS of batched affine-gap score-only DP on one MI355X, with the CPU oracle timed
+beside it on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload protein|dna150|uneven|long|mixed]
+
+A step is one pass of the DP over the whole batch (all pairs of the workload), with inputs
+already resident in HBM.  `value` is GCUPS = sum over pairs of m*n / step time / 1e9 (interior
+cells, as the reference's benchmark fixture counts them).  The line printed last is the result:
+one JSON object.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from pairwise_aligner_amd import device, tables, workloads  # noqa: E402
+
+# MI355X peaks: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz; packed int16 issues two ops per lane.
+PEAK_INT16_TOPS = 256 * 4 * 16 * 2.4e9 * 2 / 1e12   # 78.6
+PEAK_INT32_TOPS = PEAK_INT16_TOPS / 2                # 39.3
+PEAK_HBM_GBS = 8000.0
+OPS_PER_CELL = {"global": 8, "local": 10}
+
+
+def log(msg):
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
+
+
+def blosum_config(method, go, ge, policy):
+    t = tables.blosum62_standard()
+    return device.DeviceConfig(method=method, score_kind=device.SCORE_MATRIX,
+                               symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
+                               gap_open=go, gap_extension=ge, lane_policy=policy)
+
+
+def make_workload(name):
+    if name == "dna150":
+        w = workloads.config_dna150()
+        dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=4, mismatch=-5, gap_open=-10, gap_extension=-1)
+        return w, dc, "global"
+    if name == "protein":
+        w = workloads.config_protein()
+        dc = blosum_config(device.METHOD_LOCAL, -11, -1, device.LANES_AUTO)
+        return w, dc, "local"
+    if name == "uneven":
+        w = workloads.config_uneven()
+        dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
+        return w, dc, "mixed"
+    if name == "long":
+        w = workloads.config_long()
+        dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=1, mismatch=-4, gap_open=-6, gap_extension=-1,
+                                 lane_policy=device.LANES_INT32)
+        return w, dc, "global"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def oracle_cfg(dc):
+    from oracle import oracle as O
+    return O.OracleConfig(mode=dc.method, first_column_free=dc.first_column_free, first_row_free=dc.first_row_free,
+                          last_column_free=dc.last_column_free, last_row_free=dc.last_row_free,
+                          gap_open=dc.gap_open, gap_extension=dc.gap_extension, score_kind=dc.score_kind,
+                          match=dc.match, mismatch=dc.mismatch, symbols=dc.symbols, matrix=dc.matrix)
+
+
+def cpu_baseline(w, dc, gpu_scores, budget_s=12.0):
+    """Times the CPU oracle (64-bit scalar Gotoh, C, one POSIX thread per core) on a bounded
+    sample of the same pairs and checks the GPU scores of that sample."""
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    b1, o1, b2, o2 = w["b1"], w["o1"], w["b2"], w["o2"]
+    n = len(o1) - 1
+    oc = oracle_cfg(dc)
+    pm_all = w.get("pair_method")
+    # calibrate on a small prefix, then size the sample to the budget
+    sample = min(n, 256)
+    t_cal = None
+    while True:
+        sub = slice(0, sample)
+        so1 = o1[:sample + 1]
+        so2 = o2[:sample + 1]
+        pm = None if pm_all is None else pm_all[:sample]
+        t0 = time.perf_counter()
+        want = O.score_packed(b1, so1, b2, so2, oc, pair_method=pm, threads=threads)
+        dt = time.perf_counter() - t0
+        if sample == n or dt > budget_s / 3:
+            break
+        t_cal = dt
+        sample = min(n, int(sample * max(2.0, min(64.0, budget_s / max(dt, 1e-3) / 2))))
+    cells = int(np.sum(np.diff(o1[:sample + 1]).astype(np.int64) * np.diff(o2[:sample + 1]).astype(np.int64)))
+    mism = int(np.count_nonzero(gpu_scores[:sample].astype(np.int64) != want))
+    return dict(value=cells / dt / 1e9, unit="GCUPS", cores=threads, kind="port",
+                sample=f"first {sample} of {n} pairs ({cells} cells), {dt:.2f} s, scalar 64-bit oracle "
+                       f"(oracle/gotoh_oracle.c) on {threads} threads",
+                mismatches_vs_gpu=mism, sample_pairs=sample)
+
+
+def run(name, steps, warmup, with_cpu=True):
+    t0 = time.perf_counter()
+    w, dc, method = make_workload(name)
+    n = len(w["o1"]) - 1
+    cells = int(np.sum(np.diff(w["o1"]).astype(np.int64) * np.diff(w["o2"]).astype(np.int64)))
+    log(f"[bench] {name}: {n} pairs, {cells:.3e} cells, generated in {time.perf_counter() - t0:.1f} s")
+    ctx = device.default_context(0)
+    batch = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
+    st = batch.stats()
+    log(f"[bench] plan: {st}")
+    for _ in range(warmup):
+        batch.run()
+    try:
+        import torch
+        torch.cuda.synchronize()
+    except Exception:
+        pass
+    t1 = time.perf_counter()
+    total_ms, main_ms = batch.time(steps)
+    wall = time.perf_counter() - t1
+    scores, flags = batch.results()
+    st = batch.stats()
+    ms_step = total_ms / steps
+    gcups = cells / (ms_step / 1e3) / 1e9
+    ops = OPS_PER_CELL["local" if method == "local" else "global"] if method != "mixed" else 9
+    peak = PEAK_INT16_TOPS if dc.lane_policy != device.LANES_INT32 else PEAK_INT32_TOPS
+    achieved = ops * cells / (main_ms / 1e3) / 1e12 if main_ms > 0 else 0.0
+    res = dict(name=name, pairs=n, cells=cells, ms_per_step=ms_step, main_kernel_ms=main_ms, wall_s=wall,
+               gcups=gcups, achieved_tops=achieved, peak_tops=peak, ops_per_cell=ops, stats=st,
+               inputs_sha256=workloads.digest(w["b1"], w["o1"], w["b2"], w["o2"]))
+    if with_cpu:
+        res["cpu"] = cpu_baseline(w, dc, scores)
+    log(f"[bench] {name}: {gcups:.1f} GCUPS ({ms_step:.3f} ms/step, main kernels {main_ms:.3f} ms), "
+        f"{achieved:.2f} of {peak:.1f} TOP/s")
+    batch.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="protein")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary DNA workload")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (pairs are independent; multi-GPU sharding is not built); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    main_res = run(args.workload, args.steps, args.warmup)
+    extra = None
+    if not args.no_extra and args.workload != "dna150":
+        extra = run("dna150", max(args.steps, 50), args.warmup, with_cpu=False)
+
+    cpu = main_res.pop("cpu")
+    frac = main_res["achieved_tops"] / main_res["peak_tops"]
+    line = {
+        "metric": "GCUPS score-only affine DP on 1 GPU vs CPU ref; learned dispatch vs fixed int32",
+        "value": round(main_res["gcups"], 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_res["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16" if main_res["stats"]["n_int32"] == 0 else "int16+int32",
+        "data": "synthetic (seeded generators in pairwise_aligner_amd/workloads.py)",
+        "config": {"workload": {"protein": "protein local BLOSUM62, 100,000 pairs, lengths 50-500, gap (-11,-1)",
+                                "dna150": "DNA global, 10,000 pairs x 150 nt, +4/-5, gap (-10,-1)",
+                                "uneven": "1,000,000 DNA pairs lognormal lengths, half global half local",
+                                "long": "2,000 DNA pairs 10-100 kbp, int32"}.get(args.workload, args.workload),
+                   "pairs": main_res["pairs"], "cells": main_res["cells"],
+                   "inputs_sha256": main_res["inputs_sha256"]},
+        "roofline": {"bound": "valu", "achieved": round(main_res["achieved_tops"], 3),
+                     "peak": round(main_res["peak_tops"], 2), "unit": "TOP/s", "frac": round(frac, 4),
+                     "traffic": None,
+                     "ops_per_cell": main_res["ops_per_cell"], "kernel_ms": round(main_res["main_kernel_ms"], 4)},
+        "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "mismatches_vs_oracle_sample": cpu["mismatches_vs_gpu"],
+        "plan": {k: main_res["stats"][k] for k in ("n_int16", "n_int32", "n_recomputed", "padded_cells", "n_launches")},
+    }
+    if extra is not None:
+        line["dna150_global_int16"] = {"gcups": round(extra["gcups"], 2), "ms_per_step": round(extra["ms_per_step"], 4),
+                                       "kernel_ms": round(extra["main_kernel_ms"], 4),
+                                       "roofline_frac": round(extra["achieved_tops"] / extra["peak_tops"], 4)}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+/*
+ * pa_device.h -- C ABI of the MI355X (gfx950) batched affine-gap score-only aligner.
+ *
+ * The reference (rrahn/pairwise_aligner) is a header-only C++20 template library with no ABI:
+ * an aligner is configured by a compile-time rule chain and called with compute().  This
+ * header is the plain-C boundary that sits exactly where the reference's compute() calls
+ * its DP (interface_one_to_one_bulk.hpp:73-100 -> dp_algorithm_template_standard.hpp:43-100).
+ * The header-only C++ host layer (include/pairwise_aligner/...) and the Python mirror
+ * (pairwise_aligner_amd) both translate the reference's configuration into pa_config and
+ * call the functions below.  No C++ or torch types cross this boundary.
+ *
+ * Conventions: seq1 indexes DP rows, seq2 DP columns (as in the reference).  Sequences are
+ * passed as one concatenated byte buffer per side with n_pairs+1 uint64 offsets.  Scores are
+ * returned as int32 with the reference's scalar (int32) semantics; a local alignment with an
+ * empty sequence returns INT32_MIN (local_affine_scalar_test.cpp:100-105).
+ */
+#ifndef PA_DEVICE_H
+#define PA_DEVICE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PA_ABI_VERSION 1
+
+typedef enum pa_status {
+    PA_OK = 0,
+    PA_ERR_INVALID_ARGUMENT = 1,  /* null pointer, bad enum, inconsistent offsets */
+    PA_ERR_INVALID_SYMBOL = 2,    /* a residue outside the substitution-matrix alphabet */
+    PA_ERR_UNSUPPORTED = 3,       /* gap model or alphabet the device path does not handle */
+    PA_ERR_DEVICE = 4,            /* HIP runtime error (message in pa_ctx_last_error) */
+    PA_ERR_OUT_OF_MEMORY = 5,
+    PA_ERR_RANGE = 6              /* a score could leave the int32 range */
+} pa_status;
+
+/* method: cfg::method_global / cfg::method_local (method_global.hpp:35-53, method_local.hpp:41-49) */
+enum { PA_METHOD_GLOBAL = 0, PA_METHOD_LOCAL = 1 };
+/* score model: cfg::score_model_unitary* / cfg::score_model_matrix* (score_model_unitary.hpp:52-59,
+ * score_model_matrix.hpp:52-64) */
+enum { PA_SCORE_UNITARY = 0, PA_SCORE_MATRIX = 1 };
+/* lane policy: which integer width the DP runs in.  AUTO = int16 wherever a bound derived from
+ * lengths and scores proves it cannot saturate, int32 elsewhere; LEARNED = int16 wherever the
+ * learned range predictor says the scores fit; INT16 = always try int16 first.  Every int16
+ * result that reaches the int16 limit is recomputed at int32 (PA_PAIR_RECOMPUTED). */
+enum { PA_LANES_AUTO = 0, PA_LANES_LEARNED = 1, PA_LANES_INT16 = 16, PA_LANES_INT32 = 32 };
+
+typedef struct pa_config {
+    int32_t method;             /* PA_METHOD_* */
+    int32_t first_column_free;  /* cfg::leading_end_gap::first_column == end_gap::free */
+    int32_t first_row_free;     /* cfg::leading_end_gap::first_row */
+    int32_t last_column_free;   /* cfg::trailing_end_gap::last_column */
+    int32_t last_row_free;      /* cfg::trailing_end_gap::last_row */
+    int32_t gap_open;           /* cfg::gap_model_affine(open, extension): a gap of k costs open + k*extension */
+    int32_t gap_extension;
+    int32_t score_kind;         /* PA_SCORE_* */
+    int32_t match;              /* unitary */
+    int32_t mismatch;
+    int32_t matrix_dim;         /* matrix: number of symbols (<= 63) */
+    const int32_t *matrix;      /* matrix: dim*dim scores, [rank of seq1 symbol][rank of seq2 symbol] */
+    const uint8_t *matrix_symbols; /* matrix: the dim symbols; rank = position (alphabet_rank_map_scalar.hpp:39-51) */
+    int32_t lane_policy;        /* PA_LANES_* */
+} pa_config;
+
+/* per-pair result flags */
+#define PA_PAIR_INT16 0x01      /* final score computed in packed int16 lanes */
+#define PA_PAIR_INT32 0x02      /* final score computed in int32 lanes */
+#define PA_PAIR_RECOMPUTED 0x04 /* an int16 pass reached the int16 limit; recomputed at int32 */
+#define PA_PAIR_EMPTY 0x08      /* an empty sequence: closed-form score */
+
+typedef struct pa_ctx pa_ctx;
+typedef struct pa_batch pa_batch;
+
+typedef struct pa_batch_stats {
+    uint64_t n_pairs;
+    uint64_t cells;             /* sum of m*n over the batch (interior cells) */
+    uint64_t padded_cells;      /* cells the kernels actually sweep, padding included */
+    uint64_t n_int16;           /* pairs planned in int16 lanes */
+    uint64_t n_int32;           /* pairs planned in int32 lanes */
+    uint64_t n_recomputed;      /* pairs recomputed at int32 after the last run */
+    uint64_t n_launches;        /* kernel launches per run */
+    double plan_ms;             /* host planning time */
+} pa_batch_stats;
+
+/* Context: one HIP device and one HIP stream.  Not thread-safe; use one per thread. */
+pa_status pa_ctx_create(pa_ctx **out, int device);
+pa_status pa_ctx_destroy(pa_ctx *ctx);
+const char *pa_ctx_last_error(const pa_ctx *ctx);
+int pa_abi_version(void);
+
+/* One-shot: upload, plan, run, download.  Replaces aligner.compute(seqs1, seqs2)
+ * (interface_one_to_one_bulk.hpp:47-100 and, for one pair, interface_one_to_one_single.hpp:40-67).
+ * pair_method (nullable) overrides cfg->method per pair.  flags (nullable) receives PA_PAIR_*. */
+pa_status pa_score_batch(pa_ctx *ctx, const pa_config *cfg,
+                         const uint8_t *seq1, const uint64_t *off1,
+                         const uint8_t *seq2, const uint64_t *off2,
+                         uint64_t n_pairs, const uint8_t *pair_method,
+                         int32_t *scores, uint8_t *flags);
+
+/* Device-resident batch: the same work split into plan/upload, repeated runs, and download,
+ * so a caller can time runs with inputs already in HBM. */
+pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg,
+                          const uint8_t *seq1, const uint64_t *off1,
+                          const uint8_t *seq2, const uint64_t *off2,
+                          uint64_t n_pairs, const uint8_t *pair_method, pa_batch **out);
+/* Enqueue every kernel of one pass on the context stream and, when int16 results reached
+ * the limit, the int32 recomputation.  Returns when the pass is complete. */
+pa_status pa_batch_run(pa_ctx *ctx, pa_batch *batch);
+pa_status pa_batch_results(pa_ctx *ctx, pa_batch *batch, int32_t *scores, uint8_t *flags);
+pa_status pa_batch_stats_get(const pa_batch *batch, pa_batch_stats *out);
+/* Runs the batch `iters` times.  *ms_total is the HIP-event time of all passes on the context
+ * stream; *ms_main is the HIP-event time of the main DP kernel launches alone (averaged per
+ * pass), used for the roofline. */
+pa_status pa_batch_time(pa_ctx *ctx, pa_batch *batch, int iters, double *ms_total, double *ms_main);
+pa_status pa_batch_destroy(pa_ctx *ctx, pa_batch *batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PA_DEVICE_H */

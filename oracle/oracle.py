@@ -90,7 +90,7 @@ def pack(seqs: Sequence[bytes]):
     return np.ascontiguousarray(buf), off
 
 
-def score_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_mode=None, threads: int = 1) -> np.ndarray:
+def score_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_method=None, threads: int = 1) -> np.ndarray:
     """Score pairs given as packed buffers + offsets.  Returns int64 scores."""
     n = len(off1) - 1
     scores = np.zeros(max(n, 1), dtype=np.int64)
@@ -104,8 +104,8 @@ def score_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_mode=None, thre
         c.matrix = mat
         c.symbols = sym
     pm = None
-    if pair_mode is not None:
-        pm = np.ascontiguousarray(pair_mode, dtype=np.uint8)
+    if pair_method is not None:
+        pm = np.ascontiguousarray(pair_method, dtype=np.uint8)
     buf1 = np.ascontiguousarray(buf1, dtype=np.uint8)
     buf2 = np.ascontiguousarray(buf2, dtype=np.uint8)
     off1 = np.ascontiguousarray(off1, dtype=np.uint64)
@@ -119,10 +119,10 @@ def score_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_mode=None, thre
 
 
 def score_pairs(seqs1: Sequence[bytes], seqs2: Sequence[bytes], cfg: OracleConfig,
-                pair_mode=None, threads: int = 1) -> np.ndarray:
+                pair_method=None, threads: int = 1) -> np.ndarray:
     b1, o1 = pack(seqs1)
     b2, o2 = pack(seqs2)
-    return score_packed(b1, o1, b2, o2, cfg, pair_mode, threads)
+    return score_packed(b1, o1, b2, o2, cfg, pair_method, threads)
 
 
 def score(s1: bytes, s2: bytes, cfg: OracleConfig) -> int:

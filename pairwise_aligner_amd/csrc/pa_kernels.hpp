@@ -1,0 +1,43 @@
+// pa_kernels.hpp -- launch interface of the gfx950 kernels (host side of pa_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pa_plan.hpp"
+
+namespace pa {
+
+// Device-side flag bit (internal): an int16 pass reached the int16 limit.
+inline constexpr uint32_t kFlagSaturated = 0x100u;
+
+struct KernelArgs {
+    const Task *tasks;
+    uint32_t ntasks;
+    const uint32_t *rowcodes;
+    const uint32_t *colcodes;
+    uint32_t *bnd;
+    int32_t *scores;
+    uint32_t *flags;
+    const int16_t *table;
+    uint32_t table_entries;
+    int32_t go, ge;
+    uint32_t tbl_lo, tbl_hi;
+    int32_t fc, fr, lc, lr;
+};
+
+struct InitArgs {
+    const PairMeta *pairs;
+    const uint32_t *subset;   // nullable: only these pair indices
+    uint64_t n_pairs;         // pairs, or entries of subset
+    int32_t *scores;
+    uint32_t *flags;
+    int32_t go, ge;
+    int32_t fc, fr, lc, lr;
+    uint32_t flag_or;         // extra flag bits written with the initial value
+};
+
+hipError_t launch_stripe(const VariantKey &key, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st);
+hipError_t launch_init(const InitArgs &a, hipStream_t st);
+
+} // namespace pa

@@ -1,0 +1,332 @@
+// pa_plan.cpp -- host-side planning for the striped DP kernels.  See pa_plan.hpp.
+#include "pa_plan.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <tuple>
+
+namespace pa {
+
+bool VariantKey::operator<(const VariantKey &o) const
+{
+    return std::tie(width, G, R, score, local, track, rowmax) <
+           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax);
+}
+bool VariantKey::operator==(const VariantKey &o) const
+{
+    return std::tie(width, G, R, score, local, track, rowmax) ==
+           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax);
+}
+
+// Rows per lane that kernels are instantiated for (lanes per group G = 16).
+static const uint8_t kRowsPerLane[] = {4, 8, 10, 12, 16};
+
+void choose_shape(uint32_t m, uint32_t width, uint8_t &G, uint8_t &R)
+{
+    (void)width;
+    G = 16;
+    for (uint8_t r : kRowsPerLane) {
+        if (uint32_t(G) * r >= m) { R = r; return; }
+    }
+    R = 16;  // multi-stripe
+}
+
+RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n)
+{
+    const int64_t go = std::min<int64_t>(plan.go, 0), ge = std::min<int64_t>(plan.ge, 0);
+    const int64_t sp = std::max<int64_t>(plan.smax, 0);
+    RangeBound b;
+    b.upper = sp * int64_t(std::min(m, n));
+    if (local)
+        b.lower = go + ge;                       // H >= 0, gap states >= H + go + ge
+    else
+        b.lower = 3 * go + ge * (int64_t(m) + int64_t(n));  // L-shaped path plus one gap opening
+    return b;
+}
+
+namespace {
+
+struct Alphabet {
+    bool dna = false;
+    int16_t code[256];   // symbol -> code (DNA 0..3) or rank (matrix), -1 if absent
+    uint32_t size = 0;
+};
+
+pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, Alphabet &alpha, std::string &err)
+{
+    for (int c = 0; c < 256; ++c) alpha.code[c] = -1;
+    if (cfg.score_kind == PA_SCORE_UNITARY) {
+        bool seen[256] = {false};
+        const uint64_t l1 = b.off1[b.n_pairs], l2 = b.off2[b.n_pairs];
+        for (uint64_t i = b.off1[0]; i < l1; ++i) seen[b.seq1[i]] = true;
+        for (uint64_t i = b.off2[0]; i < l2; ++i) seen[b.seq2[i]] = true;
+        uint32_t k = 0;
+        for (int c = 0; c < 256; ++c) if (seen[c]) alpha.code[c] = int16_t(k++);
+        alpha.size = k;
+        if (cfg.match < -32768 || cfg.match > 32767 || cfg.mismatch < -32768 || cfg.mismatch > 32767) {
+            err = "match/mismatch scores must fit in 16 bits";
+            return PA_ERR_UNSUPPORTED;
+        }
+        plan.smax = std::max(cfg.match, cfg.mismatch);
+        plan.smin = std::min(cfg.match, cfg.mismatch);
+        if (k <= 4) {
+            alpha.dna = true;
+            plan.score = kScoreDna;
+            const uint32_t M = uint32_t(cfg.match) & 0xFFFFu, X = uint32_t(cfg.mismatch) & 0xFFFFu;
+            plan.tbl_lo = M | (X << 16);
+            plan.tbl_hi = X | (X << 16);
+            return PA_OK;
+        }
+        if (k > 63) {
+            err = "unitary scoring over more than 63 distinct symbols is not supported on the device";
+            return PA_ERR_UNSUPPORTED;
+        }
+        plan.score = kScoreMatrix;
+        plan.alphabet = k + 1;
+        plan.stride = plan.alphabet;
+        plan.table.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
+        for (uint32_t a = 0; a < k; ++a)
+            for (uint32_t c = 0; c < k; ++c)
+                plan.table[a * plan.stride + c] = int16_t(a == c ? cfg.match : cfg.mismatch);
+        return PA_OK;
+    }
+    if (cfg.score_kind != PA_SCORE_MATRIX || cfg.matrix == nullptr || cfg.matrix_symbols == nullptr) {
+        err = "invalid score model";
+        return PA_ERR_INVALID_ARGUMENT;
+    }
+    const int32_t K = cfg.matrix_dim;
+    if (K < 1 || K > 63) {
+        err = "substitution matrix dimension must be 1..63";
+        return PA_ERR_UNSUPPORTED;
+    }
+    for (int32_t r = 0; r < K; ++r) alpha.code[cfg.matrix_symbols[r]] = int16_t(r);
+    alpha.size = uint32_t(K);
+    plan.score = kScoreMatrix;
+    plan.alphabet = uint32_t(K) + 1;
+    plan.stride = plan.alphabet;
+    plan.table.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
+    plan.smax = INT32_MIN;
+    plan.smin = INT32_MAX;
+    for (int32_t a = 0; a < K; ++a)
+        for (int32_t c = 0; c < K; ++c) {
+            int32_t v = cfg.matrix[a * K + c];
+            if (v < -32768 || v > 32767) {
+                err = "substitution scores must fit in 16 bits";
+                return PA_ERR_UNSUPPORTED;
+            }
+            plan.table[a * plan.stride + c] = int16_t(v);
+            plan.smax = std::max(plan.smax, v);
+            plan.smin = std::min(plan.smin, v);
+        }
+    // every residue must be in the alphabet (alphabet_rank_map_scalar.hpp:59-63 asserts it)
+    const uint64_t l1 = b.off1[b.n_pairs], l2 = b.off2[b.n_pairs];
+    for (uint64_t i = b.off1[0]; i < l1; ++i)
+        if (alpha.code[b.seq1[i]] < 0) { err = "residue outside the substitution matrix alphabet"; return PA_ERR_INVALID_SYMBOL; }
+    for (uint64_t i = b.off2[0]; i < l2; ++i)
+        if (alpha.code[b.seq2[i]] < 0) { err = "residue outside the substitution matrix alphabet"; return PA_ERR_INVALID_SYMBOL; }
+    return PA_OK;
+}
+
+struct Item {
+    uint32_t pair;
+    uint32_t m, n;
+    uint32_t S;
+};
+
+} // namespace
+
+pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *forced_width,
+                    const std::vector<uint32_t> *only, Plan &plan, std::string &err)
+{
+    if (cfg.method != PA_METHOD_GLOBAL && cfg.method != PA_METHOD_LOCAL) { err = "invalid method"; return PA_ERR_INVALID_ARGUMENT; }
+    if (cfg.gap_extension > 0 || int64_t(cfg.gap_open) + cfg.gap_extension > 0) {
+        err = "the device path needs gap_extension <= 0 and gap_open + gap_extension <= 0";
+        return PA_ERR_UNSUPPORTED;
+    }
+    for (uint64_t p = 0; p < b.n_pairs; ++p) {
+        if (b.off1[p + 1] < b.off1[p] || b.off2[p + 1] < b.off2[p]) { err = "offsets must be non-decreasing"; return PA_ERR_INVALID_ARGUMENT; }
+        if (b.pair_method && b.pair_method[p] > 1) { err = "invalid per-pair method"; return PA_ERR_INVALID_ARGUMENT; }
+    }
+    plan = Plan{};
+    plan.go = cfg.gap_open;
+    plan.ge = cfg.gap_extension;
+    int fc = cfg.first_column_free != 0, fr = cfg.first_row_free != 0;
+    int lc = cfg.last_column_free != 0, lr = cfg.last_row_free != 0;
+    // A free last row without a free last column is turned into a free last column by
+    // exchanging the roles of seq1 and seq2 (the score is invariant under that exchange
+    // together with first_row<->first_column and last_row<->last_column).
+    bool swap_all = lr && !lc;
+    if (swap_all) { std::swap(fc, fr); std::swap(lc, lr); }
+    const bool symmetric = (fc == fr) && (lc == lr);
+    plan.fc = fc; plan.fr = fr; plan.lc = lc; plan.lr = lr;
+
+    Alphabet alpha;
+    if (pa_status st = build_alphabet(cfg, b, plan, alpha, err); st != PA_OK) return st;
+
+    const uint64_t N = b.n_pairs;
+    plan.pairs.resize(N);
+    std::vector<uint8_t> selected(N, only ? 0 : 1);
+    if (only) for (uint32_t p : *only) selected[p] = 1;
+
+    std::map<VariantKey, std::vector<Item>> groups;
+    for (uint64_t p = 0; p < N; ++p) {
+        const bool local = b.pair_method ? b.pair_method[p] == PA_METHOD_LOCAL : cfg.method == PA_METHOD_LOCAL;
+        uint32_t l1 = uint32_t(b.off1[p + 1] - b.off1[p]), l2 = uint32_t(b.off2[p + 1] - b.off2[p]);
+        PairMeta &pm = plan.pairs[p];
+        pm.local = local;
+        bool swapped = local ? (l1 > l2) : (symmetric ? (l1 > l2) : swap_all);
+        pm.swapped = swapped;
+        pm.m = swapped ? l2 : l1;
+        pm.n = swapped ? l1 : l2;
+        pm.width = 0;
+        if (!selected[p] || pm.m == 0 || pm.n == 0) continue;
+        plan.cells += uint64_t(pm.m) * pm.n;
+        const RangeBound rb = range_bound(plan, local, pm.m, pm.n);
+        if (rb.lower < INT32_MIN / 2 || rb.upper > INT32_MAX / 2) { err = "scores may leave the int32 range"; return PA_ERR_RANGE; }
+        const bool lower16 = rb.lower > -32700;
+        const bool upper16 = rb.upper < 32767;
+        uint8_t width;
+        if (forced_width && forced_width[p]) width = forced_width[p];
+        else if (cfg.lane_policy == PA_LANES_INT32) width = 32;
+        else if (cfg.lane_policy == PA_LANES_INT16 || cfg.lane_policy == PA_LANES_LEARNED) width = lower16 ? 16 : 32;
+        else width = (lower16 && upper16) ? 16 : 32;
+        if (width == 16 && !lower16) width = 32;   // the lower bound is never speculated on
+        pm.width = width;
+        VariantKey key{};
+        key.width = width;
+        choose_shape(pm.m, width, key.G, key.R);
+        key.score = plan.score;
+        key.local = local;
+        key.track = (width == 16 && !local && !upper16) ? 1 : 0;
+        key.rowmax = (!local && lc && lr) ? 1 : 0;
+        const uint32_t rows = uint32_t(key.G) * key.R;
+        groups[key].push_back(Item{uint32_t(p), pm.m, pm.n, (pm.m + rows - 1) / rows});
+        if (width == 16) ++plan.n16; else ++plan.n32;
+    }
+
+    // Residue code of row/column symbol for one half.
+    auto row_code = [&](uint8_t sym, uint32_t width) -> uint32_t {
+        const uint32_t c = uint32_t(alpha.code[sym]);
+        if (plan.score == kScoreDna) return width == 16 ? dna_row16(c) : dna_row32(c);
+        return c * plan.stride * 2;
+    };
+    auto col_code = [&](uint8_t sym, uint32_t width) -> uint32_t {
+        const uint32_t c = uint32_t(alpha.code[sym]);
+        if (plan.score == kScoreDna) return width == 16 ? dna_col16(c) : dna_col32(c);
+        return c * 2;
+    };
+    const uint32_t pad_rank = plan.alphabet - 1;
+    auto row_pad = [&](uint32_t width) -> uint32_t {
+        if (plan.score == kScoreDna) return width == 16 ? kDnaRowPad16 : kDnaRowPad32;
+        return pad_rank * plan.stride * 2;
+    };
+    auto col_pad = [&](uint32_t width) -> uint32_t {
+        if (plan.score == kScoreDna) return width == 16 ? kDnaColPad16 : kDnaColPad32;
+        return pad_rank * 2;
+    };
+
+    for (auto &[key, items] : groups) {
+        // Pack int16 pairs two per task: similar stripe count and column count side by side.
+        std::sort(items.begin(), items.end(), [](const Item &a, const Item &c) {
+            return std::tie(a.S, a.n, a.m, a.pair) > std::tie(c.S, c.n, c.m, c.pair);
+        });
+        std::vector<std::pair<int64_t, int64_t>> packs;  // indices into items
+        if (key.width == 16) {
+            size_t i = 0;
+            while (i < items.size()) {
+                if (i + 1 < items.size() && (!key.rowmax || items[i].n == items[i + 1].n)) {
+                    packs.push_back({int64_t(i), int64_t(i + 1)});
+                    i += 2;
+                } else {
+                    packs.push_back({int64_t(i), -1});
+                    i += 1;
+                }
+            }
+        } else {
+            for (size_t i = 0; i < items.size(); ++i) packs.push_back({int64_t(i), -1});
+        }
+        const uint32_t G = key.G, R = key.R, rows = G * R;
+        auto work = [&](const std::pair<int64_t, int64_t> &pk) {
+            const Item &a = items[pk.first];
+            uint32_t S = a.S, n = a.n;
+            if (pk.second >= 0) { S = std::max(S, items[pk.second].S); n = std::max(n, items[pk.second].n); }
+            return uint64_t(S) * (n + G - 1);
+        };
+        // Longest tasks first so that the tail of the launch is made of short tasks.
+        std::stable_sort(packs.begin(), packs.end(), [&](auto &x, auto &y) { return work(x) > work(y); });
+
+        Launch L{key, uint32_t(plan.tasks.size()), 0, 0};
+        for (auto &pk : packs) {
+            const Item *h[2] = {&items[pk.first], pk.second >= 0 ? &items[pk.second] : nullptr};
+            Task t{};
+            t.S = std::max(h[0]->S, h[1] ? h[1]->S : 0u);
+            t.N = std::max(h[0]->n, h[1] ? h[1]->n : 0u);
+            for (int q = 0; q < 2; ++q) {
+                t.pair[q] = h[q] ? h[q]->pair : kNone;
+                t.m[q] = h[q] ? h[q]->m : 0;
+                t.n[q] = h[q] ? h[q]->n : 0;
+            }
+            t.row_off = uint32_t(plan.rowcodes.size());
+            t.col_off = uint32_t(plan.colcodes.size());
+            t.bnd_off = kNone;
+            const uint32_t cw = col_words(t.N, G);
+            if (t.S > 1) {
+                t.bnd_off = uint32_t(plan.bnd_entries);
+                plan.bnd_entries += cw;
+            }
+            // rows
+            const uint32_t nrows = t.S * rows;
+            const size_t r0 = plan.rowcodes.size();
+            plan.rowcodes.resize(r0 + nrows);
+            for (uint32_t i = 0; i < nrows; ++i) {
+                uint32_t w = 0;
+                for (int q = 0; q < (key.width == 16 ? 2 : 1); ++q) {
+                    uint32_t code;
+                    if (h[q] && i < h[q]->m) {
+                        const PairMeta &pm = plan.pairs[h[q]->pair];
+                        const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h[q]->pair] : b.seq1 + b.off1[h[q]->pair];
+                        code = row_code(s[i], key.width);
+                    } else {
+                        code = row_pad(key.width);
+                    }
+                    w |= key.width == 16 ? (code & 0xFFFFu) << (16 * q) : code;
+                }
+                plan.rowcodes[r0 + i] = w;
+            }
+            // columns
+            const size_t c0 = plan.colcodes.size();
+            plan.colcodes.resize(c0 + cw);
+            for (uint32_t j = 0; j < cw; ++j) {
+                uint32_t w = 0;
+                for (int q = 0; q < (key.width == 16 ? 2 : 1); ++q) {
+                    uint32_t code;
+                    if (h[q] && j < h[q]->n) {
+                        const PairMeta &pm = plan.pairs[h[q]->pair];
+                        const uint8_t *s = pm.swapped ? b.seq1 + b.off1[h[q]->pair] : b.seq2 + b.off2[h[q]->pair];
+                        code = col_code(s[j], key.width);
+                    } else {
+                        code = col_pad(key.width);
+                    }
+                    w |= key.width == 16 ? (code & 0xFFFFu) << (16 * q) : code;
+                }
+                plan.colcodes[c0 + j] = w;
+            }
+            const uint64_t pc = uint64_t(t.S) * rows * t.N;
+            L.padded_cells += pc;
+            plan.padded_cells += pc;
+            plan.tasks.push_back(t);
+        }
+        L.task_end = uint32_t(plan.tasks.size());
+        if (L.task_end > L.task_begin) plan.launches.push_back(L);
+    }
+    if (plan.rowcodes.size() > 0xFFFFFFF0ull || plan.colcodes.size() > 0xFFFFFFF0ull || plan.bnd_entries > 0x7FFFFFF0ull) {
+        err = "batch too large for one plan; split it";
+        return PA_ERR_INVALID_ARGUMENT;
+    }
+    return PA_OK;
+}
+
+} // namespace pa

@@ -1,0 +1,137 @@
+// pa_plan.hpp -- host-side planning for the striped DP kernels (plain C++17, no HIP).
+//
+// The planner turns a batch of pairs plus a pa_config into
+//   * per-pair metadata (sizes after orientation, method, lane width),
+//   * tasks: one task = one group of G lanes sweeping one pair (int32) or two pairs
+//     packed into the 16-bit halves of each 32-bit lane (int16),
+//   * transposed, pre-encoded residue streams (one 32-bit word per DP row and per DP column of
+//     a task, the two halves holding the two pairs), so every load in the kernel is one dword,
+//   * launch groups: contiguous task ranges that share one kernel variant.
+//
+// Layout of one task in HBM (offsets are in 32-bit words):
+//   rowcodes[row_off + s*G*R + t*R + r]  residue of DP row i = s*G*R + t*R + r + 1 (stripe s,
+//                                         lane t, register r), padded to S*G*R rows
+//   colcodes[col_off + j]                residue of DP column j+1, padded to col_words(N) words
+//   bnd[2*(bnd_off + j) + {0,1}]          H and vertical-gap state below the current stripe at
+//                                         column j+1 (multi-stripe tasks only)
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pa_device.h"
+
+namespace pa {
+
+inline constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+enum ScoreCode : uint8_t { kScoreDna = 0, kScoreMatrix = 1 };
+
+// Residue encodings.  Unitary scoring over at most four symbols uses a byte-permute lookup:
+// score = perm(tbl_hi, tbl_lo, rowcode ^ colcode), where the xor of the two codes yields the
+// selector bytes of the 16-bit score for "equal" (entry 0) or "different" (entries 1..3).
+//   int16 half : row (2c)|(2c<<8),           column (2c)|((2c+1)<<8)
+//   int32 word : row 2c|2c<<8|c<<16|c<<24,   column 2c|(2c+1)<<8|(8+c)<<16|(8+c)<<24
+//                (selector bytes 8..11 replicate the sign of the selected 16-bit entry)
+// Padding codes make every pad cell score -1 (selector bytes >= 13 give 0xFF).
+inline constexpr uint32_t kDnaRowPad16 = 0x1010u, kDnaColPad16 = 0x2020u;
+inline constexpr uint32_t kDnaRowPad32 = 0x10101010u, kDnaColPad32 = 0x20202020u;
+
+inline uint32_t dna_row16(uint32_t c) { return (2 * c) | ((2 * c) << 8); }
+inline uint32_t dna_col16(uint32_t c) { return (2 * c) | ((2 * c + 1) << 8); }
+inline uint32_t dna_row32(uint32_t c) { return (2 * c) | ((2 * c) << 8) | (c << 16) | (c << 24); }
+inline uint32_t dna_col32(uint32_t c) { return (2 * c) | ((2 * c + 1) << 8) | ((8 + c) << 16) | ((8 + c) << 24); }
+
+// Matrix scoring: an int16 table in LDS with row stride `stride` entries; the symbol of rank
+// K (the alphabet size) is the pad symbol and scores -1 against everything.
+//   row code = rank * stride * 2 (byte offset of the table row), column code = rank * 2.
+// int16 tasks keep the two halves' codes in the two 16-bit halves of one word.
+
+struct Task {
+    uint32_t row_off;
+    uint32_t col_off;
+    uint32_t bnd_off;   // kNone if S == 1
+    uint32_t N;         // DP columns swept (max over the halves)
+    uint32_t S;         // stripes of G*R rows
+    uint32_t pair[2];   // output index per half, kNone if the half is unused
+    uint32_t m[2];      // DP rows of each half (>= 1)
+    uint32_t n[2];      // DP columns of each half (>= 1)
+    uint32_t pad_;
+};
+static_assert(sizeof(Task) == 48, "Task layout is shared with the device");
+
+struct VariantKey {
+    uint8_t width;   // 16 (two pairs per lane, saturating int16) or 32 (one pair per lane, int32)
+    uint8_t G;       // lanes per group
+    uint8_t R;       // rows per lane
+    uint8_t score;   // ScoreCode
+    uint8_t local;   // method_local
+    uint8_t track;   // int16 global: keep the running maximum to detect saturation
+    uint8_t rowmax;  // global with trailing gaps free in both dimensions: per-row maxima
+    bool operator<(const VariantKey &o) const;
+    bool operator==(const VariantKey &o) const;
+};
+
+struct Launch {
+    VariantKey key;
+    uint32_t task_begin, task_end;
+    uint64_t padded_cells;
+};
+
+struct PairMeta {
+    uint32_t m, n;        // after orientation: m rows (seq1 side), n columns
+    uint8_t local;
+    uint8_t swapped;      // rows/columns exchanged relative to the caller's seq1/seq2
+    uint8_t width;        // 0 (empty: closed form), 16 or 32
+    uint8_t pad_;
+};
+
+struct Plan {
+    // scoring
+    int32_t go = 0, ge = 0;
+    int32_t fc = 0, fr = 0, lc = 0, lr = 0;   // after orientation
+    uint8_t score = kScoreDna;
+    uint32_t tbl_lo = 0, tbl_hi = 0;          // DNA byte-permute table
+    uint32_t stride = 0, alphabet = 0;        // matrix table geometry (alphabet includes the pad)
+    std::vector<int16_t> table;               // matrix: alphabet*stride entries
+    int32_t smax = 0, smin = 0;               // extreme substitution scores
+
+    std::vector<PairMeta> pairs;
+    std::vector<Task> tasks;
+    std::vector<Launch> launches;
+    std::vector<uint32_t> rowcodes, colcodes;
+    uint64_t bnd_entries = 0;                 // (H,E) entries of the boundary buffer
+    uint64_t cells = 0, padded_cells = 0;
+    uint64_t n16 = 0, n32 = 0;
+};
+
+struct BatchView {
+    const uint8_t *seq1;
+    const uint64_t *off1;
+    const uint8_t *seq2;
+    const uint64_t *off2;
+    uint64_t n_pairs;
+    const uint8_t *pair_method;  // nullable
+};
+
+// Widths chosen per pair before packing: 16 or 32 (0 = let the policy decide).
+// `only` restricts planning to a subset of pairs (nullptr = all).
+pa_status make_plan(const pa_config &cfg, const BatchView &batch, const uint8_t *forced_width,
+                    const std::vector<uint32_t> *only, Plan &plan, std::string &err);
+
+// Number of colcodes / bnd words reserved per task: the kernels prefetch up to three
+// conveyor chunks of G words past the last column.
+inline uint32_t col_words(uint32_t N, uint32_t G) { return ((N + G - 1) / G) * G + 3 * G; }
+
+// Conservative int16 range analysis of one pair (rows m, columns n).
+struct RangeBound {
+    int64_t lower;   // every real H and gap state is >= lower
+    int64_t upper;   // every real H is <= upper
+};
+RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n);
+
+// Choice of (G, R) for a pair with m rows.
+void choose_shape(uint32_t m, uint32_t width, uint8_t &G, uint8_t &R);
+
+} // namespace pa

@@ -1,0 +1,228 @@
+"""ctypes binding of the C ABI in include/pa_device.h (libpa_device.so, built for gfx950).
+
+There is no CPU fallback: if the library is missing or no GPU is visible, every entry point
+raises.  Sequences are passed as concatenated bytes plus uint64 offsets.
+"""
+from __future__ import annotations
+
+import ctypes
+import functools
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpa_device.so")
+
+PA_OK = 0
+STATUS_NAMES = {0: "PA_OK", 1: "PA_ERR_INVALID_ARGUMENT", 2: "PA_ERR_INVALID_SYMBOL", 3: "PA_ERR_UNSUPPORTED",
+                4: "PA_ERR_DEVICE", 5: "PA_ERR_OUT_OF_MEMORY", 6: "PA_ERR_RANGE"}
+METHOD_GLOBAL, METHOD_LOCAL = 0, 1
+SCORE_UNITARY, SCORE_MATRIX = 0, 1
+LANES_AUTO, LANES_LEARNED, LANES_INT16, LANES_INT32 = 0, 1, 16, 32
+PAIR_INT16, PAIR_INT32, PAIR_RECOMPUTED, PAIR_EMPTY = 0x01, 0x02, 0x04, 0x08
+
+# Every symbol include/pa_device.h declares.
+EXPORTS = ["pa_abi_version", "pa_ctx_create", "pa_ctx_destroy", "pa_ctx_last_error", "pa_score_batch",
+           "pa_batch_create", "pa_batch_run", "pa_batch_results", "pa_batch_stats_get", "pa_batch_time",
+           "pa_batch_destroy"]
+
+
+class PaError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class CConfig(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int32),
+        ("first_column_free", ctypes.c_int32),
+        ("first_row_free", ctypes.c_int32),
+        ("last_column_free", ctypes.c_int32),
+        ("last_row_free", ctypes.c_int32),
+        ("gap_open", ctypes.c_int32),
+        ("gap_extension", ctypes.c_int32),
+        ("score_kind", ctypes.c_int32),
+        ("match", ctypes.c_int32),
+        ("mismatch", ctypes.c_int32),
+        ("matrix_dim", ctypes.c_int32),
+        ("matrix", ctypes.POINTER(ctypes.c_int32)),
+        ("matrix_symbols", ctypes.POINTER(ctypes.c_uint8)),
+        ("lane_policy", ctypes.c_int32),
+    ]
+
+
+class CStats(ctypes.Structure):
+    _fields_ = [("n_pairs", ctypes.c_uint64), ("cells", ctypes.c_uint64), ("padded_cells", ctypes.c_uint64),
+                ("n_int16", ctypes.c_uint64), ("n_int32", ctypes.c_uint64), ("n_recomputed", ctypes.c_uint64),
+                ("n_launches", ctypes.c_uint64), ("plan_ms", ctypes.c_double)]
+
+
+@dataclass(frozen=True)
+class DeviceConfig:
+    """Flat form of an aligner configuration, one-to-one with pa_config."""
+    method: int = METHOD_GLOBAL
+    first_column_free: bool = False
+    first_row_free: bool = False
+    last_column_free: bool = False
+    last_row_free: bool = False
+    gap_open: int = -10
+    gap_extension: int = -1
+    score_kind: int = SCORE_UNITARY
+    match: int = 4
+    mismatch: int = -5
+    symbols: bytes = b""
+    matrix: Optional[tuple] = None
+    lane_policy: int = LANES_AUTO
+
+
+@functools.lru_cache(maxsize=1)
+def lib() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"device library missing: {LIB_PATH}; run __graft_entry__.build() (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+    L.pa_abi_version.restype = i32
+    L.pa_ctx_create.argtypes = [ctypes.POINTER(vp), i32]
+    L.pa_ctx_destroy.argtypes = [vp]
+    L.pa_ctx_last_error.argtypes = [vp]
+    L.pa_ctx_last_error.restype = ctypes.c_char_p
+    L.pa_score_batch.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp, vp]
+    L.pa_batch_create.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, ctypes.POINTER(vp)]
+    L.pa_batch_run.argtypes = [vp, vp]
+    L.pa_batch_results.argtypes = [vp, vp, vp, vp]
+    L.pa_batch_stats_get.argtypes = [vp, ctypes.POINTER(CStats)]
+    L.pa_batch_time.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.pa_batch_destroy.argtypes = [vp, vp]
+    for name in ["pa_ctx_create", "pa_ctx_destroy", "pa_score_batch", "pa_batch_create", "pa_batch_run",
+                 "pa_batch_results", "pa_batch_stats_get", "pa_batch_time", "pa_batch_destroy"]:
+        getattr(L, name).restype = i32
+    return L
+
+
+def pack_sequences(seqs: Sequence[bytes]):
+    lens = np.fromiter((len(s) for s in seqs), dtype=np.uint64, count=len(seqs))
+    off = np.zeros(len(seqs) + 1, dtype=np.uint64)
+    np.cumsum(lens, out=off[1:])
+    joined = b"".join(seqs)
+    buf = np.frombuffer(joined, dtype=np.uint8).copy() if joined else np.zeros(1, dtype=np.uint8)
+    return buf, off
+
+
+class _CConfigHolder:
+    """Keeps the ctypes arrays a CConfig points to alive."""
+
+    def __init__(self, cfg: DeviceConfig):
+        self.c = CConfig(cfg.method, int(cfg.first_column_free), int(cfg.first_row_free),
+                         int(cfg.last_column_free), int(cfg.last_row_free), cfg.gap_open, cfg.gap_extension,
+                         cfg.score_kind, cfg.match, cfg.mismatch, 0, None, None, cfg.lane_policy)
+        if cfg.score_kind == SCORE_MATRIX:
+            self.mat = (ctypes.c_int32 * len(cfg.matrix))(*cfg.matrix)
+            self.sym = (ctypes.c_uint8 * len(cfg.symbols))(*cfg.symbols)
+            self.c.matrix_dim = len(cfg.symbols)
+            self.c.matrix = self.mat
+            self.c.matrix_symbols = self.sym
+
+
+class Context:
+    """One HIP device + stream (pa_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = lib()
+        h = ctypes.c_void_p()
+        st = self._lib.pa_ctx_create(ctypes.byref(h), device)
+        if st != PA_OK:
+            raise PaError(st, f"cannot create a context on device {device}")
+        self.handle = h
+
+    def error(self) -> str:
+        return self._lib.pa_ctx_last_error(self.handle).decode()
+
+    def check(self, st: int):
+        if st != PA_OK:
+            raise PaError(st, self.error())
+
+    def close(self):
+        if self.handle:
+            self._lib.pa_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def score(self, seqs1: Sequence[bytes], seqs2: Sequence[bytes], cfg: DeviceConfig, pair_method=None):
+        b1, o1 = pack_sequences(seqs1)
+        b2, o2 = pack_sequences(seqs2)
+        return self.score_packed(b1, o1, b2, o2, cfg, pair_method)
+
+    def score_packed(self, b1, o1, b2, o2, cfg: DeviceConfig, pair_method=None):
+        n = len(o1) - 1
+        holder = _CConfigHolder(cfg)
+        scores = np.zeros(max(n, 1), dtype=np.int32)
+        flags = np.zeros(max(n, 1), dtype=np.uint8)
+        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
+        st = self._lib.pa_score_batch(self.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
+                                      b2.ctypes.data, o2.ctypes.data, n, None if pm is None else pm.ctypes.data,
+                                      scores.ctypes.data, flags.ctypes.data)
+        self.check(st)
+        return scores[:n], flags[:n]
+
+
+class Batch:
+    """A device-resident batch (pa_batch): planned and uploaded once, run many times."""
+
+    def __init__(self, ctx: Context, b1, o1, b2, o2, cfg: DeviceConfig, pair_method=None):
+        self.ctx = ctx
+        self.n = len(o1) - 1
+        self._keep = (b1, o1, b2, o2)
+        holder = _CConfigHolder(cfg)
+        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
+        h = ctypes.c_void_p()
+        st = ctx._lib.pa_batch_create(ctx.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
+                                      b2.ctypes.data, o2.ctypes.data, self.n,
+                                      None if pm is None else pm.ctypes.data, ctypes.byref(h))
+        ctx.check(st)
+        self.handle = h
+
+    def run(self):
+        self.ctx.check(self.ctx._lib.pa_batch_run(self.ctx.handle, self.handle))
+
+    def results(self):
+        scores = np.zeros(max(self.n, 1), dtype=np.int32)
+        flags = np.zeros(max(self.n, 1), dtype=np.uint8)
+        self.ctx.check(self.ctx._lib.pa_batch_results(self.ctx.handle, self.handle, scores.ctypes.data,
+                                                      flags.ctypes.data))
+        return scores[:self.n], flags[:self.n]
+
+    def stats(self) -> dict:
+        s = CStats()
+        self.ctx.check(self.ctx._lib.pa_batch_stats_get(self.handle, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in CStats._fields_}
+
+    def time(self, iters: int):
+        tot, main = ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(self.ctx._lib.pa_batch_time(self.ctx.handle, self.handle, iters, ctypes.byref(tot),
+                                                   ctypes.byref(main)))
+        return tot.value, main.value
+
+    def close(self):
+        if self.handle:
+            self.ctx._lib.pa_batch_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@functools.lru_cache(maxsize=None)
+def default_context(device: int = 0) -> Context:
+    return Context(device)

@@ -1,0 +1,56 @@
+"""CPU-side checks of the device library: it loads, exports every symbol include/pa_device.h
+declares, and the configuration mirror validates rule chains like the reference."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from pairwise_aligner_amd import cfg, device, tables
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "pa_device.h")).read()
+    return sorted(set(re.findall(r"\b(pa_\w+)\s*\(", text)))
+
+
+def test_header_declares_the_binding_list():
+    assert declared_symbols() == sorted(device.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    lib = ctypes.CDLL(device.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    assert lib.pa_abi_version() == 1
+
+
+def test_configure_aligner_requires_score_and_gap_models():
+    with pytest.raises(TypeError):
+        cfg.configure_aligner(cfg.method_global(cfg.gap_model_affine(-10, -1)))
+    with pytest.raises(TypeError):
+        cfg.configure_aligner(cfg.score_model_unitary(4, -5))
+
+
+def test_configure_aligner_translation():
+    a = cfg.configure_aligner(cfg.method_global(cfg.gap_model_affine(cfg.score_model_unitary(4, -5), -10, -1),
+                                                cfg.leading_end_gap(first_row=cfg.end_gap.free),
+                                                cfg.trailing_end_gap(last_row=cfg.end_gap.free)))
+    c = a.config
+    assert (c.method, c.first_column_free, c.first_row_free, c.last_column_free, c.last_row_free) == \
+        (device.METHOD_GLOBAL, False, True, False, True)
+    assert (c.gap_open, c.gap_extension, c.match, c.mismatch) == (-10, -1, 4, -5)
+    loc = cfg.configure_aligner(cfg.score_model_matrix(cfg.method_local(cfg.gap_model_affine(-11, -1)),
+                                                       tables.blosum62_extended())).config
+    assert loc.method == device.METHOD_LOCAL and loc.first_column_free and loc.last_row_free
+    assert loc.symbols == b"ABCDEFGHIJKLMNOPQRSTUVWXYZ*" and len(loc.matrix) == 27 * 27
+
+
+def test_no_gpu_means_a_loud_error():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(device.PaError):
+        device.Context(0)

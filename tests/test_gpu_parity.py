@@ -1,0 +1,178 @@
+"""GPU parity: the device library against the CPU oracle, bit-exact, through the C ABI.
+
+Mirrors the reference's own test strategy: the known-answer vectors of
+test/unit/affine/*_scalar*_test.cpp through the configuration API, and seeded differential
+tests (alignment_simd_test_template.hpp:195-227) over every method, end-gap combination, score
+model and lane width, with equal and variable lengths, empty sequences and saturating pairs.
+"""
+import itertools
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pairwise_aligner_amd import cfg, device, tables, workloads
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")
+
+
+def log(msg):
+    sys.stdout.write(msg + "\n")
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return device.Context(0)
+
+
+def oracle_cfg(dc: device.DeviceConfig, method=None) -> O.OracleConfig:
+    return O.OracleConfig(mode=dc.method if method is None else method, first_column_free=dc.first_column_free,
+                          first_row_free=dc.first_row_free, last_column_free=dc.last_column_free,
+                          last_row_free=dc.last_row_free, gap_open=dc.gap_open, gap_extension=dc.gap_extension,
+                          score_kind=dc.score_kind, match=dc.match, mismatch=dc.mismatch, symbols=dc.symbols,
+                          matrix=dc.matrix)
+
+
+def check(ctx, s1, s2, dc, pair_method=None, label=""):
+    log(f"case {label}: {len(s1)} pairs, cfg={dc}")
+    got, flags = ctx.score(s1, s2, dc, pair_method)
+    if pair_method is None:
+        want = O.score_pairs(s1, s2, oracle_cfg(dc), threads=8)
+    else:
+        want = O.score_pairs(s1, s2, oracle_cfg(dc), pair_method=pair_method, threads=8)
+    bad = np.nonzero(got.astype(np.int64) != want)[0]
+    assert len(bad) == 0, (f"{label}: {len(bad)} mismatches; first {bad[:5].tolist()}: device "
+                           f"{got[bad[:5]].tolist()} oracle {want[bad[:5]].tolist()} "
+                           f"lens {[(len(s1[i]), len(s2[i])) for i in bad[:5]]}")
+    return got, flags
+
+
+def test_known_answers_through_cfg_api():
+    vecs = json.load(open(GOLDEN))
+    for v in vecs:
+        c = v["config"]
+        lead = cfg.leading_end_gap(cfg.end_gap(c["first_column_free"]), cfg.end_gap(c["first_row_free"]))
+        trail = cfg.trailing_end_gap(cfg.end_gap(c["last_column_free"]), cfg.end_gap(c["last_row_free"]))
+        gap = cfg.gap_model_affine(c["gap_open"], c["gap_extension"])
+        method = cfg.method_local(gap) if c["mode"] == "local" else cfg.method_global(gap, lead, trail)
+        sc = c["score"]
+        rule = (cfg.score_model_unitary(method, sc["match"], sc["mismatch"]) if sc["kind"] == "unitary"
+                else cfg.score_model_matrix(method, getattr(tables, sc["matrix"])()))
+        aligner = cfg.configure_aligner(rule)
+        log(f"known answer {v['name']}")
+        assert aligner.compute(v["seq1"], v["seq2"]).score() == v["expected"], v["source"]
+
+
+@pytest.mark.parametrize("method", [device.METHOD_GLOBAL, device.METHOD_LOCAL])
+@pytest.mark.parametrize("lengths", [(1, 20), (140, 160), (10, 300), (300, 700)])
+def test_dna_unitary(ctx, method, lengths):
+    rng = np.random.default_rng(100 + lengths[0] + method)
+    s1, s2 = workloads.dna_pairs(rng, 2000, lengths[0], lengths[1], divergence=0.15)
+    dc = device.DeviceConfig(method=method, match=4, mismatch=-5, gap_open=-10, gap_extension=-1)
+    check(ctx, s1, s2, dc, label=f"dna method={method} lengths={lengths}")
+
+
+@pytest.mark.parametrize("flags", list(itertools.product([False, True], repeat=4)),
+                         ids=lambda f: "".join("F" if x else "p" for x in f))
+def test_end_gaps(ctx, flags):
+    fc, fr, lc, lr = flags
+    rng = np.random.default_rng(7)
+    s1, s2 = workloads.dna_pairs(rng, 600, 0, 90, divergence=0.3)
+    # unrelated pairs too, so that free ends matter
+    u1, u2 = workloads.dna_pairs(rng, 300, 0, 90, divergence=0.0)
+    s2 = s2[:300] + u2[::-1]
+    dc = device.DeviceConfig(first_column_free=fc, first_row_free=fr, last_column_free=lc, last_row_free=lr,
+                             match=4, mismatch=-5, gap_open=-10, gap_extension=-1)
+    check(ctx, s1, s2, dc, label=f"end gaps {flags}")
+
+
+@pytest.mark.parametrize("method", [device.METHOD_GLOBAL, device.METHOD_LOCAL])
+def test_blosum62(ctx, method):
+    rng = np.random.default_rng(55 + method)
+    s1, s2 = workloads.protein_pairs(rng, 1500, 0, 420)
+    t = tables.blosum62_standard()
+    dc = device.DeviceConfig(method=method, score_kind=device.SCORE_MATRIX, symbols="".join(s for s, _ in t).encode(),
+                             matrix=tuple(v for _, r in t for v in r), gap_open=-11, gap_extension=-1)
+    check(ctx, s1, s2, dc, label=f"blosum62 method={method}")
+
+
+def test_unitary_many_symbols(ctx):
+    """More than four distinct symbols (e.g. N): unitary equality over an LDS table."""
+    rng = np.random.default_rng(9)
+    s1, s2 = workloads.dna_pairs(rng, 800, 0, 200, divergence=0.2)
+    s1 = [s.replace(b"A", b"N", 3) for s in s1]
+    for method in (device.METHOD_GLOBAL, device.METHOD_LOCAL):
+        dc = device.DeviceConfig(method=method, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
+        check(ctx, s1, s2, dc, label=f"unitary with N method={method}")
+
+
+@pytest.mark.parametrize("policy", [device.LANES_INT32, device.LANES_INT16, device.LANES_AUTO])
+def test_lane_policies(ctx, policy):
+    rng = np.random.default_rng(13)
+    s1, s2 = workloads.dna_pairs(rng, 1000, 50, 400, divergence=0.1)
+    pm = (np.arange(1000) % 2).astype(np.uint8)
+    dc = device.DeviceConfig(match=2, mismatch=-3, gap_open=-5, gap_extension=-2, lane_policy=policy)
+    got, flags = check(ctx, s1, s2, dc, pair_method=pm, label=f"policy {policy}")
+    if policy == device.LANES_INT32:
+        assert np.all(flags & device.PAIR_INT32)
+
+
+def test_saturation_recompute(ctx):
+    """Scores beyond int16: identical 9 kbp sequences with match +4 (score 36,000) in int16 lanes
+    must be flagged, recomputed in int32 and come out exact."""
+    rng = np.random.default_rng(21)
+    base = workloads.unpack(*workloads.random_packed(rng, np.full(6, 9000), workloads.DNA))
+    short1, short2 = workloads.dna_pairs(rng, 40, 100, 200, divergence=0.05)
+    s1 = base + short1
+    s2 = list(base) + short2
+    for method in (device.METHOD_GLOBAL, device.METHOD_LOCAL):
+        dc = device.DeviceConfig(method=method, match=4, mismatch=-5, gap_open=-10, gap_extension=-1,
+                                 lane_policy=device.LANES_INT16)
+        got, flags = check(ctx, s1, s2, dc, label=f"saturation method={method}")
+        assert np.all(flags[:6] & device.PAIR_RECOMPUTED), flags[:6]
+        assert np.all(got[:6] == 36000)
+        assert not np.any(flags[6:] & device.PAIR_RECOMPUTED)
+
+
+def test_empty_and_tiny(ctx):
+    s1 = [b"", b"A", b"", b"ACGT", b"G", b"", b"AC"]
+    s2 = [b"", b"", b"A", b"", b"G", b"ACGTT", b"CA"]
+    for flags in itertools.product([False, True], repeat=4):
+        fc, fr, lc, lr = flags
+        for method in (device.METHOD_GLOBAL, device.METHOD_LOCAL):
+            dc = device.DeviceConfig(method=method, first_column_free=fc, first_row_free=fr, last_column_free=lc,
+                                     last_row_free=lr)
+            check(ctx, s1, s2, dc, label=f"empty {flags} {method}")
+
+
+def test_invalid_symbol_is_an_error(ctx):
+    t = tables.blosum62_standard()
+    dc = device.DeviceConfig(score_kind=device.SCORE_MATRIX, symbols="".join(s for s, _ in t).encode(),
+                             matrix=tuple(v for _, r in t for v in r))
+    with pytest.raises(device.PaError) as e:
+        ctx.score([b"ACDZ"], [b"ACD"], dc)
+    assert e.value.status == 2
+
+
+def test_batch_repeat_runs(ctx):
+    rng = np.random.default_rng(31)
+    s1, s2 = workloads.dna_pairs(rng, 3000, 100, 200)
+    b1, o1 = workloads.pack(s1)
+    b2, o2 = workloads.pack(s2)
+    dc = device.DeviceConfig()
+    bt = device.Batch(ctx, b1, o1, b2, o2, dc)
+    bt.run()
+    first, _ = bt.results()
+    bt.run()
+    second, _ = bt.results()
+    assert np.array_equal(first, second)
+    want = O.score_pairs(s1, s2, oracle_cfg(dc), threads=8)
+    assert np.array_equal(first.astype(np.int64), want)
+    st = bt.stats()
+    assert st["n_pairs"] == 3000 and st["cells"] == sum(len(a) * len(b) for a, b in zip(s1, s2))
