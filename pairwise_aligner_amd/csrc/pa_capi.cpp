@@ -1,6 +1,6 @@
 // pa_capi.cpp -- implementation of the C ABI in include/pa_device.h on top of HIP.
 //
-// A context owns one device and one stream.  A batch owns the plan (pa_plan.hpp) and its
+// A context owns one device, one stream and a few side streams.  A batch owns the plan (pa_plan.hpp) and its
 // device buffers.  A pass enqueues the initial-score kernel and one striped-DP launch per
 // kernel variant; when an int16 launch can report saturation, the pass then reads the flags,
 // re-plans the flagged pairs at int32 (cached for the next pass) and runs them.
@@ -30,9 +30,16 @@ struct PlanDevice {
     std::vector<uint32_t> subset_host;
 };
 
+// Kernel variants of one pass run concurrently on kSide side streams (default priority), forked
+// from and joined back into the context stream with events, so that their tails overlap.
+static constexpr int kSide = 4;
+
 struct pa_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side[kSide] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kSide] = {};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string last_error;
 };
@@ -126,11 +133,31 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
 pa_status launch_plan(pa_ctx *ctx, pa_batch *b, PlanDevice &pd, bool timed)
 {
     if (timed) PA_HIP(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
-    for (const pa::Launch &L : pd.plan.launches) {
+    const size_t nl = pd.plan.launches.size();
+    const int nside = int(std::min<size_t>(nl, kSide));
+    if (nside > 1) {
+        PA_HIP(hipEventRecord(ctx->fork, ctx->stream), "hipEventRecord");
+        for (int i = 0; i < nside; ++i) PA_HIP(hipStreamWaitEvent(ctx->side[i], ctx->fork, 0), "hipStreamWaitEvent");
+    }
+    // launches are ordered by decreasing work: the largest start first
+    std::vector<size_t> order(nl);
+    for (size_t i = 0; i < nl; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return pd.plan.launches[x].padded_cells > pd.plan.launches[y].padded_cells;
+    });
+    for (size_t i = 0; i < nl; ++i) {
+        const pa::Launch &L = pd.plan.launches[order[i]];
         const pa::KernelArgs a = kernel_args(pd, b, L);
-        const uint32_t lds = uint32_t(pd.plan.table.size() * sizeof(int16_t));
-        hipError_t e = pa::launch_stripe(L.key, a, lds, ctx->stream);
+        const uint32_t lds = pd.plan.table.empty() ? 0 : pa::lds_table_bytes(pd.plan.alphabet, pd.plan.stride);
+        hipStream_t st = nside > 1 ? ctx->side[i % nside] : ctx->stream;
+        hipError_t e = pa::launch_stripe(L.key, a, lds, st);
         if (e != hipSuccess) return hip_fail(ctx, e, "striped DP launch");
+    }
+    if (nside > 1) {
+        for (int i = 0; i < nside; ++i) {
+            PA_HIP(hipEventRecord(ctx->join[i], ctx->side[i]), "hipEventRecord");
+            PA_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[i], 0), "hipStreamWaitEvent");
+        }
     }
     if (timed) {
         PA_HIP(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
@@ -219,8 +246,13 @@ pa_status pa_ctx_create(pa_ctx **out, int device)
     if (e != hipSuccess || device < 0 || device >= count) return PA_ERR_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return PA_ERR_DEVICE;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return PA_ERR_DEVICE;
+    for (auto &s : ctx->side)
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PA_ERR_DEVICE;
     for (auto &ev : ctx->ev)
         if (hipEventCreate(&ev) != hipSuccess) return PA_ERR_DEVICE;
+    if (hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess) return PA_ERR_DEVICE;
+    for (auto &ev : ctx->join)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return PA_ERR_DEVICE;
     *out = ctx.release();
     return PA_OK;
 }
@@ -230,6 +262,9 @@ pa_status pa_ctx_destroy(pa_ctx *ctx)
     if (!ctx) return PA_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     for (auto &ev : ctx->ev) if (ev) (void)hipEventDestroy(ev);
+    for (auto &ev : ctx->join) if (ev) (void)hipEventDestroy(ev);
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    for (auto &s : ctx->side) if (s) (void)hipStreamDestroy(s);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PA_OK;
@@ -269,8 +304,8 @@ pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1
     pa_status st = pa::make_plan(b->cfg, view, nullptr, nullptr, b->main.plan, err);
     if (st != PA_OK) return fail(ctx, st, err);
     const pa::Plan &p = b->main.plan;
-    for (const pa::Launch &L : p.launches)
-        if (L.key.width == 16 && (L.key.local || L.key.track)) b->can_saturate = true;
+    // Only int16 pairs whose bound does not exclude saturation need the flag read-back.
+    b->can_saturate = p.n_speculative > 0;
 
     if ((st = upload_plan(ctx, b->main)) != PA_OK) return st;
     if ((st = upload(ctx, b->pairs, p.pairs)) != PA_OK) return st;

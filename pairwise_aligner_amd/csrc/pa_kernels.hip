@@ -90,6 +90,18 @@ __device__ __forceinline__ int h0_value(int i, int free_flag, int go, int ge)
     return (i == 0 || free_flag) ? 0 : go + ge * i;
 }
 
+// Smallest capture column greater than `after` (INT32_MAX if none).
+__device__ __forceinline__ int next_event(const int cap[2], const int lcap[2], int after)
+{
+    int e = INT32_MAX;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        if (cap[q] > after && cap[q] < e) e = cap[q];
+        if (lcap[q] > after && lcap[q] < e) e = lcap[q];
+    }
+    return e;
+}
+
 template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
 __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 {
@@ -98,14 +110,20 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     extern __shared__ char lds_raw[];
 
     if constexpr (SCORE == kScoreMatrix) {
+        // 32 bank-swizzled copies: entry e for lane l at dword (e >> 1) * 32 + l % 32, half e & 1
         int16_t *lds16 = reinterpret_cast<int16_t *>(lds_raw);
-        for (uint32_t e = threadIdx.x; e < a.table_entries; e += blockDim.x) lds16[e] = a.table[e];
+        const uint32_t total = a.table_entries * 32;
+        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+            const uint32_t e = i >> 5, copy = i & 31;
+            lds16[(((e >> 1) << 5) + copy) * 2 + (e & 1)] = a.table[e];
+        }
         __syncthreads();
     }
 
     const uint32_t gid = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     if (gid >= a.ntasks) return;
     const int t = int(threadIdx.x % G);
+    const uint32_t lane_bank = (threadIdx.x & 31u) * 4u;   // this lane's copy of the LDS table
     const Task tk = a.tasks[gid];
     const int go = a.go, ge = a.ge;
     // method_local.hpp:41-49: a local alignment has every end gap free
@@ -127,8 +145,11 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         for (int r = 0; r < R; ++r) {
             const uint32_t code = a.rowcodes[tk.row_off + uint32_t(row0 + r)];
             if constexpr (SCORE == kScoreMatrix && W == 16) {
-                A[r] = code & 0xFFFFu;
-                Ahi[r] = code >> 16;
+                A[r] = (code & 0xFFFFu) + lane_bank;
+                Ahi[r] = (code >> 16) + lane_bank;
+            } else if constexpr (SCORE == kScoreMatrix) {
+                A[r] = code + lane_bank;
+                Ahi[r] = 0;
             } else {
                 A[r] = code;
                 Ahi[r] = 0;
@@ -139,7 +160,9 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         }
         V Hdiag = L::splat(h0_value(row0, fc, go, ge));
 
-        // capture positions of the two halves (global): column index or -1
+        // Capture events of the two halves (global), as column indices (-1: none).  cap: this
+        // lane owns row m and reads H(m, n) at column n; lcap (free last column): this lane
+        // reads the maximum over its real rows at column n.  `ev` is the next event column.
         int cap[2] = {-1, -1}, capr[2] = {0, 0};
         int lcap[2] = {-1, -1}, lvr[2] = {0, 0};
 #pragma unroll
@@ -149,6 +172,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             if (m - 1 >= row0 && m - 1 < row0 + R) { cap[q] = n - 1; capr[q] = m - 1 - row0; }
             if (lc && row0 < m) { lcap[q] = n - 1; lvr[q] = m - row0 < R ? m - row0 : R; }
         }
+        int ev = next_event(cap, lcap, -1);
 
         const bool has_bnd_in = s > 0;
         const bool has_bnd_out = s + 1 < tk.S;
@@ -167,7 +191,8 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         int hb_top = h0_value(1, fr, go, ge);   // H(0, k+1) for the first stripe
 
         const int nsteps = N + G - 1;
-        for (int k = 0; k < nsteps; ++k) {
+        int c = -t;   // DP column (0-based) this lane computes at step k
+        for (int k = 0; k < nsteps; ++k, ++c) {
             if (k > 0 && (k % G) == 0) {
                 convB = nextB;
                 nextB = colp[k + G + t];
@@ -182,9 +207,16 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 topH = convH;
                 topE = convE;
             } else {
-                const V h = L::splat(hb_top);
-                topH = L::bits(h);
-                topE = L::bits(L::add(h, vgogo));
+                // H(0, k+1) and the vertical gap state below it: uniform, computed in scalar
+                // registers.  Both are <= 0 because go + ge <= 0 and ge <= 0.
+                const int e = hb_top + go + ge;
+                if constexpr (W == 16) {
+                    topH = (uint32_t(hb_top < -32768 ? -32768 : hb_top) & 0xFFFFu) * 0x10001u;
+                    topE = (uint32_t(e < -32768 ? -32768 : e) & 0xFFFFu) * 0x10001u;
+                } else {
+                    topH = uint32_t(hb_top);
+                    topE = uint32_t(e);
+                }
                 hb_top = fr ? 0 : hb_top + ge;
             }
             const uint32_t Bin = dpp<kRowShr1>(convB, Bout);
@@ -195,8 +227,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 convH = dpp<kRowShl1>(convH, convH);
                 convE = dpp<kRowShl1>(convE, convE);
             }
-            const int c = k - t;
-            if (c >= 0 && c < N) {
+            if (uint32_t(c) < uint32_t(N)) {
                 V Eup = L::from(Ein);
                 V diag = Hdiag;
                 uint32_t Blo = Bin, Bhi = 0;
@@ -228,31 +259,31 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 Hout = L::bits(H[R - 1]);
                 Eout = L::bits(Eup);
                 Bout = Bin;
-                if (has_bnd_out && t == G - 1) {
-                    bndp[2 * c] = Hout;
-                    bndp[2 * c + 1] = Eout;
+                if (has_bnd_out) {
+                    if (t == G - 1) {
+                        bndp[2 * c] = Hout;
+                        bndp[2 * c + 1] = Eout;
+                    }
                 }
                 if constexpr (!LOCAL) {
-                    if (c == cap[0] || c == cap[1]) {
+                    if (c == ev) {
 #pragma unroll
                         for (int q = 0; q < L::NH; ++q) {
-                            if (c != cap[q]) continue;
-                            int v = 0;
+                            if (c == cap[q]) {
+                                int v = 0;
 #pragma unroll
-                            for (int r = 0; r < R; ++r) if (r == capr[q]) v = L::half(H[r], q);
-                            atomicMax(a.scores + tk.pair[q], v);
+                                for (int r = 0; r < R; ++r) if (r == capr[q]) v = L::half(H[r], q);
+                                atomicMax(a.scores + tk.pair[q], v);
+                            }
+                            if (c == lcap[q]) {
+                                int v = INT32_MIN;
+#pragma unroll
+                                for (int r = 0; r < R; ++r)
+                                    if (r < lvr[q]) { const int h = L::half(H[r], q); v = h > v ? h : v; }
+                                atomicMax(a.scores + tk.pair[q], v);
+                            }
                         }
-                    }
-                    if (c == lcap[0] || c == lcap[1]) {
-#pragma unroll
-                        for (int q = 0; q < L::NH; ++q) {
-                            if (c != lcap[q]) continue;
-                            int v = INT32_MIN;
-#pragma unroll
-                            for (int r = 0; r < R; ++r)
-                                if (r < lvr[q]) { const int h = L::half(H[r], q); v = h > v ? h : v; }
-                            atomicMax(a.scores + tk.pair[q], v);
-                        }
+                        ev = next_event(cap, lcap, c);
                     }
                 }
             }
@@ -367,9 +398,11 @@ static hipError_t dispatch_rows(const VariantKey &k, const KernelArgs &a, uint32
 {
     switch (k.R) {
     case 4: return dispatch_score<W, 4>(k, a, lds, st);
+    case 6: return dispatch_score<W, 6>(k, a, lds, st);
     case 8: return dispatch_score<W, 8>(k, a, lds, st);
     case 10: return dispatch_score<W, 10>(k, a, lds, st);
     case 12: return dispatch_score<W, 12>(k, a, lds, st);
+    case 14: return dispatch_score<W, 14>(k, a, lds, st);
     case 16: return dispatch_score<W, 16>(k, a, lds, st);
     default: return hipErrorInvalidValue;
     }

@@ -22,16 +22,24 @@ bool VariantKey::operator==(const VariantKey &o) const
 }
 
 // Rows per lane that kernels are instantiated for (lanes per group G = 16).
-static const uint8_t kRowsPerLane[] = {4, 8, 10, 12, 16};
+static const uint8_t kRowsPerLane[] = {4, 6, 8, 10, 12, 14, 16};
 
-void choose_shape(uint32_t m, uint32_t width, uint8_t &G, uint8_t &R)
+// Relative cost of one step of one lane: about 13 VALU instructions per cell and a fixed
+// per-step part (lane exchange, conveyors, loop control) of about 24.
+static constexpr double kCellCost = 13.0, kStepCost = 24.0;
+
+void choose_shape(uint32_t m, uint32_t n, uint32_t width, uint8_t &G, uint8_t &R)
 {
     (void)width;
     G = 16;
+    double best = 1e300;
+    R = 16;
     for (uint8_t r : kRowsPerLane) {
-        if (uint32_t(G) * r >= m) { R = r; return; }
+        const uint32_t rows = uint32_t(G) * r;
+        const uint32_t S = (m + rows - 1) / rows;
+        const double cost = double(S) * (double(n) + G - 1) * (kCellCost * r + kStepCost);
+        if (cost < best * 0.999) { best = cost; R = r; }
     }
-    R = 16;  // multi-stripe
 }
 
 RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n)
@@ -80,13 +88,13 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
             plan.tbl_hi = X | (X << 16);
             return PA_OK;
         }
-        if (k > 63) {
-            err = "unitary scoring over more than 63 distinct symbols is not supported on the device";
+        if (k > kMaxMatrixSymbols) {
+            err = "unitary scoring over more than 31 distinct symbols is not supported on the device";
             return PA_ERR_UNSUPPORTED;
         }
         plan.score = kScoreMatrix;
         plan.alphabet = k + 1;
-        plan.stride = plan.alphabet;
+        plan.stride = (plan.alphabet + 1) & ~1u;
         plan.table.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
         for (uint32_t a = 0; a < k; ++a)
             for (uint32_t c = 0; c < k; ++c)
@@ -98,15 +106,15 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
         return PA_ERR_INVALID_ARGUMENT;
     }
     const int32_t K = cfg.matrix_dim;
-    if (K < 1 || K > 63) {
-        err = "substitution matrix dimension must be 1..63";
+    if (K < 1 || K > int32_t(kMaxMatrixSymbols)) {
+        err = "substitution matrix dimension must be 1..31";
         return PA_ERR_UNSUPPORTED;
     }
     for (int32_t r = 0; r < K; ++r) alpha.code[cfg.matrix_symbols[r]] = int16_t(r);
     alpha.size = uint32_t(K);
     plan.score = kScoreMatrix;
     plan.alphabet = uint32_t(K) + 1;
-    plan.stride = plan.alphabet;
+    plan.stride = (plan.alphabet + 1) & ~1u;
     plan.table.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
     plan.smax = INT32_MIN;
     plan.smin = INT32_MAX;
@@ -197,7 +205,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         pm.width = width;
         VariantKey key{};
         key.width = width;
-        choose_shape(pm.m, width, key.G, key.R);
+        choose_shape(pm.m, pm.n, width, key.G, key.R);
         key.score = plan.score;
         key.local = local;
         key.track = (width == 16 && !local && !upper16) ? 1 : 0;
@@ -205,27 +213,28 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         const uint32_t rows = uint32_t(key.G) * key.R;
         groups[key].push_back(Item{uint32_t(p), pm.m, pm.n, (pm.m + rows - 1) / rows});
         if (width == 16) ++plan.n16; else ++plan.n32;
+        if (width == 16 && !upper16) ++plan.n_speculative;
     }
 
     // Residue code of row/column symbol for one half.
     auto row_code = [&](uint8_t sym, uint32_t width) -> uint32_t {
         const uint32_t c = uint32_t(alpha.code[sym]);
         if (plan.score == kScoreDna) return width == 16 ? dna_row16(c) : dna_row32(c);
-        return c * plan.stride * 2;
+        return lds_row_offset(c, plan.stride);
     };
     auto col_code = [&](uint8_t sym, uint32_t width) -> uint32_t {
         const uint32_t c = uint32_t(alpha.code[sym]);
         if (plan.score == kScoreDna) return width == 16 ? dna_col16(c) : dna_col32(c);
-        return c * 2;
+        return lds_col_offset(c);
     };
     const uint32_t pad_rank = plan.alphabet - 1;
     auto row_pad = [&](uint32_t width) -> uint32_t {
         if (plan.score == kScoreDna) return width == 16 ? kDnaRowPad16 : kDnaRowPad32;
-        return pad_rank * plan.stride * 2;
+        return lds_row_offset(pad_rank, plan.stride);
     };
     auto col_pad = [&](uint32_t width) -> uint32_t {
         if (plan.score == kScoreDna) return width == 16 ? kDnaColPad16 : kDnaColPad32;
-        return pad_rank * 2;
+        return lds_col_offset(pad_rank);
     };
 
     for (auto &[key, items] : groups) {
@@ -314,7 +323,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 }
                 plan.colcodes[c0 + j] = w;
             }
-            const uint64_t pc = uint64_t(t.S) * rows * t.N;
+            const uint64_t pc = uint64_t(t.S) * rows * t.N * (key.width == 16 ? 2 : 1);
             L.padded_cells += pc;
             plan.padded_cells += pc;
             plan.tasks.push_back(t);

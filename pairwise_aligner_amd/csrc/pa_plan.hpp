@@ -43,10 +43,18 @@ inline uint32_t dna_col16(uint32_t c) { return (2 * c) | ((2 * c + 1) << 8); }
 inline uint32_t dna_row32(uint32_t c) { return (2 * c) | ((2 * c) << 8) | (c << 16) | (c << 24); }
 inline uint32_t dna_col32(uint32_t c) { return (2 * c) | ((2 * c + 1) << 8) | ((8 + c) << 16) | ((8 + c) << 24); }
 
-// Matrix scoring: an int16 table in LDS with row stride `stride` entries; the symbol of rank
-// K (the alphabet size) is the pad symbol and scores -1 against everything.
-//   row code = rank * stride * 2 (byte offset of the table row), column code = rank * 2.
+// Matrix scoring: an int16 table [alphabet][stride] (stride even); the symbol of rank K (the
+// alphabet size without the pad) is the pad symbol and scores -1 against everything.  In LDS
+// the table is replicated 32 times so that lane l of every half-wave reads only bank l % 32:
+// entry e lives at byte ((e >> 1) * 32 + l % 32) * 4 + (e & 1) * 2.  With e = a*stride + b and
+// an even stride that address splits into a row part, a column part and a lane part:
+//   row code = a * (stride/2) * 128,  column code = (b >> 1) * 128 + (b & 1) * 2,
+//   lane part = (l % 32) * 4 (added by the kernel when it loads the row codes).
 // int16 tasks keep the two halves' codes in the two 16-bit halves of one word.
+inline constexpr uint32_t kMaxMatrixSymbols = 31;   // plus the pad: 32 x 16 x 128 B = 64 KiB of LDS
+inline uint32_t lds_row_offset(uint32_t a, uint32_t stride) { return a * (stride / 2) * 128; }
+inline uint32_t lds_col_offset(uint32_t b) { return (b >> 1) * 128 + (b & 1) * 2; }
+inline uint32_t lds_table_bytes(uint32_t alphabet, uint32_t stride) { return alphabet * (stride / 2) * 128; }
 
 struct Task {
     uint32_t row_off;
@@ -76,7 +84,7 @@ struct VariantKey {
 struct Launch {
     VariantKey key;
     uint32_t task_begin, task_end;
-    uint64_t padded_cells;
+    uint64_t padded_cells;   // lane-cells swept, both int16 halves counted
 };
 
 struct PairMeta {
@@ -104,6 +112,7 @@ struct Plan {
     uint64_t bnd_entries = 0;                 // (H,E) entries of the boundary buffer
     uint64_t cells = 0, padded_cells = 0;
     uint64_t n16 = 0, n32 = 0;
+    uint64_t n_speculative = 0;               // int16 pairs whose bound does not exclude saturation
 };
 
 struct BatchView {
@@ -131,7 +140,8 @@ struct RangeBound {
 };
 RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n);
 
-// Choice of (G, R) for a pair with m rows.
-void choose_shape(uint32_t m, uint32_t width, uint8_t &G, uint8_t &R);
+// Choice of (G, R) for a pair with m rows and n columns: the rows per lane with the lowest
+// modelled sweep cost (stripes x steps x per-step work).
+void choose_shape(uint32_t m, uint32_t n, uint32_t width, uint8_t &G, uint8_t &R);
 
 } // namespace pa
