@@ -120,6 +120,7 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     a.bnd = static_cast<uint32_t *>(pd.bnd.ptr);
     a.scores = static_cast<int32_t *>(b->scores.ptr);
     a.flags = static_cast<uint32_t *>(b->flags.ptr);
+    a.error_slot = uint32_t(b->n_pairs);
     a.table = static_cast<const int16_t *>(pd.table.ptr);
     a.table_entries = uint32_t(p.table.size());
     a.go = p.go;
@@ -229,6 +230,15 @@ pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
     return launch_plan(ctx, b, *b->recompute, false);
 }
 
+// The wavefront kernel's bounded waits set flags[n_pairs] when they give up.
+pa_status check_error_slot(pa_ctx *ctx, pa_batch *b)
+{
+    uint32_t err = 0;
+    PA_HIP(hipMemcpy(&err, static_cast<uint32_t *>(b->flags.ptr) + b->n_pairs, 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (err) return fail(ctx, PA_ERR_DEVICE, "wavefront kernel: a boundary wait exceeded its limit");
+    return PA_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -310,7 +320,7 @@ pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1
     if ((st = upload_plan(ctx, b->main)) != PA_OK) return st;
     if ((st = upload(ctx, b->pairs, p.pairs)) != PA_OK) return st;
     PA_HIP(hipMalloc(&b->scores.ptr, std::max<size_t>(n_pairs * 4, 16)), "hipMalloc");
-    PA_HIP(hipMalloc(&b->flags.ptr, std::max<size_t>(n_pairs * 4, 16)), "hipMalloc");
+    PA_HIP(hipMalloc(&b->flags.ptr, (n_pairs + 1) * 4), "hipMalloc");   // + error slot
     PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 
     b->stats.n_pairs = n_pairs;
@@ -331,7 +341,7 @@ pa_status pa_batch_run(pa_ctx *ctx, pa_batch *b)
     pa_status st = run_pass(ctx, b, false);
     if (st != PA_OK) return st;
     PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    return PA_OK;
+    return check_error_slot(ctx, b);
 }
 
 pa_status pa_batch_results(pa_ctx *ctx, pa_batch *b, int32_t *scores, uint8_t *flags)
@@ -377,7 +387,7 @@ pa_status pa_batch_time(pa_ctx *ctx, pa_batch *b, int iters, double *ms_total, d
     PA_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
     if (ms_total) *ms_total = ms;
     if (ms_main) *ms_main = b->main_ms_count ? b->main_ms_accum / double(b->main_ms_count) : 0.0;
-    return PA_OK;
+    return check_error_slot(ctx, b);
 }
 
 pa_status pa_batch_destroy(pa_ctx *ctx, pa_batch *b)

@@ -1,0 +1,233 @@
+// pa_cells.hpp -- device building blocks shared by the striped and the wavefront kernels:
+// lane types (packed saturating int16 / int32), DPP helpers, the R-row cell update and the
+// score-capture bookkeeping.
+//
+// Cell recurrence (rrahn/pairwise_aligner affine/affine_dp_algorithm.hpp:139-156):
+//   best = H(i-1,j-1) + s(a_i, b_j)         [local: max(best, 0), dp_matrix_local.hpp:39-47]
+//   best = max(best, V(i,j), W(i,j))         H(i,j) = best
+//   open = best + go + ge
+//   V(i+1,j) = max(V(i,j) + ge, open)        vertical gap state, flows down the rows
+//   W(i,j+1) = max(W(i,j) + ge, open)        horizontal gap state, kept per row in registers
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pa_plan.hpp"
+
+namespace pa {
+
+typedef short v2i16 __attribute__((ext_vector_type(2)));
+
+template <int W> struct Lane;
+
+template <> struct Lane<16> {
+    typedef v2i16 V;
+    static constexpr int NH = 2;
+    static constexpr int kMax = 32767;
+    __device__ static int clamp(int x) { return x < -32768 ? -32768 : (x > 32767 ? 32767 : x); }
+    __device__ static V splat(int x) { short s = short(clamp(x)); return V{s, s}; }
+    __device__ static V add(V a, V b) { return __builtin_elementwise_add_sat(a, b); }
+    __device__ static V vmax(V a, V b) { return __builtin_elementwise_max(a, b); }
+    __device__ static uint32_t bits(V v) { return __builtin_bit_cast(uint32_t, v); }
+    __device__ static V from(uint32_t u) { return __builtin_bit_cast(V, u); }
+    __device__ static int half(V v, int q) { return q == 0 ? int(v.x) : int(v.y); }
+    __device__ static V lowest() { return V{short(-32768), short(-32768)}; }
+    // packed top-boundary word of an int value (<= 0): both halves, clamped at -32768
+    __device__ static uint32_t top_word(int x) { return (uint32_t(x < -32768 ? -32768 : x) & 0xFFFFu) * 0x10001u; }
+};
+
+template <> struct Lane<32> {
+    typedef int V;
+    static constexpr int NH = 1;
+    static constexpr int kMax = INT32_MAX;
+    __device__ static V splat(int x) { return x; }
+    __device__ static V add(V a, V b) { return a + b; }
+    __device__ static V vmax(V a, V b) { return a > b ? a : b; }
+    __device__ static uint32_t bits(V v) { return uint32_t(v); }
+    __device__ static V from(uint32_t u) { return int(u); }
+    __device__ static int half(V v, int) { return v; }
+    __device__ static V lowest() { return INT32_MIN; }
+    __device__ static uint32_t top_word(int x) { return uint32_t(x); }
+};
+
+// DPP controls (gfx9).  Lanes without a source keep `old` (bound_ctrl off).
+constexpr int kRowShr1 = 0x111;   // lane t-1 -> t inside a row of 16 lanes
+constexpr int kRowShl1 = 0x101;   // lane t+1 -> t inside a row of 16 lanes
+constexpr int kRowRor1 = 0x121;
+constexpr int kRowRor2 = 0x122;
+constexpr int kRowRor4 = 0x124;
+constexpr int kRowRor8 = 0x128;
+constexpr int kWaveShr1 = 0x138;  // lane t-1 -> t across the wave
+constexpr int kWaveShl1 = 0x130;  // lane t+1 -> t across the wave
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src)
+{
+    return uint32_t(__builtin_amdgcn_update_dpp(int(old), int(src), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ int h0_value(int i, int free_flag, int go, int ge)
+{
+    return (i == 0 || free_flag) ? 0 : go + ge * i;
+}
+
+// Register state of one lane: R consecutive DP rows of the current stripe.
+template <int W, int R, bool ROWMAX>
+struct Rows {
+    using V = typename Lane<W>::V;
+    uint32_t A[R];        // residue codes (matrix: LDS byte offsets of the table rows)
+    uint32_t Ahi[R];      // int16 matrix path: codes of the high half
+    V H[R], F[R];
+    V rowacc[ROWMAX ? R : 1];
+};
+
+// Loads the residue codes of rows row0+1 .. row0+R and sets H(i,0) and W(i,1).
+template <int W, int R, int SCORE, bool ROWMAX>
+__device__ __forceinline__ void init_rows(Rows<W, R, ROWMAX> &st, const uint32_t *codes, int row0, int fc, int go,
+                                          int ge, uint32_t lane_bank)
+{
+    using L = Lane<W>;
+    const typename L::V vgogo = L::splat(go + ge);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t code = codes[row0 + r];
+        if constexpr (SCORE == kScoreMatrix && W == 16) {
+            st.A[r] = (code & 0xFFFFu) + lane_bank;
+            st.Ahi[r] = (code >> 16) + lane_bank;
+        } else if constexpr (SCORE == kScoreMatrix) {
+            st.A[r] = code + lane_bank;
+            st.Ahi[r] = 0;
+        } else {
+            st.A[r] = code;
+            st.Ahi[r] = 0;
+        }
+        st.H[r] = L::splat(h0_value(row0 + r + 1, fc, go, ge));
+        st.F[r] = L::add(st.H[r], vgogo);
+        if constexpr (ROWMAX) st.rowacc[r] = L::lowest();
+    }
+}
+
+// One step of one lane: the R cells of DP column j for this lane's rows.  `Eup` enters as the
+// vertical gap state from the row above and leaves as the one below the last row; `diag` is
+// H of the row above at column j-1.
+template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
+__device__ __forceinline__ void cell_block(Rows<W, R, ROWMAX> &st, typename Lane<W>::V &Eup,
+                                           typename Lane<W>::V diag, typename Lane<W>::V &runmax, uint32_t Bin,
+                                           uint32_t tbl_lo, uint32_t tbl_hi, const char *lds,
+                                           typename Lane<W>::V vgogo, typename Lane<W>::V vge)
+{
+    using L = Lane<W>;
+    using V = typename L::V;
+    uint32_t Blo = Bin, Bhi = 0;
+    if constexpr (SCORE == kScoreMatrix && W == 16) { Blo = Bin & 0xFFFFu; Bhi = Bin >> 16; }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        V sc;
+        if constexpr (SCORE == kScoreDna) {
+            sc = L::from(__builtin_amdgcn_perm(tbl_hi, tbl_lo, st.A[r] ^ Bin));
+        } else if constexpr (W == 16) {
+            const short s0 = *reinterpret_cast<const short *>(lds + st.A[r] + Blo);
+            const short s1 = *reinterpret_cast<const short *>(lds + st.Ahi[r] + Bhi);
+            sc = V{s0, s1};
+        } else {
+            sc = L::splat(int(*reinterpret_cast<const short *>(lds + st.A[r] + Blo)));
+        }
+        V best = L::add(diag, sc);
+        if constexpr (LOCAL) best = L::vmax(best, L::splat(0));
+        best = L::vmax(L::vmax(best, Eup), st.F[r]);
+        diag = st.H[r];
+        st.H[r] = best;
+        if constexpr (LOCAL || TRACK) runmax = L::vmax(runmax, best);
+        if constexpr (ROWMAX) st.rowacc[r] = L::vmax(st.rowacc[r], best);
+        const V open = L::add(best, vgogo);
+        Eup = L::vmax(L::add(Eup, vge), open);
+        st.F[r] = L::vmax(L::add(st.F[r], vge), open);
+    }
+}
+
+// Score capture of one lane in one stripe (global method).  cap: this lane owns row m of half
+// q and reads H(m, n) at column n; lcap (free last column): this lane reads the maximum over its
+// real rows at column n.  Contributions meet in an int32 atomicMax on the pair's score.
+template <int W, int R>
+struct Capture {
+    int cap[2], capr[2], lcap[2], lvr[2];
+    int ev;   // next event column (INT32_MAX: none)
+
+    __device__ __forceinline__ int next(int after) const
+    {
+        int e = INT32_MAX;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (cap[q] > after && cap[q] < e) e = cap[q];
+            if (lcap[q] > after && lcap[q] < e) e = lcap[q];
+        }
+        return e;
+    }
+
+    __device__ __forceinline__ void init(const Task &tk, int row0, int lc)
+    {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            cap[q] = -1; capr[q] = 0; lcap[q] = -1; lvr[q] = 0;
+            if (q >= Lane<W>::NH || tk.pair[q] == kNone) continue;
+            const int m = int(tk.m[q]), n = int(tk.n[q]);
+            if (m - 1 >= row0 && m - 1 < row0 + R) { cap[q] = n - 1; capr[q] = m - 1 - row0; }
+            if (lc && row0 < m) { lcap[q] = n - 1; lvr[q] = m - row0 < R ? m - row0 : R; }
+        }
+        ev = next(-1);
+    }
+
+    template <bool ROWMAX>
+    __device__ __forceinline__ void at_column(int c, const Rows<W, R, ROWMAX> &st, const Task &tk, int32_t *scores)
+    {
+        if (c != ev) return;
+#pragma unroll
+        for (int q = 0; q < Lane<W>::NH; ++q) {
+            if (c == cap[q]) {
+                int v = 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) if (r == capr[q]) v = Lane<W>::half(st.H[r], q);
+                atomicMax(scores + tk.pair[q], v);
+            }
+            if (c == lcap[q]) {
+                int v = INT32_MIN;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (r < lvr[q]) { const int h = Lane<W>::half(st.H[r], q); v = h > v ? h : v; }
+                atomicMax(scores + tk.pair[q], v);
+            }
+        }
+        ev = next(c);
+    }
+
+    // free last row: the owner of row m contributes the maximum of that row
+    template <bool ROWMAX>
+    __device__ __forceinline__ void row_maximum(const Rows<W, R, ROWMAX> &st, const Task &tk, int32_t *scores) const
+    {
+        if constexpr (ROWMAX) {
+#pragma unroll
+            for (int q = 0; q < Lane<W>::NH; ++q) {
+                if (cap[q] < 0) continue;
+                int v = 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) if (r == capr[q]) v = Lane<W>::half(st.rowacc[r], q);
+                atomicMax(scores + tk.pair[q], v);
+            }
+        }
+    }
+};
+
+// Copies the substitution table into LDS as 32 bank-swizzled replicas: entry e for lane l at
+// dword (e >> 1) * 32 + l % 32, half e & 1 (see pa_plan.hpp).
+__device__ __forceinline__ void load_lds_table(char *lds_raw, const int16_t *table, uint32_t entries)
+{
+    int16_t *lds16 = reinterpret_cast<int16_t *>(lds_raw);
+    const uint32_t total = entries * 32;
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+        const uint32_t e = i >> 5, copy = i & 31;
+        lds16[(((e >> 1) << 5) + copy) * 2 + (e & 1)] = table[e];
+    }
+}
+
+} // namespace pa

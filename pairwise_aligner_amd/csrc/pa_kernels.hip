@@ -1,105 +1,59 @@
 // pa_kernels.hip -- gfx950 kernels of the batched affine-gap (Gotoh) score-only aligner.
 //
-// Striped wavefront kernel.  One group of G = 16 lanes (one DPP row) sweeps one task: one
-// pair in int32 lanes, or two pairs in the two 16-bit halves of every 32-bit lane (packed
-// int16, saturating v_pk_add_i16 / v_pk_max_i16).  Lane t of the group owns R consecutive DP
-// rows of the current stripe of G*R rows and keeps their H and horizontal-gap state in
-// registers.  The lanes move along the DP columns skewed by one step per lane: at step k lane
-// t computes column c = k - t for its R rows, receiving from lane t-1 (DPP row_shr:1) the H
-// and vertical-gap state of the row above its block and the column residue.  Lane 0 of the
-// group receives the stripe's top boundary instead: the initial DP row for the first stripe,
-// otherwise the bottom row the previous stripe left in the boundary buffer.  Column residues
-// and boundary values reach lane 0 through "conveyors": G consecutive words loaded coalesced,
-// one per lane, then shifted down the group by DPP row_shl:1 once per step.
+// Both DP kernels are systolic sweeps over the DP matrix.  A lane owns R consecutive DP rows
+// of the current stripe and keeps their H and horizontal-gap state in registers.  The lanes of
+// a group move along the DP columns skewed by one step per lane: at step k lane t computes
+// column c = k - t for its R rows, receiving from lane t-1 by DPP the H and vertical-gap state
+// of the row above its block and the column residue.  Lane 0 receives the stripe's top
+// boundary instead: the initial DP row for the first stripe, otherwise the bottom row of the
+// stripe above.  Column residues and boundary values reach lane 0 through "conveyors": one
+// chunk of consecutive words loaded coalesced, one word per lane, then shifted towards lane 0
+// by one lane per step.
 //
-// Cell recurrence (rrahn/pairwise_aligner affine/affine_dp_algorithm.hpp:139-156):
-//   best = H(i-1,j-1) + s(a_i, b_j)         [local: max(best, 0), dp_matrix_local.hpp:39-47]
-//   best = max(best, V(i,j), W(i,j))         H(i,j) = best
-//   open = best + go + ge
-//   V(i+1,j) = max(V(i,j) + ge, open)        vertical gap state, flows down the rows
-//   W(i,j+1) = max(W(i,j) + ge, open)        horizontal gap state, kept per row in registers
+//   stripe_kernel     short pairs.  A group is one DPP row of G = 16 lanes (row_shr:1) and
+//                     sweeps one task: one pair in int32 lanes or two pairs in the two 16-bit
+//                     halves of every lane (packed int16, v_pk_add_i16 clamp / v_pk_max_i16).
+//                     Stripes of 16*R rows run one after the other; the boundary row between
+//                     them goes through HBM.
+//   wavefront_kernel  long pairs.  A workgroup of 8 waves sweeps one task; every wave is one
+//                     64-lane group (wave_shr:1) owning a stripe of 64*R rows, and the waves
+//                     take the stripes round robin, so 8 stripes are in flight at once, each
+//                     wave a fixed distance behind the one above.  The boundary row between
+//                     consecutive waves passes through an LDS ring buffer with progress
+//                     counters; from the last wave back to the first it goes through HBM.
+//
 // Boundary (affine_initialisation_strategy.hpp:38-51): H(i,0) = first_column free ? 0 : go+ge*i,
 // H(0,j) likewise with first_row; the gap state entering the matrix is H + go + ge.
-//
 // Score extraction (tracker_global_scalar.hpp:36-61, tracker_local_scalar.hpp:38-49):
 //   global  H(m,n) is captured by the lane owning row m at the step of column n; with a free
 //           last column every lane contributes max over its real rows at column n; with a
 //           free last row the per-row maxima are kept and the owner of row m contributes its
 //           row's maximum.  Contributions meet in an int32 atomicMax.
-//   local   running maximum over all cells, reduced over the group.  Pad rows and columns
-//           score -1 against everything, so with gap scores <= 0 no pad cell can exceed the
-//           maximum of the real cells.
+//   local   running maximum over all cells.  Pad rows and columns score -1 against
+//           everything, so with gap scores <= 0 no pad cell exceeds the maximum of the real
+//           cells.
 // Saturation: int16 lanes saturate; a running maximum equal to 32767 flags the pair for an
 // int32 recomputation.  Lower bounds are proven on the host before int16 is chosen.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "pa_plan.hpp"
+#include "pa_cells.hpp"
 #include "pa_kernels.hpp"
+#include "pa_plan.hpp"
 
 namespace pa {
 
-typedef short v2i16 __attribute__((ext_vector_type(2)));
-
-template <int W> struct Lane;
-
-template <> struct Lane<16> {
-    typedef v2i16 V;
-    static constexpr int NH = 2;
-    static constexpr int kMax = 32767;
-    __device__ static int clamp(int x) { return x < -32768 ? -32768 : (x > 32767 ? 32767 : x); }
-    __device__ static V splat(int x) { short s = short(clamp(x)); return V{s, s}; }
-    __device__ static V add(V a, V b) { return __builtin_elementwise_add_sat(a, b); }
-    __device__ static V vmax(V a, V b) { return __builtin_elementwise_max(a, b); }
-    __device__ static uint32_t bits(V v) { return __builtin_bit_cast(uint32_t, v); }
-    __device__ static V from(uint32_t u) { return __builtin_bit_cast(V, u); }
-    __device__ static int half(V v, int q) { return q == 0 ? int(v.x) : int(v.y); }
-    __device__ static V lowest() { return V{short(-32768), short(-32768)}; }
-};
-
-template <> struct Lane<32> {
-    typedef int V;
-    static constexpr int NH = 1;
-    static constexpr int kMax = INT32_MAX;
-    __device__ static V splat(int x) { return x; }
-    __device__ static V add(V a, V b) { return a + b; }
-    __device__ static V vmax(V a, V b) { return a > b ? a : b; }
-    __device__ static uint32_t bits(V v) { return uint32_t(v); }
-    __device__ static V from(uint32_t u) { return int(u); }
-    __device__ static int half(V v, int) { return v; }
-    __device__ static V lowest() { return INT32_MIN; }
-};
-
-// DPP controls (gfx9): row_shr:1 moves lane t-1 -> t inside a row of 16 lanes, row_shl:1
-// moves lane t+1 -> t.  Lanes without a source keep `old` (bound_ctrl off).
-constexpr int kRowShr1 = 0x111;
-constexpr int kRowShl1 = 0x101;
-constexpr int kRowRor1 = 0x121;
-constexpr int kRowRor2 = 0x122;
-constexpr int kRowRor4 = 0x124;
-constexpr int kRowRor8 = 0x128;
-
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src)
+// Group reduction of the running maximum over 16 lanes (one DPP row).
+template <int W>
+__device__ __forceinline__ typename Lane<W>::V row_reduce_max(typename Lane<W>::V v)
 {
-    return uint32_t(__builtin_amdgcn_update_dpp(int(old), int(src), CTRL, 0xF, 0xF, false));
-}
-
-__device__ __forceinline__ int h0_value(int i, int free_flag, int go, int ge)
-{
-    return (i == 0 || free_flag) ? 0 : go + ge * i;
-}
-
-// Smallest capture column greater than `after` (INT32_MAX if none).
-__device__ __forceinline__ int next_event(const int cap[2], const int lcap[2], int after)
-{
-    int e = INT32_MAX;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        if (cap[q] > after && cap[q] < e) e = cap[q];
-        if (lcap[q] > after && lcap[q] < e) e = lcap[q];
-    }
-    return e;
+    using L = Lane<W>;
+    uint32_t m = L::bits(v);
+    m = L::bits(L::vmax(L::from(m), L::from(dpp<kRowRor8>(m, m))));
+    m = L::bits(L::vmax(L::from(m), L::from(dpp<kRowRor4>(m, m))));
+    m = L::bits(L::vmax(L::from(m), L::from(dpp<kRowRor2>(m, m))));
+    m = L::bits(L::vmax(L::from(m), L::from(dpp<kRowRor1>(m, m))));
+    return L::from(m);
 }
 
 template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
@@ -110,13 +64,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     extern __shared__ char lds_raw[];
 
     if constexpr (SCORE == kScoreMatrix) {
-        // 32 bank-swizzled copies: entry e for lane l at dword (e >> 1) * 32 + l % 32, half e & 1
-        int16_t *lds16 = reinterpret_cast<int16_t *>(lds_raw);
-        const uint32_t total = a.table_entries * 32;
-        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-            const uint32_t e = i >> 5, copy = i & 31;
-            lds16[(((e >> 1) << 5) + copy) * 2 + (e & 1)] = a.table[e];
-        }
+        load_lds_table(lds_raw, a.table, a.table_entries);
         __syncthreads();
     }
 
@@ -137,42 +85,11 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 
     for (uint32_t s = 0; s < tk.S; ++s) {
         const int row0 = int(s) * ROWS + t * R;   // DP row above this lane's block
-        uint32_t A[R];
-        uint32_t Ahi[R];
-        V H[R], F[R];
-        V rowacc[ROWMAX ? R : 1];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t code = a.rowcodes[tk.row_off + uint32_t(row0 + r)];
-            if constexpr (SCORE == kScoreMatrix && W == 16) {
-                A[r] = (code & 0xFFFFu) + lane_bank;
-                Ahi[r] = (code >> 16) + lane_bank;
-            } else if constexpr (SCORE == kScoreMatrix) {
-                A[r] = code + lane_bank;
-                Ahi[r] = 0;
-            } else {
-                A[r] = code;
-                Ahi[r] = 0;
-            }
-            H[r] = L::splat(h0_value(row0 + r + 1, fc, go, ge));
-            F[r] = L::add(H[r], vgogo);
-            if constexpr (ROWMAX) rowacc[r] = L::lowest();
-        }
+        Rows<W, R, ROWMAX> st;
+        init_rows<W, R, SCORE, ROWMAX>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank);
         V Hdiag = L::splat(h0_value(row0, fc, go, ge));
-
-        // Capture events of the two halves (global), as column indices (-1: none).  cap: this
-        // lane owns row m and reads H(m, n) at column n; lcap (free last column): this lane
-        // reads the maximum over its real rows at column n.  `ev` is the next event column.
-        int cap[2] = {-1, -1}, capr[2] = {0, 0};
-        int lcap[2] = {-1, -1}, lvr[2] = {0, 0};
-#pragma unroll
-        for (int q = 0; q < L::NH; ++q) {
-            if (tk.pair[q] == kNone) continue;
-            const int m = int(tk.m[q]), n = int(tk.n[q]);
-            if (m - 1 >= row0 && m - 1 < row0 + R) { cap[q] = n - 1; capr[q] = m - 1 - row0; }
-            if (lc && row0 < m) { lcap[q] = n - 1; lvr[q] = m - row0 < R ? m - row0 : R; }
-        }
-        int ev = next_event(cap, lcap, -1);
+        Capture<W, R> cap;
+        if constexpr (!LOCAL) cap.init(tk, row0, lc);
 
         const bool has_bnd_in = s > 0;
         const bool has_bnd_out = s + 1 < tk.S;
@@ -207,16 +124,10 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 topH = convH;
                 topE = convE;
             } else {
-                // H(0, k+1) and the vertical gap state below it: uniform, computed in scalar
-                // registers.  Both are <= 0 because go + ge <= 0 and ge <= 0.
-                const int e = hb_top + go + ge;
-                if constexpr (W == 16) {
-                    topH = (uint32_t(hb_top < -32768 ? -32768 : hb_top) & 0xFFFFu) * 0x10001u;
-                    topE = (uint32_t(e < -32768 ? -32768 : e) & 0xFFFFu) * 0x10001u;
-                } else {
-                    topH = uint32_t(hb_top);
-                    topE = uint32_t(e);
-                }
+                // H(0, k+1) and the vertical gap state below it: uniform values in scalar
+                // registers; both are <= 0 because go + ge <= 0 and ge <= 0.
+                topH = L::top_word(hb_top);
+                topE = L::top_word(hb_top + go + ge);
                 hb_top = fr ? 0 : hb_top + ge;
             }
             const uint32_t Bin = dpp<kRowShr1>(convB, Bout);
@@ -229,34 +140,10 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             }
             if (uint32_t(c) < uint32_t(N)) {
                 V Eup = L::from(Ein);
-                V diag = Hdiag;
-                uint32_t Blo = Bin, Bhi = 0;
-                if constexpr (SCORE == kScoreMatrix && W == 16) { Blo = Bin & 0xFFFFu; Bhi = Bin >> 16; }
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    V sc;
-                    if constexpr (SCORE == kScoreDna) {
-                        sc = L::from(__builtin_amdgcn_perm(tbl_hi, tbl_lo, A[r] ^ Bin));
-                    } else if constexpr (W == 16) {
-                        const short s0 = *reinterpret_cast<const short *>(lds_raw + A[r] + Blo);
-                        const short s1 = *reinterpret_cast<const short *>(lds_raw + Ahi[r] + Bhi);
-                        sc = V{s0, s1};
-                    } else {
-                        sc = L::splat(int(*reinterpret_cast<const short *>(lds_raw + A[r] + Blo)));
-                    }
-                    V best = L::add(diag, sc);
-                    if constexpr (LOCAL) best = L::vmax(best, L::splat(0));
-                    best = L::vmax(L::vmax(best, Eup), F[r]);
-                    diag = H[r];
-                    H[r] = best;
-                    if constexpr (LOCAL || TRACK) runmax = L::vmax(runmax, best);
-                    if constexpr (ROWMAX) rowacc[r] = L::vmax(rowacc[r], best);
-                    const V open = L::add(best, vgogo);
-                    Eup = L::vmax(L::add(Eup, vge), open);
-                    F[r] = L::vmax(L::add(F[r], vge), open);
-                }
+                cell_block<W, R, SCORE, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, Bin, tbl_lo, tbl_hi, lds_raw,
+                                                              vgogo, vge);
                 Hdiag = L::from(Hin);
-                Hout = L::bits(H[R - 1]);
+                Hout = L::bits(st.H[R - 1]);
                 Eout = L::bits(Eup);
                 Bout = Bin;
                 if (has_bnd_out) {
@@ -265,39 +152,10 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                         bndp[2 * c + 1] = Eout;
                     }
                 }
-                if constexpr (!LOCAL) {
-                    if (c == ev) {
-#pragma unroll
-                        for (int q = 0; q < L::NH; ++q) {
-                            if (c == cap[q]) {
-                                int v = 0;
-#pragma unroll
-                                for (int r = 0; r < R; ++r) if (r == capr[q]) v = L::half(H[r], q);
-                                atomicMax(a.scores + tk.pair[q], v);
-                            }
-                            if (c == lcap[q]) {
-                                int v = INT32_MIN;
-#pragma unroll
-                                for (int r = 0; r < R; ++r)
-                                    if (r < lvr[q]) { const int h = L::half(H[r], q); v = h > v ? h : v; }
-                                atomicMax(a.scores + tk.pair[q], v);
-                            }
-                        }
-                        ev = next_event(cap, lcap, c);
-                    }
-                }
+                if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores);
             }
         }
-        if constexpr (ROWMAX) {
-#pragma unroll
-            for (int q = 0; q < L::NH; ++q) {
-                if (cap[q] < 0) continue;
-                int v = 0;
-#pragma unroll
-                for (int r = 0; r < R; ++r) if (r == capr[q]) v = L::half(rowacc[r], q);
-                atomicMax(a.scores + tk.pair[q], v);
-            }
-        }
+        if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores);
         if (has_bnd_out) {
             // the boundary row written by lane G-1 is read by lane 0 of the next stripe
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -306,19 +164,203 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     }
 
     if constexpr (LOCAL || TRACK) {
-        // maximum over the 16 lanes of the group
-        uint32_t m = L::bits(runmax);
-        m = L::bits(L::vmax(L::from(m), L::from(dpp<kRowRor8>(m, m))));
-        m = L::bits(L::vmax(L::from(m), L::from(dpp<kRowRor4>(m, m))));
-        m = L::bits(L::vmax(L::from(m), L::from(dpp<kRowRor2>(m, m))));
-        m = L::bits(L::vmax(L::from(m), L::from(dpp<kRowRor1>(m, m))));
+        const V mv = row_reduce_max<W>(runmax);
         if (t == 0) {
-            const V mv = L::from(m);
 #pragma unroll
             for (int q = 0; q < L::NH; ++q) {
                 if (tk.pair[q] == kNone) continue;
                 const int v = L::half(mv, q);
                 if constexpr (LOCAL) a.scores[tk.pair[q]] = v;
+                if (W == 16 && v >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Wavefront kernel for long pairs.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int kWfWaves = 8;     // waves per workgroup
+constexpr int kWfRing = 256;    // columns per LDS ring between consecutive waves
+// Bounded waits: after this many polls a wait gives up and flags the batch, so a protocol
+// error ends as a reported failure instead of a hang.
+constexpr uint32_t kWfSpinLimit = 1u << 24;
+
+struct WfShared {
+    uint32_t prod[kWfWaves];   // positions (round * N + column) published by wave w
+    uint32_t cons[kWfWaves];   // positions consumed by wave w from its predecessor's ring
+    uint32_t ring[kWfWaves - 1][kWfRing][2];   // (H, V) from wave w to wave w+1
+};
+
+__device__ __forceinline__ uint32_t lds_load_acquire(uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void wait_at_least(uint32_t *p, uint32_t need, uint32_t *err)
+{
+    uint32_t spins = 0;
+    while (lds_load_acquire(p) < need) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kWfSpinLimit) { atomicOr(err, 1u); break; }
+    }
+}
+
+template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
+__global__ __launch_bounds__(512) void wavefront_kernel(KernelArgs a)
+{
+    using L = Lane<W>;
+    using V = typename L::V;
+    constexpr int G = 64;
+    constexpr int ROWS = G * R;
+    extern __shared__ char lds_raw[];
+    __shared__ WfShared sh;
+
+    if (threadIdx.x < kWfWaves) { sh.prod[threadIdx.x] = 0; sh.cons[threadIdx.x] = 0; }
+    if constexpr (SCORE == kScoreMatrix) load_lds_table(lds_raw, a.table, a.table_entries);
+    __syncthreads();
+
+    const Task tk = a.tasks[blockIdx.x];
+    const int wave = int(threadIdx.x / G);
+    const int t = int(threadIdx.x % G);
+    const uint32_t lane_bank = (threadIdx.x & 31u) * 4u;
+    const int go = a.go, ge = a.ge;
+    const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
+    const V vgogo = L::splat(go + ge), vge = L::splat(ge);
+    const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
+    const int N = int(tk.N);
+    const int S = int(tk.S);
+    const uint32_t *colp = a.colcodes + tk.col_off;
+    uint32_t *gbnd = a.bnd + 2ull * tk.bnd_off;   // last wave -> first wave, one full row
+
+    V runmax = LOCAL ? L::splat(0) : L::lowest();
+
+    for (int s = wave; s < S; s += kWfWaves) {
+        const int round = s / kWfWaves;
+        const uint32_t pos0 = uint32_t(round) * uint32_t(N);
+        const int row0 = s * ROWS + t * R;
+        Rows<W, R, ROWMAX> st;
+        init_rows<W, R, SCORE, ROWMAX>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank);
+        V Hdiag = L::splat(h0_value(row0, fc, go, ge));
+        Capture<W, R> cap;
+        if constexpr (!LOCAL) cap.init(tk, row0, lc);
+
+        // top boundary source: 0 = initial row, 1 = LDS ring of wave-1, 2 = HBM row of the last wave
+        const int src = s == 0 ? 0 : (wave > 0 ? 1 : 2);
+        // bottom boundary sink: 0 = none (last stripe), 1 = LDS ring of this wave, 2 = HBM row
+        const int dst = s + 1 >= S ? 0 : (wave < kWfWaves - 1 ? 1 : 2);
+        uint32_t *src_prog = src == 1 ? &sh.prod[wave - 1] : &sh.prod[kWfWaves - 1];
+        const uint32_t src_pos0 = src == 1 ? pos0 : pos0 - uint32_t(N);   // HBM row: previous round
+
+        auto need_top = [&](int upto) {   // columns [0, upto) of the top boundary are published
+            if (src == 0) return;
+            wait_at_least(src_prog, src_pos0 + uint32_t(upto < N ? upto : N), a.flags + a.error_slot);
+            if (src == 2) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        };
+        auto read_top = [&](int col, uint32_t &h, uint32_t &e) {
+            if (src == 1) {
+                const uint32_t slot = (pos0 + uint32_t(col)) % kWfRing;
+                h = sh.ring[wave - 1][slot][0];
+                e = sh.ring[wave - 1][slot][1];
+            } else {
+                h = gbnd[2 * col];
+                e = gbnd[2 * col + 1];
+            }
+        };
+
+        uint32_t convB = colp[t], nextB = colp[G + t];
+        uint32_t convH = 0, convE = 0, nextH = 0, nextE = 0;
+        if (src != 0) {
+            need_top(2 * G);
+            if (t < N) read_top(t, convH, convE);
+            if (G + t < N) read_top(G + t, nextH, nextE);
+            if (src == 1 && t == 0)
+                __hip_atomic_store(&sh.cons[wave], pos0 + uint32_t(2 * G < N ? 2 * G : N), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        uint32_t Hout = 0, Eout = 0, Bout = 0;
+        int hb_top = h0_value(1, fr, go, ge);
+
+        const int nsteps = N + G - 1;
+        int c = -t;
+        for (int k = 0; k < nsteps; ++k, ++c) {
+            if (k > 0 && (k % G) == 0) {
+                convB = nextB;
+                nextB = colp[k + G + t];
+                if (src != 0) {
+                    convH = nextH; convE = nextE;
+                    need_top(k + 2 * G);
+                    if (k + G + t < N) read_top(k + G + t, nextH, nextE);
+                    if (src == 1 && t == 0)
+                        __hip_atomic_store(&sh.cons[wave], pos0 + uint32_t(k + 2 * G < N ? k + 2 * G : N),
+                                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+            // the last lane starts a new chunk of output columns: make room in the ring
+            const int cout = k - (G - 1);
+            if (dst == 1 && cout >= 0 && cout < N && (cout % G) == 0) {
+                // every ring position below pos0 + cout + G - kWfRing, this round's or the
+                // previous round's, must have been consumed before [cout, cout + G) is written
+                const int64_t need = int64_t(pos0) + cout + G - kWfRing;
+                if (need > 0) wait_at_least(&sh.cons[wave + 1], uint32_t(need), a.flags + a.error_slot);
+            }
+            uint32_t topH, topE;
+            if (src != 0) {
+                topH = convH;
+                topE = convE;
+            } else {
+                topH = L::top_word(hb_top);
+                topE = L::top_word(hb_top + go + ge);
+                hb_top = fr ? 0 : hb_top + ge;
+            }
+            const uint32_t Bin = dpp<kWaveShr1>(convB, Bout);
+            const uint32_t Hin = dpp<kWaveShr1>(topH, Hout);
+            const uint32_t Ein = dpp<kWaveShr1>(topE, Eout);
+            convB = dpp<kWaveShl1>(convB, convB);
+            if (src != 0) {
+                convH = dpp<kWaveShl1>(convH, convH);
+                convE = dpp<kWaveShl1>(convE, convE);
+            }
+            if (uint32_t(c) < uint32_t(N)) {
+                V Eup = L::from(Ein);
+                cell_block<W, R, SCORE, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, Bin, tbl_lo, tbl_hi, lds_raw,
+                                                              vgogo, vge);
+                Hdiag = L::from(Hin);
+                Hout = L::bits(st.H[R - 1]);
+                Eout = L::bits(Eup);
+                Bout = Bin;
+                if (dst != 0 && t == G - 1) {
+                    if (dst == 1) {
+                        const uint32_t slot = (pos0 + uint32_t(c)) % kWfRing;
+                        sh.ring[wave][slot][0] = Hout;
+                        sh.ring[wave][slot][1] = Eout;
+                    } else {
+                        gbnd[2 * c] = Hout;
+                        gbnd[2 * c + 1] = Eout;
+                    }
+                }
+                if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores);
+            }
+            // publish every G output columns and at the end of the row
+            if (dst != 0 && cout >= 0 && cout < N && (((cout + 1) % G) == 0 || cout + 1 == N)) {
+                if (dst == 2) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                if (t == G - 1)
+                    __hip_atomic_store(&sh.prod[wave], pos0 + uint32_t(cout + 1), __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores);
+    }
+
+    if constexpr (LOCAL || TRACK) {
+        const V mv = row_reduce_max<W>(runmax);
+        if ((t & 15) == 0) {
+#pragma unroll
+            for (int q = 0; q < L::NH; ++q) {
+                if (tk.pair[q] == kNone) continue;
+                const int v = L::half(mv, q);
+                if constexpr (LOCAL) atomicMax(a.scores + tk.pair[q], v);
                 if (W == 16 && v >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
             }
         }
@@ -338,9 +380,7 @@ __global__ void init_scores_kernel(InitArgs a)
     const int64_t m = pm.m, n = pm.n;
     uint32_t flags = (pm.width == 16 ? 0x01u : (pm.width == 32 ? 0x02u : 0x08u)) | a.flag_or;
     int64_t s = INT32_MIN;
-    if (pm.local) {
-        s = INT32_MIN;
-    } else {
+    if (!pm.local) {
         const int fc = a.fc, fr = a.fr, lc = a.lc, lr = a.lr;
         auto h0row = [&](int64_t j) -> int64_t { return (j == 0 || fr) ? 0 : go + ge * j; };  // H(0,j)
         auto h0col = [&](int64_t i) -> int64_t { return (i == 0 || fc) ? 0 : go + ge * i; };  // H(i,0)
@@ -357,29 +397,40 @@ __global__ void init_scores_kernel(InitArgs a)
     }
     a.scores[p] = int32_t(s);
     a.flags[p] = flags;
+    if (idx == 0 && a.subset == nullptr) a.flags[a.n_pairs] = 0;   // error slot of the wavefront kernel
 }
 
-template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
-static hipError_t launch_one(const KernelArgs &a, uint32_t lds_bytes, hipStream_t st)
+// ---------------------------------------------------------------------------------------------
+// Launch dispatch
+// ---------------------------------------------------------------------------------------------
+
+template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
+static hipError_t launch_variant(const VariantKey &k, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st)
 {
+    const uint32_t lds = SCORE == kScoreMatrix ? lds_bytes : 0;
+    if (k.G == 64) {
+        hipLaunchKernelGGL((wavefront_kernel<W, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(a.ntasks),
+                           dim3(64 * kWfWaves), lds, st, a);
+        return hipGetLastError();
+    }
     const uint32_t threads = 256;
-    const uint64_t total = uint64_t(a.ntasks) * G;
+    const uint64_t total = uint64_t(a.ntasks) * 16;
     const uint32_t blocks = uint32_t((total + threads - 1) / threads);
-    hipLaunchKernelGGL((stripe_kernel<W, G, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(blocks), dim3(threads),
-                       SCORE == kScoreMatrix ? lds_bytes : 0, st, a);
+    hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(blocks), dim3(threads), lds, st,
+                       a);
     return hipGetLastError();
 }
 
 template <int W, int R, int SCORE, bool LOCAL>
 static hipError_t dispatch_flags(const VariantKey &k, const KernelArgs &a, uint32_t lds, hipStream_t st)
 {
-    if (LOCAL) return launch_one<W, 16, R, SCORE, LOCAL, false, false>(a, lds, st);
+    if (LOCAL) return launch_variant<W, R, SCORE, LOCAL, false, false>(k, a, lds, st);
     if (k.rowmax) {
-        if (k.track) return launch_one<W, 16, R, SCORE, LOCAL, true, true>(a, lds, st);
-        return launch_one<W, 16, R, SCORE, LOCAL, false, true>(a, lds, st);
+        if (k.track) return launch_variant<W, R, SCORE, LOCAL, true, true>(k, a, lds, st);
+        return launch_variant<W, R, SCORE, LOCAL, false, true>(k, a, lds, st);
     }
-    if (k.track) return launch_one<W, 16, R, SCORE, LOCAL, true, false>(a, lds, st);
-    return launch_one<W, 16, R, SCORE, LOCAL, false, false>(a, lds, st);
+    if (k.track) return launch_variant<W, R, SCORE, LOCAL, true, false>(k, a, lds, st);
+    return launch_variant<W, R, SCORE, LOCAL, false, false>(k, a, lds, st);
 }
 
 template <int W, int R>
@@ -396,6 +447,10 @@ static hipError_t dispatch_score(const VariantKey &k, const KernelArgs &a, uint3
 template <int W>
 static hipError_t dispatch_rows(const VariantKey &k, const KernelArgs &a, uint32_t lds, hipStream_t st)
 {
+    if (k.G == 64) {
+        if (k.R == 16) return dispatch_score<W, 16>(k, a, lds, st);
+        return hipErrorInvalidValue;
+    }
     switch (k.R) {
     case 4: return dispatch_score<W, 4>(k, a, lds, st);
     case 6: return dispatch_score<W, 6>(k, a, lds, st);
@@ -410,7 +465,7 @@ static hipError_t dispatch_rows(const VariantKey &k, const KernelArgs &a, uint32
 
 hipError_t launch_stripe(const VariantKey &k, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st)
 {
-    if (k.G != 16 || a.ntasks == 0) return k.G != 16 ? hipErrorInvalidValue : hipSuccess;
+    if ((k.G != 16 && k.G != 64) || a.ntasks == 0) return a.ntasks == 0 ? hipSuccess : hipErrorInvalidValue;
     return k.width == 16 ? dispatch_rows<16>(k, a, lds_bytes, st) : dispatch_rows<32>(k, a, lds_bytes, st);
 }
 

@@ -18,7 +18,8 @@ struct KernelArgs {
     const uint32_t *colcodes;
     uint32_t *bnd;
     int32_t *scores;
-    uint32_t *flags;
+    uint32_t *flags;            // per pair; flags[error_slot] is set when a bounded wait gives up
+    uint32_t error_slot;
     const int16_t *table;
     uint32_t table_entries;
     int32_t go, ge;

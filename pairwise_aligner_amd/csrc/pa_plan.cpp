@@ -31,6 +31,11 @@ static constexpr double kCellCost = 13.0, kStepCost = 24.0;
 void choose_shape(uint32_t m, uint32_t n, uint32_t width, uint8_t &G, uint8_t &R)
 {
     (void)width;
+    if (m > kWavefrontMinRows) {   // long pair: one workgroup of 8 waves, stripes of 64*16 rows
+        G = 64;
+        R = 16;
+        return;
+    }
     G = 16;
     double best = 1e300;
     R = 16;

@@ -140,8 +140,11 @@ struct RangeBound {
 };
 RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n);
 
-// Choice of (G, R) for a pair with m rows and n columns: the rows per lane with the lowest
-// modelled sweep cost (stripes x steps x per-step work).
+// Pairs with more rows than this go to the wavefront kernel (G = 64, a workgroup per task).
+inline constexpr uint32_t kWavefrontMinRows = 3072;
+
+// Choice of (G, R) for a pair with m rows and n columns: the wavefront kernel for long pairs,
+// else the rows per lane with the lowest modelled sweep cost (stripes x steps x per-step work).
 void choose_shape(uint32_t m, uint32_t n, uint32_t width, uint8_t &G, uint8_t &R);
 
 } // namespace pa

@@ -176,3 +176,39 @@ def test_batch_repeat_runs(ctx):
     assert np.array_equal(first.astype(np.int64), want)
     st = bt.stats()
     assert st["n_pairs"] == 3000 and st["cells"] == sum(len(a) * len(b) for a, b in zip(s1, s2))
+
+
+@pytest.mark.parametrize("policy", [device.LANES_INT32, device.LANES_AUTO])
+@pytest.mark.parametrize("method", [device.METHOD_GLOBAL, device.METHOD_LOCAL])
+def test_long_pairs_wavefront(ctx, policy, method):
+    """Pairs longer than 3072 rows go to the wavefront kernel (8 waves per task, LDS rings
+    between waves, HBM between rounds); 9,000-12,000 rows make 9-12 stripes of 1,024 rows,
+    i.e. two rounds.  Short pairs in the same batch go to the striped kernel."""
+    rng = np.random.default_rng(41 + policy + method)
+    l1, l2 = workloads.dna_pairs(rng, 6, 3100, 12000, divergence=0.15, indel_fraction=0.6, geom_p=0.5)
+    s1_, s2_ = workloads.dna_pairs(rng, 200, 50, 400, divergence=0.1)
+    s1 = l1 + s1_
+    s2 = l2 + s2_
+    dc = device.DeviceConfig(method=method, match=1, mismatch=-4, gap_open=-6, gap_extension=-1, lane_policy=policy)
+    check(ctx, s1, s2, dc, label=f"long policy={policy} method={method}")
+
+
+@pytest.mark.parametrize("flags", [(True, False, True, False), (False, True, False, True), (True, True, True, True)])
+def test_long_pairs_free_ends(ctx, flags):
+    fc, fr, lc, lr = flags
+    rng = np.random.default_rng(77)
+    s1, s2 = workloads.dna_pairs(rng, 4, 3500, 9000, divergence=0.3)
+    s2 = [s2[0], s2[1][: len(s2[1]) // 2], s2[2], s2[3]]
+    dc = device.DeviceConfig(first_column_free=fc, first_row_free=fr, last_column_free=lc, last_row_free=lr,
+                             match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
+    check(ctx, s1, s2, dc, label=f"long free ends {flags}")
+
+
+def test_long_protein_local(ctx):
+    rng = np.random.default_rng(91)
+    s1, s2 = workloads.protein_pairs(rng, 4, 3200, 6000)
+    t = tables.blosum62_standard()
+    dc = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
+                             symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
+                             gap_open=-11, gap_extension=-1)
+    check(ctx, s1, s2, dc, label="long protein local")
