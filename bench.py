@@ -73,29 +73,29 @@ def oracle_cfg(dc):
 
 def cpu_baseline(w, dc, gpu_scores, budget_s=12.0):
     """Times the CPU oracle (64-bit scalar Gotoh, C, one POSIX thread per core) on a bounded
-    sample of the same pairs and checks the GPU scores of that sample."""
+    sample of the same pairs (a prefix of the batch) and checks the GPU scores of that sample."""
     from oracle import oracle as O
     threads = min(os.cpu_count() or 1, 16)
     b1, o1, b2, o2 = w["b1"], w["o1"], w["b2"], w["o2"]
     n = len(o1) - 1
     oc = oracle_cfg(dc)
     pm_all = w.get("pair_method")
-    # calibrate on a small prefix, then size the sample to the budget
-    sample = min(n, 256)
-    t_cal = None
-    while True:
-        sub = slice(0, sample)
-        so1 = o1[:sample + 1]
-        so2 = o2[:sample + 1]
+    cells_cum = np.cumsum(np.diff(o1).astype(np.int64) * np.diff(o2).astype(np.int64))
+
+    def timed(sample):
         pm = None if pm_all is None else pm_all[:sample]
         t0 = time.perf_counter()
-        want = O.score_packed(b1, so1, b2, so2, oc, pair_method=pm, threads=threads)
-        dt = time.perf_counter() - t0
-        if sample == n or dt > budget_s / 3:
-            break
-        t_cal = dt
-        sample = min(n, int(sample * max(2.0, min(64.0, budget_s / max(dt, 1e-3) / 2))))
-    cells = int(np.sum(np.diff(o1[:sample + 1]).astype(np.int64) * np.diff(o2[:sample + 1]).astype(np.int64)))
+        want = O.score_packed(b1, o1[:sample + 1], b2, o2[:sample + 1], oc, pair_method=pm, threads=threads)
+        return want, time.perf_counter() - t0
+
+    # calibrate on a prefix of about 2e8 cells (at least `threads` pairs), then size the sample
+    cal = int(min(n, max(threads, np.searchsorted(cells_cum, 2e8) + 1)))
+    want, dt = timed(cal)
+    rate = cells_cum[cal - 1] / max(dt, 1e-6)
+    sample = int(min(n, max(cal, np.searchsorted(cells_cum, rate * budget_s) + 1)))
+    if sample > cal:
+        want, dt = timed(sample)
+    cells = int(cells_cum[sample - 1])
     mism = int(np.count_nonzero(gpu_scores[:sample].astype(np.int64) != want))
     return dict(value=cells / dt / 1e9, unit="GCUPS", cores=threads, kind="port",
                 sample=f"first {sample} of {n} pairs ({cells} cells), {dt:.2f} s, scalar 64-bit oracle "

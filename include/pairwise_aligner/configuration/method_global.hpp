@@ -1,0 +1,4 @@
+// Drop-in for rrahn/pairwise_aligner include/pairwise_aligner/configuration/method_global.hpp:
+// the whole configuration API lives in pairwise_aligner/amd/host_api.hpp.
+#pragma once
+#include <pairwise_aligner/amd/host_api.hpp>

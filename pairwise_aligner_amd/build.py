@@ -42,5 +42,20 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+def build_cpp_tests() -> str:
+    """Compiles tests/cpp/test_host_api (the header-only C++ host API against libpa_device.so)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "test_host_api")
+    cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), src,
+           "-L", HERE, "-lpa_device", "-Wl,-rpath,$ORIGIN/../../pairwise_aligner_amd",
+           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-o", out]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building tests/cpp/test_host_api")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_cpp_tests())

@@ -1,0 +1,182 @@
+// C++ host API test: the reference's scalar known-answer tests
+// (test/unit/affine/*_scalar*_test.cpp, test/unit/configuration/*.cpp), written the way the
+// reference's tests configure and call the aligner, plus one-to-one bulk and one-to-many calls
+// checked against the single-pair call.
+//
+//   test_host_api --config   checks the configuration layer only (no GPU needed)
+//   test_host_api            runs everything on device 0
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <string_view>
+#include <vector>
+
+#include <pairwise_aligner/configuration/configure_aligner.hpp>
+#include <pairwise_aligner/configuration/gap_model_affine.hpp>
+#include <pairwise_aligner/configuration/method_global.hpp>
+#include <pairwise_aligner/configuration/method_local.hpp>
+#include <pairwise_aligner/configuration/score_model_matrix.hpp>
+#include <pairwise_aligner/configuration/score_model_unitary.hpp>
+#include <pairwise_aligner/configuration/score_model_unitary_simd.hpp>
+#include <pairwise_aligner/score_model/substitution_matrix.hpp>
+
+using namespace std::literals;
+namespace aligner = seqan::pairwise_aligner;
+
+static int failures = 0;
+#define EXPECT_EQ(a, b)                                                                                   \
+    do {                                                                                                  \
+        auto va_ = (a);                                                                                   \
+        auto vb_ = (b);                                                                                   \
+        if (!(va_ == vb_)) {                                                                              \
+            std::printf("FAIL %s:%d: %s == %lld, expected %lld\n", __FILE__, __LINE__, #a, (long long)va_, \
+                        (long long)vb_);                                                                  \
+            ++failures;                                                                                   \
+        }                                                                                                 \
+    } while (0)
+
+inline constexpr auto standard = aligner::cfg::method_global(aligner::cfg::gap_model_affine(-10, -1),
+                                                             aligner::cfg::leading_end_gap{},
+                                                             aligner::cfg::trailing_end_gap{});
+inline constexpr auto overlap = aligner::cfg::method_global(
+    aligner::cfg::gap_model_affine(-10, -1),
+    aligner::cfg::leading_end_gap{aligner::cfg::end_gap::free, aligner::cfg::end_gap::free},
+    aligner::cfg::trailing_end_gap{aligner::cfg::end_gap::free, aligner::cfg::end_gap::free});
+inline constexpr auto semi_first = aligner::cfg::method_global(
+    aligner::cfg::gap_model_affine(-10, -1), aligner::cfg::leading_end_gap{.first_column = aligner::cfg::end_gap::free},
+    aligner::cfg::trailing_end_gap{.last_column = aligner::cfg::end_gap::free});
+inline constexpr auto semi_second = aligner::cfg::method_global(
+    aligner::cfg::gap_model_affine(-10, -1), aligner::cfg::leading_end_gap{.first_row = aligner::cfg::end_gap::free},
+    aligner::cfg::trailing_end_gap{.last_row = aligner::cfg::end_gap::free});
+inline constexpr auto local = aligner::cfg::method_local(aligner::cfg::gap_model_affine(-10, -1));
+
+template <typename config_t>
+static int32_t score_of(config_t const &config, std::string_view s1, std::string_view s2)
+{
+    auto a = aligner::cfg::configure_aligner(config);
+    return a.compute(s1, s2).score();
+}
+
+static void config_checks()
+{
+    auto a = aligner::cfg::configure_aligner(aligner::cfg::method_global(
+        aligner::cfg::gap_model_affine(aligner::cfg::score_model_unitary(4, -5), -10, -1),
+        aligner::cfg::leading_end_gap{.first_column = aligner::cfg::end_gap::penalised,
+                                      .first_row = aligner::cfg::end_gap::free},
+        aligner::cfg::trailing_end_gap{.last_column = aligner::cfg::end_gap::penalised,
+                                       .last_row = aligner::cfg::end_gap::free}));
+    const pa_config &c = a.config();
+    EXPECT_EQ(c.method, PA_METHOD_GLOBAL);
+    EXPECT_EQ(c.first_column_free, 0);
+    EXPECT_EQ(c.first_row_free, 1);
+    EXPECT_EQ(c.last_column_free, 0);
+    EXPECT_EQ(c.last_row_free, 1);
+    EXPECT_EQ(c.gap_open, -10);
+    EXPECT_EQ(c.gap_extension, -1);
+    EXPECT_EQ(c.match, 4);
+    EXPECT_EQ(c.mismatch, -5);
+    EXPECT_EQ(c.lane_policy, PA_LANES_AUTO);
+    auto m = aligner::cfg::configure_aligner(aligner::cfg::score_model_matrix(local, aligner::blosum62_extended<>));
+    EXPECT_EQ(m.config().method, PA_METHOD_LOCAL);
+    EXPECT_EQ(m.config().matrix_dim, 27);
+    EXPECT_EQ(int(m.symbols()[26]), int('*'));
+    EXPECT_EQ(m.matrix()[26 * 27 + 26], 1);
+    auto s = aligner::cfg::configure_aligner(aligner::cfg::score_model_unitary_simd(standard, (int16_t)4, (int16_t)-5));
+    EXPECT_EQ(s.config().lane_policy, PA_LANES_INT16);
+    auto d = aligner::cfg::configure_aligner(aligner::cfg::gap_model_affine(aligner::cfg::score_model_unitary(1, -1), -2, -1));
+    EXPECT_EQ(d.config().method, PA_METHOD_GLOBAL);   // default method
+}
+
+static void known_answers()
+{
+    using aligner::cfg::score_model_unitary;
+    using aligner::cfg::score_model_matrix;
+    // global_standard_affine_scalar_test.cpp:34-53
+    EXPECT_EQ(score_of(score_model_unitary(standard, 4, -5), "ACGTGACTGACACTACGACT"sv, "ACGTGACTGACACTACGACT"sv), 80);
+    EXPECT_EQ(score_of(score_model_unitary(standard, 4, -5), "AAAAAAAAAA"sv, "TTTTTTTTTT"sv), -40);
+    EXPECT_EQ(score_of(score_model_unitary(standard, 4, -5), "AGATCGACTAGCGAGCTACGAGCTAGC"sv, "AGACGATCGACGAGCGACTACGTACGA"sv), 26);
+    // global_overlap_affine_scalar_test.cpp:35-54
+    EXPECT_EQ(score_of(score_model_unitary(overlap, 4, -5), "ACGTGACTGACACTACGACT"sv, "ACGTGACTGACACTACGACT"sv), 80);
+    EXPECT_EQ(score_of(score_model_unitary(overlap, 4, -5), "TTTTTCTGACACACT"sv, "CTGCACACGACGGGGGG"sv), 16);
+    EXPECT_EQ(score_of(score_model_unitary(overlap, 4, -5), "CTGCACACGACGGGGGG"sv, "TTTTTCTGACACACT"sv), 16);
+    // global_semi_first_affine_scalar_test.cpp:35-54
+    EXPECT_EQ(score_of(score_model_unitary(semi_first, 4, -5), "TTTTTACGTATGTCCCCC"sv, "ACGTAAAACGT"sv), 10);
+    EXPECT_EQ(score_of(score_model_unitary(semi_first, 4, -5), "AAGACTACGGGGGG"sv, "ACGAGGCTAC"sv), 11);
+    EXPECT_EQ(score_of(score_model_unitary(semi_first, 4, -5), "GGGGGGAAGACTAC"sv, "ACGAGGCTAC"sv), 11);
+    // global_semi_second_affine_scalar_test.cpp:35-54
+    EXPECT_EQ(score_of(score_model_unitary(semi_second, 4, -5), "ACGTAAAACGT"sv, "TTTTTACGTATGTCCCCC"sv), 10);
+    EXPECT_EQ(score_of(score_model_unitary(semi_second, 4, -5), "ACGAGGCTAC"sv, "AAGACTACGGGGGG"sv), 11);
+    EXPECT_EQ(score_of(score_model_unitary(semi_second, 4, -5), "ACGAGGCTAC"sv, "GGGGGGAAGACTAC"sv), 11);
+    // global_standard_affine_scalar_matrix_test.cpp:32-58
+    EXPECT_EQ(score_of(score_model_matrix(standard, aligner::blosum62_standard<>), "ACKLMNPQRRTVWYNMPQHIK"sv, "ACKLMNPQRRTVWYNMPQHIK"sv), 122);
+    EXPECT_EQ(score_of(score_model_matrix(standard, aligner::blosum62_standard<>), "AAAAAAAAAAAAAAAAAAAA"sv, "CDEFGHIKLMNPQRSTVWLY"sv), -21);
+    EXPECT_EQ(score_of(score_model_matrix(standard, aligner::blosum62_standard<>), "FNQSAEYPDISHCGVMQLKWRATLGT"sv, "EIKSDVLLHRWSMKNPGNILMIDVGMQVAESYFAT"sv), -26);
+    EXPECT_EQ(score_of(score_model_matrix(standard, aligner::blosum62_standard<>), "RKFCYMD"sv, "GAYQW"sv), -11);
+    // local_affine_scalar_test.cpp:39-105
+    EXPECT_EQ(score_of(score_model_unitary(local, 4, -5), "TTTTTTTTTTTTTTT"sv, "TTTTTTTTTTTTTTT"sv), 60);
+    EXPECT_EQ(score_of(score_model_unitary(local, 4, -5), "CTGCACACGACGGGGGG"sv, "TTTTTCTGACACACT"sv), 21);
+    EXPECT_EQ(score_of(score_model_unitary(local, 4, -5), "AAAAAAAAAA"sv, "TTTTTTTTTT"sv), 0);
+    EXPECT_EQ(score_of(score_model_unitary(local, 4, -5), "AAAAATAAAA"sv, "CCCCCTCCCC"sv), 4);
+    EXPECT_EQ(score_of(score_model_unitary(local, 4, -5), "AACCGGTTTAACCGGTT"sv, "ACGTCTACGTA"sv), 11);
+    EXPECT_EQ(score_of(score_model_unitary(local, 4, -5), "AAAAAATCCCCCC"sv, "CCCCCCTAAAAAA"sv), 24);
+    EXPECT_EQ(score_of(score_model_unitary(aligner::cfg::method_local(aligner::cfg::gap_model_affine(-1, -1)), 2, -1),
+                       "ATAAGCGTCTCG"sv, "TCATAGAGTTGC"sv), 9);
+    EXPECT_EQ(score_of(score_model_matrix(local, aligner::blosum62_extended<>), "ALIGATOR"sv, "GALORA"sv), 13);
+    EXPECT_EQ(score_of(score_model_matrix(local, aligner::blosum62_extended<>), "ALIGATOR"sv, ""sv),
+              std::numeric_limits<int32_t>::lowest());
+    // configuration_test.cpp:18-37
+    auto proto = aligner::cfg::configure_aligner(aligner::cfg::method_global(
+        aligner::cfg::gap_model_affine(aligner::cfg::score_model_unitary(4, -5), -10, -1),
+        aligner::cfg::leading_end_gap{.first_column = aligner::cfg::end_gap::penalised,
+                                      .first_row = aligner::cfg::end_gap::free},
+        aligner::cfg::trailing_end_gap{.last_column = aligner::cfg::end_gap::penalised,
+                                       .last_row = aligner::cfg::end_gap::free}));
+    std::string_view seq{"ACGTGACTGACACTACGACT"};
+    EXPECT_EQ(proto.compute(seq, seq).score(), 80);
+    // configure_aligner_saturated_test.cpp:18-42 (bulk of one)
+    std::vector collection1{seq};
+    std::vector collection2{seq};
+    auto sat = aligner::cfg::configure_aligner(aligner::cfg::method_global(
+        aligner::cfg::gap_model_affine(aligner::cfg::score_model_unitary_simd((int16_t)4, (int16_t)-5), -10, -1),
+        aligner::cfg::leading_end_gap{}, aligner::cfg::trailing_end_gap{}));
+    EXPECT_EQ(sat.compute(collection1, collection2)[0].score(), 80);
+}
+
+static void bulk_calls()
+{
+    // one-to-one bulk larger than any SIMD width, and one-to-many, against single calls
+    std::vector<std::string> s1, s2;
+    unsigned x = 12345;
+    auto rnd = [&] { x = x * 1103515245u + 12345u; return (x >> 16) & 0x7fff; };
+    for (int i = 0; i < 300; ++i) {
+        std::string a, b;
+        for (int k = 0, n = 1 + rnd() % 120; k < n; ++k) a.push_back("ACGT"[rnd() % 4]);
+        for (int k = 0, n = 1 + rnd() % 120; k < n; ++k) b.push_back("ACGT"[rnd() % 4]);
+        s1.push_back(a);
+        s2.push_back(b);
+    }
+    auto a = aligner::cfg::configure_aligner(aligner::cfg::score_model_unitary(local, 4, -5));
+    auto bulk = a.compute(s1, s2);
+    EXPECT_EQ(bulk.size(), s1.size());
+    for (std::size_t i = 0; i < s1.size(); i += 37) {
+        EXPECT_EQ(bulk[i].score(), a.compute(s1[i], s2[i]).score());
+        EXPECT_EQ(bulk[i].sequence1() == s1[i], true);
+    }
+    auto many = a.compute(s1[0], s2);
+    for (std::size_t i = 0; i < s2.size(); i += 41) {
+        EXPECT_EQ(many[i].score(), a.compute(s1[0], s2[i]).score());
+        EXPECT_EQ(many[i].sequence1() == s1[0], true);
+    }
+}
+
+int main(int argc, char **argv)
+{
+    config_checks();
+    if (!(argc > 1 && std::strcmp(argv[1], "--config") == 0)) {
+        known_answers();
+        bulk_calls();
+    }
+    std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "OK", failures);
+    return failures ? 1 : 0;
+}
