@@ -141,23 +141,98 @@ def run(name, steps, warmup, with_cpu=True):
     return res
 
 
+def run_mixed(count, steps, warmup, with_oracle=True):
+    """Learned lane dispatch on the mixed workload: the same pairs under fixed int32 lanes, the
+    length-bound policy and the learned policy.  Both halves are aligned locally (DNA +2/-3,
+    gap (-5,-2); protein BLOSUM62, gap (-11,-1)).  The network is trained first, on labels the
+    DP produces for separately seeded training pairs."""
+    from pairwise_aligner_amd import learned
+    t0 = time.perf_counter()
+    w = workloads.config_mixed(count=count)
+    log(f"[bench] mixed: {count} pairs generated in {time.perf_counter() - t0:.1f} s")
+    ctx = device.default_context(0)
+    dna = device.DeviceConfig(method=device.METHOD_LOCAL, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
+    prot = blosum_config(device.METHOD_LOCAL, -11, -1, device.LANES_AUTO)
+    t1 = time.perf_counter()
+    sets = [(prot, learned.training_pairs("protein", 1001, 4000)), (dna, learned.training_pairs("dna", 1002, 2000))]
+    x, y = learned.build_dataset(ctx, sets)
+    model, losses = learned.train(ctx, x, y, steps=1500, batch=2048, log=log)
+    train_s = time.perf_counter() - t1
+    halves = [("dna", dna, w["dna"]), ("protein", prot, w["protein"])]
+    cells = {k: int(np.sum(np.diff(h["o1"]).astype(np.int64) * np.diff(h["o2"]).astype(np.int64)))
+             for k, _, h in halves}
+    out = {"pairs": count, "cells": sum(cells.values()), "train_seconds": round(train_s, 2),
+           "train_loss_first_last": [round(float(np.mean(losses[:20])), 5), round(float(np.mean(losses[-20:])), 5)]}
+    ref_scores = {}
+    for name, policy in (("int32", device.LANES_INT32), ("bound", device.LANES_AUTO), ("learned", device.LANES_LEARNED)):
+        total_ms = 0.0
+        n16 = n32 = nrec = 0
+        mism = 0
+        for kind, dc, h in halves:
+            dcp = device.DeviceConfig(**{**dc.__dict__, "lane_policy": policy,
+                                         "model": model if policy == device.LANES_LEARNED else None})
+            bt = device.Batch(ctx, h["b1"], h["o1"], h["b2"], h["o2"], dcp)
+            for _ in range(warmup):
+                bt.run()
+            ms, _ = bt.time(steps)
+            total_ms += ms / steps
+            sc, _ = bt.results()
+            st = bt.stats()
+            n16 += st["n_int16"]; n32 += st["n_int32"]; nrec += st["n_recomputed"]
+            if kind in ref_scores:
+                mism += int(np.count_nonzero(ref_scores[kind] != sc))
+            else:
+                ref_scores[kind] = sc
+            bt.close()
+        out[name] = {"gcups": round(out["cells"] / (total_ms / 1e3) / 1e9, 2), "ms_per_step": round(total_ms, 3),
+                     "n_int16": n16, "n_int32": n32, "n_recomputed": nrec, "score_mismatches_vs_int32": mism}
+        log(f"[bench] mixed/{name}: {out[name]}")
+    out["speedup_learned_vs_int32"] = round(out["learned"]["gcups"] / out["int32"]["gcups"], 3)
+    out["speedup_bound_vs_int32"] = round(out["bound"]["gcups"] / out["int32"]["gcups"], 3)
+    if with_oracle:
+        from oracle import oracle as O
+        bad = 0
+        for kind, dc, h in halves:
+            k = 40
+            want = O.score_packed(h["b1"], h["o1"][:k + 1], h["b2"], h["o2"][:k + 1], oracle_cfg(dc),
+                                  threads=min(os.cpu_count() or 1, 16))
+            bad += int(np.count_nonzero(ref_scores[kind][:k].astype(np.int64) != want))
+        out["oracle_sample_mismatches"] = bad
+        out["oracle_sample_pairs"] = 80
+    model.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="protein")
-    ap.add_argument("--no-extra", action="store_true", help="skip the secondary DNA workload")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--mixed-pairs", type=int, default=100_000,
+                    help="pairs of the mixed workload used for the learned-dispatch comparison (config: 500,000)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (pairs are independent; multi-GPU sharding is not built); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "mixed":
+        res = run_mixed(args.mixed_pairs, max(1, args.steps), args.warmup)
+        print(json.dumps({"metric": "GCUPS score-only affine DP on 1 GPU vs CPU ref; learned dispatch vs fixed int32",
+                          "value": res["learned"]["gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": res["learned"]["ms_per_step"],
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int16+int32",
+                          "data": "synthetic", "config": {"workload": f"mixed DNA/protein, {args.mixed_pairs} pairs"},
+                          "mixed_learned_dispatch": res}))
+        return
     main_res = run(args.workload, args.steps, args.warmup)
-    extra = None
+    extra = mixed = None
     if not args.no_extra and args.workload != "dna150":
         extra = run("dna150", max(args.steps, 50), args.warmup, with_cpu=False)
+    if not args.no_extra:
+        mixed = run_mixed(args.mixed_pairs, 1, 1)
 
     cpu = main_res.pop("cpu")
     frac = main_res["achieved_tops"] / main_res["peak_tops"]
@@ -188,6 +263,8 @@ def main():
         "mismatches_vs_oracle_sample": cpu["mismatches_vs_gpu"],
         "plan": {k: main_res["stats"][k] for k in ("n_int16", "n_int32", "n_recomputed", "padded_cells", "n_launches")},
     }
+    if mixed is not None:
+        line["mixed_learned_dispatch"] = mixed
     if extra is not None:
         line["dna150_global_int16"] = {"gcups": round(extra["gcups"], 2), "ms_per_step": round(extra["ms_per_step"], 4),
                                        "kernel_ms": round(extra["main_kernel_ms"], 4),

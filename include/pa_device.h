@@ -57,10 +57,11 @@ typedef struct pa_config {
     int32_t score_kind;         /* PA_SCORE_* */
     int32_t match;              /* unitary */
     int32_t mismatch;
-    int32_t matrix_dim;         /* matrix: number of symbols (<= 63) */
+    int32_t matrix_dim;         /* matrix: number of symbols (<= 31) */
     const int32_t *matrix;      /* matrix: dim*dim scores, [rank of seq1 symbol][rank of seq2 symbol] */
     const uint8_t *matrix_symbols; /* matrix: the dim symbols; rank = position (alphabet_rank_map_scalar.hpp:39-51) */
     int32_t lane_policy;        /* PA_LANES_* */
+    const struct pa_model *model; /* PA_LANES_LEARNED: the trained range predictor (else ignored) */
 } pa_config;
 
 /* per-pair result flags */
@@ -71,6 +72,7 @@ typedef struct pa_config {
 
 typedef struct pa_ctx pa_ctx;
 typedef struct pa_batch pa_batch;
+typedef struct pa_model pa_model;
 
 typedef struct pa_batch_stats {
     uint64_t n_pairs;
@@ -114,6 +116,44 @@ pa_status pa_batch_stats_get(const pa_batch *batch, pa_batch_stats *out);
  * pass), used for the roofline. */
 pa_status pa_batch_time(pa_ctx *ctx, pa_batch *batch, int iters, double *ms_total, double *ms_main);
 pa_status pa_batch_destroy(pa_ctx *ctx, pa_batch *batch);
+
+/* ---------------------------------------------------------------------------------------------
+ * Learned lane dispatch.  No counterpart in the reference (its lane width is a compile-time
+ * choice of the score-model rule, score_model_unitary_simd.hpp:51-137).  A 3-layer FP32 MLP
+ * (PA_FEATURES -> 64 -> 64 -> 2) maps per-pair features to
+ *   y0 = max over the DP of H / (max substitution score * min(m, n))   (score range)
+ *   y1 = log2(modelled sweep cost of the pair) / 32                     (cell cost)
+ * Feature extraction, forward, backward and the Adam update run on the GPU.
+ * ------------------------------------------------------------------------------------------- */
+#define PA_FEATURES 8
+#define PA_MODEL_OUTPUTS 2
+#define PA_MODEL_HIDDEN 64
+
+/* Creates a model with seeded random weights (He initialisation, zero biases). */
+pa_status pa_model_create(pa_ctx *ctx, uint64_t seed, pa_model **out);
+pa_status pa_model_destroy(pa_model *model);
+int pa_model_param_count(void);
+/* Copies the flat parameter vector [W1 (F x H), b1, W2 (H x H), b2, W3 (H x 2), b3] in or out. */
+pa_status pa_model_set_params(pa_ctx *ctx, pa_model *model, const float *params);
+pa_status pa_model_get_params(pa_ctx *ctx, const pa_model *model, float *params);
+/* Per-pair features (n_pairs x PA_FEATURES, row-major) of a batch, computed on the GPU:
+ * log lengths, length ratio, hashed 3-mer composition distance, k-mer containment and the
+ * identity it implies, substitution score scale. */
+pa_status pa_features(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                      const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, float *features);
+/* Forward pass: y (n x 2). */
+pa_status pa_model_forward(pa_ctx *ctx, const pa_model *model, const float *x, uint64_t n, float *y);
+/* Mean-squared-error loss and its gradient with respect to the parameters (flat, as above);
+ * y weights (n x 2, nullable) scale each output's error. */
+pa_status pa_model_loss_grad(pa_ctx *ctx, const pa_model *model, const float *x, const float *target,
+                             const float *weight, uint64_t n, float *loss, float *grad);
+/* One Adam step on a minibatch; returns the loss before the step. */
+pa_status pa_model_train_step(pa_ctx *ctx, pa_model *model, const float *x, const float *target,
+                              const float *weight, uint64_t n, float learning_rate, float *loss);
+/* Per-pair labels produced by the DP itself: the maximum of H over the matrix (int32 pass with
+ * the running maximum kept) and the modelled sweep cost of the pair's task in the plan. */
+pa_status pa_dp_labels(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                       const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, int32_t *max_h, float *cost);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -16,6 +17,7 @@
 
 #include "pa_device.h"
 #include "pa_kernels.hpp"
+#include "pa_learned.hpp"
 #include "pa_plan.hpp"
 
 struct DeviceBuffer {
@@ -60,6 +62,12 @@ struct pa_batch {
     pa_batch_stats stats{};
     double main_ms_accum = 0;
     uint64_t main_ms_count = 0;
+};
+
+struct pa_model {
+    int device = 0;
+    DeviceBuffer params, grad, m1, m2, loss;
+    int step = 0;
 };
 
 namespace {
@@ -230,6 +238,81 @@ pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
     return launch_plan(ctx, b, *b->recompute, false);
 }
 
+// ---- learned dispatch helpers ---------------------------------------------------------------
+
+// Feature-kernel arguments for a batch: k-mers over 2-bit codes when the batch uses unitary
+// scoring over at most four symbols, else 5-bit residue codes (byte & 31) and k = 3.
+void feature_setup(const pa_config &cfg, const pa::BatchView &v, pa::FeatureArgs &fa)
+{
+    bool seen[256] = {false};
+    for (uint64_t i = v.off1[0]; i < v.off1[v.n_pairs]; ++i) seen[v.seq1[i]] = true;
+    for (uint64_t i = v.off2[0]; i < v.off2[v.n_pairs]; ++i) seen[v.seq2[i]] = true;
+    int k = 0;
+    for (int c = 0; c < 256; ++c) if (seen[c]) ++k;
+    const bool small = cfg.score_kind == PA_SCORE_UNITARY && k <= 4;
+    int r = 0;
+    for (int c = 0; c < 256; ++c) fa.code[c] = small ? uint8_t(seen[c] ? r++ : 0) : uint8_t(c & 31);
+    fa.k = small ? 8 : 3;
+    fa.bits = small ? 2 : 5;
+    fa.smax = float(pa::max_substitution(cfg));
+}
+
+pa_status device_features(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView &v, DeviceBuffer &dfeat)
+{
+    pa::FeatureArgs fa{};
+    feature_setup(cfg, v, fa);
+    std::vector<uint8_t> s1(v.seq1 + v.off1[0], v.seq1 + v.off1[v.n_pairs]);
+    std::vector<uint8_t> s2(v.seq2 + v.off2[0], v.seq2 + v.off2[v.n_pairs]);
+    std::vector<uint64_t> o1(v.off1, v.off1 + v.n_pairs + 1), o2(v.off2, v.off2 + v.n_pairs + 1);
+    for (auto &o : o1) o -= v.off1[0];
+    for (auto &o : o2) o -= v.off2[0];
+    DeviceBuffer ds1, ds2, do1, do2;
+    pa_status st;
+    if ((st = upload(ctx, ds1, s1)) != PA_OK) return st;
+    if ((st = upload(ctx, ds2, s2)) != PA_OK) return st;
+    if ((st = upload(ctx, do1, o1)) != PA_OK) return st;
+    if ((st = upload(ctx, do2, o2)) != PA_OK) return st;
+    PA_HIP(hipMalloc(&dfeat.ptr, std::max<size_t>(v.n_pairs * pa::kModelFeatures * 4, 16)), "hipMalloc");
+    fa.seq1 = static_cast<const uint8_t *>(ds1.ptr);
+    fa.off1 = static_cast<const uint64_t *>(do1.ptr);
+    fa.seq2 = static_cast<const uint8_t *>(ds2.ptr);
+    fa.off2 = static_cast<const uint64_t *>(do2.ptr);
+    fa.n_pairs = v.n_pairs;
+    fa.out = static_cast<float *>(dfeat.ptr);
+    hipError_t e = pa::launch_features(fa, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "feature launch");
+    PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return PA_OK;
+}
+
+// Lane widths from the learned range predictor: int16 where the bound proves it, or where the
+// predicted maximum of H with a 10% + 256 margin stays below the int16 limit (then the pair is
+// tracked and recomputed in int32 if it saturates after all); int32 otherwise.
+pa_status learned_widths(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView &v, std::vector<uint8_t> &forced)
+{
+    if (!cfg.model) return fail(ctx, PA_ERR_INVALID_ARGUMENT, "PA_LANES_LEARNED needs pa_config.model");
+    DeviceBuffer dfeat, dy;
+    pa_status st;
+    if ((st = device_features(ctx, cfg, v, dfeat)) != PA_OK) return st;
+    PA_HIP(hipMalloc(&dy.ptr, std::max<size_t>(v.n_pairs * pa::kModelOutputs * 4, 16)), "hipMalloc");
+    hipError_t e = pa::launch_forward(static_cast<const float *>(cfg.model->params.ptr),
+                                      static_cast<const float *>(dfeat.ptr), v.n_pairs,
+                                      static_cast<float *>(dy.ptr), ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "forward launch");
+    std::vector<float> y(v.n_pairs * pa::kModelOutputs);
+    if (!y.empty())
+        PA_HIP(hipMemcpyAsync(y.data(), dy.ptr, y.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    const double smax = std::max(1, pa::max_substitution(cfg));
+    forced.assign(v.n_pairs, 0);
+    for (uint64_t p = 0; p < v.n_pairs; ++p) {
+        const double mn = double(std::min(v.off1[p + 1] - v.off1[p], v.off2[p + 1] - v.off2[p]));
+        const double pred = std::max(0.0, double(y[p * pa::kModelOutputs])) * smax * mn;
+        forced[p] = (pred * 1.10 + 256.0 < 32767.0) ? 16 : 32;
+    }
+    return PA_OK;
+}
+
 // The wavefront kernel's bounded waits set flags[n_pairs] when they give up.
 pa_status check_error_slot(pa_ctx *ctx, pa_batch *b)
 {
@@ -311,7 +394,12 @@ pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1
     pa::BatchView view{b->seq1.data(), b->off1.data(), b->seq2.data(), b->off2.data(), n_pairs,
                        pair_method ? b->pair_method.data() : nullptr};
     std::string err;
-    pa_status st = pa::make_plan(b->cfg, view, nullptr, nullptr, b->main.plan, err);
+    std::vector<uint8_t> forced;
+    pa_status st;
+    if (cfg->lane_policy == PA_LANES_LEARNED) {
+        if ((st = learned_widths(ctx, b->cfg, view, forced)) != PA_OK) return st;
+    }
+    st = pa::make_plan(b->cfg, view, forced.empty() ? nullptr : forced.data(), nullptr, b->main.plan, err);
     if (st != PA_OK) return fail(ctx, st, err);
     const pa::Plan &p = b->main.plan;
     // Only int16 pairs whose bound does not exclude saturation need the flag read-back.
@@ -395,6 +483,207 @@ pa_status pa_batch_destroy(pa_ctx *ctx, pa_batch *b)
     if (!b) return PA_ERR_INVALID_ARGUMENT;
     if (ctx) (void)hipSetDevice(ctx->device);
     delete b;
+    return PA_OK;
+}
+
+int pa_model_param_count(void) { return pa::kModelParams; }
+
+pa_status pa_model_create(pa_ctx *ctx, uint64_t seed, pa_model **out)
+{
+    if (!ctx || !out) return PA_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto m = std::make_unique<pa_model>();
+    m->device = ctx->device;
+    // He initialisation with a splitmix64 + Box-Muller generator; biases zero
+    std::vector<float> p(pa::kModelParams, 0.0f);
+    uint64_t state = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto next = [&]() {
+        uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return double((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto gauss = [&]() { return std::sqrt(-2.0 * std::log(next() + 1e-300)) * std::cos(6.283185307179586 * next()); };
+    const int F = pa::kModelFeatures, H = pa::kModelHidden, O = pa::kModelOutputs;
+    int off = 0;
+    for (int i = 0; i < F * H; ++i) p[off + i] = float(gauss() * std::sqrt(2.0 / F));
+    off += F * H + H;
+    for (int i = 0; i < H * H; ++i) p[off + i] = float(gauss() * std::sqrt(2.0 / H));
+    off += H * H + H;
+    for (int i = 0; i < H * O; ++i) p[off + i] = float(gauss() * std::sqrt(1.0 / H));
+    pa_status st;
+    if ((st = upload(ctx, m->params, p)) != PA_OK) return st;
+    std::vector<float> zeros(pa::kModelParams, 0.0f);
+    if ((st = upload(ctx, m->grad, zeros)) != PA_OK) return st;
+    if ((st = upload(ctx, m->m1, zeros)) != PA_OK) return st;
+    if ((st = upload(ctx, m->m2, zeros)) != PA_OK) return st;
+    std::vector<float> one(4, 0.0f);
+    if ((st = upload(ctx, m->loss, one)) != PA_OK) return st;
+    PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    *out = m.release();
+    return PA_OK;
+}
+
+pa_status pa_model_destroy(pa_model *model)
+{
+    if (!model) return PA_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(model->device);
+    delete model;
+    return PA_OK;
+}
+
+pa_status pa_model_set_params(pa_ctx *ctx, pa_model *model, const float *params)
+{
+    if (!ctx || !model || !params) return PA_ERR_INVALID_ARGUMENT;
+    PA_HIP(hipMemcpy(model->params.ptr, params, pa::kModelParams * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    std::vector<float> zeros(pa::kModelParams, 0.0f);
+    PA_HIP(hipMemcpy(model->m1.ptr, zeros.data(), pa::kModelParams * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    PA_HIP(hipMemcpy(model->m2.ptr, zeros.data(), pa::kModelParams * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    model->step = 0;
+    return PA_OK;
+}
+
+pa_status pa_model_get_params(pa_ctx *ctx, const pa_model *model, float *params)
+{
+    if (!ctx || !model || !params) return PA_ERR_INVALID_ARGUMENT;
+    PA_HIP(hipMemcpy(params, model->params.ptr, pa::kModelParams * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    return PA_OK;
+}
+
+pa_status pa_features(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                      const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, float *features)
+{
+    if (!ctx || !cfg || !off1 || !off2 || !features || (n_pairs && (!seq1 || !seq2)))
+        return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_pairs == 0) return PA_OK;
+    pa::BatchView v{seq1, off1, seq2, off2, n_pairs, nullptr};
+    DeviceBuffer dfeat;
+    pa_status st = device_features(ctx, *cfg, v, dfeat);
+    if (st != PA_OK) return st;
+    PA_HIP(hipMemcpy(features, dfeat.ptr, n_pairs * pa::kModelFeatures * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    return PA_OK;
+}
+
+pa_status pa_model_forward(pa_ctx *ctx, const pa_model *model, const float *x, uint64_t n, float *y)
+{
+    if (!ctx || !model || (n && (!x || !y))) return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return PA_OK;
+    DeviceBuffer dx, dy;
+    std::vector<float> hx(x, x + n * pa::kModelFeatures);
+    pa_status st;
+    if ((st = upload(ctx, dx, hx)) != PA_OK) return st;
+    PA_HIP(hipMalloc(&dy.ptr, n * pa::kModelOutputs * 4), "hipMalloc");
+    hipError_t e = pa::launch_forward(static_cast<const float *>(model->params.ptr), static_cast<const float *>(dx.ptr),
+                                      n, static_cast<float *>(dy.ptr), ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "forward launch");
+    PA_HIP(hipMemcpyAsync(y, dy.ptr, n * pa::kModelOutputs * 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return PA_OK;
+}
+
+static pa_status loss_grad_device(pa_ctx *ctx, const pa_model *model, const float *x, const float *target,
+                                  const float *weight, uint64_t n, DeviceBuffer &dx, DeviceBuffer &dt, DeviceBuffer &dw)
+{
+    std::vector<float> hx(x, x + n * pa::kModelFeatures), ht(target, target + n * pa::kModelOutputs);
+    pa_status st;
+    if ((st = upload(ctx, dx, hx)) != PA_OK) return st;
+    if ((st = upload(ctx, dt, ht)) != PA_OK) return st;
+    if (weight) {
+        std::vector<float> hw(weight, weight + n * pa::kModelOutputs);
+        if ((st = upload(ctx, dw, hw)) != PA_OK) return st;
+    }
+    hipError_t e = pa::launch_loss_grad(static_cast<const float *>(model->params.ptr),
+                                        static_cast<const float *>(dx.ptr), static_cast<const float *>(dt.ptr),
+                                        weight ? static_cast<const float *>(dw.ptr) : nullptr, n,
+                                        static_cast<float *>(model->grad.ptr), static_cast<float *>(model->loss.ptr),
+                                        ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "loss/gradient launch");
+    return PA_OK;
+}
+
+pa_status pa_model_loss_grad(pa_ctx *ctx, const pa_model *model, const float *x, const float *target,
+                             const float *weight, uint64_t n, float *loss, float *grad)
+{
+    if (!ctx || !model || !x || !target || n == 0) return fail(ctx, PA_ERR_INVALID_ARGUMENT, "invalid argument");
+    DeviceBuffer dx, dt, dw;
+    pa_status st = loss_grad_device(ctx, model, x, target, weight, n, dx, dt, dw);
+    if (st != PA_OK) return st;
+    if (grad)
+        PA_HIP(hipMemcpyAsync(grad, model->grad.ptr, pa::kModelParams * 4, hipMemcpyDeviceToHost, ctx->stream),
+               "hipMemcpyAsync");
+    if (loss) PA_HIP(hipMemcpyAsync(loss, model->loss.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return PA_OK;
+}
+
+pa_status pa_model_train_step(pa_ctx *ctx, pa_model *model, const float *x, const float *target, const float *weight,
+                              uint64_t n, float learning_rate, float *loss)
+{
+    if (!ctx || !model || !x || !target || n == 0) return fail(ctx, PA_ERR_INVALID_ARGUMENT, "invalid argument");
+    DeviceBuffer dx, dt, dw;
+    pa_status st = loss_grad_device(ctx, model, x, target, weight, n, dx, dt, dw);
+    if (st != PA_OK) return st;
+    model->step += 1;
+    hipError_t e = pa::launch_adam(static_cast<float *>(model->params.ptr), static_cast<const float *>(model->grad.ptr),
+                                   static_cast<float *>(model->m1.ptr), static_cast<float *>(model->m2.ptr),
+                                   learning_rate, model->step, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "adam launch");
+    if (loss) PA_HIP(hipMemcpyAsync(loss, model->loss.ptr, 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return PA_OK;
+}
+
+pa_status pa_dp_labels(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                       const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, int32_t *max_h, float *cost)
+{
+    if (!ctx || !cfg || !off1 || !off2 || !max_h || (n_pairs && (!seq1 || !seq2)))
+        return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_pairs == 0) return PA_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PA_ERR_DEVICE, "hipSetDevice");
+    // every pair in int32 lanes with the running maximum kept; launches run one after the other
+    // so that each one's HIP-event time can be shared out over its tasks by modelled cost
+    pa_batch b{};
+    b.cfg = *cfg;
+    b.n_pairs = n_pairs;
+    pa::BatchView view{seq1, off1, seq2, off2, n_pairs, nullptr};
+    std::vector<uint8_t> forced(n_pairs, 32);
+    std::string err;
+    pa_status st = pa::make_plan(*cfg, view, forced.data(), nullptr, b.main.plan, err, true);
+    if (st != PA_OK) return fail(ctx, st, err);
+    if ((st = upload_plan(ctx, b.main)) != PA_OK) return st;
+    if ((st = upload(ctx, b.pairs, b.main.plan.pairs)) != PA_OK) return st;
+    PA_HIP(hipMalloc(&b.scores.ptr, n_pairs * 4), "hipMalloc");
+    PA_HIP(hipMalloc(&b.flags.ptr, (n_pairs + 1) * 4), "hipMalloc");
+    DeviceBuffer dmax;
+    std::vector<int32_t> init(n_pairs, INT32_MIN);
+    if ((st = upload(ctx, dmax, init)) != PA_OK) return st;
+    hipError_t e = pa::launch_init(init_args(&b, b.main.plan), ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "init launch");
+    std::vector<double> pair_cost(n_pairs, 0.0);
+    for (const pa::Launch &L : b.main.plan.launches) {
+        pa::KernelArgs a = kernel_args(b.main, &b, L);
+        a.maxh = static_cast<int32_t *>(dmax.ptr);
+        const uint32_t lds = b.main.plan.table.empty() ? 0 : pa::lds_table_bytes(b.main.plan.alphabet, b.main.plan.stride);
+        PA_HIP(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
+        e = pa::launch_stripe(L.key, a, lds, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "striped DP launch");
+        PA_HIP(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
+        PA_HIP(hipEventSynchronize(ctx->ev[3]), "hipEventSynchronize");
+        float ms = 0;
+        PA_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
+        double total = 0;
+        for (uint32_t t = L.task_begin; t < L.task_end; ++t) total += pa::task_cost(L.key, b.main.plan.tasks[t]);
+        for (uint32_t t = L.task_begin; t < L.task_end; ++t) {
+            const pa::Task &tk = b.main.plan.tasks[t];
+            const double share = double(ms) * 1e6 * pa::task_cost(L.key, tk) / std::max(total, 1e-9);
+            for (int q = 0; q < 2; ++q)
+                if (tk.pair[q] != pa::kNone) pair_cost[tk.pair[q]] = share / (tk.pair[1] != pa::kNone ? 2.0 : 1.0);
+        }
+    }
+    PA_HIP(hipMemcpy(max_h, dmax.ptr, n_pairs * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    if ((st = check_error_slot(ctx, &b)) != PA_OK) return st;
+    if (cost)
+        for (uint64_t p = 0; p < n_pairs; ++p) cost[p] = float(pair_cost[p]);   // nanoseconds
     return PA_OK;
 }
 

@@ -172,6 +172,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 const int v = L::half(mv, q);
                 if constexpr (LOCAL) a.scores[tk.pair[q]] = v;
                 if (W == 16 && v >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
+                if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
             }
         }
     }
@@ -362,6 +363,7 @@ __global__ __launch_bounds__(512) void wavefront_kernel(KernelArgs a)
                 const int v = L::half(mv, q);
                 if constexpr (LOCAL) atomicMax(a.scores + tk.pair[q], v);
                 if (W == 16 && v >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
+                if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
             }
         }
     }

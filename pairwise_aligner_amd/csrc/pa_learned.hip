@@ -1,0 +1,276 @@
+// pa_learned.hip -- learned lane dispatch: per-pair features and a 3-layer FP32 MLP with its own
+// forward, backward and Adam kernels (no library GEMM: the network is 8 -> 64 -> 64 -> 2 and one
+// thread carries one sample through it, with the weights staged in LDS).
+//
+// Features of pair (s1, s2) with lengths (m, n), computed by one wave per pair:
+//   0 log2(m + 1) / 16           1 log2(n + 1) / 16          2 min(m, n) / max(m, n)
+//   3 hashed 3-mer composition distance: L1 distance of the normalised 64-bin histograms / 2
+//   4 k-mer containment: fraction of the k-mers of s2 found in a 32768-bit set of s1's k-mers
+//     (k = 8 over 2-bit codes for alphabets of at most four symbols, else k = 3 over 5-bit codes)
+//   5 identity estimate containment^(1/k)
+//   6 max substitution score / 16                        7 1
+// Outputs: y0 = max of H over the DP / (max substitution score * min(m, n)), y1 = log2(cost)/32.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pa_learned.hpp"
+
+namespace pa {
+
+constexpr int F = kModelFeatures, HID = kModelHidden, OUT = kModelOutputs;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + F * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + HID * HID,
+              OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID * OUT, NPARAM = OFF_B3 + OUT;
+static_assert(NPARAM == kModelParams, "parameter layout");
+
+constexpr int kSetWords = 1024;   // 32768-bit k-mer set per wave
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void features_kernel(FeatureArgs a)
+{
+    __shared__ uint32_t comp[4][2][64];
+    __shared__ uint32_t kset[4][kSetWords];
+    __shared__ uint32_t hits[4];
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint64_t p = uint64_t(blockIdx.x) * 4 + w;
+    for (int i = lane; i < 128; i += 64) comp[w][i / 64][i % 64] = 0;
+    for (int i = lane; i < kSetWords; i += 64) kset[w][i] = 0;
+    if (lane == 0) hits[w] = 0;
+    __syncthreads();
+    const bool live = p < a.n_pairs;
+    const uint8_t *s1 = live ? a.seq1 + a.off1[p] : nullptr;
+    const uint8_t *s2 = live ? a.seq2 + a.off2[p] : nullptr;
+    const int m = live ? int(a.off1[p + 1] - a.off1[p]) : 0;
+    const int n = live ? int(a.off2[p + 1] - a.off2[p]) : 0;
+    const int k = a.k, bits = a.bits;
+    const uint32_t kmask = (1u << (k * bits)) - 1u;
+    const uint32_t smask = (1u << bits) - 1u;
+    // 3-mer histograms and the k-mer set of s1
+    for (int i = lane; i < m; i += 64) {
+        if (i + 2 < m) {
+            const uint32_t h = mix32((uint32_t(s1[i]) << 16) | (uint32_t(s1[i + 1]) << 8) | s1[i + 2]) >> 26;
+            atomicAdd(&comp[w][0][h], 1u);
+        }
+        if (i + k <= m) {
+            uint32_t v = 0;
+            for (int j = 0; j < k; ++j) v = (v << bits) | (uint32_t(a.code[s1[i + j]]) & smask);
+            const uint32_t h = mix32(v & kmask) & (kSetWords * 32 - 1);
+            atomicOr(&kset[w][h >> 5], 1u << (h & 31));
+        }
+    }
+    for (int i = lane; i + 2 < n; i += 64) {
+        const uint32_t h = mix32((uint32_t(s2[i]) << 16) | (uint32_t(s2[i + 1]) << 8) | s2[i + 2]) >> 26;
+        atomicAdd(&comp[w][1][h], 1u);
+    }
+    __syncthreads();
+    uint32_t local_hits = 0;
+    for (int i = lane; i + k <= n; i += 64) {
+        uint32_t v = 0;
+        for (int j = 0; j < k; ++j) v = (v << bits) | (uint32_t(a.code[s2[i + j]]) & smask);
+        const uint32_t h = mix32(v & kmask) & (kSetWords * 32 - 1);
+        local_hits += (kset[w][h >> 5] >> (h & 31)) & 1u;
+    }
+    atomicAdd(&hits[w], local_hits);
+    __syncthreads();
+    if (!live) return;
+    // L1 distance of the normalised histograms, reduced over the wave
+    const float t1 = m > 2 ? 1.0f / float(m - 2) : 0.0f, t2 = n > 2 ? 1.0f / float(n - 2) : 0.0f;
+    float d = fabsf(float(comp[w][0][lane]) * t1 - float(comp[w][1][lane]) * t2);
+    for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off);
+    if (lane == 0) {
+        float *f = a.out + p * F;
+        const float mn = float(m < n ? m : n), mx = float(m > n ? m : n);
+        f[0] = log2f(float(m) + 1.0f) / 16.0f;
+        f[1] = log2f(float(n) + 1.0f) / 16.0f;
+        f[2] = mx > 0 ? mn / mx : 0.0f;
+        f[3] = (m > 2 && n > 2) ? d * 0.5f : 1.0f;
+        const float cont = (m >= k && n >= k) ? float(hits[w]) / float(n - k + 1) : 0.0f;
+        f[4] = cont;
+        f[5] = cont > 0 ? powf(cont, 1.0f / float(k)) : 0.0f;
+        f[6] = a.smax / 16.0f;
+        f[7] = 1.0f;
+    }
+}
+
+// One thread carries one sample through the network; parameters staged in LDS.
+__device__ __forceinline__ void load_params(float *sp, const float *params)
+{
+    for (int i = threadIdx.x; i < NPARAM; i += blockDim.x) sp[i] = params[i];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void forward_kernel(const float *params, const float *x, uint64_t n, float *y)
+{
+    __shared__ float sp[NPARAM];
+    load_params(sp, params);
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float in[F], h1[HID], h2[HID];
+#pragma unroll
+    for (int f = 0; f < F; ++f) in[f] = x[i * F + f];
+#pragma unroll
+    for (int h = 0; h < HID; ++h) {
+        float acc = sp[OFF_B1 + h];
+#pragma unroll
+        for (int f = 0; f < F; ++f) acc += in[f] * sp[OFF_W1 + f * HID + h];
+        h1[h] = acc > 0.0f ? acc : 0.0f;
+    }
+#pragma unroll
+    for (int h = 0; h < HID; ++h) {
+        float acc = sp[OFF_B2 + h];
+#pragma unroll 8
+        for (int g = 0; g < HID; ++g) acc += h1[g] * sp[OFF_W2 + g * HID + h];
+        h2[h] = acc > 0.0f ? acc : 0.0f;
+    }
+#pragma unroll
+    for (int o = 0; o < OUT; ++o) {
+        float acc = sp[OFF_B3 + o];
+#pragma unroll
+        for (int h = 0; h < HID; ++h) acc += h2[h] * sp[OFF_W3 + h * OUT + o];
+        y[i * OUT + o] = acc;
+    }
+}
+
+// Mean squared error (weighted per output) and its gradient.  Gradients of one block are summed
+// in LDS, then added to the global gradient once per block.
+__global__ __launch_bounds__(256) void loss_grad_kernel(const float *params, const float *x, const float *target,
+                                                        const float *weight, uint64_t n, float inv_n, float *grad,
+                                                        float *loss)
+{
+    __shared__ float sp[NPARAM];
+    __shared__ float sg[NPARAM];
+    __shared__ float sloss;
+    for (int i = threadIdx.x; i < NPARAM; i += blockDim.x) sg[i] = 0.0f;
+    if (threadIdx.x == 0) sloss = 0.0f;
+    load_params(sp, params);
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float in[F], h1[HID], h2[HID], dy[OUT], dh2[HID];
+#pragma unroll
+        for (int f = 0; f < F; ++f) in[f] = x[i * F + f];
+#pragma unroll
+        for (int h = 0; h < HID; ++h) {
+            float acc = sp[OFF_B1 + h];
+#pragma unroll
+            for (int f = 0; f < F; ++f) acc += in[f] * sp[OFF_W1 + f * HID + h];
+            h1[h] = acc > 0.0f ? acc : 0.0f;
+        }
+#pragma unroll
+        for (int h = 0; h < HID; ++h) {
+            float acc = sp[OFF_B2 + h];
+#pragma unroll 8
+            for (int g = 0; g < HID; ++g) acc += h1[g] * sp[OFF_W2 + g * HID + h];
+            h2[h] = acc > 0.0f ? acc : 0.0f;
+        }
+        float l = 0.0f;
+#pragma unroll
+        for (int o = 0; o < OUT; ++o) {
+            float acc = sp[OFF_B3 + o];
+#pragma unroll
+            for (int h = 0; h < HID; ++h) acc += h2[h] * sp[OFF_W3 + h * OUT + o];
+            const float wt = weight ? weight[i * OUT + o] : 1.0f;
+            const float e = acc - target[i * OUT + o];
+            l += wt * e * e;
+            dy[o] = 2.0f * wt * e * inv_n;
+        }
+        atomicAdd(&sloss, l * inv_n);
+        // layer 3
+#pragma unroll
+        for (int o = 0; o < OUT; ++o) atomicAdd(&sg[OFF_B3 + o], dy[o]);
+#pragma unroll
+        for (int h = 0; h < HID; ++h) {
+            float g = 0.0f;
+#pragma unroll
+            for (int o = 0; o < OUT; ++o) {
+                atomicAdd(&sg[OFF_W3 + h * OUT + o], h2[h] * dy[o]);
+                g += sp[OFF_W3 + h * OUT + o] * dy[o];
+            }
+            dh2[h] = h2[h] > 0.0f ? g : 0.0f;
+        }
+        // layer 2
+#pragma unroll
+        for (int h = 0; h < HID; ++h) atomicAdd(&sg[OFF_B2 + h], dh2[h]);
+        float dh1[HID];
+#pragma unroll
+        for (int g = 0; g < HID; ++g) {
+            float acc = 0.0f;
+#pragma unroll 8
+            for (int h = 0; h < HID; ++h) {
+                atomicAdd(&sg[OFF_W2 + g * HID + h], h1[g] * dh2[h]);
+                acc += sp[OFF_W2 + g * HID + h] * dh2[h];
+            }
+            dh1[g] = h1[g] > 0.0f ? acc : 0.0f;
+        }
+        // layer 1
+#pragma unroll
+        for (int h = 0; h < HID; ++h) {
+            atomicAdd(&sg[OFF_B1 + h], dh1[h]);
+#pragma unroll
+            for (int f = 0; f < F; ++f) atomicAdd(&sg[OFF_W1 + f * HID + h], in[f] * dh1[h]);
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < NPARAM; j += blockDim.x)
+        if (sg[j] != 0.0f) atomicAdd(&grad[j], sg[j]);
+    if (threadIdx.x == 0) atomicAdd(loss, sloss);
+}
+
+__global__ void fill_kernel(float *p, uint32_t n, float v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void adam_kernel(float *params, const float *grad, float *m1, float *m2, uint32_t n, float lr, float b1,
+                            float b2, float eps, float bc1, float bc2)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = grad[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * g;
+    const float b = b2 * m2[i] + (1.0f - b2) * g * g;
+    m1[i] = a;
+    m2[i] = b;
+    params[i] -= lr * (a / bc1) / (sqrtf(b / bc2) + eps);
+}
+
+hipError_t launch_features(const FeatureArgs &a, hipStream_t st)
+{
+    if (a.n_pairs == 0) return hipSuccess;
+    hipLaunchKernelGGL(features_kernel, dim3(uint32_t((a.n_pairs + 3) / 4)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_forward(const float *params, const float *x, uint64_t n, float *y, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(forward_kernel, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, params, x, n, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_loss_grad(const float *params, const float *x, const float *target, const float *weight, uint64_t n,
+                            float *grad, float *loss, hipStream_t st)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, grad, uint32_t(NPARAM), 0.0f);
+    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(32), 0, st, loss, 1u, 0.0f);
+    if (n == 0) return hipGetLastError();
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, params, x, target, weight,
+                       n, 1.0f / float(n), grad, loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float *params, const float *grad, float *m1, float *m2, float lr, int step, hipStream_t st)
+{
+    const float b1 = 0.9f, b2 = 0.999f;
+    float bc1 = 1.0f, bc2 = 1.0f;
+    for (int i = 0; i < step; ++i) { bc1 *= b1; bc2 *= b2; }
+    hipLaunchKernelGGL(adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, params, grad, m1, m2,
+                       uint32_t(NPARAM), lr, b1, b2, 1e-8f, 1.0f - bc1, 1.0f - bc2);
+    return hipGetLastError();
+}
+
+} // namespace pa

@@ -151,8 +151,22 @@ struct Item {
 
 } // namespace
 
+double task_cost(const VariantKey &key, const Task &t)
+{
+    return double(t.S) * (double(t.N) + key.G - 1) * (kCellCost * key.R + kStepCost) * (key.G == 64 ? 4.0 : 1.0);
+}
+
+int32_t max_substitution(const pa_config &cfg)
+{
+    if (cfg.score_kind == PA_SCORE_UNITARY) return std::max(cfg.match, cfg.mismatch);
+    int32_t m = INT32_MIN;
+    if (cfg.matrix)
+        for (int32_t i = 0; i < cfg.matrix_dim * cfg.matrix_dim; ++i) m = std::max(m, cfg.matrix[i]);
+    return m;
+}
+
 pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *forced_width,
-                    const std::vector<uint32_t> *only, Plan &plan, std::string &err)
+                    const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all)
 {
     if (cfg.method != PA_METHOD_GLOBAL && cfg.method != PA_METHOD_LOCAL) { err = "invalid method"; return PA_ERR_INVALID_ARGUMENT; }
     if (cfg.gap_extension > 0 || int64_t(cfg.gap_open) + cfg.gap_extension > 0) {
@@ -213,7 +227,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         choose_shape(pm.m, pm.n, width, key.G, key.R);
         key.score = plan.score;
         key.local = local;
-        key.track = (width == 16 && !local && !upper16) ? 1 : 0;
+        key.track = (!local && ((width == 16 && !upper16) || track_all)) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;
         const uint32_t rows = uint32_t(key.G) * key.R;
         groups[key].push_back(Item{uint32_t(p), pm.m, pm.n, (pm.m + rows - 1) / rows});

@@ -126,8 +126,16 @@ struct BatchView {
 
 // Widths chosen per pair before packing: 16 or 32 (0 = let the policy decide).
 // `only` restricts planning to a subset of pairs (nullptr = all).
+// `track_all` keeps the running maximum of H in every global variant (labels for the learned
+// range predictor).
 pa_status make_plan(const pa_config &cfg, const BatchView &batch, const uint8_t *forced_width,
-                    const std::vector<uint32_t> *only, Plan &plan, std::string &err);
+                    const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all = false);
+
+// Modelled sweep cost of one task (arbitrary units): stripes x steps x per-step work.
+double task_cost(const VariantKey &key, const Task &t);
+
+// Largest substitution score of a configuration (int16 range analysis and model features).
+int32_t max_substitution(const pa_config &cfg);
 
 // Number of colcodes / bnd words reserved per task: the kernels prefetch up to three
 // conveyor chunks of G words past the last column.

@@ -27,7 +27,10 @@ PAIR_INT16, PAIR_INT32, PAIR_RECOMPUTED, PAIR_EMPTY = 0x01, 0x02, 0x04, 0x08
 # Every symbol include/pa_device.h declares.
 EXPORTS = ["pa_abi_version", "pa_ctx_create", "pa_ctx_destroy", "pa_ctx_last_error", "pa_score_batch",
            "pa_batch_create", "pa_batch_run", "pa_batch_results", "pa_batch_stats_get", "pa_batch_time",
-           "pa_batch_destroy"]
+           "pa_batch_destroy", "pa_model_create", "pa_model_destroy", "pa_model_param_count",
+           "pa_model_set_params", "pa_model_get_params", "pa_features", "pa_model_forward", "pa_model_loss_grad",
+           "pa_model_train_step", "pa_dp_labels"]
+N_FEATURES, N_OUTPUTS, N_HIDDEN = 8, 2, 64
 
 
 class PaError(RuntimeError):
@@ -52,6 +55,7 @@ class CConfig(ctypes.Structure):
         ("matrix", ctypes.POINTER(ctypes.c_int32)),
         ("matrix_symbols", ctypes.POINTER(ctypes.c_uint8)),
         ("lane_policy", ctypes.c_int32),
+        ("model", ctypes.c_void_p),
     ]
 
 
@@ -77,6 +81,7 @@ class DeviceConfig:
     symbols: bytes = b""
     matrix: Optional[tuple] = None
     lane_policy: int = LANES_AUTO
+    model: Optional["Model"] = None     # PA_LANES_LEARNED
 
 
 @functools.lru_cache(maxsize=1)
@@ -97,8 +102,20 @@ def lib() -> ctypes.CDLL:
     L.pa_batch_stats_get.argtypes = [vp, ctypes.POINTER(CStats)]
     L.pa_batch_time.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.pa_batch_destroy.argtypes = [vp, vp]
+    L.pa_model_create.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.pa_model_destroy.argtypes = [vp]
+    L.pa_model_param_count.restype = i32
+    L.pa_model_set_params.argtypes = [vp, vp, vp]
+    L.pa_model_get_params.argtypes = [vp, vp, vp]
+    L.pa_features.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp]
+    L.pa_model_forward.argtypes = [vp, vp, vp, u64, vp]
+    L.pa_model_loss_grad.argtypes = [vp, vp, vp, vp, vp, u64, vp, vp]
+    L.pa_model_train_step.argtypes = [vp, vp, vp, vp, vp, u64, ctypes.c_float, vp]
+    L.pa_dp_labels.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp]
     for name in ["pa_ctx_create", "pa_ctx_destroy", "pa_score_batch", "pa_batch_create", "pa_batch_run",
-                 "pa_batch_results", "pa_batch_stats_get", "pa_batch_time", "pa_batch_destroy"]:
+                 "pa_batch_results", "pa_batch_stats_get", "pa_batch_time", "pa_batch_destroy", "pa_model_create",
+                 "pa_model_destroy", "pa_model_set_params", "pa_model_get_params", "pa_features",
+                 "pa_model_forward", "pa_model_loss_grad", "pa_model_train_step", "pa_dp_labels"]:
         getattr(L, name).restype = i32
     return L
 
@@ -118,7 +135,8 @@ class _CConfigHolder:
     def __init__(self, cfg: DeviceConfig):
         self.c = CConfig(cfg.method, int(cfg.first_column_free), int(cfg.first_row_free),
                          int(cfg.last_column_free), int(cfg.last_row_free), cfg.gap_open, cfg.gap_extension,
-                         cfg.score_kind, cfg.match, cfg.mismatch, 0, None, None, cfg.lane_policy)
+                         cfg.score_kind, cfg.match, cfg.mismatch, 0, None, None, cfg.lane_policy,
+                         cfg.model.handle if cfg.model is not None else None)
         if cfg.score_kind == SCORE_MATRIX:
             self.mat = (ctypes.c_int32 * len(cfg.matrix))(*cfg.matrix)
             self.sym = (ctypes.c_uint8 * len(cfg.symbols))(*cfg.symbols)
@@ -221,6 +239,86 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Model:
+    """The learned range/cost predictor (pa_model): a 3-layer FP32 MLP living on the GPU."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.pa_model_create(ctx.handle, seed, ctypes.byref(h)))
+        self.handle = h
+        self.n_params = ctx._lib.pa_model_param_count()
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self.ctx.check(self.ctx._lib.pa_model_get_params(self.ctx.handle, self.handle, p.ctypes.data))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self.ctx.check(self.ctx._lib.pa_model_set_params(self.ctx.handle, self.handle, p.ctypes.data))
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((len(x), N_OUTPUTS), dtype=np.float32)
+        self.ctx.check(self.ctx._lib.pa_model_forward(self.ctx.handle, self.handle, x.ctypes.data, len(x),
+                                                      y.ctypes.data))
+        return y
+
+    def loss_grad(self, x, target, weight=None):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        grad = np.zeros(self.n_params, dtype=np.float32)
+        self.ctx.check(self.ctx._lib.pa_model_loss_grad(self.ctx.handle, self.handle, x.ctypes.data, t.ctypes.data,
+                                                        None if w is None else w.ctypes.data, len(x),
+                                                        loss.ctypes.data, grad.ctypes.data))
+        return float(loss[0]), grad
+
+    def train_step(self, x, target, weight=None, lr: float = 1e-3) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self.ctx.check(self.ctx._lib.pa_model_train_step(self.ctx.handle, self.handle, x.ctypes.data, t.ctypes.data,
+                                                         None if w is None else w.ctypes.data, len(x),
+                                                         ctypes.c_float(lr), loss.ctypes.data))
+        return float(loss[0])
+
+    def close(self):
+        if self.handle:
+            self.ctx._lib.pa_model_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def features(ctx: Context, cfg: DeviceConfig, b1, o1, b2, o2) -> np.ndarray:
+    holder = _CConfigHolder(cfg)
+    n = len(o1) - 1
+    out = np.zeros((max(n, 1), N_FEATURES), dtype=np.float32)
+    ctx.check(ctx._lib.pa_features(ctx.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
+                                   b2.ctypes.data, o2.ctypes.data, n, out.ctypes.data))
+    return out[:n]
+
+
+def dp_labels(ctx: Context, cfg: DeviceConfig, b1, o1, b2, o2):
+    """(max of H over each pair's DP matrix, modelled+measured cost in ns) from an int32 DP pass."""
+    holder = _CConfigHolder(cfg)
+    n = len(o1) - 1
+    max_h = np.zeros(max(n, 1), dtype=np.int32)
+    cost = np.zeros(max(n, 1), dtype=np.float32)
+    ctx.check(ctx._lib.pa_dp_labels(ctx.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
+                                    b2.ctypes.data, o2.ctypes.data, n, max_h.ctypes.data, cost.ctypes.data))
+    return max_h[:n], cost[:n]
 
 
 @functools.lru_cache(maxsize=None)

@@ -1,0 +1,67 @@
+"""Training driver of the learned lane dispatch.
+
+The DP itself produces the labels (pa_dp_labels: an int32 pass that keeps the maximum of H over
+every pair's matrix, and the launch timings shared out over the pairs by modelled cost); the
+features and the network's forward, backward and Adam steps run on the GPU (pa_features,
+pa_model_*).  The trained network predicts, per pair,
+    y0 = max of H / (max substitution score * min(m, n))    -> lane width (int16 when it fits)
+    y1 = log2(cost in ns + 1) / 32                           -> cost estimate
+"""
+from __future__ import annotations
+
+import time
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from . import device, workloads
+
+
+def targets(dc: device.DeviceConfig, o1, o2, max_h, cost, smax: int) -> np.ndarray:
+    mn = np.minimum(np.diff(o1), np.diff(o2)).astype(np.float64)
+    y0 = np.where(mn > 0, max_h.astype(np.float64) / (max(smax, 1) * np.maximum(mn, 1)), 0.0)
+    y1 = np.log2(cost.astype(np.float64) + 1.0) / 32.0
+    return np.stack([y0, y1], axis=1).astype(np.float32)
+
+
+def max_substitution(dc: device.DeviceConfig) -> int:
+    if dc.score_kind == device.SCORE_UNITARY:
+        return max(dc.match, dc.mismatch)
+    return max(dc.matrix)
+
+
+def training_pairs(kind: str, seed: int, count: int, lo: int = 100, hi: int = 5000):
+    """Pairs like the mixed workload (identity uniform 50-100%), from an independent seed."""
+    rng = np.random.default_rng(seed)
+    alpha, p = (workloads.DNA, None) if kind == "dna" else (workloads.AMINO, workloads.AMINO_FREQ)
+    b1, o1 = workloads.random_packed(rng, rng.integers(lo, hi + 1, size=count), alpha, p)
+    div = 1.0 - rng.uniform(0.5, 1.0, size=count)
+    b2, o2 = workloads.mutate_packed(rng, b1, o1, div, 0.1, 0.7, alpha, p)
+    return b1, o1, b2, o2
+
+
+def build_dataset(ctx: device.Context, sets: Sequence[Tuple[device.DeviceConfig, tuple]]):
+    xs, ys = [], []
+    for dc, (b1, o1, b2, o2) in sets:
+        max_h, cost = device.dp_labels(ctx, dc, b1, o1, b2, o2)
+        x = device.features(ctx, dc, b1, o1, b2, o2)
+        xs.append(x)
+        ys.append(targets(dc, o1, o2, max_h, cost, max_substitution(dc)))
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def train(ctx: device.Context, x: np.ndarray, y: np.ndarray, steps: int = 1500, batch: int = 2048,
+          lr: float = 2e-3, seed: int = 0, log=None) -> Tuple[device.Model, List[float]]:
+    """Adam on minibatches; the range output carries 4x the weight of the cost output."""
+    model = device.Model(ctx, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    w = np.tile(np.array([[4.0, 1.0]], dtype=np.float32), (batch, 1))
+    losses = []
+    t0 = time.perf_counter()
+    for step in range(steps):
+        idx = rng.integers(0, len(x), size=batch)
+        rate = lr * (0.1 ** (step / steps))
+        losses.append(model.train_step(x[idx], y[idx], w, lr=rate))
+        if log and (step % 250 == 0 or step == steps - 1):
+            log(f"[learned] step {step}: loss {losses[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+    return model, losses

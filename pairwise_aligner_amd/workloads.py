@@ -56,7 +56,11 @@ def digest(*arrays) -> str:
 def random_residues(rng, alphabet: np.ndarray, n: int, p=None) -> np.ndarray:
     if p is None:
         return alphabet[rng.integers(0, len(alphabet), size=n)]
-    return alphabet[rng.choice(len(alphabet), size=n, p=p)]
+    # inverse CDF through a 65536-entry table (frequencies exact to 1/65536)
+    cdf = np.cumsum(p)
+    cdf[-1] = 1.0
+    table = np.searchsorted(cdf, (np.arange(65536) + 0.5) / 65536.0, side="right").clip(0, len(alphabet) - 1)
+    return alphabet[table[rng.integers(0, 65536, size=n, dtype=np.uint32)]]
 
 
 def mutate_packed(rng, buf: np.ndarray, off: np.ndarray, event_rate, indel_fraction: float, geom_p: float,
@@ -67,41 +71,40 @@ def mutate_packed(rng, buf: np.ndarray, off: np.ndarray, event_rate, indel_fract
     different residue.  Returns the packed mutated batch."""
     n_seq = len(off) - 1
     total = int(off[-1])
-    seq_id = np.repeat(np.arange(n_seq), np.diff(off).astype(np.int64))
-    rate = np.broadcast_to(np.asarray(event_rate, dtype=np.float64), (n_seq,))[seq_id] if total else np.zeros(0)
-    ev = rng.random(total) < rate
-    kind = rng.random(total)
-    is_indel = ev & (kind < indel_fraction)
-    is_sub = ev & ~is_indel
-    is_ins = is_indel & (rng.random(total) < 0.5)
-    is_del = is_indel & ~is_ins
-    lengths = rng.geometric(geom_p, size=total)
+    lens = np.diff(off).astype(np.int64)
+    rate = np.asarray(event_rate, dtype=np.float64)
+    u = rng.random(total)
+    r = np.repeat(np.broadcast_to(rate, (n_seq,)), lens) if rate.ndim else rate
+    is_sub = (u < r) & (u >= r * indel_fraction)
+    indel_pos = np.nonzero(u < r * indel_fraction)[0]
+    del u, r
+    ins_flag = rng.random(len(indel_pos)) < 0.5
+    lengths = rng.geometric(geom_p, size=len(indel_pos))
 
     out = buf[:total].copy()
     # substitutions: shift by 1..k-1 within the alphabet (always a different residue)
-    if is_sub.any():
-        idx = np.nonzero(is_sub)[0]
+    idx = np.nonzero(is_sub)[0]
+    del is_sub
+    if len(idx):
         lut = np.full(256, 0, dtype=np.int64)
         lut[alphabet] = np.arange(len(alphabet))
-        ranks = lut[out[idx]]
         shift = rng.integers(1, len(alphabet), size=len(idx))
-        out[idx] = alphabet[(ranks + shift) % len(alphabet)]
+        out[idx] = alphabet[(lut[out[idx]] + shift) % len(alphabet)]
     # deletions: drop `length` positions starting here, never across a sequence end
     keep = np.ones(total, dtype=bool)
-    if is_del.any():
-        starts = np.nonzero(is_del)[0]
-        L = lengths[starts]
-        ends = np.minimum(starts + L, off[seq_id[starts] + 1].astype(np.int64))
-        cover = np.zeros(total + 1, dtype=np.int64)
-        np.add.at(cover, starts, 1)
-        np.add.at(cover, ends, -1)
+    starts = indel_pos[~ins_flag]
+    if len(starts):
+        seq_end = np.repeat(off[1:].astype(np.int64), lens)
+        ends = np.minimum(starts + lengths[~ins_flag], seq_end[starts])
+        del seq_end
+        cover = np.bincount(starts, minlength=total + 1) - np.bincount(ends, minlength=total + 1)
         keep = np.cumsum(cover[:-1]) == 0
-    ins_len = np.where(is_ins, lengths, 0)
-    counts = keep.astype(np.int64) + ins_len
+    counts = keep.astype(np.int64)
+    counts[indel_pos[ins_flag]] += lengths[ins_flag]
     new_total = int(counts.sum())
     res = np.empty(new_total, dtype=np.uint8)
     # position of each source position's first output slot
-    starts_out = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    starts_out = np.cumsum(counts) - counts
     kept = np.nonzero(keep)[0]
     res[starts_out[kept]] = out[kept]
     ins_mask = np.ones(new_total, dtype=bool)
@@ -110,9 +113,8 @@ def mutate_packed(rng, buf: np.ndarray, off: np.ndarray, event_rate, indel_fract
     if n_ins:
         res[ins_mask] = random_residues(rng, alphabet, n_ins, p)
     # new offsets
-    per_seq = np.bincount(seq_id, weights=counts, minlength=n_seq).astype(np.uint64) if total else np.zeros(n_seq, np.uint64)
-    new_off = np.zeros(n_seq + 1, dtype=np.uint64)
-    np.cumsum(per_seq, out=new_off[1:])
+    csum = np.concatenate([[0], np.cumsum(counts)])
+    new_off = csum[off.astype(np.int64)].astype(np.uint64)
     return res, new_off
 
 
@@ -181,17 +183,38 @@ def config_long(seed: int = 4, count: int = 2_000):
                 scoring=dict(kind="unitary", match=1, mismatch=-4), gap=(-6, -1), method="global")
 
 
-def config_mixed(seed: int = 5, count: int = 500_000):
-    """Half DNA, half protein pairs; identity uniform 50-100% (divergence 0-50% of positions,
-    90% of events substitutions).  Returns the DNA and protein halves separately because they
-    use different scoring."""
+def config_mixed(seed: int = 5, count: int = 500_000, chunk: int = 25_000):
+    """Half DNA, half protein pairs; lengths uniform 100-5,000; identity uniform 50-100%
+    (divergence 0-50% of positions, 90% of events substitutions).  Returns the DNA and protein
+    halves separately because they use different scoring.  Generated in chunks of `chunk` pairs
+    to bound the temporaries."""
     rng = np.random.default_rng(seed)
     half = count // 2
     out = {}
     for kind, alpha, p, n in (("dna", DNA, None, half), ("protein", AMINO, AMINO_FREQ, count - half)):
-        b1, o1 = random_packed(rng, rng.integers(100, 5_001, size=n), alpha, p)
-        div = 1.0 - rng.uniform(0.5, 1.0, size=n)
-        b2, o2 = mutate_packed(rng, b1, o1, div, 0.1, 0.7, alpha, p)
-        out[kind] = dict(b1=b1, o1=o1, b2=b2, o2=o2)
+        parts = []
+        for start in range(0, n, chunk):
+            k = min(chunk, n - start)
+            b1, o1 = random_packed(rng, rng.integers(100, 5_001, size=k), alpha, p)
+            div = 1.0 - rng.uniform(0.5, 1.0, size=k)
+            b2, o2 = mutate_packed(rng, b1, o1, div, 0.1, 0.7, alpha, p)
+            parts.append((b1[:int(o1[-1])], o1, b2[:int(o2[-1])], o2))
+        out[kind] = dict(zip(("b1", "o1", "b2", "o2"), concat_packed(parts)))
     out["name"] = "mixed"
     return out
+
+
+def concat_packed(parts):
+    """Concatenates packed chunks [(b1, o1, b2, o2), ...] into one packed batch."""
+    res = []
+    for side in (0, 2):
+        bufs = [p[side] for p in parts]
+        offs = [p[side + 1] for p in parts]
+        total = 0
+        cat_off = [np.zeros(1, dtype=np.uint64)]
+        for o in offs:
+            cat_off.append(o[1:] + np.uint64(total))
+            total += int(o[-1])
+        buf = np.concatenate(bufs) if total else np.zeros(1, dtype=np.uint8)
+        res += [buf, np.concatenate(cat_off)]
+    return res

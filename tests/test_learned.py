@@ -1,0 +1,109 @@
+"""Learned lane dispatch: the MLP's GPU forward/backward kernels against the numpy oracle (itself
+pinned by finite differences), the GPU features, and what the trained network gains over the
+bound-based policy on the same pairs with zero score differences."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+from oracle import oracle as O
+from pairwise_aligner_amd import device, learned, tables, workloads
+
+
+def _random_problem(seed, n=96):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.3, size=M.F * M.H + M.H + M.H * M.H + M.H + M.H * M.O + M.O)
+    x = rng.uniform(0, 1, size=(n, M.F))
+    t = rng.uniform(0, 1, size=(n, M.O))
+    w = rng.uniform(0.5, 2.0, size=(n, M.O))
+    return p, x, t, w
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    p, x, t, w = _random_problem(0)
+    _, g = M.loss_grad(p, x, t, w)
+    rng = np.random.default_rng(1)
+    idx = np.concatenate([rng.choice(len(p), 60, replace=False), [len(p) - 1, len(p) - 2, 0]])
+    fd = M.finite_difference(p, x, t, idx, w)
+    assert np.allclose(g[idx], fd, rtol=1e-5, atol=1e-8)
+
+
+def test_mlp_oracle_planted_mistake_is_caught():
+    """Dropping the ReLU derivative of layer 1 must be visible against finite differences."""
+    p, x, t, w = _random_problem(2)
+    W1, b1, W2, b2, W3, b3 = M.unpack(p)
+    y, (xx, z1, h1, z2, h2) = M.forward(p, x)
+    dy = 2.0 * w * (y - t) / len(x)
+    dh2 = (dy @ W3.T) * (z2 > 0)
+    wrong_gb1 = (dh2 @ W2.T).sum(0)                       # missing * (z1 > 0)
+    off = M.F * M.H
+    fd = M.finite_difference(p, x, t, list(range(off, off + M.H)), w)
+    assert not np.allclose(wrong_gb1, fd, rtol=1e-3, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_gpu_forward_and_gradient_match_oracle():
+    ctx = device.default_context(0)
+    model = device.Model(ctx, seed=3)
+    p64 = model.get_params().astype(np.float64)
+    p, x, t, w = _random_problem(4, n=1000)
+    x = x.astype(np.float32)
+    y_gpu = model.forward(x)
+    y_ref, _ = M.forward(p64, x)
+    assert np.allclose(y_gpu, y_ref, rtol=1e-4, atol=1e-5)
+    loss_gpu, g_gpu = model.loss_grad(x, t, w)
+    loss_ref, g_ref = M.loss_grad(p64, x.astype(np.float64), t, w)
+    assert abs(loss_gpu - loss_ref) <= 1e-4 * abs(loss_ref) + 1e-6
+    scale = np.max(np.abs(g_ref))
+    assert np.max(np.abs(g_gpu - g_ref)) <= 1e-4 * scale, np.max(np.abs(g_gpu - g_ref)) / scale
+
+
+@pytest.mark.gpu
+def test_gpu_features():
+    ctx = device.default_context(0)
+    rng = np.random.default_rng(5)
+    s1, _ = workloads.dna_pairs(rng, 4, 300, 600)
+    s2 = [s1[0], s1[1][::-1], s1[2][:150], b"A" * 200]
+    b1, o1 = workloads.pack(s1)
+    b2, o2 = workloads.pack(s2)
+    f = device.features(ctx, device.DeviceConfig(match=2, mismatch=-3), b1, o1, b2, o2)
+    assert f.shape == (4, device.N_FEATURES)
+    assert abs(f[0, 4] - 1.0) < 1e-6 and abs(f[0, 5] - 1.0) < 1e-6 and f[0, 3] < 1e-6   # identical
+    assert f[2, 4] > 0.99                          # a prefix is contained
+    assert f[3, 4] < 0.2                           # poly-A against random DNA
+    assert np.allclose(f[:, 0], np.log2(np.diff(o1) + 1) / 16, rtol=1e-6)
+    assert np.all(f[:, 7] == 1.0) and np.allclose(f[:, 6], 3 / 16)
+
+
+@pytest.mark.gpu
+def test_learned_dispatch_gains_over_bound_policy():
+    """Train on DP-generated labels, then score held-out protein pairs whose length bound does
+    not prove int16 safety: the learned policy runs more of them in int16 than the bound policy,
+    and every policy returns the same scores (all equal to the oracle on a sample)."""
+    ctx = device.default_context(0)
+    t = tables.blosum62_standard()
+    prot = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
+                               symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
+                               gap_open=-11, gap_extension=-1)
+    dna = device.DeviceConfig(method=device.METHOD_LOCAL, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
+    sets = [(prot, learned.training_pairs("protein", 100, 3000, 100, 4000)),
+            (dna, learned.training_pairs("dna", 101, 1500, 100, 4000))]
+    x, y = learned.build_dataset(ctx, sets)
+    model, losses = learned.train(ctx, x, y, steps=800, batch=1024)
+    assert np.mean(losses[-50:]) < 0.25 * np.mean(losses[:10])
+
+    b1, o1, b2, o2 = learned.training_pairs("protein", 202, 600, 3000, 5000)   # held out
+    results = {}
+    for name, policy, mdl in (("int32", device.LANES_INT32, None), ("auto", device.LANES_AUTO, None),
+                              ("learned", device.LANES_LEARNED, model)):
+        dc = device.DeviceConfig(**{**prot.__dict__, "lane_policy": policy, "model": mdl})
+        bt = device.Batch(ctx, b1, o1, b2, o2, dc)
+        bt.run()
+        results[name] = bt.results() + (bt.stats(),)
+    s32, _, st32 = results["int32"]
+    sa, _, sta = results["auto"]
+    sl, fl, stl = results["learned"]
+    assert np.array_equal(s32, sa) and np.array_equal(s32, sl)
+    assert stl["n_int16"] > sta["n_int16"], (stl, sta)
+    want = O.score_packed(b1, o1[:41], b2, o2[:41], O.OracleConfig(mode=O.LOCAL, score_kind=O.MATRIX,
+                          symbols=prot.symbols, matrix=prot.matrix, gap_open=-11, gap_extension=-1), threads=8)
+    assert np.array_equal(s32[:40].astype(np.int64), want)
