@@ -71,7 +71,7 @@ def test_gpu_features():
     assert f[2, 4] > 0.99                          # a prefix is contained
     assert f[3, 4] < 0.2                           # poly-A against random DNA
     assert np.allclose(f[:, 0], np.log2(np.diff(o1) + 1) / 16, rtol=1e-6)
-    assert np.all(f[:, 7] == 1.0) and np.allclose(f[:, 6], 3 / 16)
+    assert np.all(f[:, 7] == 1.0) and np.allclose(f[:, 6], 2 / 16)
 
 
 @pytest.mark.gpu
