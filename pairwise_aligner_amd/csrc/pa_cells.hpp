@@ -108,42 +108,105 @@ __device__ __forceinline__ void init_rows(Rows<W, R, ROWMAX> &st, const uint32_t
     }
 }
 
-// One step of one lane: the R cells of DP column j for this lane's rows.  `Eup` enters as the
-// vertical gap state from the row above and leaves as the one below the last row; `diag` is
-// H of the row above at column j-1.
+// Substitution scores of one lane's R rows against column residue `Bin`, packed like H.
+//   DNA codes: v_perm_b32 over an 8-byte table selected by row code ^ column code.
+//   matrix: reads of the lane's bank-swizzled LDS table copy at row offset + column offset;
+//           split in two so the reads of step k+1 can be issued before the cells of step k.
+template <int W, int R, int SCORE>
+struct ScoreReads {
+    uint32_t lo[SCORE == kScoreMatrix ? R : 1];
+    uint32_t hi[SCORE == kScoreMatrix && W == 16 ? R : 1];
+};
+
+template <int W, int R, int SCORE, bool ROWMAX>
+__device__ __forceinline__ void issue_scores(const Rows<W, R, ROWMAX> &st, uint32_t Bin, const char *lds,
+                                             ScoreReads<W, R, SCORE> &rd)
+{
+    if constexpr (SCORE == kScoreMatrix) {
+        uint32_t Blo = Bin, Bhi = 0;
+        if constexpr (W == 16) { Blo = Bin & 0xFFFFu; Bhi = Bin >> 16; }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if constexpr (W == 16) {
+                rd.lo[r] = *reinterpret_cast<const unsigned short *>(lds + st.A[r] + Blo);
+                rd.hi[r] = *reinterpret_cast<const unsigned short *>(lds + st.Ahi[r] + Bhi);
+            } else {
+                rd.lo[r] = uint32_t(int(*reinterpret_cast<const short *>(lds + st.A[r] + Blo)));
+            }
+        }
+    }
+}
+
+template <int W, int R, int SCORE, bool ROWMAX>
+__device__ __forceinline__ void finish_scores(const Rows<W, R, ROWMAX> &st, uint32_t Bin, uint32_t tbl_lo,
+                                              uint32_t tbl_hi, const ScoreReads<W, R, SCORE> &rd,
+                                              typename Lane<W>::V (&sc)[R])
+{
+    using L = Lane<W>;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if constexpr (SCORE == kScoreDna) {
+            sc[r] = L::from(__builtin_amdgcn_perm(tbl_hi, tbl_lo, st.A[r] ^ Bin));
+        } else if constexpr (W == 16) {
+            sc[r] = L::from((rd.hi[r] << 16) | rd.lo[r]);
+        } else {
+            sc[r] = L::from(rd.lo[r]);
+        }
+    }
+}
+
+// The R cells of DP column j for one lane's rows, given their substitution scores.  `Eup`
+// enters as the vertical gap state from the row above and leaves as the one below the last
+// row; `diag` is H of the row above at column j-1.
+template <int W, int R, bool LOCAL, bool TRACK, bool ROWMAX>
+__device__ __forceinline__ void update_cells(Rows<W, R, ROWMAX> &st, typename Lane<W>::V &Eup,
+                                             typename Lane<W>::V diag, typename Lane<W>::V &runmax,
+                                             const typename Lane<W>::V (&sc)[R], typename Lane<W>::V vgogo,
+                                             typename Lane<W>::V vge)
+{
+    using L = Lane<W>;
+    using V = typename L::V;
+    // Diagonal terms of all R rows first: they read only the previous column's H, so every
+    // H[r] can then be overwritten in place (no register rotation from step to step).
+    V D[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        D[r] = L::add(r == 0 ? diag : st.H[r - 1], sc[r]);
+        if constexpr (LOCAL) D[r] = L::vmax(D[r], L::splat(0));
+    }
+    // With go <= 0 (so go+ge <= ge) the vertical gap state needs only max(D, W) of its own row:
+    //   V(i+1,j) = max(V + ge, H + go + ge) = max(V + ge, max(D, W) + go + ge)
+    // because the dropped term V + go + ge never exceeds V + ge.  The row-to-row dependency
+    // chain is then one add and one max per row; H and W are updated off the chain.
+    V DF[R], DFo[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        DF[r] = L::vmax(D[r], st.F[r]);
+        DFo[r] = L::add(DF[r], vgogo);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const V best = L::vmax(DF[r], Eup);
+        Eup = L::vmax(L::add(Eup, vge), DFo[r]);
+        st.H[r] = best;
+        if constexpr (LOCAL || TRACK) runmax = L::vmax(runmax, best);
+        if constexpr (ROWMAX) st.rowacc[r] = L::vmax(st.rowacc[r], best);
+        st.F[r] = L::vmax(L::add(st.F[r], vge), L::add(best, vgogo));
+    }
+}
+
+// Scores and cells of one step in one go (no read-ahead).
 template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
 __device__ __forceinline__ void cell_block(Rows<W, R, ROWMAX> &st, typename Lane<W>::V &Eup,
                                            typename Lane<W>::V diag, typename Lane<W>::V &runmax, uint32_t Bin,
                                            uint32_t tbl_lo, uint32_t tbl_hi, const char *lds,
                                            typename Lane<W>::V vgogo, typename Lane<W>::V vge)
 {
-    using L = Lane<W>;
-    using V = typename L::V;
-    uint32_t Blo = Bin, Bhi = 0;
-    if constexpr (SCORE == kScoreMatrix && W == 16) { Blo = Bin & 0xFFFFu; Bhi = Bin >> 16; }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        V sc;
-        if constexpr (SCORE == kScoreDna) {
-            sc = L::from(__builtin_amdgcn_perm(tbl_hi, tbl_lo, st.A[r] ^ Bin));
-        } else if constexpr (W == 16) {
-            const short s0 = *reinterpret_cast<const short *>(lds + st.A[r] + Blo);
-            const short s1 = *reinterpret_cast<const short *>(lds + st.Ahi[r] + Bhi);
-            sc = V{s0, s1};
-        } else {
-            sc = L::splat(int(*reinterpret_cast<const short *>(lds + st.A[r] + Blo)));
-        }
-        V best = L::add(diag, sc);
-        if constexpr (LOCAL) best = L::vmax(best, L::splat(0));
-        best = L::vmax(L::vmax(best, Eup), st.F[r]);
-        diag = st.H[r];
-        st.H[r] = best;
-        if constexpr (LOCAL || TRACK) runmax = L::vmax(runmax, best);
-        if constexpr (ROWMAX) st.rowacc[r] = L::vmax(st.rowacc[r], best);
-        const V open = L::add(best, vgogo);
-        Eup = L::vmax(L::add(Eup, vge), open);
-        st.F[r] = L::vmax(L::add(st.F[r], vge), open);
-    }
+    ScoreReads<W, R, SCORE> rd;
+    issue_scores<W, R, SCORE>(st, Bin, lds, rd);
+    typename Lane<W>::V sc[R];
+    finish_scores<W, R, SCORE>(st, Bin, tbl_lo, tbl_hi, rd, sc);
+    update_cells<W, R, LOCAL, TRACK, ROWMAX>(st, Eup, diag, runmax, sc, vgogo, vge);
 }
 
 // Score capture of one lane in one stripe (global method).  cap: this lane owns row m of half

@@ -96,63 +96,86 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         const uint32_t *colp = a.colcodes + tk.col_off;
         uint32_t *bndp = has_bnd_in || has_bnd_out ? a.bnd + 2ull * tk.bnd_off : nullptr;
 
-        // conveyors: current chunk and the next one
-        uint32_t convB = colp[t], nextB = colp[G + t];
+        // Column-code conveyor: chunk j holds the residues of columns jG+1 .. jG+G, one per
+        // lane, so at the start of step k lane 0 of `conv` holds the residue of column k+1;
+        // `nextB` is the chunk after it.  The residue of step k+1 is thus known one step early,
+        // and its substitution scores are read while the cells of step k are computed.
+        uint32_t Bin = colp[t];   // step 0: lane 0 computes column 0
+        uint32_t conv = colp[1 + t];
+        uint32_t nextB = colp[G + 1 + t];
+        // Top-boundary conveyor: at step k lane 0 of convH/convE holds column k.
         uint32_t convH = 0, convE = 0, nextH = 0, nextE = 0;
         if (has_bnd_in) {
             convH = bndp[2 * t]; convE = bndp[2 * t + 1];
             nextH = bndp[2 * (G + t)]; nextE = bndp[2 * (G + t) + 1];
         }
         // outputs of this lane from the previous step (read by lane t+1)
-        uint32_t Hout = 0, Eout = 0, Bout = 0;
+        uint32_t Hout = 0, Eout = 0;
+        // bottom-row outputs of lane G-1 over one chunk of G steps, shifted down one lane per
+        // step: at the end of the chunk lane j holds the output of the chunk's step j
+        uint32_t qH = 0, qE = 0;
         int hb_top = h0_value(1, fr, go, ge);   // H(0, k+1) for the first stripe
 
-        const int nsteps = N + G - 1;
-        int c = -t;   // DP column (0-based) this lane computes at step k
-        for (int k = 0; k < nsteps; ++k, ++c) {
-            if (k > 0 && (k % G) == 0) {
-                convB = nextB;
-                nextB = colp[k + G + t];
+        ScoreReads<W, R, SCORE> rd;
+        issue_scores<W, R, SCORE>(st, Bin, lds_raw, rd);
+        V sc[R];
+        finish_scores<W, R, SCORE>(st, Bin, tbl_lo, tbl_hi, rd, sc);
+
+        // whole chunks of G steps; lane t is active for columns 0 <= k - t < N
+        const int nsteps = ((N + 2 * G - 2) / G) * G;
+        for (int k0 = 0; k0 < nsteps; k0 += G) {
+#pragma unroll 1
+            for (int u = 0; u < G; ++u) {
+                const int c = k0 + u - t;   // DP column (0-based) of this lane at this step
+                const uint32_t Bnext = dpp<kRowShr1>(conv, Bin);
+                conv = dpp<kRowShl1>(conv, conv);
+                issue_scores<W, R, SCORE>(st, Bnext, lds_raw, rd);
+                uint32_t topH, topE;
                 if (has_bnd_in) {
-                    convH = nextH; convE = nextE;
-                    nextH = bndp[2 * (k + G + t)];
-                    nextE = bndp[2 * (k + G + t) + 1];
+                    topH = convH;
+                    topE = convE;
+                } else {
+                    // H(0, k+1) and the vertical gap state below it: both <= 0 because
+                    // go <= 0 and ge <= 0.
+                    topH = L::top_word(hb_top);
+                    topE = L::top_word(hb_top + go + ge);
+                    hb_top = fr ? 0 : hb_top + ge;
                 }
-            }
-            uint32_t topH, topE;
-            if (has_bnd_in) {
-                topH = convH;
-                topE = convE;
-            } else {
-                // H(0, k+1) and the vertical gap state below it: uniform values in scalar
-                // registers; both are <= 0 because go + ge <= 0 and ge <= 0.
-                topH = L::top_word(hb_top);
-                topE = L::top_word(hb_top + go + ge);
-                hb_top = fr ? 0 : hb_top + ge;
-            }
-            const uint32_t Bin = dpp<kRowShr1>(convB, Bout);
-            const uint32_t Hin = dpp<kRowShr1>(topH, Hout);
-            const uint32_t Ein = dpp<kRowShr1>(topE, Eout);
-            convB = dpp<kRowShl1>(convB, convB);
-            if (has_bnd_in) {
-                convH = dpp<kRowShl1>(convH, convH);
-                convE = dpp<kRowShl1>(convE, convE);
-            }
-            if (uint32_t(c) < uint32_t(N)) {
-                V Eup = L::from(Ein);
-                cell_block<W, R, SCORE, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, Bin, tbl_lo, tbl_hi, lds_raw,
-                                                              vgogo, vge);
-                Hdiag = L::from(Hin);
-                Hout = L::bits(st.H[R - 1]);
-                Eout = L::bits(Eup);
-                Bout = Bin;
+                const uint32_t Hin = dpp<kRowShr1>(topH, Hout);
+                const uint32_t Ein = dpp<kRowShr1>(topE, Eout);
+                if (has_bnd_in) {
+                    convH = dpp<kRowShl1>(convH, convH);
+                    convE = dpp<kRowShl1>(convE, convE);
+                }
+                if (uint32_t(c) < uint32_t(N)) {
+                    V Eup = L::from(Ein);
+                    update_cells<W, R, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, sc, vgogo, vge);
+                    Hdiag = L::from(Hin);
+                    Hout = L::bits(st.H[R - 1]);
+                    Eout = L::bits(Eup);
+                    if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores);
+                }
                 if (has_bnd_out) {
-                    if (t == G - 1) {
-                        bndp[2 * c] = Hout;
-                        bndp[2 * c + 1] = Eout;
-                    }
+                    qH = dpp<kRowShl1>(Hout, qH);
+                    qE = dpp<kRowShl1>(Eout, qE);
                 }
-                if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores);
+                finish_scores<W, R, SCORE>(st, Bnext, tbl_lo, tbl_hi, rd, sc);
+                Bin = Bnext;
+            }
+            conv = nextB;
+            nextB = colp[k0 + 2 * G + 1 + t];
+            if (has_bnd_in) {
+                convH = nextH; convE = nextE;
+                nextH = bndp[2 * (k0 + 2 * G + t)];
+                nextE = bndp[2 * (k0 + 2 * G + t) + 1];
+            }
+            if (has_bnd_out) {
+                // lane G-1 computed column k0 + j - (G-1) at the chunk's step j
+                const int col = k0 + t - (G - 1);
+                if (col >= 0 && col < N) {
+                    bndp[2 * col] = qH;
+                    bndp[2 * col + 1] = qE;
+                }
             }
         }
         if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores);

@@ -169,8 +169,10 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                     const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all)
 {
     if (cfg.method != PA_METHOD_GLOBAL && cfg.method != PA_METHOD_LOCAL) { err = "invalid method"; return PA_ERR_INVALID_ARGUMENT; }
-    if (cfg.gap_extension > 0 || int64_t(cfg.gap_open) + cfg.gap_extension > 0) {
-        err = "the device path needs gap_extension <= 0 and gap_open + gap_extension <= 0";
+    // The cell update drops V + go + ge from the vertical gap state (it never exceeds V + ge
+    // when go <= 0), so both penalties must be non-positive.
+    if (cfg.gap_extension > 0 || cfg.gap_open > 0) {
+        err = "the device path needs gap_open <= 0 and gap_extension <= 0";
         return PA_ERR_UNSUPPORTED;
     }
     for (uint64_t p = 0; p < b.n_pairs; ++p) {

@@ -139,7 +139,7 @@ int32_t max_substitution(const pa_config &cfg);
 
 // Number of colcodes / bnd words reserved per task: the kernels prefetch up to three
 // conveyor chunks of G words past the last column.
-inline uint32_t col_words(uint32_t N, uint32_t G) { return ((N + G - 1) / G) * G + 3 * G; }
+inline uint32_t col_words(uint32_t N, uint32_t G) { return ((N + G - 1) / G) * G + 4 * G; }
 
 // Conservative int16 range analysis of one pair (rows m, columns n).
 struct RangeBound {

@@ -160,6 +160,14 @@ def test_invalid_symbol_is_an_error(ctx):
     assert e.value.status == 2
 
 
+def test_positive_gap_penalties_are_unsupported(ctx):
+    for go, ge in ((1, -2), (0, 1)):
+        dc = device.DeviceConfig(gap_open=go, gap_extension=ge)
+        with pytest.raises(device.PaError) as e:
+            ctx.score([b"ACGT"], [b"ACG"], dc)
+        assert e.value.status == 3
+
+
 def test_batch_repeat_runs(ctx):
     rng = np.random.default_rng(31)
     s1, s2 = workloads.dna_pairs(rng, 3000, 100, 200)
