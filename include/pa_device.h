@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PA_ABI_VERSION 1
+#define PA_ABI_VERSION 2
 
 typedef enum pa_status {
     PA_OK = 0,
@@ -46,6 +46,18 @@ enum { PA_SCORE_UNITARY = 0, PA_SCORE_MATRIX = 1 };
  * result that reaches the int16 limit is recomputed at int32 (PA_PAIR_RECOMPUTED). */
 enum { PA_LANES_AUTO = 0, PA_LANES_LEARNED = 1, PA_LANES_INT16 = 16, PA_LANES_INT32 = 32 };
 
+/* Planner and launch parameters: the autotuner's search space (pa_tune).  No counterpart in the
+ * reference, whose tile and lane shapes are compile-time constants (dp_matrix_lane_width.hpp:25-55).
+ * A zero field takes the default given in brackets. */
+typedef struct pa_tuning {
+    float cell_cost;              /* shape model: relative cost of one row of one lane per step [13] */
+    float step_cost;              /* shape model: relative fixed cost of one lane step [24] */
+    uint32_t rows_mask;           /* rows per lane R in {4,6,...,16} allowed: bit R/2-2 [all] */
+    uint32_t wavefront_min_rows;  /* pairs with more DP rows use the wavefront kernel [3072] */
+    uint32_t streams;             /* concurrent launch streams, 1..4 [4] */
+    uint32_t eager;               /* 1: launch kernels one by one; 0: replay a captured hipGraph [0] */
+} pa_tuning;
+
 typedef struct pa_config {
     int32_t method;             /* PA_METHOD_* */
     int32_t first_column_free;  /* cfg::leading_end_gap::first_column == end_gap::free */
@@ -62,6 +74,7 @@ typedef struct pa_config {
     const uint8_t *matrix_symbols; /* matrix: the dim symbols; rank = position (alphabet_rank_map_scalar.hpp:39-51) */
     int32_t lane_policy;        /* PA_LANES_* */
     const struct pa_model *model; /* PA_LANES_LEARNED: the trained range predictor (else ignored) */
+    const pa_tuning *tuning;    /* planner/launch parameters; NULL = defaults */
 } pa_config;
 
 /* per-pair result flags */
@@ -116,6 +129,14 @@ pa_status pa_batch_stats_get(const pa_batch *batch, pa_batch_stats *out);
  * pass), used for the roofline. */
 pa_status pa_batch_time(pa_ctx *ctx, pa_batch *batch, int iters, double *ms_total, double *ms_main);
 pa_status pa_batch_destroy(pa_ctx *ctx, pa_batch *batch);
+
+/* Autotuner.  No counterpart in the reference.  Plans and times candidate tunings on a sample
+ * of the batch (every k-th pair, about sample_pairs pairs; 0 = 5% of the batch, at least 2,000)
+ * by coordinate descent over the pa_tuning fields, and writes the fastest to *out.  `iters`
+ * passes are timed per candidate (0 = 3). */
+pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                  const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, const uint8_t *pair_method,
+                  uint64_t sample_pairs, int iters, pa_tuning *out);
 
 /* ---------------------------------------------------------------------------------------------
  * Learned lane dispatch.  No counterpart in the reference (its lane width is a compile-time

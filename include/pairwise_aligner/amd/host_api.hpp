@@ -32,6 +32,7 @@
 #include <concepts>
 #include <cstdint>
 #include <memory>
+#include <optional>
 #include <mutex>
 #include <ranges>
 #include <stdexcept>
@@ -325,6 +326,7 @@ class aligner {
     std::vector<uint8_t> _symbols{};
     int _device{0};
     std::shared_ptr<pa_ctx> _ctx{};
+    std::optional<pa_tuning> _tuning{};
 
     void run(std::vector<uint8_t> const &b1, std::vector<uint64_t> const &o1, std::vector<uint8_t> const &b2,
              std::vector<uint64_t> const &o2, std::vector<int32_t> &scores, std::vector<uint8_t> &flags)
@@ -338,6 +340,7 @@ class aligner {
             cfg.matrix = _matrix.data();
             cfg.matrix_symbols = _symbols.data();
         }
+        if (_tuning) cfg.tuning = &*_tuning;
         static const uint8_t empty = 0;
         const pa_status st = pa_score_batch(_ctx.get(), &cfg, b1.empty() ? &empty : b1.data(), o1.data(),
                                             b2.empty() ? &empty : b2.data(), o2.data(), n, nullptr, scores.data(),
@@ -351,6 +354,8 @@ public:
         _cfg{cfg}, _matrix{std::move(matrix)}, _symbols{std::move(symbols)}, _device{device} {}
 
     pa_config const &config() const noexcept { return _cfg; }
+    // Planner/launch parameters, e.g. from pa_tune (no counterpart in the reference).
+    void tuning(pa_tuning const &t) { _tuning = t; }
     std::vector<int32_t> const &matrix() const noexcept { return _matrix; }
     std::vector<uint8_t> const &symbols() const noexcept { return _symbols; }
 

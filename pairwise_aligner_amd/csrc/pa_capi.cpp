@@ -59,9 +59,16 @@ struct pa_batch {
     std::vector<uint32_t> recompute_set;
     DeviceBuffer pairs, scores, flags;
     bool can_saturate = false;
+    hipGraph_t graph = nullptr;            // captured main pass (see launch_main)
+    hipGraphExec_t graph_exec = nullptr;
     pa_batch_stats stats{};
     double main_ms_accum = 0;
     uint64_t main_ms_count = 0;
+    ~pa_batch()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
 };
 
 struct pa_model {
@@ -139,42 +146,32 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     return a;
 }
 
-pa_status launch_plan(pa_ctx *ctx, pa_batch *b, PlanDevice &pd, bool timed)
+// Enqueues the launches of a plan: forked from the context stream onto the plan's streams in
+// the plan's order, joined back into the context stream.
+pa_status launch_plan(pa_ctx *ctx, pa_batch *b, PlanDevice &pd)
 {
-    if (timed) PA_HIP(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
-    const size_t nl = pd.plan.launches.size();
-    const int nside = int(std::min<size_t>(nl, kSide));
-    if (nside > 1) {
+    const pa::Plan &p = pd.plan;
+    const size_t nl = p.launches.size();
+    int nside = 0;
+    for (size_t i = 0; i < nl; ++i) nside = std::max(nside, int(p.launch_stream[i]) + 1);
+    const bool fork = nside > 1;
+    if (fork) {
         PA_HIP(hipEventRecord(ctx->fork, ctx->stream), "hipEventRecord");
         for (int i = 0; i < nside; ++i) PA_HIP(hipStreamWaitEvent(ctx->side[i], ctx->fork, 0), "hipStreamWaitEvent");
     }
-    // launches are ordered by decreasing work: the largest start first
-    std::vector<size_t> order(nl);
-    for (size_t i = 0; i < nl; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-        return pd.plan.launches[x].padded_cells > pd.plan.launches[y].padded_cells;
-    });
-    for (size_t i = 0; i < nl; ++i) {
-        const pa::Launch &L = pd.plan.launches[order[i]];
+    const uint32_t lds = p.table.empty() ? 0 : pa::lds_table_bytes(p.alphabet, p.stride);
+    for (uint32_t li : p.launch_order) {
+        const pa::Launch &L = p.launches[li];
         const pa::KernelArgs a = kernel_args(pd, b, L);
-        const uint32_t lds = pd.plan.table.empty() ? 0 : pa::lds_table_bytes(pd.plan.alphabet, pd.plan.stride);
-        hipStream_t st = nside > 1 ? ctx->side[i % nside] : ctx->stream;
+        hipStream_t st = fork ? ctx->side[p.launch_stream[li]] : ctx->stream;
         hipError_t e = pa::launch_stripe(L.key, a, lds, st);
         if (e != hipSuccess) return hip_fail(ctx, e, "striped DP launch");
     }
-    if (nside > 1) {
+    if (fork) {
         for (int i = 0; i < nside; ++i) {
             PA_HIP(hipEventRecord(ctx->join[i], ctx->side[i]), "hipEventRecord");
             PA_HIP(hipStreamWaitEvent(ctx->stream, ctx->join[i], 0), "hipStreamWaitEvent");
         }
-    }
-    if (timed) {
-        PA_HIP(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
-        PA_HIP(hipEventSynchronize(ctx->ev[3]), "hipEventSynchronize");
-        float ms = 0;
-        PA_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
-        b->main_ms_accum += ms;
-        b->main_ms_count += 1;
     }
     return PA_OK;
 }
@@ -193,12 +190,46 @@ pa::InitArgs init_args(pa_batch *b, const pa::Plan &p)
     return ia;
 }
 
+// The main pass: initial scores, then every launch of the main plan.  Unless the tuning asks
+// for eager launches it is captured once into a hipGraph and replayed (one launch call per pass,
+// no per-kernel launch gaps).  The graph holds kernels and event dependencies only.
+pa_status enqueue_main(pa_ctx *ctx, pa_batch *b)
+{
+    hipError_t e = pa::launch_init(init_args(b, b->main.plan), ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "init launch");
+    return launch_plan(ctx, b, b->main);
+}
+
+pa_status launch_main(pa_ctx *ctx, pa_batch *b)
+{
+    if (b->main.plan.tuning.eager) return enqueue_main(ctx, b);
+    if (!b->graph_exec) {
+        PA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        pa_status st = enqueue_main(ctx, b);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (st != PA_OK) { if (g) (void)hipGraphDestroy(g); return st; }
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+        b->graph = g;
+        PA_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    PA_HIP(hipGraphLaunch(b->graph_exec, ctx->stream), "hipGraphLaunch");
+    return PA_OK;
+}
+
 pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
 {
     pa_status st;
-    hipError_t e = pa::launch_init(init_args(b, b->main.plan), ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "init launch");
-    if ((st = launch_plan(ctx, b, b->main, timed)) != PA_OK) return st;
+    if (timed) PA_HIP(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
+    if ((st = launch_main(ctx, b)) != PA_OK) return st;
+    if (timed) {
+        PA_HIP(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
+        PA_HIP(hipEventSynchronize(ctx->ev[3]), "hipEventSynchronize");
+        float ms = 0;
+        PA_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
+        b->main_ms_accum += ms;
+        b->main_ms_count += 1;
+    }
     b->stats.n_recomputed = 0;
     if (!b->can_saturate) return PA_OK;
 
@@ -233,9 +264,9 @@ pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
     ia.pairs = static_cast<const pa::PairMeta *>(b->pairs.ptr);
     ia.flag_or = 0x04u;  // PA_PAIR_RECOMPUTED
     // the per-pair width in the main plan says 16; the recomputed pairs report int32
-    e = pa::launch_init(ia, ctx->stream);
+    hipError_t e = pa::launch_init(ia, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "init launch");
-    return launch_plan(ctx, b, *b->recompute, false);
+    return launch_plan(ctx, b, *b->recompute);
 }
 
 // ---- learned dispatch helpers ---------------------------------------------------------------
@@ -672,10 +703,10 @@ pa_status pa_dp_labels(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, c
         float ms = 0;
         PA_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
         double total = 0;
-        for (uint32_t t = L.task_begin; t < L.task_end; ++t) total += pa::task_cost(L.key, b.main.plan.tasks[t]);
+        for (uint32_t t = L.task_begin; t < L.task_end; ++t) total += pa::task_cost(L.key, b.main.plan.tasks[t], b.main.plan.tuning);
         for (uint32_t t = L.task_begin; t < L.task_end; ++t) {
             const pa::Task &tk = b.main.plan.tasks[t];
-            const double share = double(ms) * 1e6 * pa::task_cost(L.key, tk) / std::max(total, 1e-9);
+            const double share = double(ms) * 1e6 * pa::task_cost(L.key, tk, b.main.plan.tuning) / std::max(total, 1e-9);
             for (int q = 0; q < 2; ++q)
                 if (tk.pair[q] != pa::kNone) pair_cost[tk.pair[q]] = share / (tk.pair[1] != pa::kNone ? 2.0 : 1.0);
         }
@@ -684,6 +715,89 @@ pa_status pa_dp_labels(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, c
     if ((st = check_error_slot(ctx, &b)) != PA_OK) return st;
     if (cost)
         for (uint64_t p = 0; p < n_pairs; ++p) cost[p] = float(pair_cost[p]);   // nanoseconds
+    return PA_OK;
+}
+
+pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                  const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, const uint8_t *pair_method,
+                  uint64_t sample_pairs, int iters, pa_tuning *out)
+{
+    if (!ctx || !cfg || !off1 || !off2 || !out || (n_pairs && (!seq1 || !seq2)))
+        return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
+    if (iters <= 0) iters = 3;
+    // the sample: every k-th pair, which keeps the batch's length distribution
+    if (sample_pairs == 0) sample_pairs = std::max<uint64_t>(2000, n_pairs / 20);
+    const uint64_t k = std::max<uint64_t>(1, n_pairs / std::max<uint64_t>(sample_pairs, 1));
+    std::vector<uint8_t> s1, s2, pm;
+    std::vector<uint64_t> o1{0}, o2{0};
+    for (uint64_t p = 0; p < n_pairs; p += k) {
+        s1.insert(s1.end(), seq1 + off1[p], seq1 + off1[p + 1]);
+        s2.insert(s2.end(), seq2 + off2[p], seq2 + off2[p + 1]);
+        o1.push_back(s1.size());
+        o2.push_back(s2.size());
+        if (pair_method) pm.push_back(pair_method[p]);
+    }
+    const uint64_t ns = o1.size() - 1;
+    if (s1.empty()) s1.push_back(0);
+    if (s2.empty()) s2.push_back(0);
+
+    pa::Tuning base = pa::resolve_tuning(cfg->tuning);
+    auto to_c = [](const pa::Tuning &t) {
+        pa_tuning c{};
+        c.cell_cost = float(t.cell_cost);
+        c.step_cost = float(t.step_cost);
+        c.rows_mask = t.rows_mask;
+        c.wavefront_min_rows = t.wavefront_min_rows;
+        c.streams = t.streams;
+        c.eager = t.eager ? 1 : 0;
+        return c;
+    };
+    // time of `iters` passes over the sample with tuning t (after one untimed pass)
+    auto measure = [&](const pa_tuning &t, double &ms) -> pa_status {
+        pa_config c = *cfg;
+        c.tuning = &t;
+        pa_batch *b = nullptr;
+        pa_status st = pa_batch_create(ctx, &c, s1.data(), o1.data(), s2.data(), o2.data(), ns,
+                                       pm.empty() ? nullptr : pm.data(), &b);
+        if (st != PA_OK) return st;
+        st = pa_batch_run(ctx, b);
+        double total = 0;
+        if (st == PA_OK) st = pa_batch_time(ctx, b, iters, &total, nullptr);
+        pa_batch_destroy(ctx, b);
+        ms = total / iters;
+        return st;
+    };
+    pa_tuning best = to_c(base);
+    double best_ms = 0;
+    pa_status st = measure(best, best_ms);
+    if (st != PA_OK) return st;
+
+    // coordinate descent, two sweeps over the parameters
+    const uint32_t wf[] = {1024, 2048, 3072, 4096, 6144};
+    const float step[] = {8, 16, 24, 40, 64};
+    const uint32_t rows[] = {0x7F, 0x55, 0x2A, 0x7E, 0x3F, 0x78, 0x1E};
+    const uint32_t streams[] = {1, 2, 3, 4};
+    const uint32_t eager[] = {0, 1};
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        for (int param = 0; param < 5; ++param) {
+            const int nv = param == 0 ? 5 : param == 1 ? 5 : param == 2 ? 7 : param == 3 ? 4 : 2;
+            for (int v = 0; v < nv; ++v) {
+                pa_tuning t = best;
+                switch (param) {
+                case 0: t.wavefront_min_rows = wf[v]; break;
+                case 1: t.step_cost = step[v]; break;
+                case 2: t.rows_mask = rows[v]; break;
+                case 3: t.streams = streams[v]; break;
+                default: t.eager = eager[v]; break;
+                }
+                if (std::memcmp(&t, &best, sizeof t) == 0) continue;
+                double ms = 0;
+                if ((st = measure(t, ms)) != PA_OK) return st;
+                if (ms < best_ms * 0.98) { best = t; best_ms = ms; }
+            }
+        }
+    }
+    *out = best;
     return PA_OK;
 }
 

@@ -24,14 +24,22 @@ bool VariantKey::operator==(const VariantKey &o) const
 // Rows per lane that kernels are instantiated for (lanes per group G = 16).
 static const uint8_t kRowsPerLane[] = {4, 6, 8, 10, 12, 14, 16};
 
-// Relative cost of one step of one lane: about 13 VALU instructions per cell and a fixed
-// per-step part (lane exchange, conveyors, loop control) of about 24.
-static constexpr double kCellCost = 13.0, kStepCost = 24.0;
-
-void choose_shape(uint32_t m, uint32_t n, uint32_t width, uint8_t &G, uint8_t &R)
+Tuning resolve_tuning(const pa_tuning *t)
 {
-    (void)width;
-    if (m > kWavefrontMinRows) {   // long pair: one workgroup of 8 waves, stripes of 64*16 rows
+    Tuning r;
+    if (!t) return r;
+    if (t->cell_cost > 0) r.cell_cost = t->cell_cost;
+    if (t->step_cost > 0) r.step_cost = t->step_cost;
+    if (t->rows_mask & 0x7F) r.rows_mask = t->rows_mask & 0x7F;
+    if (t->wavefront_min_rows) r.wavefront_min_rows = std::max<uint32_t>(t->wavefront_min_rows, 256);
+    if (t->streams) r.streams = std::min<uint32_t>(t->streams, 4);
+    r.eager = t->eager != 0;
+    return r;
+}
+
+void choose_shape(uint32_t m, uint32_t n, const Tuning &tuning, uint8_t &G, uint8_t &R)
+{
+    if (m > tuning.wavefront_min_rows) {   // long pair: one workgroup of 8 waves, stripes of 64*16 rows
         G = 64;
         R = 16;
         return;
@@ -39,10 +47,12 @@ void choose_shape(uint32_t m, uint32_t n, uint32_t width, uint8_t &G, uint8_t &R
     G = 16;
     double best = 1e300;
     R = 16;
-    for (uint8_t r : kRowsPerLane) {
+    for (int i = 0; i < 7; ++i) {
+        if (!(tuning.rows_mask & (1u << i))) continue;
+        const uint8_t r = kRowsPerLane[i];
         const uint32_t rows = uint32_t(G) * r;
         const uint32_t S = (m + rows - 1) / rows;
-        const double cost = double(S) * (double(n) + G - 1) * (kCellCost * r + kStepCost);
+        const double cost = double(S) * (double(n) + G - 1) * (tuning.cell_cost * r + tuning.step_cost);
         if (cost < best * 0.999) { best = cost; R = r; }
     }
 }
@@ -151,9 +161,47 @@ struct Item {
 
 } // namespace
 
-double task_cost(const VariantKey &key, const Task &t)
+double task_cost(const VariantKey &key, const Task &t, const Tuning &tuning)
 {
-    return double(t.S) * (double(t.N) + key.G - 1) * (kCellCost * key.R + kStepCost) * (key.G == 64 ? 4.0 : 1.0);
+    return double(t.S) * (double(t.N) + key.G - 1) * (tuning.cell_cost * key.R + tuning.step_cost) *
+           (key.G == 64 ? 4.0 : 1.0);
+}
+
+void order_launches(Plan &plan)
+{
+    // Wavefront launches first: each of their workgroups is one long serial sweep, so they
+    // set the critical path (longest task first).  Then the striped launches by decreasing
+    // work, so that the tail of the pass is made of short tasks.  Streams round robin.
+    const size_t nl = plan.launches.size();
+    std::vector<double> longest(nl, 0.0);
+    for (size_t i = 0; i < nl; ++i) {
+        const Launch &L = plan.launches[i];
+        for (uint32_t t = L.task_begin; t < L.task_end; ++t)
+            longest[i] = std::max(longest[i], task_cost(L.key, plan.tasks[t], plan.tuning));
+    }
+    plan.launch_order.resize(nl);
+    std::iota(plan.launch_order.begin(), plan.launch_order.end(), 0u);
+    std::stable_sort(plan.launch_order.begin(), plan.launch_order.end(), [&](uint32_t x, uint32_t y) {
+        const Launch &a = plan.launches[x], &b = plan.launches[y];
+        const bool wa = a.key.G == 64, wb = b.key.G == 64;
+        if (wa != wb) return wa;
+        if (wa) return longest[x] > longest[y];
+        return a.padded_cells > b.padded_cells;
+    });
+    // With two or more streams the wavefront launches share stream 0 and the striped launches
+    // take the others round robin, so a long wavefront workgroup never holds up striped work.
+    const uint32_t ns = std::max<uint32_t>(1, std::min<uint32_t>(plan.tuning.streams, uint32_t(std::max<size_t>(nl, 1))));
+    bool any_wf = false;
+    for (const Launch &L : plan.launches) any_wf |= L.key.G == 64;
+    const uint32_t first = (any_wf && ns > 1) ? 1 : 0;
+    plan.launch_stream.assign(nl, 0);
+    uint32_t next = 0;
+    for (size_t i = 0; i < nl; ++i) {
+        const uint32_t li = plan.launch_order[i];
+        if (plan.launches[li].key.G == 64 && first) continue;   // stream 0
+        plan.launch_stream[li] = uint8_t(first + next % (ns - first));
+        ++next;
+    }
 }
 
 int32_t max_substitution(const pa_config &cfg)
@@ -180,6 +228,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         if (b.pair_method && b.pair_method[p] > 1) { err = "invalid per-pair method"; return PA_ERR_INVALID_ARGUMENT; }
     }
     plan = Plan{};
+    plan.tuning = resolve_tuning(cfg.tuning);
     plan.go = cfg.gap_open;
     plan.ge = cfg.gap_extension;
     int fc = cfg.first_column_free != 0, fr = cfg.first_row_free != 0;
@@ -226,7 +275,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         pm.width = width;
         VariantKey key{};
         key.width = width;
-        choose_shape(pm.m, pm.n, width, key.G, key.R);
+        choose_shape(pm.m, pm.n, plan.tuning, key.G, key.R);
         key.score = plan.score;
         key.local = local;
         key.track = (!local && ((width == 16 && !upper16) || track_all)) ? 1 : 0;
@@ -356,6 +405,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         err = "batch too large for one plan; split it";
         return PA_ERR_INVALID_ARGUMENT;
     }
+    order_launches(plan);
     return PA_OK;
 }
 

@@ -95,7 +95,19 @@ struct PairMeta {
     uint8_t pad_;
 };
 
+// Planner and launch parameters resolved from pa_tuning (zero fields take these defaults).
+struct Tuning {
+    double cell_cost = 13.0;     // ~VALU instructions per row of a lane per step
+    double step_cost = 24.0;     // per-step work outside the cells (lane exchange, conveyors, loop)
+    uint32_t rows_mask = 0x7F;   // bit i: R = 4 + 2i allowed
+    uint32_t wavefront_min_rows = 3072;
+    uint32_t streams = 4;
+    bool eager = false;
+};
+Tuning resolve_tuning(const pa_tuning *t);
+
 struct Plan {
+    Tuning tuning;
     // scoring
     int32_t go = 0, ge = 0;
     int32_t fc = 0, fr = 0, lc = 0, lr = 0;   // after orientation
@@ -113,6 +125,8 @@ struct Plan {
     uint64_t cells = 0, padded_cells = 0;
     uint64_t n16 = 0, n32 = 0;
     uint64_t n_speculative = 0;               // int16 pairs whose bound does not exclude saturation
+    std::vector<uint32_t> launch_order;       // launches in start order (order_launches)
+    std::vector<uint8_t> launch_stream;       // stream of each launch (indexed like launches)
 };
 
 struct BatchView {
@@ -132,12 +146,12 @@ pa_status make_plan(const pa_config &cfg, const BatchView &batch, const uint8_t 
                     const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all = false);
 
 // Modelled sweep cost of one task (arbitrary units): stripes x steps x per-step work.
-double task_cost(const VariantKey &key, const Task &t);
+double task_cost(const VariantKey &key, const Task &t, const Tuning &tuning);
 
 // Largest substitution score of a configuration (int16 range analysis and model features).
 int32_t max_substitution(const pa_config &cfg);
 
-// Number of colcodes / bnd words reserved per task: the kernels prefetch up to three
+// Number of colcodes / bnd words reserved per task: the kernels prefetch up to four
 // conveyor chunks of G words past the last column.
 inline uint32_t col_words(uint32_t N, uint32_t G) { return ((N + G - 1) / G) * G + 4 * G; }
 
@@ -151,8 +165,14 @@ RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n);
 // Pairs with more rows than this go to the wavefront kernel (G = 64, a workgroup per task).
 inline constexpr uint32_t kWavefrontMinRows = 3072;
 
-// Choice of (G, R) for a pair with m rows and n columns: the wavefront kernel for long pairs,
-// else the rows per lane with the lowest modelled sweep cost (stripes x steps x per-step work).
-void choose_shape(uint32_t m, uint32_t n, uint32_t width, uint8_t &G, uint8_t &R);
+// Choice of (G, R) for a pair with m rows and n columns: the wavefront kernel for pairs with
+// more than tuning.wavefront_min_rows rows, else the allowed rows per lane with the lowest
+// modelled sweep cost (stripes x steps x per-step work).
+void choose_shape(uint32_t m, uint32_t n, const Tuning &tuning, uint8_t &G, uint8_t &R);
+
+// Order of the launches of a plan and the stream of each: wavefront launches first (longest
+// task first), then striped launches by decreasing work.  Wavefront launches share stream 0,
+// striped launches take the other streams round robin.
+void order_launches(Plan &plan);
 
 } // namespace pa

@@ -29,7 +29,7 @@ EXPORTS = ["pa_abi_version", "pa_ctx_create", "pa_ctx_destroy", "pa_ctx_last_err
            "pa_batch_create", "pa_batch_run", "pa_batch_results", "pa_batch_stats_get", "pa_batch_time",
            "pa_batch_destroy", "pa_model_create", "pa_model_destroy", "pa_model_param_count",
            "pa_model_set_params", "pa_model_get_params", "pa_features", "pa_model_forward", "pa_model_loss_grad",
-           "pa_model_train_step", "pa_dp_labels"]
+           "pa_model_train_step", "pa_dp_labels", "pa_tune"]
 N_FEATURES, N_OUTPUTS, N_HIDDEN = 8, 2, 64
 
 
@@ -37,6 +37,37 @@ class PaError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
         self.status = status
+
+
+class CTuning(ctypes.Structure):
+    _fields_ = [
+        ("cell_cost", ctypes.c_float),
+        ("step_cost", ctypes.c_float),
+        ("rows_mask", ctypes.c_uint32),
+        ("wavefront_min_rows", ctypes.c_uint32),
+        ("streams", ctypes.c_uint32),
+        ("eager", ctypes.c_uint32),
+    ]
+
+
+@dataclass(frozen=True)
+class Tuning:
+    """Planner and launch parameters (pa_tuning); 0 fields take the library defaults."""
+    cell_cost: float = 0.0
+    step_cost: float = 0.0
+    rows_mask: int = 0
+    wavefront_min_rows: int = 0
+    streams: int = 0
+    eager: int = 0
+
+    def to_c(self) -> CTuning:
+        return CTuning(self.cell_cost, self.step_cost, self.rows_mask, self.wavefront_min_rows, self.streams,
+                       self.eager)
+
+    @staticmethod
+    def from_c(c: CTuning) -> "Tuning":
+        return Tuning(round(c.cell_cost, 4), round(c.step_cost, 4), c.rows_mask, c.wavefront_min_rows, c.streams,
+                      c.eager)
 
 
 class CConfig(ctypes.Structure):
@@ -56,6 +87,7 @@ class CConfig(ctypes.Structure):
         ("matrix_symbols", ctypes.POINTER(ctypes.c_uint8)),
         ("lane_policy", ctypes.c_int32),
         ("model", ctypes.c_void_p),
+        ("tuning", ctypes.POINTER(CTuning)),
     ]
 
 
@@ -82,6 +114,7 @@ class DeviceConfig:
     matrix: Optional[tuple] = None
     lane_policy: int = LANES_AUTO
     model: Optional["Model"] = None     # PA_LANES_LEARNED
+    tuning: Optional[Tuning] = None     # planner/launch parameters (None: defaults)
 
 
 @functools.lru_cache(maxsize=1)
@@ -112,10 +145,11 @@ def lib() -> ctypes.CDLL:
     L.pa_model_loss_grad.argtypes = [vp, vp, vp, vp, vp, u64, vp, vp]
     L.pa_model_train_step.argtypes = [vp, vp, vp, vp, vp, u64, ctypes.c_float, vp]
     L.pa_dp_labels.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp]
+    L.pa_tune.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, u64, i32, ctypes.POINTER(CTuning)]
     for name in ["pa_ctx_create", "pa_ctx_destroy", "pa_score_batch", "pa_batch_create", "pa_batch_run",
                  "pa_batch_results", "pa_batch_stats_get", "pa_batch_time", "pa_batch_destroy", "pa_model_create",
                  "pa_model_destroy", "pa_model_set_params", "pa_model_get_params", "pa_features",
-                 "pa_model_forward", "pa_model_loss_grad", "pa_model_train_step", "pa_dp_labels"]:
+                 "pa_model_forward", "pa_model_loss_grad", "pa_model_train_step", "pa_dp_labels", "pa_tune"]:
         getattr(L, name).restype = i32
     return L
 
@@ -137,6 +171,9 @@ class _CConfigHolder:
                          int(cfg.last_column_free), int(cfg.last_row_free), cfg.gap_open, cfg.gap_extension,
                          cfg.score_kind, cfg.match, cfg.mismatch, 0, None, None, cfg.lane_policy,
                          cfg.model.handle if cfg.model is not None else None)
+        if cfg.tuning is not None:
+            self.tun = cfg.tuning.to_c()
+            self.c.tuning = ctypes.pointer(self.tun)
         if cfg.score_kind == SCORE_MATRIX:
             self.mat = (ctypes.c_int32 * len(cfg.matrix))(*cfg.matrix)
             self.sym = (ctypes.c_uint8 * len(cfg.symbols))(*cfg.symbols)
@@ -319,6 +356,19 @@ def dp_labels(ctx: Context, cfg: DeviceConfig, b1, o1, b2, o2):
     ctx.check(ctx._lib.pa_dp_labels(ctx.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
                                     b2.ctypes.data, o2.ctypes.data, n, max_h.ctypes.data, cost.ctypes.data))
     return max_h[:n], cost[:n]
+
+
+def tune(ctx: Context, cfg: DeviceConfig, b1, o1, b2, o2, pair_method=None, sample_pairs: int = 0,
+         iters: int = 0) -> Tuning:
+    """Runs the library's autotuner (pa_tune) on a sample of the batch; returns the fastest tuning."""
+    holder = _CConfigHolder(cfg)
+    n = len(o1) - 1
+    pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
+    out = CTuning()
+    ctx.check(ctx._lib.pa_tune(ctx.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data, b2.ctypes.data,
+                               o2.ctypes.data, n, None if pm is None else pm.ctypes.data, sample_pairs, iters,
+                               ctypes.byref(out)))
+    return Tuning.from_c(out)
 
 
 @functools.lru_cache(maxsize=None)

@@ -1,0 +1,146 @@
+"""GPU: hipGraph replay of the main pass, planner tunings and the autotuner, all bit-exact
+against the CPU oracle through the C ABI.
+
+The replay test follows the rule for captured graphs: it replays the graph several times with
+eager work and synchronisations of the program between the replays, and checks every replay
+against the eagerly launched result.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pairwise_aligner_amd import device, tables, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    sys.stdout.write(msg + "\n")
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return device.Context(0)
+
+
+def oracle_cfg(dc):
+    return O.OracleConfig(mode=dc.method, first_column_free=dc.first_column_free, first_row_free=dc.first_row_free,
+                          last_column_free=dc.last_column_free, last_row_free=dc.last_row_free,
+                          gap_open=dc.gap_open, gap_extension=dc.gap_extension, score_kind=dc.score_kind,
+                          match=dc.match, mismatch=dc.mismatch, symbols=dc.symbols, matrix=dc.matrix)
+
+
+def uneven_batch(seed, count, hi=4000):
+    """DNA pairs with log-normal lengths (median 250), half global half local."""
+    rng = np.random.default_rng(seed)
+    L = np.clip(np.round(rng.lognormal(np.log(250.0), 0.8, size=count)), 1, hi).astype(np.int64)
+    b1, o1 = workloads.random_packed(rng, L, workloads.DNA)
+    b2, o2 = workloads.mutate_packed(rng, b1, o1, rng.uniform(0.01, 0.2, size=count), 0.1, 0.7, workloads.DNA)
+    pm = (np.arange(count) % 2).astype(np.uint8)
+    return b1, o1, b2, o2, pm
+
+
+def oracle_scores(b1, o1, b2, o2, dc, pm):
+    return O.score_packed(b1, o1, b2, o2, oracle_cfg(dc), pair_method=pm, threads=8)
+
+
+def test_graph_replays_match_eager(ctx):
+    b1, o1, b2, o2, pm = uneven_batch(41, 3000)
+    base = dict(match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
+    eager = device.DeviceConfig(**base, tuning=device.Tuning(eager=1))
+    graph = device.DeviceConfig(**base)
+    want = oracle_scores(b1, o1, b2, o2, graph, pm)
+    be = device.Batch(ctx, b1, o1, b2, o2, eager, pm)
+    be.run()
+    ref, ref_flags = be.results()
+    assert np.array_equal(ref.astype(np.int64), want)
+    bg = device.Batch(ctx, b1, o1, b2, o2, graph, pm)
+    for rep in range(4):
+        log(f"graph replay {rep}")
+        bg.run()
+        got, flags = bg.results()
+        assert np.array_equal(got, ref), f"replay {rep}: {np.count_nonzero(got != ref)} mismatches"
+        assert np.array_equal(flags, ref_flags)
+        # eager work of the program between replays
+        be.run()
+        again, _ = be.results()
+        assert np.array_equal(again, ref)
+    bg.time(3)
+    got, _ = bg.results()
+    assert np.array_equal(got, ref)
+
+
+def test_graph_with_saturation_recompute(ctx):
+    """The recompute pass (int32, eager) after a replayed int16 pass."""
+    rng = np.random.default_rng(43)
+    base = workloads.unpack(*workloads.random_packed(rng, np.full(4, 9000), workloads.DNA))
+    short1, short2 = workloads.dna_pairs(rng, 200, 50, 300, divergence=0.05)
+    s1, s2 = base + short1, list(base) + short2
+    b1, o1 = workloads.pack(s1)
+    b2, o2 = workloads.pack(s2)
+    dc = device.DeviceConfig(match=4, mismatch=-5, gap_open=-10, gap_extension=-1, lane_policy=device.LANES_INT16)
+    want = O.score_packed(b1, o1, b2, o2, oracle_cfg(dc), threads=8)
+    bt = device.Batch(ctx, b1, o1, b2, o2, dc)
+    for rep in range(3):
+        log(f"saturating replay {rep}")
+        bt.run()
+        got, flags = bt.results()
+        assert np.array_equal(got.astype(np.int64), want)
+        assert np.all(flags[:4] & device.PAIR_RECOMPUTED)
+
+
+TUNINGS = [
+    device.Tuning(rows_mask=0x55),                      # R = 4, 8, 12, 16 only
+    device.Tuning(rows_mask=0x2A, streams=1),           # R = 6, 10, 14 on one stream
+    device.Tuning(wavefront_min_rows=512, streams=2),   # medium pairs in the wavefront kernel
+    device.Tuning(step_cost=64.0, eager=1),
+    device.Tuning(cell_cost=4.0, step_cost=8.0, rows_mask=0x01),   # R = 4 only
+]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: f"{t}")
+def test_tunings_are_exact(ctx, tuning):
+    b1, o1, b2, o2, pm = uneven_batch(47, 1500, hi=3000)
+    dc = device.DeviceConfig(match=2, mismatch=-3, gap_open=-5, gap_extension=-2, tuning=tuning)
+    want = oracle_scores(b1, o1, b2, o2, dc, pm)
+    got, _ = ctx.score_packed(b1, o1, b2, o2, dc, pm)
+    bad = np.nonzero(got.astype(np.int64) != want)[0]
+    assert len(bad) == 0, f"{tuning}: {len(bad)} mismatches, first {bad[:5].tolist()}"
+
+
+def test_tunings_blosum_local(ctx):
+    rng = np.random.default_rng(53)
+    s1, s2 = workloads.protein_pairs(rng, 1200, 20, 600)
+    t = tables.blosum62_standard()
+    b1, o1 = workloads.pack(s1)
+    b2, o2 = workloads.pack(s2)
+    for tuning in TUNINGS[:3]:
+        dc = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
+                                 symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
+                                 gap_open=-11, gap_extension=-1, tuning=tuning)
+        want = O.score_packed(b1, o1, b2, o2, oracle_cfg(dc), threads=8)
+        got, _ = ctx.score_packed(b1, o1, b2, o2, dc)
+        assert np.array_equal(got.astype(np.int64), want), f"{tuning}"
+
+
+def test_autotuner_result_is_exact_and_not_slower(ctx):
+    b1, o1, b2, o2, pm = uneven_batch(59, 20000)
+    dc = device.DeviceConfig(match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
+    tuned = device.tune(ctx, dc, b1, o1, b2, o2, pm, sample_pairs=4000, iters=2)
+    log(f"tuned: {tuned}")
+    assert 1 <= tuned.streams <= 4 and tuned.rows_mask & 0x7F
+    dct = device.DeviceConfig(match=2, mismatch=-3, gap_open=-5, gap_extension=-2, tuning=tuned)
+    want = oracle_scores(b1, o1, b2, o2, dc, pm)
+    bd = device.Batch(ctx, b1, o1, b2, o2, dc, pm)
+    bt = device.Batch(ctx, b1, o1, b2, o2, dct, pm)
+    bd.run()
+    bt.run()
+    assert np.array_equal(bt.results()[0].astype(np.int64), want)
+    t_default = min(bd.time(3)[0] for _ in range(2))
+    t_tuned = min(bt.time(3)[0] for _ in range(2))
+    log(f"default {t_default / 3:.3f} ms, tuned {t_tuned / 3:.3f} ms per pass")
+    # the tuner measured on a sample; on the whole batch it must not lose much
+    assert t_tuned <= t_default * 1.15
