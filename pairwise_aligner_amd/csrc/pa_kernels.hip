@@ -15,9 +15,9 @@
 //                     halves of every lane (packed int16, v_pk_add_i16 clamp / v_pk_max_i16).
 //                     Stripes of 16*R rows run one after the other; the boundary row between
 //                     them goes through HBM.
-//   wavefront_kernel  long pairs.  A workgroup of 8 waves sweeps one task; every wave is one
+//   wavefront_kernel  long pairs.  A workgroup of 4 waves sweeps one task; every wave is one
 //                     64-lane group (wave_shr:1) owning a stripe of 64*R rows, and the waves
-//                     take the stripes round robin, so 8 stripes are in flight at once, each
+//                     take the stripes round robin, so 4 stripes are in flight at once, each
 //                     wave a fixed distance behind the one above.  The boundary row between
 //                     consecutive waves passes through an LDS ring buffer with progress
 //                     counters; from the last wave back to the first it goes through HBM.
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 // Wavefront kernel for long pairs.
 // ---------------------------------------------------------------------------------------------
 
-constexpr int kWfWaves = 8;     // waves per workgroup
+constexpr int kWfWaves = 4;     // waves per workgroup
 constexpr int kWfRing = 256;    // columns per LDS ring between consecutive waves
 // Bounded waits: after this many polls a wait gives up and flags the batch, so a protocol
 // error ends as a reported failure instead of a hang.
@@ -214,6 +214,7 @@ constexpr uint32_t kWfSpinLimit = 1u << 24;
 struct WfShared {
     uint32_t prod[kWfWaves];   // positions (round * N + column) published by wave w
     uint32_t cons[kWfWaves];   // positions consumed by wave w from its predecessor's ring
+    uint32_t failed;           // a wait gave up: later waits return at once
     uint32_t ring[kWfWaves - 1][kWfRing][2];   // (H, V) from wave w to wave w+1
 };
 
@@ -222,17 +223,25 @@ __device__ __forceinline__ uint32_t lds_load_acquire(uint32_t *p)
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ __forceinline__ void wait_at_least(uint32_t *p, uint32_t need, uint32_t *err)
+// Waits until *p >= need.  After kWfSpinLimit polls it gives up, flags the batch in *err and
+// marks the workgroup failed, so that every later wait of the workgroup returns at once and a
+// protocol error costs one time-out, not one per wait.
+__device__ __forceinline__ void wait_at_least(uint32_t *p, uint32_t need, uint32_t *err, uint32_t *failed)
 {
     uint32_t spins = 0;
     while (lds_load_acquire(p) < need) {
+        if (lds_load_acquire(failed)) return;
         __builtin_amdgcn_s_sleep(1);
-        if (++spins > kWfSpinLimit) { atomicOr(err, 1u); break; }
+        if (++spins > kWfSpinLimit) {
+            atomicOr(err, 1u);
+            __hip_atomic_store(failed, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return;
+        }
     }
 }
 
 template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
-__global__ __launch_bounds__(512) void wavefront_kernel(KernelArgs a)
+__global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
 {
     using L = Lane<W>;
     using V = typename L::V;
@@ -242,6 +251,7 @@ __global__ __launch_bounds__(512) void wavefront_kernel(KernelArgs a)
     __shared__ WfShared sh;
 
     if (threadIdx.x < kWfWaves) { sh.prod[threadIdx.x] = 0; sh.cons[threadIdx.x] = 0; }
+    if (threadIdx.x == 0) sh.failed = 0;
     if constexpr (SCORE == kScoreMatrix) load_lds_table(lds_raw, a.table, a.table_entries);
     __syncthreads();
 
@@ -279,7 +289,7 @@ __global__ __launch_bounds__(512) void wavefront_kernel(KernelArgs a)
 
         auto need_top = [&](int upto) {   // columns [0, upto) of the top boundary are published
             if (src == 0) return;
-            wait_at_least(src_prog, src_pos0 + uint32_t(upto < N ? upto : N), a.flags + a.error_slot);
+            wait_at_least(src_prog, src_pos0 + uint32_t(upto < N ? upto : N), a.flags + a.error_slot, &sh.failed);
             if (src == 2) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         };
         auto read_top = [&](int col, uint32_t &h, uint32_t &e) {
@@ -292,86 +302,105 @@ __global__ __launch_bounds__(512) void wavefront_kernel(KernelArgs a)
                 e = gbnd[2 * col + 1];
             }
         };
+        auto publish_consumed = [&](int upto) {
+            if (src == 1 && t == 0)
+                __hip_atomic_store(&sh.cons[wave], pos0 + uint32_t(upto < N ? upto : N), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
 
-        uint32_t convB = colp[t], nextB = colp[G + t];
+        // Conveyors as in the striped kernel, one column per lane: chunk j of the column-code
+        // conveyor holds columns jG+1 .. jG+G, the top-boundary conveyor columns jG .. jG+G-1.
+        uint32_t Bin = colp[t];
+        uint32_t conv = colp[1 + t];
+        uint32_t nextB = colp[G + 1 + t];
         uint32_t convH = 0, convE = 0, nextH = 0, nextE = 0;
         if (src != 0) {
             need_top(2 * G);
             if (t < N) read_top(t, convH, convE);
             if (G + t < N) read_top(G + t, nextH, nextE);
-            if (src == 1 && t == 0)
-                __hip_atomic_store(&sh.cons[wave], pos0 + uint32_t(2 * G < N ? 2 * G : N), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            publish_consumed(2 * G);
         }
-        uint32_t Hout = 0, Eout = 0, Bout = 0;
+        uint32_t Hout = 0, Eout = 0;
+        uint32_t qH = 0, qE = 0;   // bottom-row outputs of lane G-1 over one chunk
         int hb_top = h0_value(1, fr, go, ge);
 
-        const int nsteps = N + G - 1;
-        int c = -t;
-        for (int k = 0; k < nsteps; ++k, ++c) {
-            if (k > 0 && (k % G) == 0) {
-                convB = nextB;
-                nextB = colp[k + G + t];
+        ScoreReads<W, R, SCORE> rd;
+        issue_scores<W, R, SCORE>(st, Bin, lds_raw, rd);
+        V sc[R];
+        finish_scores<W, R, SCORE>(st, Bin, tbl_lo, tbl_hi, rd, sc);
+
+        const int nsteps = ((N + 2 * G - 2) / G) * G;
+        for (int k0 = 0; k0 < nsteps; k0 += G) {
+#pragma unroll 1
+            for (int u = 0; u < G; ++u) {
+                const int c = k0 + u - t;
+                const uint32_t Bnext = dpp<kWaveShr1>(conv, Bin);
+                conv = dpp<kWaveShl1>(conv, conv);
+                issue_scores<W, R, SCORE>(st, Bnext, lds_raw, rd);
+                uint32_t topH, topE;
                 if (src != 0) {
-                    convH = nextH; convE = nextE;
-                    need_top(k + 2 * G);
-                    if (k + G + t < N) read_top(k + G + t, nextH, nextE);
-                    if (src == 1 && t == 0)
-                        __hip_atomic_store(&sh.cons[wave], pos0 + uint32_t(k + 2 * G < N ? k + 2 * G : N),
-                                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    topH = convH;
+                    topE = convE;
+                } else {
+                    topH = L::top_word(hb_top);
+                    topE = L::top_word(hb_top + go + ge);
+                    hb_top = fr ? 0 : hb_top + ge;
                 }
+                const uint32_t Hin = dpp<kWaveShr1>(topH, Hout);
+                const uint32_t Ein = dpp<kWaveShr1>(topE, Eout);
+                if (src != 0) {
+                    convH = dpp<kWaveShl1>(convH, convH);
+                    convE = dpp<kWaveShl1>(convE, convE);
+                }
+                if (uint32_t(c) < uint32_t(N)) {
+                    V Eup = L::from(Ein);
+                    update_cells<W, R, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, sc, vgogo, vge);
+                    Hdiag = L::from(Hin);
+                    Hout = L::bits(st.H[R - 1]);
+                    Eout = L::bits(Eup);
+                    if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores);
+                }
+                if (dst != 0) {
+                    qH = dpp<kWaveShl1>(Hout, qH);
+                    qE = dpp<kWaveShl1>(Eout, qE);
+                }
+                finish_scores<W, R, SCORE>(st, Bnext, tbl_lo, tbl_hi, rd, sc);
+                Bin = Bnext;
             }
-            // the last lane starts a new chunk of output columns: make room in the ring
-            const int cout = k - (G - 1);
-            if (dst == 1 && cout >= 0 && cout < N && (cout % G) == 0) {
-                // every ring position below pos0 + cout + G - kWfRing, this round's or the
-                // previous round's, must have been consumed before [cout, cout + G) is written
-                const int64_t need = int64_t(pos0) + cout + G - kWfRing;
-                if (need > 0) wait_at_least(&sh.cons[wave + 1], uint32_t(need), a.flags + a.error_slot);
-            }
-            uint32_t topH, topE;
+            // end of chunk: refill the conveyors one chunk ahead
+            conv = nextB;
+            nextB = colp[k0 + 2 * G + 1 + t];
             if (src != 0) {
-                topH = convH;
-                topE = convE;
-            } else {
-                topH = L::top_word(hb_top);
-                topE = L::top_word(hb_top + go + ge);
-                hb_top = fr ? 0 : hb_top + ge;
+                convH = nextH;
+                convE = nextE;
+                need_top(k0 + 3 * G);
+                if (k0 + 2 * G + t < N) read_top(k0 + 2 * G + t, nextH, nextE);
+                publish_consumed(k0 + 3 * G);
             }
-            const uint32_t Bin = dpp<kWaveShr1>(convB, Bout);
-            const uint32_t Hin = dpp<kWaveShr1>(topH, Hout);
-            const uint32_t Ein = dpp<kWaveShr1>(topE, Eout);
-            convB = dpp<kWaveShl1>(convB, convB);
-            if (src != 0) {
-                convH = dpp<kWaveShl1>(convH, convH);
-                convE = dpp<kWaveShl1>(convE, convE);
-            }
-            if (uint32_t(c) < uint32_t(N)) {
-                V Eup = L::from(Ein);
-                cell_block<W, R, SCORE, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, Bin, tbl_lo, tbl_hi, lds_raw,
-                                                              vgogo, vge);
-                Hdiag = L::from(Hin);
-                Hout = L::bits(st.H[R - 1]);
-                Eout = L::bits(Eup);
-                Bout = Bin;
-                if (dst != 0 && t == G - 1) {
-                    if (dst == 1) {
-                        const uint32_t slot = (pos0 + uint32_t(c)) % kWfRing;
-                        sh.ring[wave][slot][0] = Hout;
-                        sh.ring[wave][slot][1] = Eout;
-                    } else {
-                        gbnd[2 * c] = Hout;
-                        gbnd[2 * c + 1] = Eout;
+            if (dst != 0) {
+                // lane j holds the bottom row of column k0 + j - (G-1)
+                const int col = k0 + t - (G - 1);
+                const int hi = k0 + 1 < N ? k0 + 1 : N;   // columns [0, hi) done after this chunk
+                if (dst == 1) {
+                    // the slots of [k0+1-G, k0+1) held positions kWfRing earlier: those must
+                    // have been consumed by the next wave
+                    const int64_t need = int64_t(pos0) + hi - kWfRing;
+                    if (need > 0) wait_at_least(&sh.cons[wave + 1], uint32_t(need), a.flags + a.error_slot, &sh.failed);
+                    if (col >= 0 && col < N) {
+                        const uint32_t slot = (pos0 + uint32_t(col)) % kWfRing;
+                        sh.ring[wave][slot][0] = qH;
+                        sh.ring[wave][slot][1] = qE;
                     }
+                } else if (col >= 0 && col < N) {
+                    gbnd[2 * col] = qH;
+                    gbnd[2 * col + 1] = qE;
                 }
-                if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores);
-            }
-            // publish every G output columns and at the end of the row
-            if (dst != 0 && cout >= 0 && cout < N && (((cout + 1) % G) == 0 || cout + 1 == N)) {
-                if (dst == 2) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                if (t == G - 1)
-                    __hip_atomic_store(&sh.prod[wave], pos0 + uint32_t(cout + 1), __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (hi > k0 + 1 - G) {   // this chunk produced columns: publish them
+                    if (dst == 2) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                    if (t == G - 1)
+                        __hip_atomic_store(&sh.prod[wave], pos0 + uint32_t(hi), __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
             }
         }
         if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores);
