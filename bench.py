@@ -11,6 +11,7 @@ one JSON object.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -103,13 +104,22 @@ def cpu_baseline(w, dc, gpu_scores, budget_s=12.0):
                 mismatches_vs_gpu=mism, sample_pairs=sample)
 
 
-def run(name, steps, warmup, with_cpu=True):
+def run(name, steps, warmup, with_cpu=True, autotune=False):
     t0 = time.perf_counter()
     w, dc, method = make_workload(name)
     n = len(w["o1"]) - 1
     cells = int(np.sum(np.diff(w["o1"]).astype(np.int64) * np.diff(w["o2"]).astype(np.int64)))
     log(f"[bench] {name}: {n} pairs, {cells:.3e} cells, generated in {time.perf_counter() - t0:.1f} s")
     ctx = device.default_context(0)
+    tuned = None
+    if autotune:
+        # the library's autotuner times candidate planner/launch parameters on a 5% sample
+        t1 = time.perf_counter()
+        tuning = device.tune(ctx, dc, w["b1"], w["o1"], w["b2"], w["o2"], w.get("pair_method"),
+                             sample_pairs=max(2000, n // 20), iters=2)
+        dc = dataclasses.replace(dc, tuning=tuning)
+        tuned = dict(tuning=dataclasses.asdict(tuning), seconds=round(time.perf_counter() - t1, 2))
+        log(f"[bench] {name}: autotuned {tuned}")
     batch = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
     st = batch.stats()
     log(f"[bench] plan: {st}")
@@ -132,7 +142,7 @@ def run(name, steps, warmup, with_cpu=True):
     achieved = ops * cells / (main_ms / 1e3) / 1e12 if main_ms > 0 else 0.0
     res = dict(name=name, pairs=n, cells=cells, ms_per_step=ms_step, main_kernel_ms=main_ms, wall_s=wall,
                gcups=gcups, achieved_tops=achieved, peak_tops=peak, ops_per_cell=ops, stats=st,
-               inputs_sha256=workloads.digest(w["b1"], w["o1"], w["b2"], w["o2"]))
+               inputs_sha256=workloads.digest(w["b1"], w["o1"], w["b2"], w["o2"]), autotuned=tuned)
     if with_cpu:
         res["cpu"] = cpu_baseline(w, dc, scores)
     log(f"[bench] {name}: {gcups:.1f} GCUPS ({ms_step:.3f} ms/step, main kernels {main_ms:.3f} ms), "
@@ -227,10 +237,12 @@ def main():
                           "data": "synthetic", "config": {"workload": f"mixed DNA/protein, {args.mixed_pairs} pairs"},
                           "mixed_learned_dispatch": res}))
         return
-    main_res = run(args.workload, args.steps, args.warmup)
-    extra = mixed = None
+    main_res = run(args.workload, args.steps, args.warmup, autotune=args.workload == "uneven")
+    extra = mixed = uneven = None
     if not args.no_extra and args.workload != "dna150":
         extra = run("dna150", max(args.steps, 50), args.warmup, with_cpu=False)
+    if not args.no_extra and args.workload != "uneven":
+        uneven = run("uneven", 5, 1, with_cpu=False, autotune=True)
     if not args.no_extra:
         mixed = run_mixed(args.mixed_pairs, 1, 1)
 
@@ -263,8 +275,16 @@ def main():
         "mismatches_vs_oracle_sample": cpu["mismatches_vs_gpu"],
         "plan": {k: main_res["stats"][k] for k in ("n_int16", "n_int32", "n_recomputed", "padded_cells", "n_launches")},
     }
+    if main_res.get("autotuned"):
+        line["autotuned"] = main_res["autotuned"]
     if mixed is not None:
         line["mixed_learned_dispatch"] = mixed
+    if uneven is not None:
+        line["uneven_1m_autotuned_graph"] = {
+            "gcups": round(uneven["gcups"], 2), "ms_per_step": round(uneven["ms_per_step"], 3),
+            "pairs": uneven["pairs"], "cells": uneven["cells"], "autotuned": uneven["autotuned"],
+            "roofline_frac": round(uneven["achieved_tops"] / uneven["peak_tops"], 4),
+            "n_launches": uneven["stats"]["n_launches"], "plan_ms": round(uneven["stats"]["plan_ms"], 1)}
     if extra is not None:
         line["dna150_global_int16"] = {"gcups": round(extra["gcups"], 2), "ms_per_step": round(extra["ms_per_step"], 4),
                                        "kernel_ms": round(extra["main_kernel_ms"], 4),

@@ -130,10 +130,12 @@ pa_status pa_batch_stats_get(const pa_batch *batch, pa_batch_stats *out);
 pa_status pa_batch_time(pa_ctx *ctx, pa_batch *batch, int iters, double *ms_total, double *ms_main);
 pa_status pa_batch_destroy(pa_ctx *ctx, pa_batch *batch);
 
-/* Autotuner.  No counterpart in the reference.  Plans and times candidate tunings on a sample
- * of the batch (every k-th pair, about sample_pairs pairs; 0 = 5% of the batch, at least 2,000)
- * by coordinate descent over the pa_tuning fields, and writes the fastest to *out.  `iters`
- * passes are timed per candidate (0 = 3). */
+/* Autotuner.  No counterpart in the reference.  Plans and times candidate tunings and writes
+ * the fastest to *out, in two stages of coordinate descent: the shape-model fields (step_cost,
+ * rows_mask) on a sample of the batch (every k-th pair, about sample_pairs pairs; 0 = 5% of the
+ * batch, at least 2,000), then the launch-level fields (wavefront_min_rows, streams, eager) and
+ * the stage-1 result against the defaults on the whole batch.  `iters` passes are timed per
+ * candidate (0 = 3). */
 pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
                   const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, const uint8_t *pair_method,
                   uint64_t sample_pairs, int iters, pa_tuning *out);

@@ -18,7 +18,7 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
     # no packed-FP32 instructions anywhere (the kernels are integer-only)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unknown-warning-option",
+    "-Wno-unknown-warning-option", "-pthread",
 ]
 
 

@@ -752,13 +752,14 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
         c.eager = t.eager ? 1 : 0;
         return c;
     };
-    // time of `iters` passes over the sample with tuning t (after one untimed pass)
-    auto measure = [&](const pa_tuning &t, double &ms) -> pa_status {
+    // time of `iters` passes with tuning t (after one untimed pass), on the sample or the batch
+    auto measure = [&](const pa_tuning &t, bool full, double &ms) -> pa_status {
         pa_config c = *cfg;
         c.tuning = &t;
         pa_batch *b = nullptr;
-        pa_status st = pa_batch_create(ctx, &c, s1.data(), o1.data(), s2.data(), o2.data(), ns,
-                                       pm.empty() ? nullptr : pm.data(), &b);
+        pa_status st = full ? pa_batch_create(ctx, &c, seq1, off1, seq2, off2, n_pairs, pair_method, &b)
+                            : pa_batch_create(ctx, &c, s1.data(), o1.data(), s2.data(), o2.data(), ns,
+                                              pm.empty() ? nullptr : pm.data(), &b);
         if (st != PA_OK) return st;
         st = pa_batch_run(ctx, b);
         double total = 0;
@@ -767,36 +768,58 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
         ms = total / iters;
         return st;
     };
+    pa_status st;
+
+    // Stage 1, on the sample: the shape model (tile height R per pair and its allowed values).
+    // These decide per-pair efficiency, which a sample with the batch's length mix shows.
     pa_tuning best = to_c(base);
     double best_ms = 0;
-    pa_status st = measure(best, best_ms);
-    if (st != PA_OK) return st;
-
-    // coordinate descent, two sweeps over the parameters
-    const uint32_t wf[] = {1024, 2048, 3072, 4096, 6144};
+    if ((st = measure(best, false, best_ms)) != PA_OK) return st;
     const float step[] = {8, 16, 24, 40, 64};
     const uint32_t rows[] = {0x7F, 0x55, 0x2A, 0x7E, 0x3F, 0x78, 0x1E};
-    const uint32_t streams[] = {1, 2, 3, 4};
-    const uint32_t eager[] = {0, 1};
     for (int sweep = 0; sweep < 2; ++sweep) {
-        for (int param = 0; param < 5; ++param) {
-            const int nv = param == 0 ? 5 : param == 1 ? 5 : param == 2 ? 7 : param == 3 ? 4 : 2;
+        for (int param = 0; param < 2; ++param) {
+            const int nv = param == 0 ? 5 : 7;
             for (int v = 0; v < nv; ++v) {
                 pa_tuning t = best;
-                switch (param) {
-                case 0: t.wavefront_min_rows = wf[v]; break;
-                case 1: t.step_cost = step[v]; break;
-                case 2: t.rows_mask = rows[v]; break;
-                case 3: t.streams = streams[v]; break;
-                default: t.eager = eager[v]; break;
-                }
+                if (param == 0) t.step_cost = step[v]; else t.rows_mask = rows[v];
                 if (std::memcmp(&t, &best, sizeof t) == 0) continue;
                 double ms = 0;
-                if ((st = measure(t, ms)) != PA_OK) return st;
+                if ((st = measure(t, false, ms)) != PA_OK) return st;
                 if (ms < best_ms * 0.98) { best = t; best_ms = ms; }
             }
         }
     }
+
+    // Stage 2, on the whole batch: the launch-level parameters, whose effect depends on the
+    // batch's total work against its longest tasks (the wavefront threshold, streams, graph
+    // replay), and the stage-1 choice against the defaults.
+    const pa_tuning start = to_c(base);
+    double full_best = 0;
+    if ((st = measure(start, true, full_best)) != PA_OK) return st;
+    pa_tuning full = start;
+    if (std::memcmp(&best, &start, sizeof best) != 0) {
+        double ms = 0;
+        if ((st = measure(best, true, ms)) != PA_OK) return st;
+        if (ms < full_best) { full = best; full_best = ms; }
+    }
+    const uint32_t wf[] = {2048, 3072, 4096};
+    const uint32_t streams[] = {2, 4};
+    const uint32_t eager[] = {0, 1};
+    for (int param = 0; param < 3; ++param) {
+        const int nv = param == 0 ? 3 : 2;
+        for (int v = 0; v < nv; ++v) {
+            pa_tuning t = full;
+            if (param == 0) t.wavefront_min_rows = wf[v];
+            else if (param == 1) t.streams = streams[v];
+            else t.eager = eager[v];
+            if (std::memcmp(&t, &full, sizeof t) == 0) continue;
+            double ms = 0;
+            if ((st = measure(t, true, ms)) != PA_OK) return st;
+            if (ms < full_best * 0.98) { full = t; full_best = ms; }
+        }
+    }
+    best = full;
     *out = best;
     return PA_OK;
 }

@@ -2,10 +2,12 @@
 #include "pa_plan.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <thread>
 #include <tuple>
 
 namespace pa {
@@ -159,6 +161,31 @@ struct Item {
     uint32_t S;
 };
 
+// Runs body(i) for i in [0, n) on up to 16 threads, in blocks of 256 indices.
+template <typename F>
+void parallel_for(size_t n, F &&body)
+{
+    const size_t block = 256;
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = unsigned(std::min<size_t>(hw, (n + block - 1) / block));
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; ++i) body(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            const size_t b = next.fetch_add(block);
+            if (b >= n) return;
+            for (size_t i = b; i < std::min(n, b + block); ++i) body(i);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned k = 1; k < nt; ++k) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+}
+
 } // namespace
 
 double task_cost(const VariantKey &key, const Task &t, const Tuning &tuning)
@@ -307,6 +334,13 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         return lds_col_offset(pad_rank);
     };
 
+    struct Fill {
+        const Item *a, *b;   // the pair(s) of the task
+        uint8_t width, G;
+        uint32_t rows;       // G * R
+    };
+    std::vector<Fill> fills;
+    uint64_t n_rowcodes = 0, n_colcodes = 0;
     for (auto &[key, items] : groups) {
         // Pack int16 pairs two per task: similar stripe count and column count side by side.
         std::sort(items.begin(), items.end(), [](const Item &a, const Item &c) {
@@ -348,63 +382,71 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 t.m[q] = h[q] ? h[q]->m : 0;
                 t.n[q] = h[q] ? h[q]->n : 0;
             }
-            t.row_off = uint32_t(plan.rowcodes.size());
-            t.col_off = uint32_t(plan.colcodes.size());
+            t.row_off = uint32_t(std::min<uint64_t>(n_rowcodes, 0xFFFFFFFFull));
+            t.col_off = uint32_t(std::min<uint64_t>(n_colcodes, 0xFFFFFFFFull));
             t.bnd_off = kNone;
             const uint32_t cw = col_words(t.N, G);
             if (t.S > 1) {
-                t.bnd_off = uint32_t(plan.bnd_entries);
+                t.bnd_off = uint32_t(std::min<uint64_t>(plan.bnd_entries, 0xFFFFFFFFull));
                 plan.bnd_entries += cw;
             }
-            // rows
-            const uint32_t nrows = t.S * rows;
-            const size_t r0 = plan.rowcodes.size();
-            plan.rowcodes.resize(r0 + nrows);
-            for (uint32_t i = 0; i < nrows; ++i) {
-                uint32_t w = 0;
-                for (int q = 0; q < (key.width == 16 ? 2 : 1); ++q) {
-                    uint32_t code;
-                    if (h[q] && i < h[q]->m) {
-                        const PairMeta &pm = plan.pairs[h[q]->pair];
-                        const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h[q]->pair] : b.seq1 + b.off1[h[q]->pair];
-                        code = row_code(s[i], key.width);
-                    } else {
-                        code = row_pad(key.width);
-                    }
-                    w |= key.width == 16 ? (code & 0xFFFFu) << (16 * q) : code;
-                }
-                plan.rowcodes[r0 + i] = w;
-            }
-            // columns
-            const size_t c0 = plan.colcodes.size();
-            plan.colcodes.resize(c0 + cw);
-            for (uint32_t j = 0; j < cw; ++j) {
-                uint32_t w = 0;
-                for (int q = 0; q < (key.width == 16 ? 2 : 1); ++q) {
-                    uint32_t code;
-                    if (h[q] && j < h[q]->n) {
-                        const PairMeta &pm = plan.pairs[h[q]->pair];
-                        const uint8_t *s = pm.swapped ? b.seq1 + b.off1[h[q]->pair] : b.seq2 + b.off2[h[q]->pair];
-                        code = col_code(s[j], key.width);
-                    } else {
-                        code = col_pad(key.width);
-                    }
-                    w |= key.width == 16 ? (code & 0xFFFFu) << (16 * q) : code;
-                }
-                plan.colcodes[c0 + j] = w;
-            }
+            n_rowcodes += uint64_t(t.S) * rows;
+            n_colcodes += cw;
             const uint64_t pc = uint64_t(t.S) * rows * t.N * (key.width == 16 ? 2 : 1);
             L.padded_cells += pc;
             plan.padded_cells += pc;
             plan.tasks.push_back(t);
+            fills.push_back(Fill{h[0], h[1], key.width, key.G, rows});
         }
         L.task_end = uint32_t(plan.tasks.size());
         if (L.task_end > L.task_begin) plan.launches.push_back(L);
     }
-    if (plan.rowcodes.size() > 0xFFFFFFF0ull || plan.colcodes.size() > 0xFFFFFFF0ull || plan.bnd_entries > 0x7FFFFFF0ull) {
+    if (n_rowcodes > 0xFFFFFFF0ull || n_colcodes > 0xFFFFFFF0ull || plan.bnd_entries > 0x7FFFFFF0ull) {
         err = "batch too large for one plan; split it";
         return PA_ERR_INVALID_ARGUMENT;
     }
+
+    // Residue streams of every task, encoded in parallel (tasks write disjoint ranges).
+    plan.rowcodes.resize(n_rowcodes);
+    plan.colcodes.resize(n_colcodes);
+    auto fill_task = [&](size_t ti) {
+        const Task &t = plan.tasks[ti];
+        const Fill &f = fills[ti];
+        const Item *h[2] = {f.a, f.b};
+        const uint32_t nrows = t.S * f.rows;
+        for (uint32_t i = 0; i < nrows; ++i) {
+            uint32_t w = 0;
+            for (int q = 0; q < (f.width == 16 ? 2 : 1); ++q) {
+                uint32_t code;
+                if (h[q] && i < h[q]->m) {
+                    const PairMeta &pm = plan.pairs[h[q]->pair];
+                    const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h[q]->pair] : b.seq1 + b.off1[h[q]->pair];
+                    code = row_code(s[i], f.width);
+                } else {
+                    code = row_pad(f.width);
+                }
+                w |= f.width == 16 ? (code & 0xFFFFu) << (16 * q) : code;
+            }
+            plan.rowcodes[size_t(t.row_off) + i] = w;
+        }
+        const uint32_t cw = col_words(t.N, f.G);
+        for (uint32_t j = 0; j < cw; ++j) {
+            uint32_t w = 0;
+            for (int q = 0; q < (f.width == 16 ? 2 : 1); ++q) {
+                uint32_t code;
+                if (h[q] && j < h[q]->n) {
+                    const PairMeta &pm = plan.pairs[h[q]->pair];
+                    const uint8_t *s = pm.swapped ? b.seq1 + b.off1[h[q]->pair] : b.seq2 + b.off2[h[q]->pair];
+                    code = col_code(s[j], f.width);
+                } else {
+                    code = col_pad(f.width);
+                }
+                w |= f.width == 16 ? (code & 0xFFFFu) << (16 * q) : code;
+            }
+            plan.colcodes[size_t(t.col_off) + j] = w;
+        }
+    };
+    parallel_for(plan.tasks.size(), fill_task);
     order_launches(plan);
     return PA_OK;
 }
