@@ -29,6 +29,21 @@ PEAK_INT16_TOPS = 256 * 4 * 16 * 2.4e9 * 2 / 1e12   # 78.6
 PEAK_INT32_TOPS = PEAK_INT16_TOPS / 2                # 39.3
 PEAK_HBM_GBS = 8000.0
 OPS_PER_CELL = {"global": 8, "local": 10}
+COPY_PROBE_BYTES = 256 << 20
+# HBM bytes per pass measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one run each),
+# scaled by the counters' reading for a plain copy of known size; see profiles/.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(name, digest):
+    """HBM bytes per pass for this workload, if profiles/ holds a measurement of these inputs."""
+    try:
+        t = json.load(open(TRAFFIC_FILE)).get(name)
+    except (OSError, ValueError):
+        return None
+    if not t or t.get("inputs_sha256") != digest:
+        return None
+    return t
 
 
 def log(msg):
@@ -220,9 +235,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="protein")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (scales the rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE counters for roofline.traffic)")
     ap.add_argument("--mixed-pairs", type=int, default=100_000,
                     help="pairs of the mixed workload used for the learned-dispatch comparison (config: 500,000)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.arange(COPY_PROBE_BYTES // 4, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copies": 3, "check": int(b[-1].item())}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (pairs are independent; multi-GPU sharding is not built); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -247,6 +274,7 @@ def main():
         mixed = run_mixed(args.mixed_pairs, 1, 1)
 
     cpu = main_res.pop("cpu")
+    traffic = measured_traffic(args.workload, main_res["inputs_sha256"])
     frac = main_res["achieved_tops"] / main_res["peak_tops"]
     line = {
         "metric": "GCUPS score-only affine DP on 1 GPU vs CPU ref; learned dispatch vs fixed int32",
@@ -269,7 +297,7 @@ def main():
                    "inputs_sha256": main_res["inputs_sha256"]},
         "roofline": {"bound": "valu", "achieved": round(main_res["achieved_tops"], 3),
                      "peak": round(main_res["peak_tops"], 2), "unit": "TOP/s", "frac": round(frac, 4),
-                     "traffic": None,
+                     "traffic": None if traffic is None else traffic["bytes_per_pass"],
                      "ops_per_cell": main_res["ops_per_cell"], "kernel_ms": round(main_res["main_kernel_ms"], 4)},
         "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "mismatches_vs_oracle_sample": cpu["mismatches_vs_gpu"],
