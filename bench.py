@@ -88,8 +88,10 @@ def oracle_cfg(dc):
 
 
 def cpu_baseline(w, dc, gpu_scores, budget_s=12.0):
-    """Times the CPU oracle (64-bit scalar Gotoh, C, one POSIX thread per core) on a bounded
-    sample of the same pairs (a prefix of the batch) and checks the GPU scores of that sample."""
+    """Times the CPU baseline -- oracle/simd_port.cpp, the reference's inter-sequence SIMD scheme
+    (one pair per lane, int16 lanes where the bound allows) restated for AVX-512BW / AVX2 -- on
+    the host cores, on a bounded prefix of the same pairs, and compares the GPU scores of that
+    prefix with it.  A one-thread rate on a smaller prefix is reported beside it."""
     from oracle import oracle as O
     threads = min(os.cpu_count() or 1, 16)
     b1, o1, b2, o2 = w["b1"], w["o1"], w["b2"], w["o2"]
@@ -97,26 +99,37 @@ def cpu_baseline(w, dc, gpu_scores, budget_s=12.0):
     oc = oracle_cfg(dc)
     pm_all = w.get("pair_method")
     cells_cum = np.cumsum(np.diff(o1).astype(np.int64) * np.diff(o2).astype(np.int64))
+    isa = {512: "AVX-512BW", 256: "AVX2"}.get(O.simd_isa(), "none")
 
-    def timed(sample):
+    def timed(sample, nthreads):
         pm = None if pm_all is None else pm_all[:sample]
         t0 = time.perf_counter()
-        want = O.score_packed(b1, o1[:sample + 1], b2, o2[:sample + 1], oc, pair_method=pm, threads=threads)
-        return want, time.perf_counter() - t0
+        got = O.simd_score_packed(b1, o1[:sample + 1], b2, o2[:sample + 1], oc, pair_method=pm, threads=nthreads)
+        return got, time.perf_counter() - t0
 
-    # calibrate on a prefix of about 2e8 cells (at least `threads` pairs), then size the sample
-    cal = int(min(n, max(threads, np.searchsorted(cells_cum, 2e8) + 1)))
-    want, dt = timed(cal)
-    rate = cells_cum[cal - 1] / max(dt, 1e-6)
-    sample = int(min(n, max(cal, np.searchsorted(cells_cum, rate * budget_s) + 1)))
-    if sample > cal:
-        want, dt = timed(sample)
+    def sized(target_s, nthreads):
+        # calibrate on a prefix of about 1e8 cells, then size the sample to target_s seconds
+        cal = int(min(n, max(64 * nthreads, np.searchsorted(cells_cum, 1e8) + 1)))
+        got, dt = timed(cal, nthreads)
+        rate = cells_cum[cal - 1] / max(dt, 1e-6)
+        sample = int(min(n, max(cal, np.searchsorted(cells_cum, rate * target_s) + 1)))
+        if sample > cal:
+            got, dt = timed(sample, nthreads)
+        else:
+            sample = cal
+        return got, dt, sample
+
+    got, dt, sample = sized(budget_s, threads)
     cells = int(cells_cum[sample - 1])
-    mism = int(np.count_nonzero(gpu_scores[:sample].astype(np.int64) != want))
+    _, dt1, sample1 = sized(budget_s / 4, 1)
+    cells1 = int(cells_cum[sample1 - 1])
+    mism = int(np.count_nonzero(gpu_scores[:sample].astype(np.int64) != got.astype(np.int64)))
     return dict(value=cells / dt / 1e9, unit="GCUPS", cores=threads, kind="port",
-                sample=f"first {sample} of {n} pairs ({cells} cells), {dt:.2f} s, scalar 64-bit oracle "
-                       f"(oracle/gotoh_oracle.c) on {threads} threads",
-                mismatches_vs_gpu=mism, sample_pairs=sample)
+                sample=f"first {sample} of {n} pairs ({cells} cells) in {dt:.2f} s; inter-sequence SIMD port "
+                       f"of the reference (oracle/simd_port.cpp, {isa}, int16 lanes where the bound allows) "
+                       f"on {threads} threads",
+                value_1core=cells1 / dt1 / 1e9, sample_1core=f"first {sample1} pairs ({cells1} cells) in {dt1:.2f} s",
+                isa=isa, mismatches_vs_gpu=mism, sample_pairs=sample)
 
 
 def run(name, steps, warmup, with_cpu=True, autotune=False):
@@ -299,7 +312,8 @@ def main():
                      "peak": round(main_res["peak_tops"], 2), "unit": "TOP/s", "frac": round(frac, 4),
                      "traffic": None if traffic is None else traffic["bytes_per_pass"],
                      "ops_per_cell": main_res["ops_per_cell"], "kernel_ms": round(main_res["main_kernel_ms"], 4)},
-        "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample", "value_1core",
+                                             "sample_1core", "isa")},
         "mismatches_vs_oracle_sample": cpu["mismatches_vs_gpu"],
         "plan": {k: main_res["stats"][k] for k in ("n_int16", "n_int32", "n_recomputed", "padded_cells", "n_launches")},
     }

@@ -118,6 +118,54 @@ def score_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_method=None, th
     return scores[:n]
 
 
+SIMD_LIB_PATH = os.path.join(_HERE, "libsimdport.so")
+
+
+@functools.lru_cache(maxsize=1)
+def _simd_lib():
+    if not os.path.exists(SIMD_LIB_PATH):
+        raise RuntimeError(f"{SIMD_LIB_PATH} missing: run `make -C oracle`")
+    L = ctypes.CDLL(SIMD_LIB_PATH)
+    L.pas_isa.restype = ctypes.c_int
+    L.pas_score_batch.argtypes = [ctypes.POINTER(_CConfig), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    L.pas_score_batch.restype = ctypes.c_int
+    return L
+
+
+def simd_isa() -> int:
+    """Vector ISA of the CPU baseline on this host: 512 (AVX-512BW), 256 (AVX2) or 0."""
+    return int(_simd_lib().pas_isa())
+
+
+def simd_score_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_method=None, threads: int = 1) -> np.ndarray:
+    """The CPU baseline (oracle/simd_port.cpp): inter-sequence SIMD lanes, int16 where the bulk's
+    bound allows, int32 otherwise.  Returns int32 scores; checked against score_packed in tests."""
+    n = len(off1) - 1
+    scores = np.zeros(max(n, 1), dtype=np.int32)
+    c = _CConfig(cfg.mode, int(cfg.first_column_free), int(cfg.first_row_free),
+                 int(cfg.last_column_free), int(cfg.last_row_free), cfg.gap_open, cfg.gap_extension,
+                 cfg.score_kind, cfg.match, cfg.mismatch, len(cfg.symbols), None, None)
+    mat = sym = None
+    if cfg.score_kind == MATRIX:
+        mat = (ctypes.c_int32 * len(cfg.matrix))(*cfg.matrix)
+        sym = (ctypes.c_uint8 * len(cfg.symbols))(*cfg.symbols)
+        c.matrix = mat
+        c.symbols = sym
+    pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
+    buf1 = np.ascontiguousarray(buf1, dtype=np.uint8)
+    buf2 = np.ascontiguousarray(buf2, dtype=np.uint8)
+    off1 = np.ascontiguousarray(off1, dtype=np.uint64)
+    off2 = np.ascontiguousarray(off2, dtype=np.uint64)
+    rc = _simd_lib().pas_score_batch(ctypes.byref(c), buf1.ctypes.data, off1.ctypes.data, buf2.ctypes.data,
+                                     off2.ctypes.data, n, None if pm is None else pm.ctypes.data, threads,
+                                     scores.ctypes.data)
+    if rc != 0:
+        raise ValueError({-1: "no AVX2/AVX-512 on this CPU", -2: "symbol outside the substitution matrix alphabet",
+                          -3: "score range beyond int32"}.get(rc, f"error {rc}"))
+    return scores[:n]
+
+
 def score_pairs(seqs1: Sequence[bytes], seqs2: Sequence[bytes], cfg: OracleConfig,
                 pair_method=None, threads: int = 1) -> np.ndarray:
     b1, o1 = pack(seqs1)

@@ -222,3 +222,56 @@ def test_planted_mistakes_are_caught(bug):
             caught = True
             break
     assert caught == (bug is not None)
+
+
+# ---------------------------------------------------------------------------------------------
+# The CPU baseline of bench.py (oracle/simd_port.cpp) against the scalar oracle.
+# ---------------------------------------------------------------------------------------------
+
+def _blosum_cfg(mode):
+    from pairwise_aligner_amd import tables
+    t = tables.blosum62_standard()
+    return O.OracleConfig(mode=mode, gap_open=-11, gap_extension=-1, score_kind=O.MATRIX,
+                          symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r))
+
+
+def test_simd_baseline_matches_oracle_all_end_gaps():
+    import itertools
+    from pairwise_aligner_amd import workloads
+    if O.simd_isa() == 0:
+        pytest.skip("no AVX2 / AVX-512 on this CPU")
+    rng = np.random.default_rng(71)
+    s1, s2 = workloads.dna_pairs(rng, 300, 0, 200, divergence=0.2)
+    b1, o1 = workloads.pack(s1)
+    b2, o2 = workloads.pack(s2)
+    for fl in itertools.product([False, True], repeat=4):
+        for mode in (O.GLOBAL, O.LOCAL):
+            cfg = O.OracleConfig(mode=mode, first_column_free=fl[0], first_row_free=fl[1], last_column_free=fl[2],
+                                 last_row_free=fl[3], gap_open=-10, gap_extension=-1, match=4, mismatch=-5)
+            want = O.score_packed(b1, o1, b2, o2, cfg, threads=4)
+            got = O.simd_score_packed(b1, o1, b2, o2, cfg, threads=4)
+            assert np.array_equal(want, got.astype(np.int64)), (fl, mode)
+
+
+def test_simd_baseline_matches_oracle_blosum_mixed_methods_and_int32_bulks():
+    from pairwise_aligner_amd import workloads
+    if O.simd_isa() == 0:
+        pytest.skip("no AVX2 / AVX-512 on this CPU")
+    rng = np.random.default_rng(73)
+    s1, s2 = workloads.protein_pairs(rng, 300, 0, 300)
+    b1, o1 = workloads.pack(s1)
+    b2, o2 = workloads.pack(s2)
+    pm = (np.arange(300) % 3 == 0).astype(np.uint8)
+    cfg = _blosum_cfg(O.GLOBAL)
+    want = O.score_packed(b1, o1, b2, o2, cfg, pair_method=pm, threads=4)
+    got = O.simd_score_packed(b1, o1, b2, o2, cfg, pair_method=pm, threads=4)
+    assert np.array_equal(want, got.astype(np.int64))
+    # bulks whose bound needs int32 lanes (scores beyond int16)
+    s1, s2 = workloads.dna_pairs(rng, 20, 9000, 10000, divergence=0.02)
+    b1, o1 = workloads.pack(s1)
+    b2, o2 = workloads.pack(s2)
+    cfg = O.OracleConfig(mode=O.GLOBAL, gap_open=-10, gap_extension=-1, match=4, mismatch=-5)
+    want = O.score_packed(b1, o1, b2, o2, cfg, threads=4)
+    got = O.simd_score_packed(b1, o1, b2, o2, cfg, threads=4)
+    assert np.array_equal(want, got.astype(np.int64))
+    assert want.max() > 32767
