@@ -43,6 +43,11 @@
 
 namespace pa {
 
+// Top-boundary value for columns past the end of a pair: low enough that no real cell is
+// raised by it, high enough that adding gap penalties cannot wrap in int32 lanes
+// (int16 lanes read it as two halves of -16384).
+constexpr uint32_t kLowWord = 0xC000C000u;
+
 // Group reduction of the running maximum over 16 lanes (one DPP row).
 template <int W>
 __device__ __forceinline__ typename Lane<W>::V row_reduce_max(typename Lane<W>::V v)
@@ -103,72 +108,70 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         uint32_t Bin = colp[t];   // step 0: lane 0 computes column 0
         uint32_t conv = colp[1 + t];
         uint32_t nextB = colp[G + 1 + t];
-        // Top-boundary conveyor: at step k lane 0 of convH/convE holds column k.
-        uint32_t convH = 0, convE = 0, nextH = 0, nextE = 0;
-        if (has_bnd_in) {
-            convH = bndp[2 * t]; convE = bndp[2 * t + 1];
-            nextH = bndp[2 * (G + t)]; nextE = bndp[2 * (G + t) + 1];
-        }
+        // Top-boundary conveyor: chunk j holds H and the vertical gap state above this stripe
+        // for columns jG .. jG+G-1: the initial row H(0, c+1) for the first stripe, the stored
+        // bottom row of the stripe above otherwise.  Columns past N (computed, never read, by
+        // the unmasked steps) get a low value that no real cell can be raised by.
+        auto top_chunk = [&](int col, uint32_t &h, uint32_t &e) {
+            if (has_bnd_in) {
+                h = col < N ? bndp[2 * col] : kLowWord;
+                e = col < N ? bndp[2 * col + 1] : kLowWord;
+            } else {
+                const int hv = h0_value(col + 1, fr, go, ge);
+                h = L::top_word(hv);
+                e = L::top_word(hv + go + ge);
+            }
+        };
+        uint32_t convH, convE, nextH, nextE;
+        top_chunk(t, convH, convE);
+        top_chunk(G + t, nextH, nextE);
         // outputs of this lane from the previous step (read by lane t+1)
         uint32_t Hout = 0, Eout = 0;
         // bottom-row outputs of lane G-1 over one chunk of G steps, shifted down one lane per
         // step: at the end of the chunk lane j holds the output of the chunk's step j
         uint32_t qH = 0, qE = 0;
-        int hb_top = h0_value(1, fr, go, ge);   // H(0, k+1) for the first stripe
 
         ScoreReads<W, R, SCORE> rd;
         issue_scores<W, R, SCORE>(st, Bin, lds_raw, rd);
         V sc[R];
         finish_scores<W, R, SCORE>(st, Bin, tbl_lo, tbl_hi, rd, sc);
 
-        // whole chunks of G steps; lane t is active for columns 0 <= k - t < N
-        const int nsteps = ((N + 2 * G - 2) / G) * G;
-        for (int k0 = 0; k0 < nsteps; k0 += G) {
-#pragma unroll 1
-            for (int u = 0; u < G; ++u) {
-                const int c = k0 + u - t;   // DP column (0-based) of this lane at this step
-                const uint32_t Bnext = dpp<kRowShr1>(conv, Bin);
-                conv = dpp<kRowShl1>(conv, conv);
-                issue_scores<W, R, SCORE>(st, Bnext, lds_raw, rd);
-                uint32_t topH, topE;
-                if (has_bnd_in) {
-                    topH = convH;
-                    topE = convE;
-                } else {
-                    // H(0, k+1) and the vertical gap state below it: both <= 0 because
-                    // go <= 0 and ge <= 0.
-                    topH = L::top_word(hb_top);
-                    topE = L::top_word(hb_top + go + ge);
-                    hb_top = fr ? 0 : hb_top + ge;
-                }
-                const uint32_t Hin = dpp<kRowShr1>(topH, Hout);
-                const uint32_t Ein = dpp<kRowShr1>(topE, Eout);
-                if (has_bnd_in) {
-                    convH = dpp<kRowShl1>(convH, convH);
-                    convE = dpp<kRowShl1>(convE, convE);
-                }
-                if (uint32_t(c) < uint32_t(N)) {
-                    V Eup = L::from(Ein);
-                    update_cells<W, R, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, sc, vgogo, vge);
-                    Hdiag = L::from(Hin);
-                    Hout = L::bits(st.H[R - 1]);
-                    Eout = L::bits(Eup);
-                    if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores);
-                }
-                if (has_bnd_out) {
-                    qH = dpp<kRowShl1>(Hout, qH);
-                    qE = dpp<kRowShl1>(Eout, qE);
-                }
-                finish_scores<W, R, SCORE>(st, Bnext, tbl_lo, tbl_hi, rd, sc);
-                Bin = Bnext;
+        // One step.  MASKED: lanes outside 0 <= k - t < N leave their state alone (needed in
+        // the first chunk, where lane t waits t steps, and by the ROWMAX/TRACK variants whose
+        // per-row maxima must not see pad columns).  Unmasked, lanes past column N-1 sweep pad
+        // columns: pad cells never exceed the real ones and never feed them.
+        auto step = [&]<bool MASKED>(int k) {
+            const int c = k - t;   // DP column (0-based) of this lane at this step
+            const uint32_t Bnext = dpp<kRowShr1>(conv, Bin);
+            conv = dpp<kRowShl1>(conv, conv);
+            issue_scores<W, R, SCORE>(st, Bnext, lds_raw, rd);
+            const uint32_t Hin = dpp<kRowShr1>(convH, Hout);
+            const uint32_t Ein = dpp<kRowShr1>(convE, Eout);
+            convH = dpp<kRowShl1>(convH, convH);
+            convE = dpp<kRowShl1>(convE, convE);
+            if (!MASKED || uint32_t(c) < uint32_t(N)) {
+                V Eup = L::from(Ein);
+                update_cells<W, R, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, sc, vgogo, vge);
+                Hdiag = L::from(Hin);
+                Hout = L::bits(st.H[R - 1]);
+                Eout = L::bits(Eup);
             }
+            if constexpr (!LOCAL) {
+                // capture events are rare: test the whole wave first
+                if (__builtin_expect(__builtin_amdgcn_ballot_w64(c == cap.ev) != 0, 0))
+                    cap.at_column(c, st, tk, a.scores);
+            }
+            qH = dpp<kRowShl1>(Hout, qH);
+            qE = dpp<kRowShl1>(Eout, qE);
+            finish_scores<W, R, SCORE>(st, Bnext, tbl_lo, tbl_hi, rd, sc);
+            Bin = Bnext;
+        };
+        auto chunk_end = [&](int k0) {
             conv = nextB;
             nextB = colp[k0 + 2 * G + 1 + t];
-            if (has_bnd_in) {
-                convH = nextH; convE = nextE;
-                nextH = bndp[2 * (k0 + 2 * G + t)];
-                nextE = bndp[2 * (k0 + 2 * G + t) + 1];
-            }
+            convH = nextH;
+            convE = nextE;
+            top_chunk(k0 + 2 * G + t, nextH, nextE);
             if (has_bnd_out) {
                 // lane G-1 computed column k0 + j - (G-1) at the chunk's step j
                 const int col = k0 + t - (G - 1);
@@ -177,6 +180,18 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                     bndp[2 * col + 1] = qE;
                 }
             }
+        };
+
+        // whole chunks of G steps; the first is masked (lanes start one step apart)
+        const int nsteps = ((N + 2 * G - 2) / G) * G;
+        constexpr bool kMaskAll = ROWMAX || TRACK;
+#pragma unroll 1
+        for (int u = 0; u < G; ++u) step.template operator()<true>(u);
+        chunk_end(0);
+        for (int k0 = G; k0 < nsteps; k0 += G) {
+#pragma unroll 1
+            for (int u = 0; u < G; ++u) step.template operator()<kMaskAll>(k0 + u);
+            chunk_end(k0);
         }
         if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores);
         if (has_bnd_out) {
