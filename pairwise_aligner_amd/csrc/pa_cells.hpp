@@ -72,19 +72,34 @@ __device__ __forceinline__ int h0_value(int i, int free_flag, int go, int ge)
     return (i == 0 || free_flag) ? 0 : go + ge * i;
 }
 
-// Register state of one lane: R consecutive DP rows of the current stripe.
-template <int W, int R, bool ROWMAX>
+// Row drift (global method).  Inside a lane, register row r (r = 0..R-1) keeps H and both gap
+// states shifted up by |ge|*(r+1):  x' = x - ge*(r+1).  The vertical gap state then needs one
+// max per row,
+//   V'(r+1) = max(V'(r), max(D', W')(r) + go)        (V(r+1) = max(V(r)+ge, H(r)+go+ge)),
+// the diagonal term gains |ge| (D' = H'(r-1) + s - ge, folded into the substitution scores by
+// the planner), and values leaving the lane are shifted back: H exact, V + |ge| (the drift of
+// the next lane's first row).  Per-row maxima are kept drifted and shifted back when read.  The
+// local method keeps exact values: its zero clamp would need a different bound per row.
+template <bool DRIFT>
+__device__ __forceinline__ int drift_of(int ge, int r)
+{
+    return DRIFT ? -ge * (r + 1) : 0;
+}
+
+// Register state of one lane: R consecutive DP rows of the current stripe.  ACC: per-row
+// running maxima (local / tracked running maximum, free last row).
+template <int W, int R, bool ACC>
 struct Rows {
     using V = typename Lane<W>::V;
     uint32_t A[R];        // residue codes (matrix: LDS byte offsets of the table rows)
     uint32_t Ahi[R];      // int16 matrix path: codes of the high half
     V H[R], F[R];
-    V rowacc[ROWMAX ? R : 1];
+    V rowacc[ACC ? R : 1];
 };
 
-// Loads the residue codes of rows row0+1 .. row0+R and sets H(i,0) and W(i,1).
-template <int W, int R, int SCORE, bool ROWMAX>
-__device__ __forceinline__ void init_rows(Rows<W, R, ROWMAX> &st, const uint32_t *codes, int row0, int fc, int go,
+// Loads the residue codes of rows row0+1 .. row0+R and sets H(i,0) and W(i,1) (drifted if DRIFT).
+template <int W, int R, int SCORE, bool ACC, bool DRIFT>
+__device__ __forceinline__ void init_rows(Rows<W, R, ACC> &st, const uint32_t *codes, int row0, int fc, int go,
                                           int ge, uint32_t lane_bank)
 {
     using L = Lane<W>;
@@ -102,9 +117,25 @@ __device__ __forceinline__ void init_rows(Rows<W, R, ROWMAX> &st, const uint32_t
             st.A[r] = code;
             st.Ahi[r] = 0;
         }
-        st.H[r] = L::splat(h0_value(row0 + r + 1, fc, go, ge));
+        st.H[r] = L::splat(h0_value(row0 + r + 1, fc, go, ge) + drift_of<DRIFT>(ge, r));
         st.F[r] = L::add(st.H[r], vgogo);
-        if constexpr (ROWMAX) st.rowacc[r] = L::lowest();
+        if constexpr (ACC) st.rowacc[r] = L::lowest();
+    }
+}
+
+// Folds the per-row maxima of a stripe into the exact running maximum `runmax` and the drifted
+// maximum `satmax` (an int16 saturation test: every drifted value is >= its exact value).
+template <int W, int R, bool ACC, bool DRIFT>
+__device__ __forceinline__ void fold_rows(const Rows<W, R, ACC> &st, typename Lane<W>::V &runmax,
+                                          typename Lane<W>::V &satmax, int ge)
+{
+    using L = Lane<W>;
+    if constexpr (ACC) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            runmax = L::vmax(runmax, L::add(st.rowacc[r], L::splat(-drift_of<DRIFT>(ge, r))));
+            satmax = L::vmax(satmax, st.rowacc[r]);
+        }
     }
 }
 
@@ -118,8 +149,8 @@ struct ScoreReads {
     uint32_t hi[SCORE == kScoreMatrix && W == 16 ? R : 1];
 };
 
-template <int W, int R, int SCORE, bool ROWMAX>
-__device__ __forceinline__ void issue_scores(const Rows<W, R, ROWMAX> &st, uint32_t Bin, const char *lds,
+template <int W, int R, int SCORE, bool ACC>
+__device__ __forceinline__ void issue_scores(const Rows<W, R, ACC> &st, uint32_t Bin, const char *lds,
                                              ScoreReads<W, R, SCORE> &rd)
 {
     if constexpr (SCORE == kScoreMatrix) {
@@ -137,8 +168,8 @@ __device__ __forceinline__ void issue_scores(const Rows<W, R, ROWMAX> &st, uint3
     }
 }
 
-template <int W, int R, int SCORE, bool ROWMAX>
-__device__ __forceinline__ void finish_scores(const Rows<W, R, ROWMAX> &st, uint32_t Bin, uint32_t tbl_lo,
+template <int W, int R, int SCORE, bool ACC>
+__device__ __forceinline__ void finish_scores(const Rows<W, R, ACC> &st, uint32_t Bin, uint32_t tbl_lo,
                                               uint32_t tbl_hi, const ScoreReads<W, R, SCORE> &rd,
                                               typename Lane<W>::V (&sc)[R])
 {
@@ -146,7 +177,8 @@ __device__ __forceinline__ void finish_scores(const Rows<W, R, ROWMAX> &st, uint
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if constexpr (SCORE == kScoreDna) {
-            sc[r] = L::from(__builtin_amdgcn_perm(tbl_hi, tbl_lo, st.A[r] ^ Bin));
+            // formed where it is used (dna_score): two VALU instructions, nothing to hide
+            (void)tbl_lo; (void)tbl_hi; (void)Bin;
         } else if constexpr (W == 16) {
             sc[r] = L::from((rd.hi[r] << 16) | rd.lo[r]);
         } else {
@@ -155,14 +187,17 @@ __device__ __forceinline__ void finish_scores(const Rows<W, R, ROWMAX> &st, uint
     }
 }
 
-// The R cells of DP column j for one lane's rows, given their substitution scores.  `Eup`
-// enters as the vertical gap state from the row above and leaves as the one below the last
-// row; `diag` is H of the row above at column j-1.
-template <int W, int R, bool LOCAL, bool TRACK, bool ROWMAX>
-__device__ __forceinline__ void update_cells(Rows<W, R, ROWMAX> &st, typename Lane<W>::V &Eup,
+// The R cells of DP column j for one lane's rows, given their substitution scores.  DRIFT: the
+// rows are drifted (see drift_of) and the scores already shifted by |ge|; `Eup` enters as the
+// vertical gap state of row 0 and leaves as that of row R.  `diag` is the exact H of the row above
+// at column j-1.  RUN: keep the exact running maximum in `runmax` (local method); per-row
+// maxima go to rowacc (ACC).
+template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT>
+__device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<W>::V &Eup,
                                              typename Lane<W>::V diag, typename Lane<W>::V &runmax,
-                                             const typename Lane<W>::V (&sc)[R], typename Lane<W>::V vgogo,
-                                             typename Lane<W>::V vge)
+                                             const typename Lane<W>::V (&sc)[R], uint32_t Bin, uint32_t tbl_lo,
+                                             uint32_t tbl_hi, typename Lane<W>::V vgo,
+                                             typename Lane<W>::V vgogo, typename Lane<W>::V vge)
 {
     using L = Lane<W>;
     using V = typename L::V;
@@ -171,47 +206,50 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ROWMAX> &st, typename La
     V D[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        D[r] = L::add(r == 0 ? diag : st.H[r - 1], sc[r]);
+        V s;
+        if constexpr (SCORE == kScoreDna) s = L::from(__builtin_amdgcn_perm(tbl_hi, tbl_lo, st.A[r] ^ Bin));
+        else s = sc[r];
+        D[r] = L::add(r == 0 ? diag : st.H[r - 1], s);
         if constexpr (LOCAL) D[r] = L::vmax(D[r], L::splat(0));
     }
     // With go <= 0 (so go+ge <= ge) the vertical gap state needs only max(D, W) of its own row:
     //   V(i+1,j) = max(V + ge, H + go + ge) = max(V + ge, max(D, W) + go + ge)
-    // because the dropped term V + go + ge never exceeds V + ge.  The row-to-row dependency
-    // chain is then one add and one max per row; H and W are updated off the chain.
+    // because the dropped term V + go + ge never exceeds V + ge; drifted, V' = max(V', DF' + go).
     V DF[R], DFo[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         DF[r] = L::vmax(D[r], st.F[r]);
-        DFo[r] = L::add(DF[r], vgogo);
+        DFo[r] = L::add(DF[r], DRIFT ? vgo : vgogo);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const V best = L::vmax(DF[r], Eup);
-        Eup = L::vmax(L::add(Eup, vge), DFo[r]);
+        Eup = DRIFT ? L::vmax(Eup, DFo[r]) : L::vmax(L::add(Eup, vge), DFo[r]);
         st.H[r] = best;
-        if constexpr (LOCAL || TRACK) runmax = L::vmax(runmax, best);
-        if constexpr (ROWMAX) st.rowacc[r] = L::vmax(st.rowacc[r], best);
+        if constexpr (LOCAL) runmax = L::vmax(runmax, best);
+        if constexpr (ACC) st.rowacc[r] = L::vmax(st.rowacc[r], best);
         st.F[r] = L::vmax(L::add(st.F[r], vge), L::add(best, vgogo));
     }
 }
 
 // Scores and cells of one step in one go (no read-ahead).
-template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
-__device__ __forceinline__ void cell_block(Rows<W, R, ROWMAX> &st, typename Lane<W>::V &Eup,
+template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT>
+__device__ __forceinline__ void cell_block(Rows<W, R, ACC> &st, typename Lane<W>::V &Eup,
                                            typename Lane<W>::V diag, typename Lane<W>::V &runmax, uint32_t Bin,
-                                           uint32_t tbl_lo, uint32_t tbl_hi, const char *lds,
+                                           uint32_t tbl_lo, uint32_t tbl_hi, const char *lds, typename Lane<W>::V vgo,
                                            typename Lane<W>::V vgogo, typename Lane<W>::V vge)
 {
     ScoreReads<W, R, SCORE> rd;
     issue_scores<W, R, SCORE>(st, Bin, lds, rd);
     typename Lane<W>::V sc[R];
     finish_scores<W, R, SCORE>(st, Bin, tbl_lo, tbl_hi, rd, sc);
-    update_cells<W, R, LOCAL, TRACK, ROWMAX>(st, Eup, diag, runmax, sc, vgogo, vge);
+    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT>(st, Eup, diag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo, vge);
 }
 
 // Score capture of one lane in one stripe (global method).  cap: this lane owns row m of half
 // q and reads H(m, n) at column n; lcap (free last column): this lane reads the maximum over its
-// real rows at column n.  Contributions meet in an int32 atomicMax on the pair's score.
+// real rows at column n.  Contributions meet in an int32 atomicMax on the pair's score.  Register
+// values drifted by -dge*(r+1) (dge = ge with the row drift, else 0) are shifted back here.
 template <int W, int R>
 struct Capture {
     int cap[2], capr[2], lcap[2], lvr[2];
@@ -241,8 +279,9 @@ struct Capture {
         ev = next(-1);
     }
 
-    template <bool ROWMAX>
-    __device__ __forceinline__ void at_column(int c, const Rows<W, R, ROWMAX> &st, const Task &tk, int32_t *scores)
+    template <bool ACC>
+    __device__ __forceinline__ void at_column(int c, const Rows<W, R, ACC> &st, const Task &tk, int32_t *scores,
+                                              int dge)
     {
         if (c != ev) return;
 #pragma unroll
@@ -251,13 +290,13 @@ struct Capture {
                 int v = 0;
 #pragma unroll
                 for (int r = 0; r < R; ++r) if (r == capr[q]) v = Lane<W>::half(st.H[r], q);
-                atomicMax(scores + tk.pair[q], v);
+                atomicMax(scores + tk.pair[q], v + dge * (capr[q] + 1));
             }
             if (c == lcap[q]) {
                 int v = INT32_MIN;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    if (r < lvr[q]) { const int h = Lane<W>::half(st.H[r], q); v = h > v ? h : v; }
+                    if (r < lvr[q]) { const int h = Lane<W>::half(st.H[r], q) + dge * (r + 1); v = h > v ? h : v; }
                 atomicMax(scores + tk.pair[q], v);
             }
         }
@@ -265,17 +304,18 @@ struct Capture {
     }
 
     // free last row: the owner of row m contributes the maximum of that row
-    template <bool ROWMAX>
-    __device__ __forceinline__ void row_maximum(const Rows<W, R, ROWMAX> &st, const Task &tk, int32_t *scores) const
+    template <bool ACC>
+    __device__ __forceinline__ void row_maximum(const Rows<W, R, ACC> &st, const Task &tk, int32_t *scores,
+                                                int dge) const
     {
-        if constexpr (ROWMAX) {
+        if constexpr (ACC) {
 #pragma unroll
             for (int q = 0; q < Lane<W>::NH; ++q) {
                 if (cap[q] < 0) continue;
                 int v = 0;
 #pragma unroll
                 for (int r = 0; r < R; ++r) if (r == capr[q]) v = Lane<W>::half(st.rowacc[r], q);
-                atomicMax(scores + tk.pair[q], v);
+                atomicMax(scores + tk.pair[q], v + dge * (capr[q] + 1));
             }
         }
     }

@@ -81,17 +81,22 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     const int go = a.go, ge = a.ge;
     // method_local.hpp:41-49: a local alignment has every end gap free
     const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
-    const V vgogo = L::splat(go + ge), vge = L::splat(ge);
+    const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
+    const V vout = L::splat(ge * R);   // shifts the last row's drift back when H, V leave the lane
     const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
     const int N = int(tk.N);
     constexpr int ROWS = G * R;
+    constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
+    constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
+    const int dge = DRIFT ? ge : 0;
 
-    V runmax = LOCAL ? L::splat(0) : L::lowest();
+    V runmax = LOCAL ? L::splat(0) : L::lowest();   // exact
+    V satmax = L::lowest();                         // drifted (int16 saturation test)
 
     for (uint32_t s = 0; s < tk.S; ++s) {
         const int row0 = int(s) * ROWS + t * R;   // DP row above this lane's block
-        Rows<W, R, ROWMAX> st;
-        init_rows<W, R, SCORE, ROWMAX>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank);
+        Rows<W, R, ACC> st;
+        init_rows<W, R, SCORE, ACC, DRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank);
         V Hdiag = L::splat(h0_value(row0, fc, go, ge));
         Capture<W, R> cap;
         if constexpr (!LOCAL) cap.init(tk, row0, lc);
@@ -117,9 +122,10 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 h = col < N ? bndp[2 * col] : kLowWord;
                 e = col < N ? bndp[2 * col + 1] : kLowWord;
             } else {
+                // H(0, col+1) exact, the vertical gap state below it (drifted by |ge| for row 0)
                 const int hv = h0_value(col + 1, fr, go, ge);
                 h = L::top_word(hv);
-                e = L::top_word(hv + go + ge);
+                e = L::top_word(hv + go + ge - dge);
             }
         };
         uint32_t convH, convE, nextH, nextE;
@@ -151,15 +157,15 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             convE = dpp<kRowShl1>(convE, convE);
             if (!MASKED || uint32_t(c) < uint32_t(N)) {
                 V Eup = L::from(Ein);
-                update_cells<W, R, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, sc, vgogo, vge);
+                update_cells<W, R, SCORE, LOCAL, ACC, DRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo, vge);
                 Hdiag = L::from(Hin);
-                Hout = L::bits(st.H[R - 1]);
-                Eout = L::bits(Eup);
+                Hout = L::bits(DRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
+                Eout = L::bits(DRIFT ? L::add(Eup, vout) : Eup);
             }
             if constexpr (!LOCAL) {
                 // capture events are rare: test the whole wave first
                 if (__builtin_expect(__builtin_amdgcn_ballot_w64(c == cap.ev) != 0, 0))
-                    cap.at_column(c, st, tk, a.scores);
+                    cap.at_column(c, st, tk, a.scores, dge);
             }
             qH = dpp<kRowShl1>(Hout, qH);
             qE = dpp<kRowShl1>(Eout, qE);
@@ -193,7 +199,8 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             for (int u = 0; u < G; ++u) step.template operator()<kMaskAll>(k0 + u);
             chunk_end(k0);
         }
-        if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores);
+        if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores, dge);
+        fold_rows<W, R, ACC, DRIFT>(st, runmax, satmax, ge);
         if (has_bnd_out) {
             // the boundary row written by lane G-1 is read by lane 0 of the next stripe
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -203,13 +210,14 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 
     if constexpr (LOCAL || TRACK) {
         const V mv = row_reduce_max<W>(runmax);
+        const V sv = row_reduce_max<W>(LOCAL ? runmax : satmax);
         if (t == 0) {
 #pragma unroll
             for (int q = 0; q < L::NH; ++q) {
                 if (tk.pair[q] == kNone) continue;
                 const int v = L::half(mv, q);
                 if constexpr (LOCAL) a.scores[tk.pair[q]] = v;
-                if (W == 16 && v >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
+                if (W == 16 && L::half(sv, q) >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
                 if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
             }
         }
@@ -276,21 +284,26 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     const uint32_t lane_bank = (threadIdx.x & 31u) * 4u;
     const int go = a.go, ge = a.ge;
     const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
-    const V vgogo = L::splat(go + ge), vge = L::splat(ge);
+    const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
+    const V vout = L::splat(ge * R);   // shifts the last row's drift back when H, V leave the lane
     const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
     const int N = int(tk.N);
     const int S = int(tk.S);
+    constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
+    constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
+    const int dge = DRIFT ? ge : 0;
     const uint32_t *colp = a.colcodes + tk.col_off;
     uint32_t *gbnd = a.bnd + 2ull * tk.bnd_off;   // last wave -> first wave, one full row
 
-    V runmax = LOCAL ? L::splat(0) : L::lowest();
+    V runmax = LOCAL ? L::splat(0) : L::lowest();   // exact
+    V satmax = L::lowest();                         // drifted (int16 saturation test)
 
     for (int s = wave; s < S; s += kWfWaves) {
         const int round = s / kWfWaves;
         const uint32_t pos0 = uint32_t(round) * uint32_t(N);
         const int row0 = s * ROWS + t * R;
-        Rows<W, R, ROWMAX> st;
-        init_rows<W, R, SCORE, ROWMAX>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank);
+        Rows<W, R, ACC> st;
+        init_rows<W, R, SCORE, ACC, DRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank);
         V Hdiag = L::splat(h0_value(row0, fc, go, ge));
         Capture<W, R> cap;
         if constexpr (!LOCAL) cap.init(tk, row0, lc);
@@ -358,7 +371,7 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                     topE = convE;
                 } else {
                     topH = L::top_word(hb_top);
-                    topE = L::top_word(hb_top + go + ge);
+                    topE = L::top_word(hb_top + go + ge - dge);   // drifted by |ge| for row 0
                     hb_top = fr ? 0 : hb_top + ge;
                 }
                 const uint32_t Hin = dpp<kWaveShr1>(topH, Hout);
@@ -369,11 +382,11 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                 }
                 if (uint32_t(c) < uint32_t(N)) {
                     V Eup = L::from(Ein);
-                    update_cells<W, R, LOCAL, TRACK, ROWMAX>(st, Eup, Hdiag, runmax, sc, vgogo, vge);
+                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo, vge);
                     Hdiag = L::from(Hin);
-                    Hout = L::bits(st.H[R - 1]);
-                    Eout = L::bits(Eup);
-                    if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores);
+                    Hout = L::bits(DRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
+                    Eout = L::bits(DRIFT ? L::add(Eup, vout) : Eup);
+                    if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores, dge);
                 }
                 if (dst != 0) {
                     qH = dpp<kWaveShl1>(Hout, qH);
@@ -418,18 +431,20 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                 }
             }
         }
-        if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores);
+        if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores, dge);
+        fold_rows<W, R, ACC, DRIFT>(st, runmax, satmax, ge);
     }
 
     if constexpr (LOCAL || TRACK) {
         const V mv = row_reduce_max<W>(runmax);
+        const V sv = row_reduce_max<W>(LOCAL ? runmax : satmax);
         if ((t & 15) == 0) {
 #pragma unroll
             for (int q = 0; q < L::NH; ++q) {
                 if (tk.pair[q] == kNone) continue;
                 const int v = L::half(mv, q);
                 if constexpr (LOCAL) atomicMax(a.scores + tk.pair[q], v);
-                if (W == 16 && v >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
+                if (W == 16 && L::half(sv, q) >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
                 if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
             }
         }

@@ -64,7 +64,8 @@ RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n)
     const int64_t go = std::min<int64_t>(plan.go, 0), ge = std::min<int64_t>(plan.ge, 0);
     const int64_t sp = std::max<int64_t>(plan.smax, 0);
     RangeBound b;
-    b.upper = sp * int64_t(std::min(m, n));
+    // + the largest row drift the kernels add (|ge| * (R + 1), R <= 16)
+    b.upper = sp * int64_t(std::min(m, n)) - ge * 17;
     if (local)
         b.lower = go + ge;                       // H >= 0, gap states >= H + go + ge
     else
@@ -97,12 +98,21 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
         }
         plan.smax = std::max(cfg.match, cfg.mismatch);
         plan.smin = std::min(cfg.match, cfg.mismatch);
+        // global-method kernels add the row drift -ge to every substitution score (pa_cells.hpp)
+        const int32_t dm = cfg.match - cfg.gap_extension, dx = cfg.mismatch - cfg.gap_extension;
+        if (dm > 32767 || dx > 32767) {
+            err = "match/mismatch minus gap_extension must fit in 16 bits";
+            return PA_ERR_UNSUPPORTED;
+        }
         if (k <= 4) {
             alpha.dna = true;
             plan.score = kScoreDna;
             const uint32_t M = uint32_t(cfg.match) & 0xFFFFu, X = uint32_t(cfg.mismatch) & 0xFFFFu;
             plan.tbl_lo = M | (X << 16);
             plan.tbl_hi = X | (X << 16);
+            const uint32_t Md = uint32_t(dm) & 0xFFFFu, Xd = uint32_t(dx) & 0xFFFFu;
+            plan.tbl_lo_d = Md | (Xd << 16);
+            plan.tbl_hi_d = Xd | (Xd << 16);
             return PA_OK;
         }
         if (k > kMaxMatrixSymbols) {
@@ -113,9 +123,12 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
         plan.alphabet = k + 1;
         plan.stride = (plan.alphabet + 1) & ~1u;
         plan.table.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
+        plan.table_d = plan.table;
         for (uint32_t a = 0; a < k; ++a)
-            for (uint32_t c = 0; c < k; ++c)
+            for (uint32_t c = 0; c < k; ++c) {
                 plan.table[a * plan.stride + c] = int16_t(a == c ? cfg.match : cfg.mismatch);
+                plan.table_d[a * plan.stride + c] = int16_t(a == c ? dm : dx);
+            }
         return PA_OK;
     }
     if (cfg.score_kind != PA_SCORE_MATRIX || cfg.matrix == nullptr || cfg.matrix_symbols == nullptr) {
@@ -133,16 +146,19 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
     plan.alphabet = uint32_t(K) + 1;
     plan.stride = (plan.alphabet + 1) & ~1u;
     plan.table.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
+    plan.table_d = plan.table;
     plan.smax = INT32_MIN;
     plan.smin = INT32_MAX;
     for (int32_t a = 0; a < K; ++a)
         for (int32_t c = 0; c < K; ++c) {
             int32_t v = cfg.matrix[a * K + c];
-            if (v < -32768 || v > 32767) {
-                err = "substitution scores must fit in 16 bits";
+            if (v < -32768 || v - cfg.gap_extension > 32767) {
+                err = "substitution scores (minus gap_extension) must fit in 16 bits";
                 return PA_ERR_UNSUPPORTED;
             }
             plan.table[a * plan.stride + c] = int16_t(v);
+            // global-method kernels add the row drift -ge to every substitution score (pa_cells.hpp)
+            plan.table_d[a * plan.stride + c] = int16_t(v - cfg.gap_extension);
             plan.smax = std::max(plan.smax, v);
             plan.smin = std::min(plan.smin, v);
         }

@@ -113,8 +113,10 @@ struct Plan {
     int32_t fc = 0, fr = 0, lc = 0, lr = 0;   // after orientation
     uint8_t score = kScoreDna;
     uint32_t tbl_lo = 0, tbl_hi = 0;          // DNA byte-permute table
+    uint32_t tbl_lo_d = 0, tbl_hi_d = 0;      // the same, scores shifted by -ge (global method)
     uint32_t stride = 0, alphabet = 0;        // matrix table geometry (alphabet includes the pad)
     std::vector<int16_t> table;               // matrix: alphabet*stride entries
+    std::vector<int16_t> table_d;             // the same, scores shifted by -ge (global method)
     int32_t smax = 0, smin = 0;               // extreme substitution scores
 
     std::vector<PairMeta> pairs;
