@@ -28,7 +28,7 @@ struct DeviceBuffer {
 
 struct PlanDevice {
     pa::Plan plan;
-    DeviceBuffer tasks, rowcodes, colcodes, bnd, table, table_d, subset;
+    DeviceBuffer tasks, rowcodes, colcodes, bnd, table, table_d, ptable, ptable_d, subset;
     std::vector<uint32_t> subset_host;
 };
 
@@ -117,6 +117,8 @@ pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd)
     if ((st = upload(ctx, pd.colcodes, pd.plan.colcodes)) != PA_OK) return st;
     if ((st = upload(ctx, pd.table, pd.plan.table)) != PA_OK) return st;
     if ((st = upload(ctx, pd.table_d, pd.plan.table_d)) != PA_OK) return st;
+    if ((st = upload(ctx, pd.ptable, pd.plan.ptable)) != PA_OK) return st;
+    if ((st = upload(ctx, pd.ptable_d, pd.plan.ptable_d)) != PA_OK) return st;
     const size_t bnd_bytes = std::max<size_t>(pd.plan.bnd_entries * 8, 16);
     PA_HIP(hipMalloc(&pd.bnd.ptr, bnd_bytes), "hipMalloc");
     pd.bnd.bytes = bnd_bytes;
@@ -139,8 +141,13 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     a.error_slot = uint32_t(b->n_pairs);
     // global-method variants read the scores shifted by -ge (row drift, pa_cells.hpp)
     const bool drift = !L.key.local;
-    a.table = static_cast<const int16_t *>(drift ? pd.table_d.ptr : pd.table.ptr);
-    a.table_entries = uint32_t(p.table.size());
+    if (L.key.score == pa::kScorePairs) {
+        a.table = static_cast<const int16_t *>(drift ? pd.ptable_d.ptr : pd.ptable.ptr);
+        a.table_entries = uint32_t(p.ptable.size());   // dwords
+    } else {
+        a.table = static_cast<const int16_t *>(drift ? pd.table_d.ptr : pd.table.ptr);
+        a.table_entries = uint32_t(p.table.size());
+    }
     a.go = p.go;
     a.ge = p.ge;
     a.tbl_lo = drift ? p.tbl_lo_d : p.tbl_lo;
@@ -162,10 +169,10 @@ pa_status launch_plan(pa_ctx *ctx, pa_batch *b, PlanDevice &pd)
         PA_HIP(hipEventRecord(ctx->fork, ctx->stream), "hipEventRecord");
         for (int i = 0; i < nside; ++i) PA_HIP(hipStreamWaitEvent(ctx->side[i], ctx->fork, 0), "hipStreamWaitEvent");
     }
-    const uint32_t lds = p.table.empty() ? 0 : pa::lds_table_bytes(p.alphabet, p.stride);
     for (uint32_t li : p.launch_order) {
         const pa::Launch &L = p.launches[li];
         const pa::KernelArgs a = kernel_args(pd, b, L);
+        const uint32_t lds = pa::lds_bytes(p, L.key);
         hipStream_t st = fork ? ctx->side[p.launch_stream[li]] : ctx->stream;
         hipError_t e = pa::launch_stripe(L.key, a, lds, st);
         if (e != hipSuccess) return hip_fail(ctx, e, "striped DP launch");
@@ -697,7 +704,7 @@ pa_status pa_dp_labels(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, c
     for (const pa::Launch &L : b.main.plan.launches) {
         pa::KernelArgs a = kernel_args(b.main, &b, L);
         a.maxh = static_cast<int32_t *>(dmax.ptr);
-        const uint32_t lds = b.main.plan.table.empty() ? 0 : pa::lds_table_bytes(b.main.plan.alphabet, b.main.plan.stride);
+        const uint32_t lds = pa::lds_bytes(b.main.plan, L.key);
         PA_HIP(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
         e = pa::launch_stripe(L.key, a, lds, ctx->stream);
         if (e != hipSuccess) return hip_fail(ctx, e, "striped DP launch");

@@ -106,16 +106,22 @@ __device__ __forceinline__ void init_rows(Rows<W, R, ACC> &st, const uint32_t *c
     const typename L::V vgogo = L::splat(go + ge);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const uint32_t code = codes[row0 + r];
-        if constexpr (SCORE == kScoreMatrix && W == 16) {
-            st.A[r] = (code & 0xFFFFu) + lane_bank;
-            st.Ahi[r] = (code >> 16) + lane_bank;
-        } else if constexpr (SCORE == kScoreMatrix) {
-            st.A[r] = code + lane_bank;
+        if constexpr (SCORE == kScorePairs) {
+            // one word per two rows: the pair-row table offsets of both halves
+            if (r % 2 == 0) st.A[r / 2] = codes[row0 / 2 + r / 2];
             st.Ahi[r] = 0;
         } else {
-            st.A[r] = code;
-            st.Ahi[r] = 0;
+            const uint32_t code = codes[row0 + r];
+            if constexpr (SCORE == kScoreMatrix && W == 16) {
+                st.A[r] = (code & 0xFFFFu) + lane_bank;
+                st.Ahi[r] = (code >> 16) + lane_bank;
+            } else if constexpr (SCORE == kScoreMatrix) {
+                st.A[r] = code + lane_bank;
+                st.Ahi[r] = 0;
+            } else {
+                st.A[r] = code;
+                st.Ahi[r] = 0;
+            }
         }
         st.H[r] = L::splat(h0_value(row0 + r + 1, fc, go, ge) + drift_of<DRIFT>(ge, r));
         st.F[r] = L::add(st.H[r], vgogo);
@@ -145,15 +151,22 @@ __device__ __forceinline__ void fold_rows(const Rows<W, R, ACC> &st, typename La
 //           split in two so the reads of step k+1 can be issued before the cells of step k.
 template <int W, int R, int SCORE>
 struct ScoreReads {
-    uint32_t lo[SCORE == kScoreMatrix ? R : 1];
-    uint32_t hi[SCORE == kScoreMatrix && W == 16 ? R : 1];
+    uint32_t lo[SCORE == kScoreMatrix ? R : (SCORE == kScorePairs ? R / 2 : 1)];
+    uint32_t hi[SCORE == kScoreMatrix && W == 16 ? R : (SCORE == kScorePairs ? R / 2 : 1)];
 };
 
 template <int W, int R, int SCORE, bool ACC>
 __device__ __forceinline__ void issue_scores(const Rows<W, R, ACC> &st, uint32_t Bin, const char *lds,
                                              ScoreReads<W, R, SCORE> &rd)
 {
-    if constexpr (SCORE == kScoreMatrix) {
+    if constexpr (SCORE == kScorePairs) {
+        // rows 2p and 2p+1 of each half in one dword read
+#pragma unroll
+        for (int p = 0; p < R / 2; ++p) {
+            rd.lo[p] = *reinterpret_cast<const uint32_t *>(lds + (st.A[p] & 0xFFFFu) + (Bin & 0xFFFFu));
+            rd.hi[p] = *reinterpret_cast<const uint32_t *>(lds + (st.A[p] >> 16) + (Bin >> 16));
+        }
+    } else if constexpr (SCORE == kScoreMatrix) {
         uint32_t Blo = Bin, Bhi = 0;
         if constexpr (W == 16) { Blo = Bin & 0xFFFFu; Bhi = Bin >> 16; }
 #pragma unroll
@@ -177,8 +190,11 @@ __device__ __forceinline__ void finish_scores(const Rows<W, R, ACC> &st, uint32_
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if constexpr (SCORE == kScoreDna) {
-            // formed where it is used (dna_score): two VALU instructions, nothing to hide
+            // formed where it is used (update_cells): two VALU instructions, nothing to hide
             (void)tbl_lo; (void)tbl_hi; (void)Bin;
+        } else if constexpr (SCORE == kScorePairs) {
+            // (low halves) = row 2p of both pairs, (high halves) = row 2p+1
+            sc[r] = L::from(__builtin_amdgcn_perm(rd.hi[r / 2], rd.lo[r / 2], (r % 2) ? 0x07060302u : 0x05040100u));
         } else if constexpr (W == 16) {
             sc[r] = L::from((rd.hi[r] << 16) | rd.lo[r]);
         } else {
@@ -320,6 +336,13 @@ struct Capture {
         }
     }
 };
+
+// Copies the pair-row table (`entries` dwords) into LDS.
+__device__ __forceinline__ void load_lds_pairs(char *lds_raw, const uint32_t *table, uint32_t entries)
+{
+    uint32_t *lds32 = reinterpret_cast<uint32_t *>(lds_raw);
+    for (uint32_t i = threadIdx.x; i < entries; i += blockDim.x) lds32[i] = table[i];
+}
 
 // Copies the substitution table into LDS as 32 bank-swizzled replicas: entry e for lane l at
 // dword (e >> 1) * 32 + l % 32, half e & 1 (see pa_plan.hpp).

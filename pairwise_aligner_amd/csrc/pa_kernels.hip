@@ -71,6 +71,9 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     if constexpr (SCORE == kScoreMatrix) {
         load_lds_table(lds_raw, a.table, a.table_entries);
         __syncthreads();
+    } else if constexpr (SCORE == kScorePairs) {
+        load_lds_pairs(lds_raw, reinterpret_cast<const uint32_t *>(a.table), a.table_entries);
+        __syncthreads();
     }
 
     const uint32_t gid = (blockIdx.x * blockDim.x + threadIdx.x) / G;
@@ -276,6 +279,8 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     if (threadIdx.x < kWfWaves) { sh.prod[threadIdx.x] = 0; sh.cons[threadIdx.x] = 0; }
     if (threadIdx.x == 0) sh.failed = 0;
     if constexpr (SCORE == kScoreMatrix) load_lds_table(lds_raw, a.table, a.table_entries);
+    if constexpr (SCORE == kScorePairs)
+        load_lds_pairs(lds_raw, reinterpret_cast<const uint32_t *>(a.table), a.table_entries);
     __syncthreads();
 
     const Task tk = a.tasks[blockIdx.x];
@@ -491,7 +496,7 @@ __global__ void init_scores_kernel(InitArgs a)
 template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
 static hipError_t launch_variant(const VariantKey &k, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st)
 {
-    const uint32_t lds = SCORE == kScoreMatrix ? lds_bytes : 0;
+    const uint32_t lds = SCORE != kScoreDna ? lds_bytes : 0;
     if (k.G == 64) {
         hipLaunchKernelGGL((wavefront_kernel<W, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(a.ntasks),
                            dim3(64 * kWfWaves), lds, st, a);
@@ -523,6 +528,11 @@ static hipError_t dispatch_score(const VariantKey &k, const KernelArgs &a, uint3
     if (k.score == kScoreDna) {
         return k.local ? dispatch_flags<W, R, kScoreDna, true>(k, a, lds, st)
                        : dispatch_flags<W, R, kScoreDna, false>(k, a, lds, st);
+    }
+    if constexpr (W == 16) {
+        if (k.score == kScorePairs)
+            return k.local ? dispatch_flags<W, R, kScorePairs, true>(k, a, lds, st)
+                           : dispatch_flags<W, R, kScorePairs, false>(k, a, lds, st);
     }
     return k.local ? dispatch_flags<W, R, kScoreMatrix, true>(k, a, lds, st)
                    : dispatch_flags<W, R, kScoreMatrix, false>(k, a, lds, st);

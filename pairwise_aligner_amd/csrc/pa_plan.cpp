@@ -210,6 +210,13 @@ double task_cost(const VariantKey &key, const Task &t, const Tuning &tuning)
            (key.G == 64 ? 4.0 : 1.0);
 }
 
+uint32_t lds_bytes(const Plan &plan, const VariantKey &key)
+{
+    if (key.score == kScorePairs) return pair_table_bytes(plan.alphabet);
+    if (key.score == kScoreMatrix) return lds_table_bytes(plan.alphabet, plan.stride);
+    return 0;
+}
+
 void order_launches(Plan &plan)
 {
     // Wavefront launches first: each of their workgroups is one long serial sweep, so they
@@ -286,6 +293,22 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
 
     Alphabet alpha;
     if (pa_status st = build_alphabet(cfg, b, plan, alpha, err); st != PA_OK) return st;
+    if (plan.score == kScoreMatrix && plan.alphabet <= kPairTableMaxAlphabet) {
+        const uint32_t K = plan.alphabet;
+        plan.ptable.resize(size_t(K) * K * K);
+        plan.ptable_d.resize(size_t(K) * K * K);
+        for (uint32_t a1 = 0; a1 < K; ++a1)
+            for (uint32_t a2 = 0; a2 < K; ++a2)
+                for (uint32_t c = 0; c < K; ++c) {
+                    const size_t e = (size_t(a1) * K + a2) * K + c;
+                    auto word = [&](const std::vector<int16_t> &t) {
+                        return (uint32_t(uint16_t(t[size_t(a2) * plan.stride + c])) << 16) |
+                               uint32_t(uint16_t(t[size_t(a1) * plan.stride + c]));
+                    };
+                    plan.ptable[e] = word(plan.table);
+                    plan.ptable_d[e] = word(plan.table_d);
+                }
+    }
 
     const uint64_t N = b.n_pairs;
     plan.pairs.resize(N);
@@ -319,7 +342,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         VariantKey key{};
         key.width = width;
         choose_shape(pm.m, pm.n, plan.tuning, key.G, key.R);
-        key.score = plan.score;
+        key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable.empty()) ? kScorePairs : plan.score;
         key.local = local;
         key.track = (!local && ((width == 16 && !upper16) || track_all)) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;
@@ -335,9 +358,10 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         if (plan.score == kScoreDna) return width == 16 ? dna_row16(c) : dna_row32(c);
         return lds_row_offset(c, plan.stride);
     };
-    auto col_code = [&](uint8_t sym, uint32_t width) -> uint32_t {
+    auto col_code = [&](uint8_t sym, uint32_t width, uint8_t score) -> uint32_t {
         const uint32_t c = uint32_t(alpha.code[sym]);
         if (plan.score == kScoreDna) return width == 16 ? dna_col16(c) : dna_col32(c);
+        if (score == kScorePairs) return pair_col_offset(c);
         return lds_col_offset(c);
     };
     const uint32_t pad_rank = plan.alphabet - 1;
@@ -345,14 +369,22 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         if (plan.score == kScoreDna) return width == 16 ? kDnaRowPad16 : kDnaRowPad32;
         return lds_row_offset(pad_rank, plan.stride);
     };
-    auto col_pad = [&](uint32_t width) -> uint32_t {
+    auto col_pad = [&](uint32_t width, uint8_t score) -> uint32_t {
         if (plan.score == kScoreDna) return width == 16 ? kDnaColPad16 : kDnaColPad32;
+        if (score == kScorePairs) return pair_col_offset(pad_rank);
         return lds_col_offset(pad_rank);
+    };
+    // rank of row i of one half (the pad rank past its end)
+    auto row_rank = [&](const Item *h, uint32_t i) -> uint32_t {
+        if (!h || i >= h->m) return pad_rank;
+        const PairMeta &pm = plan.pairs[h->pair];
+        const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h->pair] : b.seq1 + b.off1[h->pair];
+        return uint32_t(alpha.code[s[i]]);
     };
 
     struct Fill {
         const Item *a, *b;   // the pair(s) of the task
-        uint8_t width, G;
+        uint8_t width, G, score;
         uint32_t rows;       // G * R
     };
     std::vector<Fill> fills;
@@ -406,13 +438,13 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 t.bnd_off = uint32_t(std::min<uint64_t>(plan.bnd_entries, 0xFFFFFFFFull));
                 plan.bnd_entries += cw;
             }
-            n_rowcodes += uint64_t(t.S) * rows;
+            n_rowcodes += uint64_t(t.S) * rows / (key.score == kScorePairs ? 2 : 1);
             n_colcodes += cw;
             const uint64_t pc = uint64_t(t.S) * rows * t.N * (key.width == 16 ? 2 : 1);
             L.padded_cells += pc;
             plan.padded_cells += pc;
             plan.tasks.push_back(t);
-            fills.push_back(Fill{h[0], h[1], key.width, key.G, rows});
+            fills.push_back(Fill{h[0], h[1], key.width, key.G, key.score, rows});
         }
         L.task_end = uint32_t(plan.tasks.size());
         if (L.task_end > L.task_begin) plan.launches.push_back(L);
@@ -430,7 +462,17 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         const Fill &f = fills[ti];
         const Item *h[2] = {f.a, f.b};
         const uint32_t nrows = t.S * f.rows;
-        for (uint32_t i = 0; i < nrows; ++i) {
+        if (f.score == kScorePairs) {
+            // one word per two rows: the pair-row table offset of each half
+            const uint32_t K = plan.alphabet;
+            for (uint32_t i = 0; i < nrows; i += 2) {
+                uint32_t w = 0;
+                for (int q = 0; q < 2; ++q)
+                    w |= (pair_row_offset(row_rank(h[q], i), row_rank(h[q], i + 1), K) & 0xFFFFu) << (16 * q);
+                plan.rowcodes[size_t(t.row_off) + i / 2] = w;
+            }
+        }
+        for (uint32_t i = 0; i < (f.score == kScorePairs ? 0 : nrows); ++i) {
             uint32_t w = 0;
             for (int q = 0; q < (f.width == 16 ? 2 : 1); ++q) {
                 uint32_t code;
@@ -453,9 +495,9 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 if (h[q] && j < h[q]->n) {
                     const PairMeta &pm = plan.pairs[h[q]->pair];
                     const uint8_t *s = pm.swapped ? b.seq1 + b.off1[h[q]->pair] : b.seq2 + b.off2[h[q]->pair];
-                    code = col_code(s[j], f.width);
+                    code = col_code(s[j], f.width, f.score);
                 } else {
-                    code = col_pad(f.width);
+                    code = col_pad(f.width, f.score);
                 }
                 w |= f.width == 16 ? (code & 0xFFFFu) << (16 * q) : code;
             }

@@ -26,7 +26,7 @@ namespace pa {
 
 inline constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-enum ScoreCode : uint8_t { kScoreDna = 0, kScoreMatrix = 1 };
+enum ScoreCode : uint8_t { kScoreDna = 0, kScoreMatrix = 1, kScorePairs = 2 };
 
 // Residue encodings.  Unitary scoring over at most four symbols uses a byte-permute lookup:
 // score = perm(tbl_hi, tbl_lo, rowcode ^ colcode), where the xor of the two codes yields the
@@ -55,6 +55,16 @@ inline constexpr uint32_t kMaxMatrixSymbols = 31;   // plus the pad: 32 x 16 x 1
 inline uint32_t lds_row_offset(uint32_t a, uint32_t stride) { return a * (stride / 2) * 128; }
 inline uint32_t lds_col_offset(uint32_t b) { return (b >> 1) * 128 + (b & 1) * 2; }
 inline uint32_t lds_table_bytes(uint32_t alphabet, uint32_t stride) { return alphabet * (stride / 2) * 128; }
+
+// Pair-row table (int16 tasks, alphabets of at most kPairTableMaxAlphabet symbols with the pad):
+// one dword per (a1, a2, b) holding s(a1, b) in the low and s(a2, b) in the high half, so one
+// LDS read gives the scores of two consecutive rows (a1, a2) of one pair against column residue
+// b.  Stored once (42.6 KB for 22 symbols); an int16 task's row code is one word per two rows,
+// the 16-bit byte offset (a1*K + a2)*K*4 of each half, its column code b*4 per half.
+inline constexpr uint32_t kPairTableMaxAlphabet = 22;
+inline uint32_t pair_row_offset(uint32_t a1, uint32_t a2, uint32_t K) { return (a1 * K + a2) * K * 4; }
+inline uint32_t pair_col_offset(uint32_t b) { return b * 4; }
+inline uint32_t pair_table_bytes(uint32_t K) { return K * K * K * 4; }
 
 struct Task {
     uint32_t row_off;
@@ -117,6 +127,7 @@ struct Plan {
     uint32_t stride = 0, alphabet = 0;        // matrix table geometry (alphabet includes the pad)
     std::vector<int16_t> table;               // matrix: alphabet*stride entries
     std::vector<int16_t> table_d;             // the same, scores shifted by -ge (global method)
+    std::vector<uint32_t> ptable, ptable_d;   // pair-row tables (empty: alphabet too large)
     int32_t smax = 0, smin = 0;               // extreme substitution scores
 
     std::vector<PairMeta> pairs;
@@ -171,6 +182,9 @@ inline constexpr uint32_t kWavefrontMinRows = 3072;
 // more than tuning.wavefront_min_rows rows, else the allowed rows per lane with the lowest
 // modelled sweep cost (stripes x steps x per-step work).
 void choose_shape(uint32_t m, uint32_t n, const Tuning &tuning, uint8_t &G, uint8_t &R);
+
+// LDS bytes a launch of variant `key` needs for its substitution table.
+uint32_t lds_bytes(const Plan &plan, const VariantKey &key);
 
 // Order of the launches of a plan and the stream of each: wavefront launches first (longest
 // task first), then striped launches by decreasing work.  Wavefront launches share stream 0,

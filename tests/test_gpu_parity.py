@@ -102,6 +102,22 @@ def test_blosum62(ctx, method):
     check(ctx, s1, s2, dc, label=f"blosum62 method={method}")
 
 
+@pytest.mark.parametrize("method", [device.METHOD_GLOBAL, device.METHOD_LOCAL])
+def test_blosum62_extended_large_alphabet(ctx, method):
+    """27 symbols + pad: too many for the pair-row LDS table, so int16 tasks take the replicated
+    per-entry table; odd and even lengths, residues from the whole extended alphabet."""
+    rng = np.random.default_rng(61 + method)
+    t = tables.blosum62_extended()
+    syms = np.frombuffer("".join(s for s, _ in t).encode(), dtype=np.uint8)
+    lens1 = rng.integers(0, 400, size=600)
+    lens2 = rng.integers(0, 400, size=600)
+    s1 = [syms[rng.integers(0, len(syms), size=n)].tobytes() for n in lens1]
+    s2 = [syms[rng.integers(0, len(syms), size=n)].tobytes() for n in lens2]
+    dc = device.DeviceConfig(method=method, score_kind=device.SCORE_MATRIX, symbols=syms.tobytes(),
+                             matrix=tuple(v for _, r in t for v in r), gap_open=-11, gap_extension=-1)
+    check(ctx, s1, s2, dc, label=f"blosum62 extended method={method}")
+
+
 def test_unitary_many_symbols(ctx):
     """More than four distinct symbols (e.g. N): unitary equality over an LDS table."""
     rng = np.random.default_rng(9)
