@@ -92,6 +92,23 @@ def test_end_gaps(ctx, flags):
     check(ctx, s1, s2, dc, label=f"end gaps {flags}")
 
 
+@pytest.mark.parametrize("policy", [device.LANES_AUTO, device.LANES_INT16, device.LANES_INT32])
+def test_blosum62_end_gaps_and_lane_policies(ctx, policy):
+    """Every end-gap combination with the matrix score (pair-row table in int16, per-entry table
+    in int32), including the per-row maxima of a free last row and the tracked running maximum
+    of speculative int16 pairs; rows of every parity so that pair rows end in a pad row."""
+    rng = np.random.default_rng(67 + policy)
+    s1, s2 = workloads.protein_pairs(rng, 400, 0, 330)
+    t = tables.blosum62_standard()
+    for flags in itertools.product([False, True], repeat=4):
+        fc, fr, lc, lr = flags
+        dc = device.DeviceConfig(first_column_free=fc, first_row_free=fr, last_column_free=lc, last_row_free=lr,
+                                 score_kind=device.SCORE_MATRIX, symbols="".join(x for x, _ in t).encode(),
+                                 matrix=tuple(v for _, r in t for v in r), gap_open=-11, gap_extension=-1,
+                                 lane_policy=policy)
+        check(ctx, s1, s2, dc, label=f"blosum62 end gaps {flags} policy {policy}")
+
+
 @pytest.mark.parametrize("method", [device.METHOD_GLOBAL, device.METHOD_LOCAL])
 def test_blosum62(ctx, method):
     rng = np.random.default_rng(55 + method)
