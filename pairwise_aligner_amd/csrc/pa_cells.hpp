@@ -228,23 +228,39 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
         D[r] = L::add(r == 0 ? diag : st.H[r - 1], s);
         if constexpr (LOCAL) D[r] = L::vmax(D[r], L::splat(0));
     }
-    // With go <= 0 (so go+ge <= ge) the vertical gap state needs only max(D, W) of its own row:
-    //   V(i+1,j) = max(V + ge, H + go + ge) = max(V + ge, max(D, W) + go + ge)
-    // because the dropped term V + go + ge never exceeds V + ge; drifted, V' = max(V', DF' + go).
-    V DF[R], DFo[R];
+    V DF[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        DF[r] = L::vmax(D[r], st.F[r]);
-        DFo[r] = L::add(DF[r], DRIFT ? vgo : vgogo);
-    }
+    for (int r = 0; r < R; ++r) DF[r] = L::vmax(D[r], st.F[r]);
+    if constexpr (DRIFT) {
+        // With go <= 0 (so go+ge <= ge) the vertical gap state needs only max(D, W) of its row:
+        //   V(i+1,j) = max(V + ge, H + go + ge) = max(V + ge, max(D, W) + go + ge)
+        // because the dropped term V + go + ge never exceeds V + ge; drifted, V' = max(V', DF' + go):
+        // one max on the row-to-row chain.
+        V DFo[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const V best = L::vmax(DF[r], Eup);
-        Eup = DRIFT ? L::vmax(Eup, DFo[r]) : L::vmax(L::add(Eup, vge), DFo[r]);
-        st.H[r] = best;
-        if constexpr (LOCAL) runmax = L::vmax(runmax, best);
-        if constexpr (ACC) st.rowacc[r] = L::vmax(st.rowacc[r], best);
-        st.F[r] = L::vmax(L::add(st.F[r], vge), L::add(best, vgogo));
+        for (int r = 0; r < R; ++r) DFo[r] = L::add(DF[r], vgo);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const V best = L::vmax(DF[r], Eup);
+            Eup = L::vmax(Eup, DFo[r]);
+            st.H[r] = best;
+            if constexpr (LOCAL) runmax = L::vmax(runmax, best);
+            if constexpr (ACC) st.rowacc[r] = L::vmax(st.rowacc[r], best);
+            st.F[r] = L::vmax(L::add(st.F[r], vge), L::add(best, vgogo));
+        }
+    } else {
+        // Exact values: the gap opening H + go + ge is shared by both gap states (one add less
+        // per cell than the drift-free form of the line above, one op more on the chain).
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const V best = L::vmax(DF[r], Eup);
+            const V open = L::add(best, vgogo);
+            Eup = L::vmax(L::add(Eup, vge), open);
+            st.H[r] = best;
+            if constexpr (LOCAL) runmax = L::vmax(runmax, best);
+            if constexpr (ACC) st.rowacc[r] = L::vmax(st.rowacc[r], best);
+            st.F[r] = L::vmax(L::add(st.F[r], vge), open);
+        }
     }
 }
 
