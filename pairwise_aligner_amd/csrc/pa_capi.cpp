@@ -28,7 +28,8 @@ struct DeviceBuffer {
 
 struct PlanDevice {
     pa::Plan plan;
-    DeviceBuffer tasks, rowcodes, colcodes, bnd, table, table_d, ptable, ptable_d, subset;
+    DeviceBuffer tasks, rowcodes, colcodes, bnd, subset;
+    DeviceBuffer table[pa::kScoreShifts], ptable[pa::kScoreShifts];
     std::vector<uint32_t> subset_host;
 };
 
@@ -115,10 +116,10 @@ pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd)
     if ((st = upload(ctx, pd.tasks, pd.plan.tasks)) != PA_OK) return st;
     if ((st = upload(ctx, pd.rowcodes, pd.plan.rowcodes)) != PA_OK) return st;
     if ((st = upload(ctx, pd.colcodes, pd.plan.colcodes)) != PA_OK) return st;
-    if ((st = upload(ctx, pd.table, pd.plan.table)) != PA_OK) return st;
-    if ((st = upload(ctx, pd.table_d, pd.plan.table_d)) != PA_OK) return st;
-    if ((st = upload(ctx, pd.ptable, pd.plan.ptable)) != PA_OK) return st;
-    if ((st = upload(ctx, pd.ptable_d, pd.plan.ptable_d)) != PA_OK) return st;
+    for (int v = 0; v < pa::kScoreShifts; ++v) {
+        if ((st = upload(ctx, pd.table[v], pd.plan.table[v])) != PA_OK) return st;
+        if ((st = upload(ctx, pd.ptable[v], pd.plan.ptable[v])) != PA_OK) return st;
+    }
     const size_t bnd_bytes = std::max<size_t>(pd.plan.bnd_entries * 8, 16);
     PA_HIP(hipMalloc(&pd.bnd.ptr, bnd_bytes), "hipMalloc");
     pd.bnd.bytes = bnd_bytes;
@@ -139,19 +140,19 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     a.scores = static_cast<int32_t *>(b->scores.ptr);
     a.flags = static_cast<uint32_t *>(b->flags.ptr);
     a.error_slot = uint32_t(b->n_pairs);
-    // global-method variants read the scores shifted by -ge (row drift, pa_cells.hpp)
-    const bool drift = !L.key.local;
+    // global-method variants read the scores shifted by their drift (pa_cells.hpp)
+    const int v = pa::score_shift(L.key);
     if (L.key.score == pa::kScorePairs) {
-        a.table = static_cast<const int16_t *>(drift ? pd.ptable_d.ptr : pd.ptable.ptr);
-        a.table_entries = uint32_t(p.ptable.size());   // dwords
+        a.table = static_cast<const int16_t *>(pd.ptable[v].ptr);
+        a.table_entries = uint32_t(p.ptable[v].size());   // dwords
     } else {
-        a.table = static_cast<const int16_t *>(drift ? pd.table_d.ptr : pd.table.ptr);
-        a.table_entries = uint32_t(p.table.size());
+        a.table = static_cast<const int16_t *>(pd.table[v].ptr);
+        a.table_entries = uint32_t(p.table[v].size());
     }
     a.go = p.go;
     a.ge = p.ge;
-    a.tbl_lo = drift ? p.tbl_lo_d : p.tbl_lo;
-    a.tbl_hi = drift ? p.tbl_hi_d : p.tbl_hi;
+    a.tbl_lo = p.tbl_lo[v];
+    a.tbl_hi = p.tbl_hi[v];
     a.fc = p.fc; a.fr = p.fr; a.lc = p.lc; a.lr = p.lr;
     return a;
 }

@@ -80,6 +80,14 @@ __device__ __forceinline__ int h0_value(int i, int free_flag, int go, int ge)
 // the planner), and values leaving the lane are shifted back: H exact, V + |ge| (the drift of
 // the next lane's first row).  Per-row maxima are kept drifted and shifted back when read.  The
 // local method keeps exact values: its zero clamp would need a different bound per row.
+//
+// Column drift (global method without per-row maxima, on top of the row drift).  A value
+// computed at step k of a chunk of G steps (u = k mod G) also carries |ge|*u.  Consecutive
+// columns of a row are consecutive steps, so the horizontal gap state loses its extension term
+// too, W'(j+1) = max(W'(j), H'(j) + go), and the diagonal term gains another |ge| (scores shifted
+// by -2ge).  The drift is the same for every lane at a step, so values passed between lanes are
+// shifted by a constant; at the end of a chunk every live value drops by G*|ge|, which keeps the
+// drift below |ge|*(R+1+G).
 template <bool DRIFT>
 __device__ __forceinline__ int drift_of(int ge, int r)
 {
@@ -97,13 +105,14 @@ struct Rows {
     V rowacc[ACC ? R : 1];
 };
 
-// Loads the residue codes of rows row0+1 .. row0+R and sets H(i,0) and W(i,1) (drifted if DRIFT).
-template <int W, int R, int SCORE, bool ACC, bool DRIFT>
+// Loads the residue codes of rows row0+1 .. row0+R and sets H(i,0) and W(i,1) (drifted if DRIFT;
+// with the column drift CDRIFT, H carries `hcol` more and W enters one column later).
+template <int W, int R, int SCORE, bool ACC, bool DRIFT, bool CDRIFT>
 __device__ __forceinline__ void init_rows(Rows<W, R, ACC> &st, const uint32_t *codes, int row0, int fc, int go,
-                                          int ge, uint32_t lane_bank)
+                                          int ge, uint32_t lane_bank, int hcol)
 {
     using L = Lane<W>;
-    const typename L::V vgogo = L::splat(go + ge);
+    const typename L::V vgogo = L::splat(CDRIFT ? go : go + ge);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if constexpr (SCORE == kScorePairs) {
@@ -123,7 +132,7 @@ __device__ __forceinline__ void init_rows(Rows<W, R, ACC> &st, const uint32_t *c
                 st.Ahi[r] = 0;
             }
         }
-        st.H[r] = L::splat(h0_value(row0 + r + 1, fc, go, ge) + drift_of<DRIFT>(ge, r));
+        st.H[r] = L::splat(h0_value(row0 + r + 1, fc, go, ge) + drift_of<DRIFT>(ge, r) + (CDRIFT ? hcol : 0));
         st.F[r] = L::add(st.H[r], vgogo);
         if constexpr (ACC) st.rowacc[r] = L::lowest();
     }
@@ -208,7 +217,7 @@ __device__ __forceinline__ void finish_scores(const Rows<W, R, ACC> &st, uint32_
 // vertical gap state of row 0 and leaves as that of row R.  `diag` is the exact H of the row above
 // at column j-1.  RUN: keep the exact running maximum in `runmax` (local method); per-row
 // maxima go to rowacc (ACC).
-template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT>
+template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT, bool CDRIFT>
 __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<W>::V &Eup,
                                              typename Lane<W>::V diag, typename Lane<W>::V &runmax,
                                              const typename Lane<W>::V (&sc)[R], uint32_t Bin, uint32_t tbl_lo,
@@ -231,7 +240,21 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
     V DF[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) DF[r] = L::vmax(D[r], st.F[r]);
-    if constexpr (DRIFT) {
+    if constexpr (CDRIFT) {
+        // Row and column drift: both gap states lose their extension term,
+        //   V'(r+1) = max(V'(r), H'(r) + go),   W'(j+1) = max(W'(j), H'(j) + go),
+        // and share the opening term.
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const V best = L::vmax(DF[r], Eup);
+            const V open = L::add(best, vgo);
+            Eup = L::vmax(Eup, open);
+            st.H[r] = best;
+            if constexpr (LOCAL) runmax = L::vmax(runmax, best);
+            if constexpr (ACC) st.rowacc[r] = L::vmax(st.rowacc[r], best);
+            st.F[r] = L::vmax(st.F[r], open);
+        }
+    } else if constexpr (DRIFT) {
         // With go <= 0 (so go+ge <= ge) the vertical gap state needs only max(D, W) of its row:
         //   V(i+1,j) = max(V + ge, H + go + ge) = max(V + ge, max(D, W) + go + ge)
         // because the dropped term V + go + ge never exceeds V + ge; drifted, V' = max(V', DF' + go):
@@ -265,7 +288,7 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
 }
 
 // Scores and cells of one step in one go (no read-ahead).
-template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT>
+template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT, bool CDRIFT>
 __device__ __forceinline__ void cell_block(Rows<W, R, ACC> &st, typename Lane<W>::V &Eup,
                                            typename Lane<W>::V diag, typename Lane<W>::V &runmax, uint32_t Bin,
                                            uint32_t tbl_lo, uint32_t tbl_hi, const char *lds, typename Lane<W>::V vgo,
@@ -275,13 +298,15 @@ __device__ __forceinline__ void cell_block(Rows<W, R, ACC> &st, typename Lane<W>
     issue_scores<W, R, SCORE>(st, Bin, lds, rd);
     typename Lane<W>::V sc[R];
     finish_scores<W, R, SCORE>(st, Bin, tbl_lo, tbl_hi, rd, sc);
-    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT>(st, Eup, diag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo, vge);
+    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT>(st, Eup, diag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo,
+                                                         vge);
 }
 
 // Score capture of one lane in one stripe (global method).  cap: this lane owns row m of half
 // q and reads H(m, n) at column n; lcap (free last column): this lane reads the maximum over its
 // real rows at column n.  Contributions meet in an int32 atomicMax on the pair's score.  Register
-// values drifted by -dge*(r+1) (dge = ge with the row drift, else 0) are shifted back here.
+// values drifted by -dge*(r+1) (dge = ge with the row drift, else 0) and by -dcol (the column
+// drift of the current step) are shifted back here.
 template <int W, int R>
 struct Capture {
     int cap[2], capr[2], lcap[2], lvr[2];
@@ -313,7 +338,7 @@ struct Capture {
 
     template <bool ACC>
     __device__ __forceinline__ void at_column(int c, const Rows<W, R, ACC> &st, const Task &tk, int32_t *scores,
-                                              int dge)
+                                              int dge, int dcol = 0)
     {
         if (c != ev) return;
 #pragma unroll
@@ -322,13 +347,13 @@ struct Capture {
                 int v = 0;
 #pragma unroll
                 for (int r = 0; r < R; ++r) if (r == capr[q]) v = Lane<W>::half(st.H[r], q);
-                atomicMax(scores + tk.pair[q], v + dge * (capr[q] + 1));
+                atomicMax(scores + tk.pair[q], v + dge * (capr[q] + 1) + dcol);
             }
             if (c == lcap[q]) {
                 int v = INT32_MIN;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    if (r < lvr[q]) { const int h = Lane<W>::half(st.H[r], q) + dge * (r + 1); v = h > v ? h : v; }
+                    if (r < lvr[q]) { const int h = Lane<W>::half(st.H[r], q) + dge * (r + 1) + dcol; v = h > v ? h : v; }
                 atomicMax(scores + tk.pair[q], v);
             }
         }

@@ -84,14 +84,19 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     const int go = a.go, ge = a.ge;
     // method_local.hpp:41-49: a local alignment has every end gap free
     const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
+    constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
+    constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
+    constexpr bool CDRIFT = DRIFT && !ACC;   // column drift on top
+    const int dge = DRIFT ? ge : 0;
+    const int cg = CDRIFT ? -ge : 0;         // column drift per step
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
-    const V vout = L::splat(ge * R);   // shifts the last row's drift back when H, V leave the lane
+    // shifts the last row's drift back when H, V leave the lane (and, with the column drift,
+    // forward by one step: the next lane uses them one step later)
+    const V vout = L::splat(CDRIFT ? ge * (R - 1) : ge * R);
+    const V vrebase = L::splat(ge * G);      // end of a chunk: the column drift drops by G|ge|
     const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
     const int N = int(tk.N);
     constexpr int ROWS = G * R;
-    constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
-    constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
-    const int dge = DRIFT ? ge : 0;
 
     V runmax = LOCAL ? L::splat(0) : L::lowest();   // exact
     V satmax = L::lowest();                         // drifted (int16 saturation test)
@@ -99,8 +104,10 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     for (uint32_t s = 0; s < tk.S; ++s) {
         const int row0 = int(s) * ROWS + t * R;   // DP row above this lane's block
         Rows<W, R, ACC> st;
-        init_rows<W, R, SCORE, ACC, DRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank);
-        V Hdiag = L::splat(h0_value(row0, fc, go, ge));
+        // lane t starts at step t: its initial column (j = 0) counts as computed at step t-1
+        init_rows<W, R, SCORE, ACC, DRIFT, CDRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank,
+                                                   cg * (t - 1));
+        V Hdiag = L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1));
         Capture<W, R> cap;
         if constexpr (!LOCAL) cap.init(tk, row0, lc);
 
@@ -120,15 +127,24 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         // for columns jG .. jG+G-1: the initial row H(0, c+1) for the first stripe, the stored
         // bottom row of the stripe above otherwise.  Columns past N (computed, never read, by
         // the unmasked steps) get a low value that no real cell can be raised by.
+        // Lane t of a chunk is consumed at the chunk's step t: column drift cg*t.
         auto top_chunk = [&](int col, uint32_t &h, uint32_t &e) {
             if (has_bnd_in) {
-                h = col < N ? bndp[2 * col] : kLowWord;
-                e = col < N ? bndp[2 * col + 1] : kLowWord;
+                if (col < N) {
+                    h = bndp[2 * col];
+                    e = bndp[2 * col + 1];
+                    if constexpr (CDRIFT) {
+                        h = L::bits(L::add(L::from(h), L::splat(cg * t)));
+                        e = L::bits(L::add(L::from(e), L::splat(cg * t)));
+                    }
+                } else {
+                    h = e = kLowWord;
+                }
             } else {
                 // H(0, col+1) exact, the vertical gap state below it (drifted by |ge| for row 0)
                 const int hv = h0_value(col + 1, fr, go, ge);
-                h = L::top_word(hv);
-                e = L::top_word(hv + go + ge - dge);
+                h = L::top_word(hv + cg * t);
+                e = L::top_word(hv + go + ge - dge + cg * t);
             }
         };
         uint32_t convH, convE, nextH, nextE;
@@ -160,7 +176,8 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             convE = dpp<kRowShl1>(convE, convE);
             if (!MASKED || uint32_t(c) < uint32_t(N)) {
                 V Eup = L::from(Ein);
-                update_cells<W, R, SCORE, LOCAL, ACC, DRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo, vge);
+                update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo, tbl_hi,
+                                                                     vgo, vgogo, vge);
                 Hdiag = L::from(Hin);
                 Hout = L::bits(DRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
                 Eout = L::bits(DRIFT ? L::add(Eup, vout) : Eup);
@@ -168,7 +185,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             if constexpr (!LOCAL) {
                 // capture events are rare: test the whole wave first
                 if (__builtin_expect(__builtin_amdgcn_ballot_w64(c == cap.ev) != 0, 0))
-                    cap.at_column(c, st, tk, a.scores, dge);
+                    cap.at_column(c, st, tk, a.scores, dge, -cg * (k % G));
             }
             qH = dpp<kRowShl1>(Hout, qH);
             qE = dpp<kRowShl1>(Eout, qE);
@@ -182,12 +199,28 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             convE = nextE;
             top_chunk(k0 + 2 * G + t, nextH, nextE);
             if (has_bnd_out) {
-                // lane G-1 computed column k0 + j - (G-1) at the chunk's step j
+                // lane G-1 computed column k0 + j - (G-1) at the chunk's step j; stored without
+                // the column drift it was sent with (cg*(j+1))
                 const int col = k0 + t - (G - 1);
                 if (col >= 0 && col < N) {
-                    bndp[2 * col] = qH;
-                    bndp[2 * col + 1] = qE;
+                    uint32_t h = qH, e = qE;
+                    if constexpr (CDRIFT) {
+                        h = L::bits(L::add(L::from(h), L::splat(-cg * (t + 1))));
+                        e = L::bits(L::add(L::from(e), L::splat(-cg * (t + 1))));
+                    }
+                    bndp[2 * col] = h;
+                    bndp[2 * col + 1] = e;
                 }
+            }
+            if constexpr (CDRIFT) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    st.H[r] = L::add(st.H[r], vrebase);
+                    st.F[r] = L::add(st.F[r], vrebase);
+                }
+                Hdiag = L::add(Hdiag, vrebase);
+                Hout = L::bits(L::add(L::from(Hout), vrebase));
+                Eout = L::bits(L::add(L::from(Eout), vrebase));
             }
         };
 
@@ -289,14 +322,19 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     const uint32_t lane_bank = (threadIdx.x & 31u) * 4u;
     const int go = a.go, ge = a.ge;
     const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
+    constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
+    constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
+    constexpr bool CDRIFT = DRIFT && !ACC;   // column drift on top
+    const int dge = DRIFT ? ge : 0;
+    const int cg = CDRIFT ? -ge : 0;         // column drift per step
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
-    const V vout = L::splat(ge * R);   // shifts the last row's drift back when H, V leave the lane
+    // shifts the last row's drift back when H, V leave the lane (and, with the column drift,
+    // forward by one step: the next lane uses them one step later)
+    const V vout = L::splat(CDRIFT ? ge * (R - 1) : ge * R);
+    const V vrebase = L::splat(ge * G);      // end of a chunk: the column drift drops by G|ge|
     const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
     const int N = int(tk.N);
     const int S = int(tk.S);
-    constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
-    constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
-    const int dge = DRIFT ? ge : 0;
     const uint32_t *colp = a.colcodes + tk.col_off;
     uint32_t *gbnd = a.bnd + 2ull * tk.bnd_off;   // last wave -> first wave, one full row
 
@@ -308,8 +346,10 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
         const uint32_t pos0 = uint32_t(round) * uint32_t(N);
         const int row0 = s * ROWS + t * R;
         Rows<W, R, ACC> st;
-        init_rows<W, R, SCORE, ACC, DRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank);
-        V Hdiag = L::splat(h0_value(row0, fc, go, ge));
+        // lane t starts at step t: its initial column (j = 0) counts as computed at step t-1
+        init_rows<W, R, SCORE, ACC, DRIFT, CDRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank,
+                                                   cg * (t - 1));
+        V Hdiag = L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1));
         Capture<W, R> cap;
         if constexpr (!LOCAL) cap.init(tk, row0, lc);
 
@@ -325,6 +365,7 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
             wait_at_least(src_prog, src_pos0 + uint32_t(upto < N ? upto : N), a.flags + a.error_slot, &sh.failed);
             if (src == 2) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         };
+        // lane t of a chunk is consumed at the chunk's step t: column drift cg*t
         auto read_top = [&](int col, uint32_t &h, uint32_t &e) {
             if (src == 1) {
                 const uint32_t slot = (pos0 + uint32_t(col)) % kWfRing;
@@ -333,6 +374,10 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
             } else {
                 h = gbnd[2 * col];
                 e = gbnd[2 * col + 1];
+            }
+            if constexpr (CDRIFT) {
+                h = L::bits(L::add(L::from(h), L::splat(cg * t)));
+                e = L::bits(L::add(L::from(e), L::splat(cg * t)));
             }
         };
         auto publish_consumed = [&](int upto) {
@@ -375,8 +420,8 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                     topH = convH;
                     topE = convE;
                 } else {
-                    topH = L::top_word(hb_top);
-                    topE = L::top_word(hb_top + go + ge - dge);   // drifted by |ge| for row 0
+                    topH = L::top_word(hb_top + cg * u);
+                    topE = L::top_word(hb_top + go + ge - dge + cg * u);   // drifted by |ge| for row 0
                     hb_top = fr ? 0 : hb_top + ge;
                 }
                 const uint32_t Hin = dpp<kWaveShr1>(topH, Hout);
@@ -387,11 +432,12 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                 }
                 if (uint32_t(c) < uint32_t(N)) {
                     V Eup = L::from(Ein);
-                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo, vge);
+                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo,
+                                                                         tbl_hi, vgo, vgogo, vge);
                     Hdiag = L::from(Hin);
                     Hout = L::bits(DRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
                     Eout = L::bits(DRIFT ? L::add(Eup, vout) : Eup);
-                    if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores, dge);
+                    if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores, dge, -cg * u);
                 }
                 if (dst != 0) {
                     qH = dpp<kWaveShl1>(Hout, qH);
@@ -411,9 +457,14 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                 publish_consumed(k0 + 3 * G);
             }
             if (dst != 0) {
-                // lane j holds the bottom row of column k0 + j - (G-1)
+                // lane j holds the bottom row of column k0 + j - (G-1), sent with column drift
+                // cg*(j+1): stored without it
                 const int col = k0 + t - (G - 1);
                 const int hi = k0 + 1 < N ? k0 + 1 : N;   // columns [0, hi) done after this chunk
+                if constexpr (CDRIFT) {
+                    qH = L::bits(L::add(L::from(qH), L::splat(-cg * (t + 1))));
+                    qE = L::bits(L::add(L::from(qE), L::splat(-cg * (t + 1))));
+                }
                 if (dst == 1) {
                     // the slots of [k0+1-G, k0+1) held positions kWfRing earlier: those must
                     // have been consumed by the next wave
@@ -434,6 +485,16 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                         __hip_atomic_store(&sh.prod[wave], pos0 + uint32_t(hi), __ATOMIC_RELEASE,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
+            }
+            if constexpr (CDRIFT) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    st.H[r] = L::add(st.H[r], vrebase);
+                    st.F[r] = L::add(st.F[r], vrebase);
+                }
+                Hdiag = L::add(Hdiag, vrebase);
+                Hout = L::bits(L::add(L::from(Hout), vrebase));
+                Eout = L::bits(L::add(L::from(Eout), vrebase));
             }
         }
         if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores, dge);

@@ -64,12 +64,13 @@ RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n)
     const int64_t go = std::min<int64_t>(plan.go, 0), ge = std::min<int64_t>(plan.ge, 0);
     const int64_t sp = std::max<int64_t>(plan.smax, 0);
     RangeBound b;
-    // + the largest row drift the kernels add (|ge| * (R + 1), R <= 16)
-    b.upper = sp * int64_t(std::min(m, n)) - ge * 17;
+    // + the largest drift the kernels add: |ge| * (R + 1) along the rows (R <= 16) and
+    // |ge| * G along the columns (G <= 64)
+    b.upper = sp * int64_t(std::min(m, n)) - ge * 81;
     if (local)
         b.lower = go + ge;                       // H >= 0, gap states >= H + go + ge
-    else
-        b.lower = 3 * go + ge * (int64_t(m) + int64_t(n));  // L-shaped path plus one gap opening
+    else   // L-shaped path plus one gap opening, and the diagonal value kept one column back
+        b.lower = 3 * go + ge * (int64_t(m) + int64_t(n) + 1);
     return b;
 }
 
@@ -98,21 +99,21 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
         }
         plan.smax = std::max(cfg.match, cfg.mismatch);
         plan.smin = std::min(cfg.match, cfg.mismatch);
-        // global-method kernels add the row drift -ge to every substitution score (pa_cells.hpp)
-        const int32_t dm = cfg.match - cfg.gap_extension, dx = cfg.mismatch - cfg.gap_extension;
-        if (dm > 32767 || dx > 32767) {
-            err = "match/mismatch minus gap_extension must fit in 16 bits";
+        // global-method kernels add their drift (up to -2ge) to every substitution score
+        const int32_t shift_max = -(kScoreShifts - 1) * cfg.gap_extension;
+        if (cfg.match + shift_max > 32767 || cfg.mismatch + shift_max > 32767) {
+            err = "match/mismatch minus twice gap_extension must fit in 16 bits";
             return PA_ERR_UNSUPPORTED;
         }
         if (k <= 4) {
             alpha.dna = true;
             plan.score = kScoreDna;
-            const uint32_t M = uint32_t(cfg.match) & 0xFFFFu, X = uint32_t(cfg.mismatch) & 0xFFFFu;
-            plan.tbl_lo = M | (X << 16);
-            plan.tbl_hi = X | (X << 16);
-            const uint32_t Md = uint32_t(dm) & 0xFFFFu, Xd = uint32_t(dx) & 0xFFFFu;
-            plan.tbl_lo_d = Md | (Xd << 16);
-            plan.tbl_hi_d = Xd | (Xd << 16);
+            for (int v = 0; v < kScoreShifts; ++v) {
+                const uint32_t M = uint32_t(cfg.match - v * cfg.gap_extension) & 0xFFFFu;
+                const uint32_t X = uint32_t(cfg.mismatch - v * cfg.gap_extension) & 0xFFFFu;
+                plan.tbl_lo[v] = M | (X << 16);
+                plan.tbl_hi[v] = X | (X << 16);
+            }
             return PA_OK;
         }
         if (k > kMaxMatrixSymbols) {
@@ -122,13 +123,13 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
         plan.score = kScoreMatrix;
         plan.alphabet = k + 1;
         plan.stride = (plan.alphabet + 1) & ~1u;
-        plan.table.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
-        plan.table_d = plan.table;
-        for (uint32_t a = 0; a < k; ++a)
-            for (uint32_t c = 0; c < k; ++c) {
-                plan.table[a * plan.stride + c] = int16_t(a == c ? cfg.match : cfg.mismatch);
-                plan.table_d[a * plan.stride + c] = int16_t(a == c ? dm : dx);
-            }
+        for (int v = 0; v < kScoreShifts; ++v) {
+            std::vector<int16_t> &t = plan.table[v];
+            t.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
+            for (uint32_t a = 0; a < k; ++a)
+                for (uint32_t c = 0; c < k; ++c)
+                    t[a * plan.stride + c] = int16_t((a == c ? cfg.match : cfg.mismatch) - v * cfg.gap_extension);
+        }
         return PA_OK;
     }
     if (cfg.score_kind != PA_SCORE_MATRIX || cfg.matrix == nullptr || cfg.matrix_symbols == nullptr) {
@@ -145,20 +146,19 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
     plan.score = kScoreMatrix;
     plan.alphabet = uint32_t(K) + 1;
     plan.stride = (plan.alphabet + 1) & ~1u;
-    plan.table.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
-    plan.table_d = plan.table;
+    for (std::vector<int16_t> &t : plan.table) t.assign(size_t(plan.alphabet) * plan.stride, int16_t(-1));
     plan.smax = INT32_MIN;
     plan.smin = INT32_MAX;
     for (int32_t a = 0; a < K; ++a)
         for (int32_t c = 0; c < K; ++c) {
             int32_t v = cfg.matrix[a * K + c];
-            if (v < -32768 || v - cfg.gap_extension > 32767) {
-                err = "substitution scores (minus gap_extension) must fit in 16 bits";
+            // global-method kernels add their drift (up to -2ge) to every substitution score
+            if (v < -32768 || v - (kScoreShifts - 1) * cfg.gap_extension > 32767) {
+                err = "substitution scores (minus twice gap_extension) must fit in 16 bits";
                 return PA_ERR_UNSUPPORTED;
             }
-            plan.table[a * plan.stride + c] = int16_t(v);
-            // global-method kernels add the row drift -ge to every substitution score (pa_cells.hpp)
-            plan.table_d[a * plan.stride + c] = int16_t(v - cfg.gap_extension);
+            for (int s = 0; s < kScoreShifts; ++s)
+                plan.table[s][a * plan.stride + c] = int16_t(v - s * cfg.gap_extension);
             plan.smax = std::max(plan.smax, v);
             plan.smin = std::min(plan.smin, v);
         }
@@ -295,19 +295,16 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     if (pa_status st = build_alphabet(cfg, b, plan, alpha, err); st != PA_OK) return st;
     if (plan.score == kScoreMatrix && plan.alphabet <= kPairTableMaxAlphabet) {
         const uint32_t K = plan.alphabet;
-        plan.ptable.resize(size_t(K) * K * K);
-        plan.ptable_d.resize(size_t(K) * K * K);
-        for (uint32_t a1 = 0; a1 < K; ++a1)
-            for (uint32_t a2 = 0; a2 < K; ++a2)
-                for (uint32_t c = 0; c < K; ++c) {
-                    const size_t e = (size_t(a1) * K + a2) * K + c;
-                    auto word = [&](const std::vector<int16_t> &t) {
-                        return (uint32_t(uint16_t(t[size_t(a2) * plan.stride + c])) << 16) |
-                               uint32_t(uint16_t(t[size_t(a1) * plan.stride + c]));
-                    };
-                    plan.ptable[e] = word(plan.table);
-                    plan.ptable_d[e] = word(plan.table_d);
-                }
+        for (int v = 0; v < kScoreShifts; ++v) {
+            const std::vector<int16_t> &t = plan.table[v];
+            std::vector<uint32_t> &pt = plan.ptable[v];
+            pt.resize(size_t(K) * K * K);
+            for (uint32_t a1 = 0; a1 < K; ++a1)
+                for (uint32_t a2 = 0; a2 < K; ++a2)
+                    for (uint32_t c = 0; c < K; ++c)
+                        pt[(size_t(a1) * K + a2) * K + c] = (uint32_t(uint16_t(t[size_t(a2) * plan.stride + c])) << 16) |
+                                                            uint32_t(uint16_t(t[size_t(a1) * plan.stride + c]));
+        }
     }
 
     const uint64_t N = b.n_pairs;
@@ -342,7 +339,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         VariantKey key{};
         key.width = width;
         choose_shape(pm.m, pm.n, plan.tuning, key.G, key.R);
-        key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable.empty()) ? kScorePairs : plan.score;
+        key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable[0].empty()) ? kScorePairs : plan.score;
         key.local = local;
         key.track = (!local && ((width == 16 && !upper16) || track_all)) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;

@@ -91,6 +91,12 @@ struct VariantKey {
     bool operator==(const VariantKey &o) const;
 };
 
+// Substitution-score shift of a kernel variant, in units of -ge (pa_cells.hpp): the local
+// method keeps exact values (0); the global method drifts along the rows (1) and, unless it
+// keeps per-row maxima, along the columns as well (2).
+inline constexpr int kScoreShifts = 3;
+inline int score_shift(const VariantKey &key) { return key.local ? 0 : ((key.track || key.rowmax) ? 1 : 2); }
+
 struct Launch {
     VariantKey key;
     uint32_t task_begin, task_end;
@@ -122,12 +128,12 @@ struct Plan {
     int32_t go = 0, ge = 0;
     int32_t fc = 0, fr = 0, lc = 0, lr = 0;   // after orientation
     uint8_t score = kScoreDna;
-    uint32_t tbl_lo = 0, tbl_hi = 0;          // DNA byte-permute table
-    uint32_t tbl_lo_d = 0, tbl_hi_d = 0;      // the same, scores shifted by -ge (global method)
+    // Every table exists in kScoreShifts versions, version k with the scores shifted by -k*ge
+    // (score_shift: the drift the kernel variant adds per diagonal step).
+    uint32_t tbl_lo[kScoreShifts] = {}, tbl_hi[kScoreShifts] = {};   // DNA byte-permute table
     uint32_t stride = 0, alphabet = 0;        // matrix table geometry (alphabet includes the pad)
-    std::vector<int16_t> table;               // matrix: alphabet*stride entries
-    std::vector<int16_t> table_d;             // the same, scores shifted by -ge (global method)
-    std::vector<uint32_t> ptable, ptable_d;   // pair-row tables (empty: alphabet too large)
+    std::vector<int16_t> table[kScoreShifts];      // matrix: alphabet*stride entries
+    std::vector<uint32_t> ptable[kScoreShifts];    // pair-row tables (empty: alphabet too large)
     int32_t smax = 0, smin = 0;               // extreme substitution scores
 
     std::vector<PairMeta> pairs;
