@@ -14,6 +14,7 @@ import argparse
 import dataclasses
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -58,13 +59,16 @@ def blosum_config(method, go, ge, policy):
                                gap_open=go, gap_extension=ge, lane_policy=policy)
 
 
-def make_workload(name):
+def make_workload(name, pairs=None):
+    """The named workload; `pairs` overrides the pair count of the DNA 150 nt and protein workloads
+    (the batch-size sweep of the small DNA configuration)."""
+    count = {} if pairs is None else {"count": pairs}
     if name == "dna150":
-        w = workloads.config_dna150()
+        w = workloads.config_dna150(**count)
         dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=4, mismatch=-5, gap_open=-10, gap_extension=-1)
         return w, dc, "global"
     if name == "protein":
-        w = workloads.config_protein()
+        w = workloads.config_protein(**count)
         dc = blosum_config(device.METHOD_LOCAL, -11, -1, device.LANES_AUTO)
         return w, dc, "local"
     if name == "uneven":
@@ -132,9 +136,9 @@ def cpu_baseline(w, dc, gpu_scores, budget_s=12.0):
                 isa=isa, mismatches_vs_gpu=mism, sample_pairs=sample)
 
 
-def run(name, steps, warmup, with_cpu=True, autotune=False):
+def run(name, steps, warmup, with_cpu=True, autotune=False, pairs=None):
     t0 = time.perf_counter()
-    w, dc, method = make_workload(name)
+    w, dc, method = make_workload(name, pairs)
     n = len(w["o1"]) - 1
     cells = int(np.sum(np.diff(w["o1"]).astype(np.int64) * np.diff(w["o2"]).astype(np.int64)))
     log(f"[bench] {name}: {n} pairs, {cells:.3e} cells, generated in {time.perf_counter() - t0:.1f} s")
@@ -251,6 +255,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy three times (scales the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters for roofline.traffic)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only run the VALU issue-rate probe (pairwise_aligner_amd/valu_probe): cycles per "
+                         "instruction of one wave and the shader clock for the kernels' instructions")
+    ap.add_argument("--pairs", type=int, default=None,
+                    help="pair count of the dna150 / protein workload (default: 10,000 / 100,000)")
     ap.add_argument("--mixed-pairs", type=int, default=100_000,
                     help="pairs of the mixed workload used for the learned-dispatch comparison (config: 500,000)")
     args = ap.parse_args()
@@ -262,6 +271,16 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copies": 3, "check": int(b[-1].item())}))
+        return
+    if args.valu_probe:
+        probe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pairwise_aligner_amd", "valu_probe")
+        if not os.path.exists(probe):
+            raise SystemExit("pairwise_aligner_amd/valu_probe is not built (python -m pairwise_aligner_amd.build)")
+        res = subprocess.run([probe], capture_output=True, text=True, timeout=120)
+        sys.stdout.write(res.stdout)
+        if res.returncode != 0:
+            sys.stderr.write(res.stderr)
+            raise SystemExit(res.returncode)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (pairs are independent; multi-GPU sharding is not built); "
@@ -277,7 +296,7 @@ def main():
                           "data": "synthetic", "config": {"workload": f"mixed DNA/protein, {args.mixed_pairs} pairs"},
                           "mixed_learned_dispatch": res}))
         return
-    main_res = run(args.workload, args.steps, args.warmup, autotune=args.workload == "uneven")
+    main_res = run(args.workload, args.steps, args.warmup, autotune=args.workload == "uneven", pairs=args.pairs)
     extra = mixed = uneven = None
     if not args.no_extra and args.workload != "dna150":
         extra = run("dna150", max(args.steps, 50), args.warmup, with_cpu=False)
@@ -305,7 +324,8 @@ def main():
         "config": {"workload": {"protein": "protein local BLOSUM62, 100,000 pairs, lengths 50-500, gap (-11,-1)",
                                 "dna150": "DNA global, 10,000 pairs x 150 nt, +4/-5, gap (-10,-1)",
                                 "uneven": "1,000,000 DNA pairs lognormal lengths, half global half local",
-                                "long": "2,000 DNA pairs 10-100 kbp, int32"}.get(args.workload, args.workload),
+                                "long": "2,000 DNA pairs 10-100 kbp, int32"}.get(args.workload, args.workload)
+                   if args.pairs is None else f"{args.workload} workload with {args.pairs} pairs",
                    "pairs": main_res["pairs"], "cells": main_res["cells"],
                    "inputs_sha256": main_res["inputs_sha256"]},
         "roofline": {"bound": "valu", "achieved": round(main_res["achieved_tops"], 3),

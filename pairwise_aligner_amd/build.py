@@ -42,6 +42,22 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+PROBE_SRC = os.path.join(CSRC, "valu_probe.hip")
+PROBE_OUT = os.path.join(HERE, "valu_probe")
+
+
+def build_probe() -> str:
+    """Compiles the VALU issue-rate probe (csrc/valu_probe.hip), run by `bench.py --valu-probe`."""
+    flags = [f for f in HIPCC_FLAGS if f not in ("-fPIC", "-shared")]
+    cmd = [hipcc(), *flags, PROBE_SRC, "-o", PROBE_OUT + ".tmp"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building valu_probe")
+    os.replace(PROBE_OUT + ".tmp", PROBE_OUT)
+    return PROBE_OUT
+
+
 def build_cpp_tests() -> str:
     """Compiles tests/cpp/test_host_api (the header-only C++ host API against libpa_device.so)."""
     src = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
@@ -59,3 +75,4 @@ def build_cpp_tests() -> str:
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_cpp_tests())
+    print(build_probe())

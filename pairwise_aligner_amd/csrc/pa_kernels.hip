@@ -231,7 +231,9 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         for (int u = 0; u < G; ++u) step.template operator()<true>(u);
         chunk_end(0);
         for (int k0 = G; k0 < nsteps; k0 += G) {
-#pragma unroll 1
+            // two steps per iteration: the register allocator then alternates the exchanged
+            // values between two sets of registers instead of copying them back every step
+#pragma unroll 2
             for (int u = 0; u < G; ++u) step.template operator()<kMaskAll>(k0 + u);
             chunk_end(k0);
         }
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
 
         const int nsteps = ((N + 2 * G - 2) / G) * G;
         for (int k0 = 0; k0 < nsteps; k0 += G) {
-#pragma unroll 1
+#pragma unroll 2
             for (int u = 0; u < G; ++u) {
                 const int c = k0 + u - t;
                 const uint32_t Bnext = dpp<kWaveShr1>(conv, Bin);
