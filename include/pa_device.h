@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PA_ABI_VERSION 2
+#define PA_ABI_VERSION 3
 
 typedef enum pa_status {
     PA_OK = 0,
@@ -96,6 +96,7 @@ typedef struct pa_batch_stats {
     uint64_t n_recomputed;      /* pairs recomputed at int32 after the last run */
     uint64_t n_launches;        /* kernel launches per run */
     double plan_ms;             /* host planning time */
+    uint64_t n_profiled;        /* int16 pairs scored through a query profile (one-to-many) */
 } pa_batch_stats;
 
 /* Context: one HIP device and one HIP stream.  Not thread-safe; use one per thread. */
@@ -119,6 +120,22 @@ pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg,
                           const uint8_t *seq1, const uint64_t *off1,
                           const uint8_t *seq2, const uint64_t *off2,
                           uint64_t n_pairs, const uint8_t *pair_method, pa_batch **out);
+/* One query against many targets.  Replaces aligner.compute(seq1, bulk2)
+ * (interface_one_to_many_bulk.hpp:47-92) and its query profile (score_model_matrix_simd_1xN.hpp:63-183,
+ * dp_matrix_lane_profile.hpp:53-80): scores[p] is the score of (query, seq2[p]), exactly as
+ * pa_score_batch with the query repeated as every seq1.  When the query is the row side of every
+ * pair (every configuration except a free last row with a penalised last column), fits the striped
+ * kernel and its profile fits 64 KiB (about 1,400 residues for a 21-letter matrix, 6,500 for DNA),
+ * the int16 tasks read their scores from a query profile in LDS instead of the residue tables. */
+pa_status pa_score_one_to_many(pa_ctx *ctx, const pa_config *cfg,
+                               const uint8_t *query, uint64_t query_len,
+                               const uint8_t *seq2, const uint64_t *off2,
+                               uint64_t n_targets, const uint8_t *pair_method,
+                               int32_t *scores, uint8_t *flags);
+pa_status pa_batch_create_one_to_many(pa_ctx *ctx, const pa_config *cfg,
+                                      const uint8_t *query, uint64_t query_len,
+                                      const uint8_t *seq2, const uint64_t *off2,
+                                      uint64_t n_targets, const uint8_t *pair_method, pa_batch **out);
 /* Enqueue every kernel of one pass on the context stream and, when int16 results reached
  * the limit, the int32 recomputation.  Returns when the pass is complete. */
 pa_status pa_batch_run(pa_ctx *ctx, pa_batch *batch);

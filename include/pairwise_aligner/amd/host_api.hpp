@@ -328,12 +328,8 @@ class aligner {
     std::shared_ptr<pa_ctx> _ctx{};
     std::optional<pa_tuning> _tuning{};
 
-    void run(std::vector<uint8_t> const &b1, std::vector<uint64_t> const &o1, std::vector<uint8_t> const &b2,
-             std::vector<uint64_t> const &o2, std::vector<int32_t> &scores, std::vector<uint8_t> &flags)
+    pa_config device_config()
     {
-        const uint64_t n = o1.size() - 1;
-        scores.assign(n, 0);
-        flags.assign(n, 0);
         if (!_ctx) _ctx = detail::device_context(_device);
         pa_config cfg = _cfg;
         if (!_matrix.empty()) {
@@ -341,11 +337,38 @@ class aligner {
             cfg.matrix_symbols = _symbols.data();
         }
         if (_tuning) cfg.tuning = &*_tuning;
-        static const uint8_t empty = 0;
-        const pa_status st = pa_score_batch(_ctx.get(), &cfg, b1.empty() ? &empty : b1.data(), o1.data(),
-                                            b2.empty() ? &empty : b2.data(), o2.data(), n, nullptr, scores.data(),
-                                            flags.data());
+        return cfg;
+    }
+
+    void check(pa_status st)
+    {
         if (st != PA_OK) throw std::runtime_error(std::string{"pairwise_aligner: "} + pa_ctx_last_error(_ctx.get()));
+    }
+
+    void run(std::vector<uint8_t> const &b1, std::vector<uint64_t> const &o1, std::vector<uint8_t> const &b2,
+             std::vector<uint64_t> const &o2, std::vector<int32_t> &scores, std::vector<uint8_t> &flags)
+    {
+        const uint64_t n = o1.size() - 1;
+        scores.assign(n, 0);
+        flags.assign(n, 0);
+        const pa_config cfg = device_config();
+        static const uint8_t empty = 0;
+        check(pa_score_batch(_ctx.get(), &cfg, b1.empty() ? &empty : b1.data(), o1.data(),
+                             b2.empty() ? &empty : b2.data(), o2.data(), n, nullptr, scores.data(), flags.data()));
+    }
+
+    // one query against every target: pa_score_one_to_many (query profile on the device)
+    void run_one_to_many(std::vector<uint8_t> const &query, std::vector<uint8_t> const &b2,
+                         std::vector<uint64_t> const &o2, std::vector<int32_t> &scores, std::vector<uint8_t> &flags)
+    {
+        const uint64_t n = o2.size() - 1;
+        scores.assign(n, 0);
+        flags.assign(n, 0);
+        const pa_config cfg = device_config();
+        static const uint8_t empty = 0;
+        check(pa_score_one_to_many(_ctx.get(), &cfg, query.empty() ? &empty : query.data(), query.size(),
+                                   b2.empty() ? &empty : b2.data(), o2.data(), n, nullptr, scores.data(),
+                                   flags.data()));
     }
 
 public:
@@ -393,20 +416,19 @@ public:
         return results;
     }
 
-    // one-to-many (interface_one_to_many_bulk.hpp:47-92)
+    // one-to-many (interface_one_to_many_bulk.hpp:47-92): the query is planned once and, on the
+    // device, scored through a query profile (score_model_matrix_simd_1xN.hpp:63-183)
     template <detail::sequence sequence1_t, detail::sequence_bulk bulk2_t>
     auto compute(sequence1_t &&sequence1, bulk2_t &&bulk2)
     {
-        std::vector<uint8_t> b1, b2;
-        std::vector<uint64_t> o1{0}, o2{0};
-        for (auto &&s : bulk2) {
-            detail::append(b1, o1, sequence1);
-            detail::append(b2, o2, s);
-        }
+        std::vector<uint8_t> q, b2;
+        std::vector<uint64_t> oq{0}, o2{0};
+        detail::append(q, oq, sequence1);
+        for (auto &&s : bulk2) detail::append(b2, o2, s);
         using result_t = bulk_result<sequence1_t, bulk2_t>;
         auto shared = std::make_shared<result_t>(result_t{std::forward<sequence1_t>(sequence1),
                                                           std::forward<bulk2_t>(bulk2), {}, {}});
-        run(b1, o1, b2, o2, shared->_scores, shared->_flags);
+        run_one_to_many(q, b2, o2, shared->_scores, shared->_flags);
         std::vector<aligner_result_bulk<result_t, true>> results;
         results.reserve(o2.size() - 1);
         for (std::size_t i = 0; i + 1 < o2.size(); ++i) results.emplace_back(shared, i);

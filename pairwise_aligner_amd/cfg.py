@@ -177,7 +177,11 @@ class Aligner:
         if single1 and single2:
             return self.compute([seq1], [seq2])[0]
         if single1:
-            seq1 = [seq1] * len(seq2)
+            # one against many: the device plans the query once (pa_score_one_to_many)
+            q = _as_bytes(seq1)
+            b2, o2 = _dev.pack_sequences([_as_bytes(s) for s in seq2])
+            scores, flags = self._ctx().score_one_to_many(q, b2, o2, self.config)
+            return [aligner_result(seq1, b, sc, fl) for b, sc, fl in zip(seq2, scores, flags)]
         s1 = [_as_bytes(s) for s in seq1]
         s2 = [_as_bytes(s) for s in seq2]
         if len(s1) != len(s2):

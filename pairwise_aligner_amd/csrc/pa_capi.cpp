@@ -29,7 +29,7 @@ struct DeviceBuffer {
 struct PlanDevice {
     pa::Plan plan;
     DeviceBuffer tasks, rowcodes, colcodes, bnd, subset;
-    DeviceBuffer table[pa::kScoreShifts], ptable[pa::kScoreShifts];
+    DeviceBuffer table[pa::kScoreShifts], ptable[pa::kScoreShifts], profile[pa::kScoreShifts];
     std::vector<uint32_t> subset_host;
 };
 
@@ -55,6 +55,7 @@ struct pa_batch {
     std::vector<uint8_t> seq1, seq2, pair_method;
     std::vector<uint64_t> off1, off2;
     uint64_t n_pairs = 0;
+    bool one_query = false;   // seq1 is one query scored against every seq2 (off1 = {0, length})
     PlanDevice main;
     std::unique_ptr<PlanDevice> recompute;   // int32 re-plan of the pairs that saturated
     std::vector<uint32_t> recompute_set;
@@ -119,6 +120,7 @@ pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd)
     for (int v = 0; v < pa::kScoreShifts; ++v) {
         if ((st = upload(ctx, pd.table[v], pd.plan.table[v])) != PA_OK) return st;
         if ((st = upload(ctx, pd.ptable[v], pd.plan.ptable[v])) != PA_OK) return st;
+        if ((st = upload(ctx, pd.profile[v], pd.plan.profile[v])) != PA_OK) return st;
     }
     const size_t bnd_bytes = std::max<size_t>(pd.plan.bnd_entries * 8, 16);
     PA_HIP(hipMalloc(&pd.bnd.ptr, bnd_bytes), "hipMalloc");
@@ -145,6 +147,9 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     if (L.key.score == pa::kScorePairs) {
         a.table = static_cast<const int16_t *>(pd.ptable[v].ptr);
         a.table_entries = uint32_t(p.ptable[v].size());   // dwords
+    } else if (L.key.score == pa::kScoreProfile) {
+        a.table = static_cast<const int16_t *>(pd.profile[v].ptr);
+        a.table_entries = uint32_t(p.profile[v].size());   // dwords
     } else {
         a.table = static_cast<const int16_t *>(pd.table[v].ptr);
         a.table_entries = uint32_t(p.table[v].size());
@@ -259,7 +264,7 @@ pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
         std::vector<uint8_t> forced(b->n_pairs, 0);
         for (uint32_t p : sat) forced[p] = 32;
         pa::BatchView view{b->seq1.data(), b->off1.data(), b->seq2.data(), b->off2.data(), b->n_pairs,
-                           b->pair_method.empty() ? nullptr : b->pair_method.data()};
+                           b->pair_method.empty() ? nullptr : b->pair_method.data(), b->one_query};
         std::string err;
         pa_config cfg = b->cfg;
         if ((st = pa::make_plan(cfg, view, forced.data(), &sat, pd->plan, err)) != PA_OK) return fail(ctx, st, err);
@@ -287,7 +292,7 @@ pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
 void feature_setup(const pa_config &cfg, const pa::BatchView &v, pa::FeatureArgs &fa)
 {
     bool seen[256] = {false};
-    for (uint64_t i = v.off1[0]; i < v.off1[v.n_pairs]; ++i) seen[v.seq1[i]] = true;
+    for (uint64_t i = v.off1[0]; i < v.end1(); ++i) seen[v.seq1[i]] = true;
     for (uint64_t i = v.off2[0]; i < v.off2[v.n_pairs]; ++i) seen[v.seq2[i]] = true;
     int k = 0;
     for (int c = 0; c < 256; ++c) if (seen[c]) ++k;
@@ -303,10 +308,22 @@ pa_status device_features(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView
 {
     pa::FeatureArgs fa{};
     feature_setup(cfg, v, fa);
-    std::vector<uint8_t> s1(v.seq1 + v.off1[0], v.seq1 + v.off1[v.n_pairs]);
+    std::vector<uint8_t> s1;
+    std::vector<uint64_t> o1(v.n_pairs + 1, 0);
+    if (v.one_query) {
+        // the feature kernel reads one first sequence per pair: repeat the query
+        const uint64_t q = v.len1(0);
+        s1.reserve(q * v.n_pairs);
+        for (uint64_t p = 0; p < v.n_pairs; ++p) {
+            s1.insert(s1.end(), v.s1(0), v.s1(0) + q);
+            o1[p + 1] = o1[p] + q;
+        }
+    } else {
+        s1.assign(v.seq1 + v.off1[0], v.seq1 + v.off1[v.n_pairs]);
+        for (uint64_t p = 0; p <= v.n_pairs; ++p) o1[p] = v.off1[p] - v.off1[0];
+    }
     std::vector<uint8_t> s2(v.seq2 + v.off2[0], v.seq2 + v.off2[v.n_pairs]);
-    std::vector<uint64_t> o1(v.off1, v.off1 + v.n_pairs + 1), o2(v.off2, v.off2 + v.n_pairs + 1);
-    for (auto &o : o1) o -= v.off1[0];
+    std::vector<uint64_t> o2(v.off2, v.off2 + v.n_pairs + 1);
     for (auto &o : o2) o -= v.off2[0];
     DeviceBuffer ds1, ds2, do1, do2;
     pa_status st;
@@ -348,7 +365,7 @@ pa_status learned_widths(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView 
     const double smax = std::max(1, pa::max_substitution(cfg));
     forced.assign(v.n_pairs, 0);
     for (uint64_t p = 0; p < v.n_pairs; ++p) {
-        const double mn = double(std::min(v.off1[p + 1] - v.off1[p], v.off2[p + 1] - v.off2[p]));
+        const double mn = double(std::min(v.len1(p), v.off2[p + 1] - v.off2[p]));
         const double pred = std::max(0.0, double(y[p * pa::kModelOutputs])) * smax * mn;
         forced[p] = (pred * 1.10 + 256.0 < 32767.0) ? 16 : 32;
     }
@@ -407,12 +424,12 @@ pa_status pa_ctx_destroy(pa_ctx *ctx)
 
 const char *pa_ctx_last_error(const pa_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
 
-pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
-                          const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs,
-                          const uint8_t *pair_method, pa_batch **out)
+// Creates a batch of pairs (seq1[p], seq2[p]), or with one_query of (seq1[off1[0], off1[1]),
+// seq2[p]) for every p.
+static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                              bool one_query, const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs,
+                              const uint8_t *pair_method, pa_batch **out)
 {
-    if (!ctx || !cfg || !out || !off1 || !off2 || (n_pairs && (!seq1 || !seq2)))
-        return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PA_ERR_DEVICE, "hipSetDevice");
     auto t0 = std::chrono::steady_clock::now();
@@ -425,16 +442,18 @@ pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1
         b->cfg.matrix_symbols = b->symbols.data();
     }
     b->n_pairs = n_pairs;
-    b->off1.assign(off1, off1 + n_pairs + 1);
+    b->one_query = one_query;
+    const uint64_t n1 = one_query ? 1 : n_pairs;
+    b->off1.assign(off1, off1 + n1 + 1);
     b->off2.assign(off2, off2 + n_pairs + 1);
-    b->seq1.assign(seq1 ? seq1 : nullptr, seq1 ? seq1 + off1[n_pairs] : nullptr);
+    b->seq1.assign(seq1 ? seq1 : nullptr, seq1 ? seq1 + off1[n1] : nullptr);
     b->seq2.assign(seq2 ? seq2 : nullptr, seq2 ? seq2 + off2[n_pairs] : nullptr);
     if (pair_method) b->pair_method.assign(pair_method, pair_method + n_pairs);
     if (b->seq1.empty()) b->seq1.push_back(0);
     if (b->seq2.empty()) b->seq2.push_back(0);
 
     pa::BatchView view{b->seq1.data(), b->off1.data(), b->seq2.data(), b->off2.data(), n_pairs,
-                       pair_method ? b->pair_method.data() : nullptr};
+                       pair_method ? b->pair_method.data() : nullptr, one_query};
     std::string err;
     std::vector<uint8_t> forced;
     pa_status st;
@@ -451,6 +470,9 @@ pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1
     if ((st = upload(ctx, b->pairs, p.pairs)) != PA_OK) return st;
     PA_HIP(hipMalloc(&b->scores.ptr, std::max<size_t>(n_pairs * 4, 16)), "hipMalloc");
     PA_HIP(hipMalloc(&b->flags.ptr, (n_pairs + 1) * 4), "hipMalloc");   // + error slot
+    // the initial-score kernel clears the flags and the error slot on every pass; a batch without
+    // pairs launches nothing, so its error slot starts cleared here (outside any graph capture)
+    PA_HIP(hipMemsetAsync(b->flags.ptr, 0, (n_pairs + 1) * 4, ctx->stream), "hipMemsetAsync");
     PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 
     b->stats.n_pairs = n_pairs;
@@ -459,9 +481,29 @@ pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1
     b->stats.n_int16 = p.n16;
     b->stats.n_int32 = p.n32;
     b->stats.n_launches = p.launches.size() + 1;
+    b->stats.n_profiled = p.n_profiled;
     b->stats.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = b.release();
     return PA_OK;
+}
+
+pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                          const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs,
+                          const uint8_t *pair_method, pa_batch **out)
+{
+    if (!ctx || !cfg || !out || !off1 || !off2 || (n_pairs && (!seq1 || !seq2)))
+        return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
+    return create_batch(ctx, cfg, seq1, off1, false, seq2, off2, n_pairs, pair_method, out);
+}
+
+pa_status pa_batch_create_one_to_many(pa_ctx *ctx, const pa_config *cfg, const uint8_t *query, uint64_t query_len,
+                                      const uint8_t *seq2, const uint64_t *off2, uint64_t n_targets,
+                                      const uint8_t *pair_method, pa_batch **out)
+{
+    if (!ctx || !cfg || !out || !off2 || (query_len && !query) || (n_targets && !seq2))
+        return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
+    const uint64_t off1[2] = {0, query_len};
+    return create_batch(ctx, cfg, query, off1, true, seq2, off2, n_targets, pair_method, out);
 }
 
 pa_status pa_batch_run(pa_ctx *ctx, pa_batch *b)
@@ -841,6 +883,19 @@ pa_status pa_score_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1,
 {
     pa_batch *b = nullptr;
     pa_status st = pa_batch_create(ctx, cfg, seq1, off1, seq2, off2, n_pairs, pair_method, &b);
+    if (st != PA_OK) return st;
+    st = pa_batch_run(ctx, b);
+    if (st == PA_OK) st = pa_batch_results(ctx, b, scores, flags);
+    pa_batch_destroy(ctx, b);
+    return st;
+}
+
+pa_status pa_score_one_to_many(pa_ctx *ctx, const pa_config *cfg, const uint8_t *query, uint64_t query_len,
+                               const uint8_t *seq2, const uint64_t *off2, uint64_t n_targets,
+                               const uint8_t *pair_method, int32_t *scores, uint8_t *flags)
+{
+    pa_batch *b = nullptr;
+    pa_status st = pa_batch_create_one_to_many(ctx, cfg, query, query_len, seq2, off2, n_targets, pair_method, &b);
     if (st != PA_OK) return st;
     st = pa_batch_run(ctx, b);
     if (st == PA_OK) st = pa_batch_results(ctx, b, scores, flags);

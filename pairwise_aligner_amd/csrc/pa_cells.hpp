@@ -119,6 +119,11 @@ __device__ __forceinline__ void init_rows(Rows<W, R, ACC> &st, const uint32_t *c
             // one word per two rows: the pair-row table offsets of both halves
             if (r % 2 == 0) st.A[r / 2] = codes[row0 / 2 + r / 2];
             st.Ahi[r] = 0;
+        } else if constexpr (SCORE == kScoreProfile) {
+            // every task's rows are the query: the profile column of row i is at byte 2i
+            (void)codes;
+            if (r == 0) st.A[0] = uint32_t(row0) * 2u;
+            st.Ahi[r] = 0;
         } else {
             const uint32_t code = codes[row0 + r];
             if constexpr (SCORE == kScoreMatrix && W == 16) {
@@ -158,10 +163,13 @@ __device__ __forceinline__ void fold_rows(const Rows<W, R, ACC> &st, typename La
 //   DNA codes: v_perm_b32 over an 8-byte table selected by row code ^ column code.
 //   matrix: reads of the lane's bank-swizzled LDS table copy at row offset + column offset;
 //           split in two so the reads of step k+1 can be issued before the cells of step k.
+//   pair-row table / query profile (int16): one dword per two rows and half, interleaved by
+//           byte permutes.
 template <int W, int R, int SCORE>
 struct ScoreReads {
-    uint32_t lo[SCORE == kScoreMatrix ? R : (SCORE == kScorePairs ? R / 2 : 1)];
-    uint32_t hi[SCORE == kScoreMatrix && W == 16 ? R : (SCORE == kScorePairs ? R / 2 : 1)];
+    static constexpr bool kDwordPairs = SCORE == kScorePairs || SCORE == kScoreProfile;
+    uint32_t lo[SCORE == kScoreMatrix ? R : (kDwordPairs ? R / 2 : 1)];
+    uint32_t hi[SCORE == kScoreMatrix && W == 16 ? R : (kDwordPairs ? R / 2 : 1)];
 };
 
 template <int W, int R, int SCORE, bool ACC>
@@ -174,6 +182,17 @@ __device__ __forceinline__ void issue_scores(const Rows<W, R, ACC> &st, uint32_t
         for (int p = 0; p < R / 2; ++p) {
             rd.lo[p] = *reinterpret_cast<const uint32_t *>(lds + (st.A[p] & 0xFFFFu) + (Bin & 0xFFFFu));
             rd.hi[p] = *reinterpret_cast<const uint32_t *>(lds + (st.A[p] >> 16) + (Bin >> 16));
+        }
+    } else if constexpr (SCORE == kScoreProfile) {
+        // the lane's consecutive rows are consecutive int16 entries of the profile column of
+        // each half's residue: one base address per half, then immediate offsets
+        static_assert(W == 16, "the query profile is an int16 path");
+        const char *plo = lds + (Bin & 0xFFFFu) + st.A[0];
+        const char *phi = lds + (Bin >> 16) + st.A[0];
+#pragma unroll
+        for (int p = 0; p < R / 2; ++p) {
+            rd.lo[p] = *reinterpret_cast<const uint32_t *>(plo + 4 * p);
+            rd.hi[p] = *reinterpret_cast<const uint32_t *>(phi + 4 * p);
         }
     } else if constexpr (SCORE == kScoreMatrix) {
         uint32_t Blo = Bin, Bhi = 0;
@@ -201,7 +220,7 @@ __device__ __forceinline__ void finish_scores(const Rows<W, R, ACC> &st, uint32_
         if constexpr (SCORE == kScoreDna) {
             // formed where it is used (update_cells): two VALU instructions, nothing to hide
             (void)tbl_lo; (void)tbl_hi; (void)Bin;
-        } else if constexpr (SCORE == kScorePairs) {
+        } else if constexpr (SCORE == kScorePairs || SCORE == kScoreProfile) {
             // (low halves) = row 2p of both pairs, (high halves) = row 2p+1
             sc[r] = L::from(__builtin_amdgcn_perm(rd.hi[r / 2], rd.lo[r / 2], (r % 2) ? 0x07060302u : 0x05040100u));
         } else if constexpr (W == 16) {
@@ -360,7 +379,8 @@ struct Capture {
         ev = next(c);
     }
 
-    // free last row: the owner of row m contributes the maximum of that row
+    // free last row (the ROWMAX variants only): the owner of row m contributes the maximum of
+    // that row
     template <bool ACC>
     __device__ __forceinline__ void row_maximum(const Rows<W, R, ACC> &st, const Task &tk, int32_t *scores,
                                                 int dge) const

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     if constexpr (SCORE == kScoreMatrix) {
         load_lds_table(lds_raw, a.table, a.table_entries);
         __syncthreads();
-    } else if constexpr (SCORE == kScorePairs) {
+    } else if constexpr (SCORE == kScorePairs || SCORE == kScoreProfile) {
         load_lds_pairs(lds_raw, reinterpret_cast<const uint32_t *>(a.table), a.table_entries);
         __syncthreads();
     }
@@ -237,7 +237,9 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             for (int u = 0; u < G; ++u) step.template operator()<kMaskAll>(k0 + u);
             chunk_end(k0);
         }
-        if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores, dge);
+        // free last row only: the int16 saturation tracking (TRACK) keeps per-row maxima too, but
+        // they are not candidates for the score
+        if constexpr (ROWMAX) cap.row_maximum(st, tk, a.scores, dge);
         fold_rows<W, R, ACC, DRIFT>(st, runmax, satmax, ge);
         if (has_bnd_out) {
             // the boundary row written by lane G-1 is read by lane 0 of the next stripe
@@ -499,7 +501,9 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                 Eout = L::bits(L::add(L::from(Eout), vrebase));
             }
         }
-        if constexpr (!LOCAL) cap.row_maximum(st, tk, a.scores, dge);
+        // free last row only: the int16 saturation tracking (TRACK) keeps per-row maxima too, but
+        // they are not candidates for the score
+        if constexpr (ROWMAX) cap.row_maximum(st, tk, a.scores, dge);
         fold_rows<W, R, ACC, DRIFT>(st, runmax, satmax, ge);
     }
 
@@ -560,17 +564,28 @@ template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
 static hipError_t launch_variant(const VariantKey &k, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st)
 {
     const uint32_t lds = SCORE != kScoreDna ? lds_bytes : 0;
-    if (k.G == 64) {
-        hipLaunchKernelGGL((wavefront_kernel<W, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(a.ntasks),
-                           dim3(64 * kWfWaves), lds, st, a);
+    const uint32_t threads = 256;
+    const uint32_t blocks = uint32_t((uint64_t(a.ntasks) * 16 + threads - 1) / threads);
+    if constexpr (SCORE == kScoreProfile) {
+        // striped kernel only, at rows per lane whose profile reads are free of bank conflicts
+        if constexpr (R == 8 || R == 16) {
+            return hipErrorInvalidValue;
+        } else {
+            if (k.G != 16) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(blocks), dim3(threads),
+                               lds, st, a);
+            return hipGetLastError();
+        }
+    } else {
+        if (k.G == 64) {
+            hipLaunchKernelGGL((wavefront_kernel<W, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(a.ntasks),
+                               dim3(64 * kWfWaves), lds, st, a);
+            return hipGetLastError();
+        }
+        hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(blocks), dim3(threads), lds,
+                           st, a);
         return hipGetLastError();
     }
-    const uint32_t threads = 256;
-    const uint64_t total = uint64_t(a.ntasks) * 16;
-    const uint32_t blocks = uint32_t((total + threads - 1) / threads);
-    hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(blocks), dim3(threads), lds, st,
-                       a);
-    return hipGetLastError();
 }
 
 template <int W, int R, int SCORE, bool LOCAL>
@@ -596,6 +611,11 @@ static hipError_t dispatch_score(const VariantKey &k, const KernelArgs &a, uint3
         if (k.score == kScorePairs)
             return k.local ? dispatch_flags<W, R, kScorePairs, true>(k, a, lds, st)
                            : dispatch_flags<W, R, kScorePairs, false>(k, a, lds, st);
+        if (k.score == kScoreProfile)
+            return k.local ? dispatch_flags<W, R, kScoreProfile, true>(k, a, lds, st)
+                           : dispatch_flags<W, R, kScoreProfile, false>(k, a, lds, st);
+    } else {
+        if (k.score == kScoreProfile) return hipErrorInvalidValue;
     }
     return k.local ? dispatch_flags<W, R, kScoreMatrix, true>(k, a, lds, st)
                    : dispatch_flags<W, R, kScoreMatrix, false>(k, a, lds, st);

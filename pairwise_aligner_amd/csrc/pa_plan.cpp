@@ -87,7 +87,7 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
     for (int c = 0; c < 256; ++c) alpha.code[c] = -1;
     if (cfg.score_kind == PA_SCORE_UNITARY) {
         bool seen[256] = {false};
-        const uint64_t l1 = b.off1[b.n_pairs], l2 = b.off2[b.n_pairs];
+        const uint64_t l1 = b.end1(), l2 = b.off2[b.n_pairs];
         for (uint64_t i = b.off1[0]; i < l1; ++i) seen[b.seq1[i]] = true;
         for (uint64_t i = b.off2[0]; i < l2; ++i) seen[b.seq2[i]] = true;
         uint32_t k = 0;
@@ -163,7 +163,7 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
             plan.smin = std::min(plan.smin, v);
         }
     // every residue must be in the alphabet (alphabet_rank_map_scalar.hpp:59-63 asserts it)
-    const uint64_t l1 = b.off1[b.n_pairs], l2 = b.off2[b.n_pairs];
+    const uint64_t l1 = b.end1(), l2 = b.off2[b.n_pairs];
     for (uint64_t i = b.off1[0]; i < l1; ++i)
         if (alpha.code[b.seq1[i]] < 0) { err = "residue outside the substitution matrix alphabet"; return PA_ERR_INVALID_SYMBOL; }
     for (uint64_t i = b.off2[0]; i < l2; ++i)
@@ -212,6 +212,7 @@ double task_cost(const VariantKey &key, const Task &t, const Tuning &tuning)
 
 uint32_t lds_bytes(const Plan &plan, const VariantKey &key)
 {
+    if (key.score == kScoreProfile) return plan.profile_ranks * plan.profile_rows * 2;
     if (key.score == kScorePairs) return pair_table_bytes(plan.alphabet);
     if (key.score == kScoreMatrix) return lds_table_bytes(plan.alphabet, plan.stride);
     return 0;
@@ -273,8 +274,12 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         err = "the device path needs gap_open <= 0 and gap_extension <= 0";
         return PA_ERR_UNSUPPORTED;
     }
+    if (b.one_query && b.off1[1] < b.off1[0]) { err = "offsets must be non-decreasing"; return PA_ERR_INVALID_ARGUMENT; }
     for (uint64_t p = 0; p < b.n_pairs; ++p) {
-        if (b.off1[p + 1] < b.off1[p] || b.off2[p + 1] < b.off2[p]) { err = "offsets must be non-decreasing"; return PA_ERR_INVALID_ARGUMENT; }
+        if ((!b.one_query && b.off1[p + 1] < b.off1[p]) || b.off2[p + 1] < b.off2[p]) {
+            err = "offsets must be non-decreasing";
+            return PA_ERR_INVALID_ARGUMENT;
+        }
         if (b.pair_method && b.pair_method[p] > 1) { err = "invalid per-pair method"; return PA_ERR_INVALID_ARGUMENT; }
     }
     plan = Plan{};
@@ -307,6 +312,43 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         }
     }
 
+    // Query profile: one query against many targets, with the query on the row side of every
+    // pair (any orientation but the forced exchange for a free last row).  Built for the rows
+    // the allowed shapes pad the query to, if it fits the LDS of a block.
+    Tuning profile_tuning = plan.tuning;
+    profile_tuning.rows_mask &= kProfileRowsMask;
+    if (!profile_tuning.rows_mask) profile_tuning.rows_mask = kProfileRowsMask;
+    const uint64_t qlen = b.one_query ? b.len1(0) : 0;
+    if (b.one_query && !swap_all && qlen > 0 && qlen <= plan.tuning.wavefront_min_rows) {
+        uint64_t rows = 0;
+        for (uint32_t i = 0; i < 7; ++i)
+            if ((profile_tuning.rows_mask >> i) & 1u) {
+                const uint64_t stripe = 16 * (4 + 2 * i);
+                rows = std::max(rows, (qlen + stripe - 1) / stripe * stripe);
+            }
+        const uint32_t K = plan.score == kScoreDna ? alpha.size + 1 : plan.alphabet;   // with the pad rank
+        if (uint64_t(K) * rows * 2 <= kProfileMaxBytes) {
+            plan.profile_rows = uint32_t(rows);
+            plan.profile_ranks = K;
+            const uint8_t *q = b.s1(0);
+            for (int v = 0; v < kScoreShifts; ++v) {
+                std::vector<uint32_t> &pf = plan.profile[v];
+                pf.assign(size_t(K) * rows / 2, 0xFFFFFFFFu);   // -1 in both halves: pad rows, pad rank
+                for (uint32_t c = 0; c + 1 < K; ++c)
+                    for (uint64_t i = 0; i < qlen; ++i) {
+                        const uint32_t a = uint32_t(alpha.code[q[i]]);
+                        const int32_t sc = plan.score == kScoreDna ? (a == c ? cfg.match : cfg.mismatch)
+                                                                   : int32_t(plan.table[0][a * plan.stride + c]);
+                        const size_t e = size_t(c) * rows + i;
+                        const uint32_t h = uint32_t(uint16_t(int16_t(sc - v * cfg.gap_extension)));
+                        uint32_t &w = pf[e / 2];
+                        w = (e & 1) ? (w & 0xFFFFu) | (h << 16) : (w & 0xFFFF0000u) | h;
+                    }
+            }
+        }
+    }
+    const bool use_profile = !plan.profile[0].empty();
+
     const uint64_t N = b.n_pairs;
     plan.pairs.resize(N);
     std::vector<uint8_t> selected(N, only ? 0 : 1);
@@ -315,10 +357,12 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     std::map<VariantKey, std::vector<Item>> groups;
     for (uint64_t p = 0; p < N; ++p) {
         const bool local = b.pair_method ? b.pair_method[p] == PA_METHOD_LOCAL : cfg.method == PA_METHOD_LOCAL;
-        uint32_t l1 = uint32_t(b.off1[p + 1] - b.off1[p]), l2 = uint32_t(b.off2[p + 1] - b.off2[p]);
+        uint32_t l1 = uint32_t(b.len1(p)), l2 = uint32_t(b.off2[p + 1] - b.off2[p]);
         PairMeta &pm = plan.pairs[p];
         pm.local = local;
-        bool swapped = local ? (l1 > l2) : (symmetric ? (l1 > l2) : swap_all);
+        // the query profile needs the query as the rows; the score does not depend on the
+        // orientation when the end gaps are symmetric (always for the local method)
+        bool swapped = use_profile ? false : local ? (l1 > l2) : (symmetric ? (l1 > l2) : swap_all);
         pm.swapped = swapped;
         pm.m = swapped ? l2 : l1;
         pm.n = swapped ? l1 : l2;
@@ -338,8 +382,10 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         pm.width = width;
         VariantKey key{};
         key.width = width;
-        choose_shape(pm.m, pm.n, plan.tuning, key.G, key.R);
-        key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable[0].empty()) ? kScorePairs : plan.score;
+        const bool profiled = use_profile && width == 16;
+        choose_shape(pm.m, pm.n, profiled ? profile_tuning : plan.tuning, key.G, key.R);
+        if (profiled) { key.score = kScoreProfile; ++plan.n_profiled; }
+        else key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable[0].empty()) ? kScorePairs : plan.score;
         key.local = local;
         key.track = (!local && ((width == 16 && !upper16) || track_all)) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;
@@ -357,6 +403,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     };
     auto col_code = [&](uint8_t sym, uint32_t width, uint8_t score) -> uint32_t {
         const uint32_t c = uint32_t(alpha.code[sym]);
+        if (score == kScoreProfile) return profile_col_offset(c, plan.profile_rows);
         if (plan.score == kScoreDna) return width == 16 ? dna_col16(c) : dna_col32(c);
         if (score == kScorePairs) return pair_col_offset(c);
         return lds_col_offset(c);
@@ -367,6 +414,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         return lds_row_offset(pad_rank, plan.stride);
     };
     auto col_pad = [&](uint32_t width, uint8_t score) -> uint32_t {
+        if (score == kScoreProfile) return profile_col_offset(plan.profile_ranks - 1, plan.profile_rows);
         if (plan.score == kScoreDna) return width == 16 ? kDnaColPad16 : kDnaColPad32;
         if (score == kScorePairs) return pair_col_offset(pad_rank);
         return lds_col_offset(pad_rank);
@@ -375,7 +423,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     auto row_rank = [&](const Item *h, uint32_t i) -> uint32_t {
         if (!h || i >= h->m) return pad_rank;
         const PairMeta &pm = plan.pairs[h->pair];
-        const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h->pair] : b.seq1 + b.off1[h->pair];
+        const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h->pair] : b.s1(h->pair);
         return uint32_t(alpha.code[s[i]]);
     };
 
@@ -435,7 +483,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 t.bnd_off = uint32_t(std::min<uint64_t>(plan.bnd_entries, 0xFFFFFFFFull));
                 plan.bnd_entries += cw;
             }
-            n_rowcodes += uint64_t(t.S) * rows / (key.score == kScorePairs ? 2 : 1);
+            if (key.score != kScoreProfile) n_rowcodes += uint64_t(t.S) * rows / (key.score == kScorePairs ? 2 : 1);
             n_colcodes += cw;
             const uint64_t pc = uint64_t(t.S) * rows * t.N * (key.width == 16 ? 2 : 1);
             L.padded_cells += pc;
@@ -469,13 +517,13 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 plan.rowcodes[size_t(t.row_off) + i / 2] = w;
             }
         }
-        for (uint32_t i = 0; i < (f.score == kScorePairs ? 0 : nrows); ++i) {
+        for (uint32_t i = 0; i < ((f.score == kScorePairs || f.score == kScoreProfile) ? 0 : nrows); ++i) {
             uint32_t w = 0;
             for (int q = 0; q < (f.width == 16 ? 2 : 1); ++q) {
                 uint32_t code;
                 if (h[q] && i < h[q]->m) {
                     const PairMeta &pm = plan.pairs[h[q]->pair];
-                    const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h[q]->pair] : b.seq1 + b.off1[h[q]->pair];
+                    const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h[q]->pair] : b.s1(h[q]->pair);
                     code = row_code(s[i], f.width);
                 } else {
                     code = row_pad(f.width);
@@ -491,7 +539,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 uint32_t code;
                 if (h[q] && j < h[q]->n) {
                     const PairMeta &pm = plan.pairs[h[q]->pair];
-                    const uint8_t *s = pm.swapped ? b.seq1 + b.off1[h[q]->pair] : b.seq2 + b.off2[h[q]->pair];
+                    const uint8_t *s = pm.swapped ? b.s1(h[q]->pair) : b.seq2 + b.off2[h[q]->pair];
                     code = col_code(s[j], f.width, f.score);
                 } else {
                     code = col_pad(f.width, f.score);

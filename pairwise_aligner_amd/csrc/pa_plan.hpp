@@ -26,7 +26,7 @@ namespace pa {
 
 inline constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-enum ScoreCode : uint8_t { kScoreDna = 0, kScoreMatrix = 1, kScorePairs = 2 };
+enum ScoreCode : uint8_t { kScoreDna = 0, kScoreMatrix = 1, kScorePairs = 2, kScoreProfile = 3 };
 
 // Residue encodings.  Unitary scoring over at most four symbols uses a byte-permute lookup:
 // score = perm(tbl_hi, tbl_lo, rowcode ^ colcode), where the xor of the two codes yields the
@@ -65,6 +65,14 @@ inline constexpr uint32_t kPairTableMaxAlphabet = 22;
 inline uint32_t pair_row_offset(uint32_t a1, uint32_t a2, uint32_t K) { return (a1 * K + a2) * K * 4; }
 inline uint32_t pair_col_offset(uint32_t b) { return b * 4; }
 inline uint32_t pair_table_bytes(uint32_t K) { return K * K * K * 4; }
+
+// Query profile (int16 tasks of one query against many targets; striped kernel): the column code
+// of target residue rank b is the byte offset b * rows * 2 of its profile column, and the lane
+// reads its R consecutive rows from there.  Rows per lane 8 and 16 are left out: their lanes'
+// reads would fall on the same LDS banks.
+inline constexpr uint32_t kProfileMaxBytes = 65536;
+inline constexpr uint32_t kProfileRowsMask = 0x7F & ~0x44u;   // R = 4, 6, 10, 12, 14
+inline uint32_t profile_col_offset(uint32_t b, uint32_t rows) { return b * rows * 2; }
 
 struct Task {
     uint32_t row_off;
@@ -134,6 +142,12 @@ struct Plan {
     uint32_t stride = 0, alphabet = 0;        // matrix table geometry (alphabet includes the pad)
     std::vector<int16_t> table[kScoreShifts];      // matrix: alphabet*stride entries
     std::vector<uint32_t> ptable[kScoreShifts];    // pair-row tables (empty: alphabet too large)
+    // Query profile (one query against many targets, int16): entry [b * profile_rows + i] is the
+    // score of query residue i against residue rank b, -1 for pad rows and the pad rank; two
+    // entries per word.  Empty: not used.
+    std::vector<uint32_t> profile[kScoreShifts];
+    uint32_t profile_rows = 0;                // query rows covered (padded for every allowed R)
+    uint32_t profile_ranks = 0;               // residue ranks including the pad
     int32_t smax = 0, smin = 0;               // extreme substitution scores
 
     std::vector<PairMeta> pairs;
@@ -144,6 +158,7 @@ struct Plan {
     uint64_t cells = 0, padded_cells = 0;
     uint64_t n16 = 0, n32 = 0;
     uint64_t n_speculative = 0;               // int16 pairs whose bound does not exclude saturation
+    uint64_t n_profiled = 0;                  // int16 pairs scored through the query profile
     std::vector<uint32_t> launch_order;       // launches in start order (order_launches)
     std::vector<uint8_t> launch_stream;       // stream of each launch (indexed like launches)
 };
@@ -155,6 +170,13 @@ struct BatchView {
     const uint64_t *off2;
     uint64_t n_pairs;
     const uint8_t *pair_method;  // nullable
+    // One query against many targets: every pair's first sequence is seq1[off1[0], off1[1]) (off1
+    // holds two entries).
+    bool one_query = false;
+
+    const uint8_t *s1(uint64_t p) const { return seq1 + off1[one_query ? 0 : p]; }
+    uint64_t len1(uint64_t p) const { return one_query ? off1[1] - off1[0] : off1[p + 1] - off1[p]; }
+    uint64_t end1() const { return off1[one_query ? 1 : n_pairs]; }   // end of the first sequences
 };
 
 // Widths chosen per pair before packing: 16 or 32 (0 = let the policy decide).

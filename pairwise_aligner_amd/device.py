@@ -29,7 +29,7 @@ EXPORTS = ["pa_abi_version", "pa_ctx_create", "pa_ctx_destroy", "pa_ctx_last_err
            "pa_batch_create", "pa_batch_run", "pa_batch_results", "pa_batch_stats_get", "pa_batch_time",
            "pa_batch_destroy", "pa_model_create", "pa_model_destroy", "pa_model_param_count",
            "pa_model_set_params", "pa_model_get_params", "pa_features", "pa_model_forward", "pa_model_loss_grad",
-           "pa_model_train_step", "pa_dp_labels", "pa_tune"]
+           "pa_model_train_step", "pa_dp_labels", "pa_tune", "pa_score_one_to_many", "pa_batch_create_one_to_many"]
 N_FEATURES, N_OUTPUTS, N_HIDDEN = 8, 2, 64
 
 
@@ -94,7 +94,7 @@ class CConfig(ctypes.Structure):
 class CStats(ctypes.Structure):
     _fields_ = [("n_pairs", ctypes.c_uint64), ("cells", ctypes.c_uint64), ("padded_cells", ctypes.c_uint64),
                 ("n_int16", ctypes.c_uint64), ("n_int32", ctypes.c_uint64), ("n_recomputed", ctypes.c_uint64),
-                ("n_launches", ctypes.c_uint64), ("plan_ms", ctypes.c_double)]
+                ("n_launches", ctypes.c_uint64), ("plan_ms", ctypes.c_double), ("n_profiled", ctypes.c_uint64)]
 
 
 @dataclass(frozen=True)
@@ -130,6 +130,9 @@ def lib() -> ctypes.CDLL:
     L.pa_ctx_last_error.restype = ctypes.c_char_p
     L.pa_score_batch.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp, vp]
     L.pa_batch_create.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, ctypes.POINTER(vp)]
+    L.pa_score_one_to_many.argtypes = [vp, ctypes.POINTER(CConfig), vp, u64, vp, vp, u64, vp, vp, vp]
+    L.pa_batch_create_one_to_many.argtypes = [vp, ctypes.POINTER(CConfig), vp, u64, vp, vp, u64, vp,
+                                              ctypes.POINTER(vp)]
     L.pa_batch_run.argtypes = [vp, vp]
     L.pa_batch_results.argtypes = [vp, vp, vp, vp]
     L.pa_batch_stats_get.argtypes = [vp, ctypes.POINTER(CStats)]
@@ -149,7 +152,8 @@ def lib() -> ctypes.CDLL:
     for name in ["pa_ctx_create", "pa_ctx_destroy", "pa_score_batch", "pa_batch_create", "pa_batch_run",
                  "pa_batch_results", "pa_batch_stats_get", "pa_batch_time", "pa_batch_destroy", "pa_model_create",
                  "pa_model_destroy", "pa_model_set_params", "pa_model_get_params", "pa_features",
-                 "pa_model_forward", "pa_model_loss_grad", "pa_model_train_step", "pa_dp_labels", "pa_tune"]:
+                 "pa_model_forward", "pa_model_loss_grad", "pa_model_train_step", "pa_dp_labels", "pa_tune",
+                 "pa_score_one_to_many", "pa_batch_create_one_to_many"]:
         getattr(L, name).restype = i32
     return L
 
@@ -228,6 +232,21 @@ class Context:
         self.check(st)
         return scores[:n], flags[:n]
 
+    def score_one_to_many(self, query, b2, o2, cfg: DeviceConfig, pair_method=None):
+        """Scores of (query, target p) for every packed target (pa_score_one_to_many)."""
+        q = np.frombuffer(bytes(query), dtype=np.uint8).copy() if len(query) else np.zeros(1, dtype=np.uint8)
+        n = len(o2) - 1
+        holder = _CConfigHolder(cfg)
+        scores = np.zeros(max(n, 1), dtype=np.int32)
+        flags = np.zeros(max(n, 1), dtype=np.uint8)
+        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
+        st = self._lib.pa_score_one_to_many(self.handle, ctypes.byref(holder.c), q.ctypes.data, len(query),
+                                            b2.ctypes.data, o2.ctypes.data, n,
+                                            None if pm is None else pm.ctypes.data, scores.ctypes.data,
+                                            flags.ctypes.data)
+        self.check(st)
+        return scores[:n], flags[:n]
+
 
 class Batch:
     """A device-resident batch (pa_batch): planned and uploaded once, run many times."""
@@ -244,6 +263,26 @@ class Batch:
                                       None if pm is None else pm.ctypes.data, ctypes.byref(h))
         ctx.check(st)
         self.handle = h
+
+    @classmethod
+    def one_to_many(cls, ctx: Context, query, b2, o2, cfg: DeviceConfig, pair_method=None) -> "Batch":
+        """A device-resident batch of one query against every packed target
+        (pa_batch_create_one_to_many)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.n = len(o2) - 1
+        q = np.frombuffer(bytes(query), dtype=np.uint8).copy() if len(query) else np.zeros(1, dtype=np.uint8)
+        self._keep = (q, b2, o2)
+        self.handle = None
+        holder = _CConfigHolder(cfg)
+        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
+        h = ctypes.c_void_p()
+        st = ctx._lib.pa_batch_create_one_to_many(ctx.handle, ctypes.byref(holder.c), q.ctypes.data, len(query),
+                                                  b2.ctypes.data, o2.ctypes.data, self.n,
+                                                  None if pm is None else pm.ctypes.data, ctypes.byref(h))
+        ctx.check(st)
+        self.handle = h
+        return self
 
     def run(self):
         self.ctx.check(self.ctx._lib.pa_batch_run(self.ctx.handle, self.handle))

@@ -173,6 +173,22 @@ def test_saturation_recompute(ctx):
         assert not np.any(flags[6:] & device.PAIR_RECOMPUTED)
 
 
+@pytest.mark.parametrize("ends", [(False, False), (True, False), (False, True), (True, True)],
+                         ids=["penalised", "last-column-free", "last-row-free", "both-free"])
+def test_tracked_int16_global_scores(ctx, ends):
+    """int16 global pairs whose bound does not rule out saturation (tracked for the int32
+    recompute) but that stay below it: the score is H(m, n) (or the free-end maximum), never the
+    maximum of the last row that the tracking keeps for its saturation test."""
+    lc, lr = ends
+    rng = np.random.default_rng(22)
+    s1, s2 = workloads.dna_pairs(rng, 120, 1100, 2200, divergence=0.4, indel_fraction=0.3)
+    dc = device.DeviceConfig(match=30, mismatch=-5, gap_open=-10, gap_extension=-1, last_column_free=lc,
+                             last_row_free=lr, lane_policy=device.LANES_INT16)
+    got, flags = check(ctx, s1, s2, dc, label=f"tracked int16 ends={ends}")
+    stayed = ((flags & device.PAIR_INT16) != 0) & ((flags & device.PAIR_RECOMPUTED) == 0)
+    assert np.count_nonzero(stayed) > 30   # scored by the tracked int16 kernels
+
+
 def test_empty_and_tiny(ctx):
     s1 = [b"", b"A", b"", b"ACGT", b"G", b"", b"AC"]
     s2 = [b"", b"", b"A", b"", b"G", b"ACGTT", b"CA"]
@@ -182,6 +198,10 @@ def test_empty_and_tiny(ctx):
             dc = device.DeviceConfig(method=method, first_column_free=fc, first_row_free=fr, last_column_free=lc,
                                      last_row_free=lr)
             check(ctx, s1, s2, dc, label=f"empty {flags} {method}")
+    # a batch without pairs launches nothing and reports no error, however often it is made
+    for _ in range(4):
+        got, _ = ctx.score([], [], device.DeviceConfig())
+        assert len(got) == 0
 
 
 def test_invalid_symbol_is_an_error(ctx):
