@@ -71,6 +71,10 @@ def make_workload(name, pairs=None):
         w = workloads.config_protein(**count)
         dc = blosum_config(device.METHOD_LOCAL, -11, -1, device.LANES_AUTO)
         return w, dc, "local"
+    if name == "protein_1xn":
+        w = workloads.config_protein_1xn(**count)
+        dc = blosum_config(device.METHOD_LOCAL, -11, -1, device.LANES_AUTO)
+        return w, dc, "local"
     if name == "uneven":
         w = workloads.config_uneven()
         dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
@@ -152,7 +156,20 @@ def run(name, steps, warmup, with_cpu=True, autotune=False, pairs=None):
         dc = dataclasses.replace(dc, tuning=tuning)
         tuned = dict(tuning=dataclasses.asdict(tuning), seconds=round(time.perf_counter() - t1, 2))
         log(f"[bench] {name}: autotuned {tuned}")
-    batch = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
+    one_to_one = None
+    if "query" in w:
+        # one query against many targets: the query-profile path, beside the one-to-one path
+        # on the same pairs (the query repeated)
+        ref = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
+        for _ in range(warmup):
+            ref.run()
+        ref_total, ref_main = ref.time(steps)
+        ref_scores, _ = ref.results()
+        ref.close()
+        one_to_one = dict(ms_per_step=ref_total / steps, main_kernel_ms=ref_main)
+        batch = device.Batch.one_to_many(ctx, w["query"], w["b2"], w["o2"], dc, w.get("pair_method"))
+    else:
+        batch = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
     st = batch.stats()
     log(f"[bench] plan: {st}")
     for _ in range(warmup):
@@ -175,6 +192,10 @@ def run(name, steps, warmup, with_cpu=True, autotune=False, pairs=None):
     res = dict(name=name, pairs=n, cells=cells, ms_per_step=ms_step, main_kernel_ms=main_ms, wall_s=wall,
                gcups=gcups, achieved_tops=achieved, peak_tops=peak, ops_per_cell=ops, stats=st,
                inputs_sha256=workloads.digest(w["b1"], w["o1"], w["b2"], w["o2"]), autotuned=tuned)
+    if one_to_one is not None:
+        one_to_one["scores_identical"] = bool(np.array_equal(ref_scores, scores))
+        one_to_one["speedup_of_profile"] = one_to_one["ms_per_step"] / ms_step
+        res["one_to_one"] = one_to_one
     if with_cpu:
         res["cpu"] = cpu_baseline(w, dc, scores)
     log(f"[bench] {name}: {gcups:.1f} GCUPS ({ms_step:.3f} ms/step, main kernels {main_ms:.3f} ms), "
@@ -302,6 +323,9 @@ def main():
         extra = run("dna150", max(args.steps, 50), args.warmup, with_cpu=False)
     if not args.no_extra and args.workload != "uneven":
         uneven = run("uneven", 5, 1, with_cpu=False, autotune=True)
+    otm = None
+    if not args.no_extra and args.workload != "protein_1xn":
+        otm = run("protein_1xn", max(args.steps, 10), args.warmup, with_cpu=False)
     if not args.no_extra:
         mixed = run_mixed(args.mixed_pairs, 1, 1)
 
@@ -324,7 +348,10 @@ def main():
         "config": {"workload": {"protein": "protein local BLOSUM62, 100,000 pairs, lengths 50-500, gap (-11,-1)",
                                 "dna150": "DNA global, 10,000 pairs x 150 nt, +4/-5, gap (-10,-1)",
                                 "uneven": "1,000,000 DNA pairs lognormal lengths, half global half local",
-                                "long": "2,000 DNA pairs 10-100 kbp, int32"}.get(args.workload, args.workload)
+                                "long": "2,000 DNA pairs 10-100 kbp, int32",
+                                "protein_1xn": "one protein query (300 aa) against 200,000 targets 50-500, "
+                                               "local BLOSUM62, gap (-11,-1), query profile"}.get(args.workload,
+                                                                                                   args.workload)
                    if args.pairs is None else f"{args.workload} workload with {args.pairs} pairs",
                    "pairs": main_res["pairs"], "cells": main_res["cells"],
                    "inputs_sha256": main_res["inputs_sha256"]},
@@ -335,8 +362,12 @@ def main():
         "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample", "value_1core",
                                              "sample_1core", "isa")},
         "mismatches_vs_oracle_sample": cpu["mismatches_vs_gpu"],
-        "plan": {k: main_res["stats"][k] for k in ("n_int16", "n_int32", "n_recomputed", "padded_cells", "n_launches")},
+        "plan": {k: main_res["stats"][k] for k in ("n_int16", "n_int32", "n_recomputed", "padded_cells", "n_launches",
+                                                   "n_profiled")},
     }
+    if main_res.get("one_to_one"):
+        line["one_to_one_same_pairs"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                                         for k, v in main_res["one_to_one"].items()}
     if main_res.get("autotuned"):
         line["autotuned"] = main_res["autotuned"]
     if mixed is not None:
@@ -347,6 +378,13 @@ def main():
             "pairs": uneven["pairs"], "cells": uneven["cells"], "autotuned": uneven["autotuned"],
             "roofline_frac": round(uneven["achieved_tops"] / uneven["peak_tops"], 4),
             "n_launches": uneven["stats"]["n_launches"], "plan_ms": round(uneven["stats"]["plan_ms"], 1)}
+    if otm is not None:
+        line["protein_1xn_query_profile"] = {
+            "gcups": round(otm["gcups"], 2), "ms_per_step": round(otm["ms_per_step"], 4),
+            "roofline_frac": round(otm["achieved_tops"] / otm["peak_tops"], 4),
+            "one_to_one_ms_per_step": round(otm["one_to_one"]["ms_per_step"], 4),
+            "speedup_of_profile": round(otm["one_to_one"]["speedup_of_profile"], 3),
+            "scores_identical": otm["one_to_one"]["scores_identical"], "n_profiled": otm["stats"]["n_profiled"]}
     if extra is not None:
         line["dna150_global_int16"] = {"gcups": round(extra["gcups"], 2), "ms_per_step": round(extra["ms_per_step"], 4),
                                        "kernel_ms": round(extra["main_kernel_ms"], 4),

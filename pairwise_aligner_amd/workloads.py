@@ -164,6 +164,19 @@ def config_protein(seed: int = 2, count: int = 100_000):
                 scoring=dict(kind="blosum62"), gap=(-11, -1), method="local")
 
 
+def config_protein_1xn(seed: int = 6, count: int = 200_000, query_len: int = 300):
+    """One protein query against `count` targets (a database search): lengths of the targets
+    uniform 50..500, background residue frequencies.  b1/o1 repeat the query for the one-to-one
+    path and the CPU baseline."""
+    rng = np.random.default_rng(seed)
+    q = random_residues(rng, AMINO, query_len, AMINO_FREQ)
+    b2, o2 = random_packed(rng, rng.integers(50, 501, size=count), AMINO, AMINO_FREQ)
+    b1 = np.tile(q, count) if count else np.zeros(1, dtype=np.uint8)
+    o1 = (np.arange(count + 1, dtype=np.uint64) * np.uint64(query_len))
+    return dict(name="protein_1xn", query=q.tobytes(), b1=b1, o1=o1, b2=b2, o2=o2, pair_method=None,
+                scoring=dict(kind="blosum62"), gap=(-11, -1), method="local")
+
+
 def config_uneven(seed: int = 3, count: int = 1_000_000):
     rng = np.random.default_rng(seed)
     L = np.clip(np.round(rng.lognormal(np.log(250.0), 0.8, size=count)), 50, 10_000).astype(np.int64)
