@@ -140,7 +140,7 @@ def cpu_baseline(w, dc, gpu_scores, budget_s=12.0):
                 isa=isa, mismatches_vs_gpu=mism, sample_pairs=sample)
 
 
-def run(name, steps, warmup, with_cpu=True, autotune=False, pairs=None):
+def run(name, steps, warmup, with_cpu=True, autotune=False, pairs=None, e2e_reps=0):
     t0 = time.perf_counter()
     w, dc, method = make_workload(name, pairs)
     n = len(w["o1"]) - 1
@@ -201,7 +201,34 @@ def run(name, steps, warmup, with_cpu=True, autotune=False, pairs=None):
     log(f"[bench] {name}: {gcups:.1f} GCUPS ({ms_step:.3f} ms/step, main kernels {main_ms:.3f} ms), "
         f"{achieved:.2f} of {peak:.1f} TOP/s")
     batch.close()
+    if e2e_reps:
+        res["end_to_end"] = end_to_end(ctx, w, dc, cells, scores, e2e_reps)
     return res
+
+
+def end_to_end(ctx, w, dc, cells, want_scores, reps):
+    """End-to-end time of the one-shot entry point (pa_score_batch / pa_score_one_to_many): host
+    planning, upload of the sequences, the pass, download of scores and flags; the median of
+    `reps` calls after one warm-up call, and the host planning time of one call."""
+    def once():
+        t0 = time.perf_counter()
+        if "query" in w:
+            got, _ = ctx.score_one_to_many(w["query"], w["b2"], w["o2"], dc, w.get("pair_method"))
+        else:
+            got, _ = ctx.score_packed(w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
+        return time.perf_counter() - t0, got
+    _, got = once()
+    times = [once()[0] for _ in range(reps)]
+    med = float(np.median(times))
+    if "query" in w:
+        b = device.Batch.one_to_many(ctx, w["query"], w["b2"], w["o2"], dc, w.get("pair_method"))
+    else:
+        b = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
+    plan_ms = b.stats()["plan_ms"]
+    b.close()
+    log(f"[bench] end to end: median {med * 1e3:.1f} ms over {reps} calls, planning {plan_ms:.1f} ms")
+    return dict(median_ms=round(med * 1e3, 3), gcups=round(cells / med / 1e9, 2), reps=reps,
+                plan_and_upload_ms=round(plan_ms, 2), scores_identical=bool(np.array_equal(got, want_scores)))
 
 
 def run_mixed(count, steps, warmup, with_oracle=True):
@@ -279,6 +306,8 @@ def main():
     ap.add_argument("--valu-probe", action="store_true",
                     help="only run the VALU issue-rate probe (pairwise_aligner_amd/valu_probe): cycles per "
                          "instruction of one wave and the shader clock for the kernels' instructions")
+    ap.add_argument("--e2e-reps", type=int, default=10,
+                    help="calls of the one-shot entry point timed end to end (0: skip)")
     ap.add_argument("--pairs", type=int, default=None,
                     help="pair count of the dna150 / protein workload (default: 10,000 / 100,000)")
     ap.add_argument("--mixed-pairs", type=int, default=100_000,
@@ -317,12 +346,13 @@ def main():
                           "data": "synthetic", "config": {"workload": f"mixed DNA/protein, {args.mixed_pairs} pairs"},
                           "mixed_learned_dispatch": res}))
         return
-    main_res = run(args.workload, args.steps, args.warmup, autotune=args.workload == "uneven", pairs=args.pairs)
+    main_res = run(args.workload, args.steps, args.warmup, autotune=args.workload == "uneven", pairs=args.pairs,
+                   e2e_reps=args.e2e_reps)
     extra = mixed = uneven = None
     if not args.no_extra and args.workload != "dna150":
         extra = run("dna150", max(args.steps, 50), args.warmup, with_cpu=False)
     if not args.no_extra and args.workload != "uneven":
-        uneven = run("uneven", 5, 1, with_cpu=False, autotune=True)
+        uneven = run("uneven", 5, 1, with_cpu=False, autotune=True, e2e_reps=min(args.e2e_reps, 5))
     otm = None
     if not args.no_extra and args.workload != "protein_1xn":
         otm = run("protein_1xn", max(args.steps, 10), args.warmup, with_cpu=False)
@@ -365,6 +395,8 @@ def main():
         "plan": {k: main_res["stats"][k] for k in ("n_int16", "n_int32", "n_recomputed", "padded_cells", "n_launches",
                                                    "n_profiled")},
     }
+    if main_res.get("end_to_end"):
+        line["end_to_end"] = main_res["end_to_end"]
     if main_res.get("one_to_one"):
         line["one_to_one_same_pairs"] = {k: (round(v, 4) if isinstance(v, float) else v)
                                          for k, v in main_res["one_to_one"].items()}
@@ -377,7 +409,8 @@ def main():
             "gcups": round(uneven["gcups"], 2), "ms_per_step": round(uneven["ms_per_step"], 3),
             "pairs": uneven["pairs"], "cells": uneven["cells"], "autotuned": uneven["autotuned"],
             "roofline_frac": round(uneven["achieved_tops"] / uneven["peak_tops"], 4),
-            "n_launches": uneven["stats"]["n_launches"], "plan_ms": round(uneven["stats"]["plan_ms"], 1)}
+            "n_launches": uneven["stats"]["n_launches"], "plan_ms": round(uneven["stats"]["plan_ms"], 1),
+            "end_to_end": uneven.get("end_to_end")}
     if otm is not None:
         line["protein_1xn_query_profile"] = {
             "gcups": round(otm["gcups"], 2), "ms_per_step": round(otm["ms_per_step"], 4),

@@ -28,7 +28,8 @@ struct DeviceBuffer {
 
 struct PlanDevice {
     pa::Plan plan;
-    DeviceBuffer tasks, rowcodes, colcodes, bnd, subset;
+    DeviceBuffer tasks, rowcodes, colcodes, bnd, subset, enc, codes;
+    std::vector<pa::CodeTables> code_host;   // per launch (kept until the encoder has read them)
     DeviceBuffer table[pa::kScoreShifts], ptable[pa::kScoreShifts], profile[pa::kScoreShifts];
     std::vector<uint32_t> subset_host;
 };
@@ -56,6 +57,7 @@ struct pa_batch {
     std::vector<uint64_t> off1, off2;
     uint64_t n_pairs = 0;
     bool one_query = false;   // seq1 is one query scored against every seq2 (off1 = {0, length})
+    DeviceBuffer residues;    // seq1 then seq2 bytes on the device (upload_residues)
     PlanDevice main;
     std::unique_ptr<PlanDevice> recompute;   // int32 re-plan of the pairs that saturated
     std::vector<uint32_t> recompute_set;
@@ -111,12 +113,53 @@ pa_status upload(pa_ctx *ctx, DeviceBuffer &buf, const std::vector<T> &host)
     return PA_OK;
 }
 
-pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd)
+// Uploads the residue bytes of a batch as one device buffer: seq1 bytes [0, end1), then the seq2
+// bytes (the layout the plan's EncodeTask offsets refer to).
+pa_status upload_residues(pa_ctx *ctx, const pa::BatchView &v, DeviceBuffer &buf)
+{
+    const uint64_t n1 = v.end1(), n2 = v.off2[v.n_pairs];
+    const size_t bytes = std::max<size_t>(n1 + n2, 16);
+    PA_HIP(hipMalloc(&buf.ptr, bytes), "hipMalloc");
+    buf.bytes = bytes;
+    char *dst = static_cast<char *>(buf.ptr);
+    if (n1) PA_HIP(hipMemcpyAsync(dst, v.seq1, n1, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+    if (n2) PA_HIP(hipMemcpyAsync(dst + n1, v.seq2, n2, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+    return PA_OK;
+}
+
+// Uploads a plan and encodes its residue streams on the device from `residues` (upload_residues).
+pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd, const DeviceBuffer &residues)
 {
     pa_status st;
-    if ((st = upload(ctx, pd.tasks, pd.plan.tasks)) != PA_OK) return st;
-    if ((st = upload(ctx, pd.rowcodes, pd.plan.rowcodes)) != PA_OK) return st;
-    if ((st = upload(ctx, pd.colcodes, pd.plan.colcodes)) != PA_OK) return st;
+    const pa::Plan &p = pd.plan;
+    if ((st = upload(ctx, pd.tasks, p.tasks)) != PA_OK) return st;
+    if ((st = upload(ctx, pd.enc, p.enc)) != PA_OK) return st;
+    PA_HIP(hipMalloc(&pd.rowcodes.ptr, std::max<size_t>(p.n_rowcodes * 4, 16)), "hipMalloc");
+    pd.rowcodes.bytes = std::max<size_t>(p.n_rowcodes * 4, 16);
+    PA_HIP(hipMalloc(&pd.colcodes.ptr, std::max<size_t>(p.n_colcodes * 4, 16)), "hipMalloc");
+    pd.colcodes.bytes = std::max<size_t>(p.n_colcodes * 4, 16);
+    pd.code_host.resize(p.launches.size());
+    for (size_t l = 0; l < p.launches.size(); ++l) pa::code_tables(p, p.launches[l].key, pd.code_host[l]);
+    if ((st = upload(ctx, pd.codes, pd.code_host)) != PA_OK) return st;
+    for (size_t l = 0; l < p.launches.size(); ++l) {
+        const pa::Launch &L = p.launches[l];
+        pa::EncodeArgs ea{};
+        ea.tasks = static_cast<const pa::Task *>(pd.tasks.ptr);
+        ea.enc = static_cast<const pa::EncodeTask *>(pd.enc.ptr);
+        ea.task_begin = L.task_begin;
+        ea.task_end = L.task_end;
+        ea.residues = static_cast<const uint8_t *>(residues.ptr);
+        ea.codes = static_cast<const pa::CodeTables *>(pd.codes.ptr) + l;
+        ea.rowcodes = static_cast<uint32_t *>(pd.rowcodes.ptr);
+        ea.colcodes = static_cast<uint32_t *>(pd.colcodes.ptr);
+        ea.stripe_rows = uint32_t(L.key.G) * L.key.R;
+        ea.G = L.key.G;
+        ea.width = L.key.width;
+        ea.score = L.key.score;
+        ea.pair_K = p.alphabet;
+        hipError_t e = pa::launch_encode(ea, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "encode launch");
+    }
     for (int v = 0; v < pa::kScoreShifts; ++v) {
         if ((st = upload(ctx, pd.table[v], pd.plan.table[v])) != PA_OK) return st;
         if ((st = upload(ctx, pd.ptable[v], pd.plan.ptable[v])) != PA_OK) return st;
@@ -269,7 +312,7 @@ pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
         pa_config cfg = b->cfg;
         if ((st = pa::make_plan(cfg, view, forced.data(), &sat, pd->plan, err)) != PA_OK) return fail(ctx, st, err);
         pd->subset_host = sat;
-        if ((st = upload_plan(ctx, *pd)) != PA_OK) return st;
+        if ((st = upload_plan(ctx, *pd, b->residues)) != PA_OK) return st;
         b->recompute = std::move(pd);
         b->recompute_set = sat;
     }
@@ -466,7 +509,8 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
     // Only int16 pairs whose bound does not exclude saturation need the flag read-back.
     b->can_saturate = p.n_speculative > 0;
 
-    if ((st = upload_plan(ctx, b->main)) != PA_OK) return st;
+    if ((st = upload_residues(ctx, view, b->residues)) != PA_OK) return st;
+    if ((st = upload_plan(ctx, b->main, b->residues)) != PA_OK) return st;
     if ((st = upload(ctx, b->pairs, p.pairs)) != PA_OK) return st;
     PA_HIP(hipMalloc(&b->scores.ptr, std::max<size_t>(n_pairs * 4, 16)), "hipMalloc");
     PA_HIP(hipMalloc(&b->flags.ptr, (n_pairs + 1) * 4), "hipMalloc");   // + error slot
@@ -734,7 +778,8 @@ pa_status pa_dp_labels(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, c
     std::string err;
     pa_status st = pa::make_plan(*cfg, view, forced.data(), nullptr, b.main.plan, err, true);
     if (st != PA_OK) return fail(ctx, st, err);
-    if ((st = upload_plan(ctx, b.main)) != PA_OK) return st;
+    if ((st = upload_residues(ctx, view, b.residues)) != PA_OK) return st;
+    if ((st = upload_plan(ctx, b.main, b.residues)) != PA_OK) return st;
     if ((st = upload(ctx, b.pairs, b.main.plan.pairs)) != PA_OK) return st;
     PA_HIP(hipMalloc(&b.scores.ptr, n_pairs * 4), "hipMalloc");
     PA_HIP(hipMalloc(&b.flags.ptr, (n_pairs + 1) * 4), "hipMalloc");

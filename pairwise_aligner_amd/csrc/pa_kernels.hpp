@@ -39,7 +39,24 @@ struct InitArgs {
     uint32_t flag_or;         // extra flag bits written with the initial value
 };
 
+// Residue-stream encoder (pa_encode.hip): writes the row and column codes of tasks
+// [task_begin, task_end) of one launch from the batch's residue bytes.
+struct EncodeArgs {
+    const Task *tasks;          // the plan's tasks
+    const EncodeTask *enc;      // per task: residue sources
+    uint32_t task_begin, task_end;
+    const uint8_t *residues;    // seq1 bytes, then seq2 bytes
+    const CodeTables *codes;    // this launch's code tables (device copy)
+    uint32_t *rowcodes, *colcodes;
+    uint32_t stripe_rows;       // G * R
+    uint32_t G;
+    uint32_t width;             // 16 or 32
+    uint32_t score;             // ScoreCode
+    uint32_t pair_K;            // pair-row table: residue ranks with the pad
+};
+
 hipError_t launch_stripe(const VariantKey &key, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st);
+hipError_t launch_encode(const EncodeArgs &a, hipStream_t st);
 hipError_t launch_init(const InitArgs &a, hipStream_t st);
 
 } // namespace pa

@@ -2,6 +2,7 @@
 #include "pa_plan.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -76,20 +77,63 @@ RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n)
 
 namespace {
 
+// Runs body(i) for i in [0, n) on up to 16 threads, in blocks of 256 indices.
+template <typename F>
+void parallel_for(size_t n, F &&body)
+{
+    const size_t block = 256;
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = unsigned(std::min<size_t>(hw, (n + block - 1) / block));
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; ++i) body(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            const size_t b = next.fetch_add(block);
+            if (b >= n) return;
+            for (size_t i = b; i < std::min(n, b + block); ++i) body(i);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned k = 1; k < nt; ++k) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+}
+
+
 struct Alphabet {
     bool dna = false;
     int16_t code[256];   // symbol -> code (DNA 0..3) or rank (matrix), -1 if absent
     uint32_t size = 0;
 };
 
+// Marks the byte values that occur in the batch's sequences (in parallel over blocks of bytes).
+void scan_residues(const BatchView &b, bool (&seen)[256])
+{
+    const uint8_t *src[2] = {b.seq1 + b.off1[0], b.seq2 + b.off2[0]};
+    const uint64_t len[2] = {b.end1() - b.off1[0], b.off2[b.n_pairs] - b.off2[0]};
+    constexpr uint64_t kBlock = uint64_t(1) << 16;
+    const uint64_t nb0 = (len[0] + kBlock - 1) / kBlock, nb1 = (len[1] + kBlock - 1) / kBlock;
+    std::vector<std::array<uint8_t, 256>> marks(nb0 + nb1);
+    parallel_for(nb0 + nb1, [&](size_t k) {
+        const int side = k < nb0 ? 0 : 1;
+        const uint64_t lo = (side ? k - nb0 : k) * kBlock, hi = std::min(len[side], lo + kBlock);
+        std::array<uint8_t, 256> &m = marks[k];
+        m.fill(0);
+        for (uint64_t i = lo; i < hi; ++i) m[src[side][i]] = 1;
+    });
+    for (const auto &m : marks)
+        for (int c = 0; c < 256; ++c) seen[c] = seen[c] || m[c];
+}
+
 pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, Alphabet &alpha, std::string &err)
 {
     for (int c = 0; c < 256; ++c) alpha.code[c] = -1;
     if (cfg.score_kind == PA_SCORE_UNITARY) {
         bool seen[256] = {false};
-        const uint64_t l1 = b.end1(), l2 = b.off2[b.n_pairs];
-        for (uint64_t i = b.off1[0]; i < l1; ++i) seen[b.seq1[i]] = true;
-        for (uint64_t i = b.off2[0]; i < l2; ++i) seen[b.seq2[i]] = true;
+        scan_residues(b, seen);
         uint32_t k = 0;
         for (int c = 0; c < 256; ++c) if (seen[c]) alpha.code[c] = int16_t(k++);
         alpha.size = k;
@@ -163,11 +207,10 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
             plan.smin = std::min(plan.smin, v);
         }
     // every residue must be in the alphabet (alphabet_rank_map_scalar.hpp:59-63 asserts it)
-    const uint64_t l1 = b.end1(), l2 = b.off2[b.n_pairs];
-    for (uint64_t i = b.off1[0]; i < l1; ++i)
-        if (alpha.code[b.seq1[i]] < 0) { err = "residue outside the substitution matrix alphabet"; return PA_ERR_INVALID_SYMBOL; }
-    for (uint64_t i = b.off2[0]; i < l2; ++i)
-        if (alpha.code[b.seq2[i]] < 0) { err = "residue outside the substitution matrix alphabet"; return PA_ERR_INVALID_SYMBOL; }
+    bool seen[256] = {false};
+    scan_residues(b, seen);
+    for (int c = 0; c < 256; ++c)
+        if (seen[c] && alpha.code[c] < 0) { err = "residue outside the substitution matrix alphabet"; return PA_ERR_INVALID_SYMBOL; }
     return PA_OK;
 }
 
@@ -176,31 +219,6 @@ struct Item {
     uint32_t m, n;
     uint32_t S;
 };
-
-// Runs body(i) for i in [0, n) on up to 16 threads, in blocks of 256 indices.
-template <typename F>
-void parallel_for(size_t n, F &&body)
-{
-    const size_t block = 256;
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const unsigned nt = unsigned(std::min<size_t>(hw, (n + block - 1) / block));
-    if (nt <= 1) {
-        for (size_t i = 0; i < n; ++i) body(i);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    auto worker = [&]() {
-        for (;;) {
-            const size_t b = next.fetch_add(block);
-            if (b >= n) return;
-            for (size_t i = b; i < std::min(n, b + block); ++i) body(i);
-        }
-    };
-    std::vector<std::thread> pool;
-    for (unsigned k = 1; k < nt; ++k) pool.emplace_back(worker);
-    worker();
-    for (auto &th : pool) th.join();
-}
 
 } // namespace
 
@@ -216,6 +234,37 @@ uint32_t lds_bytes(const Plan &plan, const VariantKey &key)
     if (key.score == kScorePairs) return pair_table_bytes(plan.alphabet);
     if (key.score == kScoreMatrix) return lds_table_bytes(plan.alphabet, plan.stride);
     return 0;
+}
+
+void code_tables(const Plan &plan, const VariantKey &key, CodeTables &out)
+{
+    const bool w16 = key.width == 16;
+    const uint32_t pad = plan.pad_rank;
+    for (int sym = 0; sym < 256; ++sym) {
+        const int16_t code = plan.code[sym];
+        const uint32_t c = code < 0 ? 0u : uint32_t(code);   // absent bytes never occur
+        uint32_t r = 0, k = 0;
+        switch (key.score) {
+        case kScoreDna:
+            r = w16 ? dna_row16(c) : dna_row32(c);
+            k = w16 ? dna_col16(c) : dna_col32(c);
+            break;
+        case kScorePairs: r = c; k = pair_col_offset(c); break;   // rows: the rank itself
+        case kScoreProfile: r = 0; k = profile_col_offset(c, plan.profile_rows); break;
+        default: r = lds_row_offset(c, plan.stride); k = lds_col_offset(c); break;
+        }
+        out.row[sym] = r;
+        out.col[sym] = k;
+    }
+    switch (key.score) {
+    case kScoreDna:
+        out.row_pad = w16 ? kDnaRowPad16 : kDnaRowPad32;
+        out.col_pad = w16 ? kDnaColPad16 : kDnaColPad32;
+        break;
+    case kScorePairs: out.row_pad = pad; out.col_pad = pair_col_offset(pad); break;
+    case kScoreProfile: out.row_pad = 0; out.col_pad = profile_col_offset(plan.profile_ranks - 1, plan.profile_rows); break;
+    default: out.row_pad = lds_row_offset(pad, plan.stride); out.col_pad = lds_col_offset(pad); break;
+    }
 }
 
 void order_launches(Plan &plan)
@@ -355,7 +404,11 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     if (only) for (uint32_t p : *only) selected[p] = 1;
 
     std::map<VariantKey, std::vector<Item>> groups;
-    for (uint64_t p = 0; p < N; ++p) {
+    // Per pair, in parallel: orientation, lane width and kernel variant.
+    std::vector<VariantKey> pair_key(N);
+    std::vector<uint8_t> speculative(N, 0);   // int16 without a proof against saturation
+    std::atomic<bool> out_of_range{false};
+    parallel_for(N, [&](size_t p) {
         const bool local = b.pair_method ? b.pair_method[p] == PA_METHOD_LOCAL : cfg.method == PA_METHOD_LOCAL;
         uint32_t l1 = uint32_t(b.len1(p)), l2 = uint32_t(b.off2[p + 1] - b.off2[p]);
         PairMeta &pm = plan.pairs[p];
@@ -367,10 +420,9 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         pm.m = swapped ? l2 : l1;
         pm.n = swapped ? l1 : l2;
         pm.width = 0;
-        if (!selected[p] || pm.m == 0 || pm.n == 0) continue;
-        plan.cells += uint64_t(pm.m) * pm.n;
+        if (!selected[p] || pm.m == 0 || pm.n == 0) return;
         const RangeBound rb = range_bound(plan, local, pm.m, pm.n);
-        if (rb.lower < INT32_MIN / 2 || rb.upper > INT32_MAX / 2) { err = "scores may leave the int32 range"; return PA_ERR_RANGE; }
+        if (rb.lower < INT32_MIN / 2 || rb.upper > INT32_MAX / 2) { out_of_range = true; return; }
         const bool lower16 = rb.lower > -32700;
         const bool upper16 = rb.upper < 32767;
         uint8_t width;
@@ -380,66 +432,54 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         else width = (lower16 && upper16) ? 16 : 32;
         if (width == 16 && !lower16) width = 32;   // the lower bound is never speculated on
         pm.width = width;
-        VariantKey key{};
+        speculative[p] = width == 16 && !upper16;
+        VariantKey &key = pair_key[p];
         key.width = width;
         const bool profiled = use_profile && width == 16;
         choose_shape(pm.m, pm.n, profiled ? profile_tuning : plan.tuning, key.G, key.R);
-        if (profiled) { key.score = kScoreProfile; ++plan.n_profiled; }
+        if (profiled) key.score = kScoreProfile;
         else key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable[0].empty()) ? kScorePairs : plan.score;
         key.local = local;
         key.track = (!local && ((width == 16 && !upper16) || track_all)) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;
+    });
+    if (out_of_range) { err = "scores may leave the int32 range"; return PA_ERR_RANGE; }
+    // Group the pairs by variant (in pair order, so that planning is deterministic).
+    const VariantKey *last_key = nullptr;
+    std::vector<Item> *last_items = nullptr;
+    for (uint64_t p = 0; p < N; ++p) {
+        const PairMeta &pm = plan.pairs[p];
+        if (pm.width == 0) continue;
+        const VariantKey &key = pair_key[p];
+        if (!last_key || !(key == *last_key)) {
+            last_items = &groups[key];
+            last_key = &key;
+        }
         const uint32_t rows = uint32_t(key.G) * key.R;
-        groups[key].push_back(Item{uint32_t(p), pm.m, pm.n, (pm.m + rows - 1) / rows});
-        if (width == 16) ++plan.n16; else ++plan.n32;
-        if (width == 16 && !upper16) ++plan.n_speculative;
+        last_items->push_back(Item{uint32_t(p), pm.m, pm.n, (pm.m + rows - 1) / rows});
+        plan.cells += uint64_t(pm.m) * pm.n;
+        if (key.width == 16) ++plan.n16; else ++plan.n32;
+        if (speculative[p]) ++plan.n_speculative;
+        if (key.score == kScoreProfile) ++plan.n_profiled;
     }
 
-    // Residue code of row/column symbol for one half.
-    auto row_code = [&](uint8_t sym, uint32_t width) -> uint32_t {
-        const uint32_t c = uint32_t(alpha.code[sym]);
-        if (plan.score == kScoreDna) return width == 16 ? dna_row16(c) : dna_row32(c);
-        return lds_row_offset(c, plan.stride);
-    };
-    auto col_code = [&](uint8_t sym, uint32_t width, uint8_t score) -> uint32_t {
-        const uint32_t c = uint32_t(alpha.code[sym]);
-        if (score == kScoreProfile) return profile_col_offset(c, plan.profile_rows);
-        if (plan.score == kScoreDna) return width == 16 ? dna_col16(c) : dna_col32(c);
-        if (score == kScorePairs) return pair_col_offset(c);
-        return lds_col_offset(c);
-    };
-    const uint32_t pad_rank = plan.alphabet - 1;
-    auto row_pad = [&](uint32_t width) -> uint32_t {
-        if (plan.score == kScoreDna) return width == 16 ? kDnaRowPad16 : kDnaRowPad32;
-        return lds_row_offset(pad_rank, plan.stride);
-    };
-    auto col_pad = [&](uint32_t width, uint8_t score) -> uint32_t {
-        if (score == kScoreProfile) return profile_col_offset(plan.profile_ranks - 1, plan.profile_rows);
-        if (plan.score == kScoreDna) return width == 16 ? kDnaColPad16 : kDnaColPad32;
-        if (score == kScorePairs) return pair_col_offset(pad_rank);
-        return lds_col_offset(pad_rank);
-    };
-    // rank of row i of one half (the pad rank past its end)
-    auto row_rank = [&](const Item *h, uint32_t i) -> uint32_t {
-        if (!h || i >= h->m) return pad_rank;
-        const PairMeta &pm = plan.pairs[h->pair];
-        const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h->pair] : b.s1(h->pair);
-        return uint32_t(alpha.code[s[i]]);
-    };
+    for (int c = 0; c < 256; ++c) plan.code[c] = alpha.code[c];
+    plan.pad_rank = plan.alphabet ? plan.alphabet - 1 : 0;
+    const uint64_t seq2_base = b.end1();   // the residue buffer holds seq1 bytes, then seq2 bytes
 
-    struct Fill {
-        const Item *a, *b;   // the pair(s) of the task
-        uint8_t width, G, score;
-        uint32_t rows;       // G * R
-    };
-    std::vector<Fill> fills;
     uint64_t n_rowcodes = 0, n_colcodes = 0;
-    for (auto &[key, items] : groups) {
+    // Sorting and packing of the groups are independent: one thread per group.
+    std::vector<std::pair<const VariantKey *, std::vector<Item> *>> group_list;
+    for (auto &[key, items] : groups) group_list.push_back({&key, &items});
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> group_packs(group_list.size());
+    auto pack_group = [&](size_t gi) {
+        const VariantKey &key = *group_list[gi].first;
+        std::vector<Item> &items = *group_list[gi].second;
         // Pack int16 pairs two per task: similar stripe count and column count side by side.
         std::sort(items.begin(), items.end(), [](const Item &a, const Item &c) {
             return std::tie(a.S, a.n, a.m, a.pair) > std::tie(c.S, c.n, c.m, c.pair);
         });
-        std::vector<std::pair<int64_t, int64_t>> packs;  // indices into items
+        std::vector<std::pair<int64_t, int64_t>> &packs = group_packs[gi];  // indices into items
         if (key.width == 16) {
             size_t i = 0;
             while (i < items.size()) {
@@ -454,7 +494,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         } else {
             for (size_t i = 0; i < items.size(); ++i) packs.push_back({int64_t(i), -1});
         }
-        const uint32_t G = key.G, R = key.R, rows = G * R;
+        const uint32_t G = key.G;
         auto work = [&](const std::pair<int64_t, int64_t> &pk) {
             const Item &a = items[pk.first];
             uint32_t S = a.S, n = a.n;
@@ -463,7 +503,19 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         };
         // Longest tasks first so that the tail of the launch is made of short tasks.
         std::stable_sort(packs.begin(), packs.end(), [&](auto &x, auto &y) { return work(x) > work(y); });
+    };
+    {
+        std::vector<std::thread> pool;
+        for (size_t gi = 1; gi < group_list.size(); ++gi) pool.emplace_back(pack_group, gi);
+        if (!group_list.empty()) pack_group(0);
+        for (auto &th : pool) th.join();
+    }
 
+    for (size_t gi = 0; gi < group_list.size(); ++gi) {
+        const VariantKey &key = *group_list[gi].first;
+        const std::vector<Item> &items = *group_list[gi].second;
+        const std::vector<std::pair<int64_t, int64_t>> &packs = group_packs[gi];
+        const uint32_t G = key.G, R = key.R, rows = G * R;
         Launch L{key, uint32_t(plan.tasks.size()), 0, 0};
         for (auto &pk : packs) {
             const Item *h[2] = {&items[pk.first], pk.second >= 0 ? &items[pk.second] : nullptr};
@@ -489,7 +541,15 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
             L.padded_cells += pc;
             plan.padded_cells += pc;
             plan.tasks.push_back(t);
-            fills.push_back(Fill{h[0], h[1], key.width, key.G, key.score, rows});
+            EncodeTask e{};
+            for (int q = 0; q < 2; ++q) {
+                if (!h[q]) continue;
+                const uint32_t p = h[q]->pair;
+                const uint64_t s1 = uint64_t(b.s1(p) - b.seq1), s2 = seq2_base + b.off2[p];
+                e.row_src[q] = plan.pairs[p].swapped ? s2 : s1;
+                e.col_src[q] = plan.pairs[p].swapped ? s1 : s2;
+            }
+            plan.enc.push_back(e);
         }
         L.task_end = uint32_t(plan.tasks.size());
         if (L.task_end > L.task_begin) plan.launches.push_back(L);
@@ -499,57 +559,8 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         return PA_ERR_INVALID_ARGUMENT;
     }
 
-    // Residue streams of every task, encoded in parallel (tasks write disjoint ranges).
-    plan.rowcodes.resize(n_rowcodes);
-    plan.colcodes.resize(n_colcodes);
-    auto fill_task = [&](size_t ti) {
-        const Task &t = plan.tasks[ti];
-        const Fill &f = fills[ti];
-        const Item *h[2] = {f.a, f.b};
-        const uint32_t nrows = t.S * f.rows;
-        if (f.score == kScorePairs) {
-            // one word per two rows: the pair-row table offset of each half
-            const uint32_t K = plan.alphabet;
-            for (uint32_t i = 0; i < nrows; i += 2) {
-                uint32_t w = 0;
-                for (int q = 0; q < 2; ++q)
-                    w |= (pair_row_offset(row_rank(h[q], i), row_rank(h[q], i + 1), K) & 0xFFFFu) << (16 * q);
-                plan.rowcodes[size_t(t.row_off) + i / 2] = w;
-            }
-        }
-        for (uint32_t i = 0; i < ((f.score == kScorePairs || f.score == kScoreProfile) ? 0 : nrows); ++i) {
-            uint32_t w = 0;
-            for (int q = 0; q < (f.width == 16 ? 2 : 1); ++q) {
-                uint32_t code;
-                if (h[q] && i < h[q]->m) {
-                    const PairMeta &pm = plan.pairs[h[q]->pair];
-                    const uint8_t *s = pm.swapped ? b.seq2 + b.off2[h[q]->pair] : b.s1(h[q]->pair);
-                    code = row_code(s[i], f.width);
-                } else {
-                    code = row_pad(f.width);
-                }
-                w |= f.width == 16 ? (code & 0xFFFFu) << (16 * q) : code;
-            }
-            plan.rowcodes[size_t(t.row_off) + i] = w;
-        }
-        const uint32_t cw = col_words(t.N, f.G);
-        for (uint32_t j = 0; j < cw; ++j) {
-            uint32_t w = 0;
-            for (int q = 0; q < (f.width == 16 ? 2 : 1); ++q) {
-                uint32_t code;
-                if (h[q] && j < h[q]->n) {
-                    const PairMeta &pm = plan.pairs[h[q]->pair];
-                    const uint8_t *s = pm.swapped ? b.s1(h[q]->pair) : b.seq2 + b.off2[h[q]->pair];
-                    code = col_code(s[j], f.width, f.score);
-                } else {
-                    code = col_pad(f.width, f.score);
-                }
-                w |= f.width == 16 ? (code & 0xFFFFu) << (16 * q) : code;
-            }
-            plan.colcodes[size_t(t.col_off) + j] = w;
-        }
-    };
-    parallel_for(plan.tasks.size(), fill_task);
+    plan.n_rowcodes = n_rowcodes;
+    plan.n_colcodes = n_colcodes;
     order_launches(plan);
     return PA_OK;
 }

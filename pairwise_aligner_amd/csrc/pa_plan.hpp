@@ -8,6 +8,9 @@
 //     a task, the two halves holding the two pairs), so every load in the kernel is one dword,
 //   * launch groups: contiguous task ranges that share one kernel variant.
 //
+// The residue streams are encoded on the device (pa_encode.hip) from the caller's bytes, which are
+// uploaded once per batch as one buffer: the seq1 bytes [0, end1) followed by the seq2 bytes.
+//
 // Layout of one task in HBM (offsets are in 32-bit words):
 //   rowcodes[row_off + s*G*R + t*R + r]  residue of DP row i = s*G*R + t*R + r + 1 (stripe s,
 //                                         lane t, register r), padded to S*G*R rows
@@ -87,6 +90,14 @@ struct Task {
 };
 static_assert(sizeof(Task) == 48, "Task layout is shared with the device");
 
+// Where the device encoder reads a task's residues: per half, the byte offsets of the row and the
+// column sequence in the batch's residue buffer (seq2 bytes start at BatchView::end1()).
+struct EncodeTask {
+    uint64_t row_src[2];
+    uint64_t col_src[2];
+};
+static_assert(sizeof(EncodeTask) == 32, "EncodeTask layout is shared with the device");
+
 struct VariantKey {
     uint8_t width;   // 16 (two pairs per lane, saturating int16) or 32 (one pair per lane, int32)
     uint8_t G;       // lanes per group
@@ -153,12 +164,15 @@ struct Plan {
     std::vector<PairMeta> pairs;
     std::vector<Task> tasks;
     std::vector<Launch> launches;
-    std::vector<uint32_t> rowcodes, colcodes;
     uint64_t bnd_entries = 0;                 // (H,E) entries of the boundary buffer
     uint64_t cells = 0, padded_cells = 0;
     uint64_t n16 = 0, n32 = 0;
     uint64_t n_speculative = 0;               // int16 pairs whose bound does not exclude saturation
     uint64_t n_profiled = 0;                  // int16 pairs scored through the query profile
+    std::vector<EncodeTask> enc;              // per task: residue sources for the device encoder
+    uint64_t n_rowcodes = 0, n_colcodes = 0;  // words of the encoded residue streams
+    int16_t code[256] = {};                   // residue -> DNA code or matrix rank (-1: absent)
+    uint32_t pad_rank = 0;                    // matrix rank of the pad symbol
     std::vector<uint32_t> launch_order;       // launches in start order (order_launches)
     std::vector<uint8_t> launch_stream;       // stream of each launch (indexed like launches)
 };
@@ -213,6 +227,15 @@ void choose_shape(uint32_t m, uint32_t n, const Tuning &tuning, uint8_t &G, uint
 
 // LDS bytes a launch of variant `key` needs for its substitution table.
 uint32_t lds_bytes(const Plan &plan, const VariantKey &key);
+
+// How the device encoder forms the residue codes of a launch of variant `key`: the row and column
+// code of every byte value (for the pair-row table the row table holds ranks) and the pad codes.
+struct CodeTables {
+    uint32_t row[256];
+    uint32_t col[256];
+    uint32_t row_pad, col_pad;
+};
+void code_tables(const Plan &plan, const VariantKey &key, CodeTables &out);
 
 // Order of the launches of a plan and the stream of each: wavefront launches first (longest
 // task first), then striped launches by decreasing work.  Wavefront launches share stream 0,
