@@ -69,9 +69,12 @@ static void build_rank_map(const pao_config *cfg, int32_t map[256])
     for (int r = 0; r < cfg->dim; ++r) map[cfg->symbols[r]] = r;
 }
 
-/* Returns 0 on success, -1 if a symbol is outside the matrix alphabet. */
-int pao_score(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
-              const pao_config *cfg, int64_t *out_score)
+/* Returns 0 on success, -1 if a symbol is outside the matrix alphabet.  last_row (n+1 entries,
+ * nullable) receives H(m,j), j = 0..n, and last_col (m+1 entries, nullable) H(i,n), i = 0..m: the
+ * scores of the reference's final DP row and column (aligner_result.hpp:41-59; the vectors the
+ * global tracker reads, tracker_global_scalar.hpp:36-61). */
+int pao_score_dp(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
+                 const pao_config *cfg, int64_t *out_score, int64_t *last_row, int64_t *last_col)
 {
     const int local = cfg->mode == PAO_LOCAL;
     const int fc = local || cfg->first_column_free;
@@ -99,6 +102,7 @@ int pao_score(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
 
     int64_t best_local = INT32_MIN;
     int64_t best_col = Hrow[n];          /* max over H(i,n), i = 0..m */
+    if (last_col) last_col[0] = Hrow[n];
 
     for (int64_t i = 1; i <= m; ++i) {
         int64_t hcol0 = fc ? 0 : go + ge * i;  /* H(i,0) */
@@ -123,7 +127,9 @@ int pao_score(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
             w = max64(w + ge, open);            /* W(i,j+1) */
         }
         best_col = max64(best_col, Hrow[n]);
+        if (last_col) last_col[i] = Hrow[n];
     }
+    if (last_row) for (int64_t j = 0; j <= n; ++j) last_row[j] = Hrow[j];
 
     int64_t score;
     if (local) {
@@ -138,12 +144,19 @@ int pao_score(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
     return 0;
 }
 
+int pao_score(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
+              const pao_config *cfg, int64_t *out_score)
+{
+    return pao_score_dp(s1, m, s2, n, cfg, out_score, NULL, NULL);
+}
+
 typedef struct batch_job {
     const uint8_t *seq1; const uint64_t *off1;
     const uint8_t *seq2; const uint64_t *off2;
     const pao_config *cfg;
     const uint8_t *pair_mode;   /* optional per-pair PAO_GLOBAL/PAO_LOCAL override */
     int64_t *scores;
+    int64_t *rows, *cols;       /* optional final DP rows / columns (pao_dp_batch) */
     int64_t begin, end;
     int status;
 } batch_job;
@@ -158,17 +171,21 @@ static void *batch_worker(void *arg)
         const uint8_t *b = job->seq2 + job->off2[p];
         int64_t m = (int64_t)(job->off1[p + 1] - job->off1[p]);
         int64_t n = (int64_t)(job->off2[p + 1] - job->off2[p]);
-        if (pao_score(a, m, b, n, &local_cfg, &job->scores[p]) != 0) job->status = -1;
+        int64_t *row = job->rows ? job->rows + (job->off2[p] - job->off2[0]) + p : NULL;
+        int64_t *col = job->cols ? job->cols + (job->off1[p] - job->off1[0]) + p : NULL;
+        if (pao_score_dp(a, m, b, n, &local_cfg, &job->scores[p], row, col) != 0) job->status = -1;
     }
     return NULL;
 }
 
-/* Scores n_pairs pairs given as concatenated sequences with n_pairs+1 offsets.
- * Splits the batch over n_threads POSIX threads.  Returns 0 or -1 (invalid symbol). */
-int pao_score_batch(const uint8_t *seq1, const uint64_t *off1,
-                    const uint8_t *seq2, const uint64_t *off2,
-                    int64_t n_pairs, const pao_config *cfg, const uint8_t *pair_mode,
-                    int64_t *scores, int n_threads)
+/* Scores n_pairs pairs given as concatenated sequences with n_pairs+1 offsets and, when rows /
+ * cols are given, their final DP rows (pair p at (off2[p]-off2[0]) + p, n_p+1 entries) and columns
+ * (pair p at (off1[p]-off1[0]) + p, m_p+1 entries).  Splits the batch over n_threads POSIX
+ * threads.  Returns 0 or -1 (invalid symbol). */
+int pao_dp_batch(const uint8_t *seq1, const uint64_t *off1,
+                 const uint8_t *seq2, const uint64_t *off2,
+                 int64_t n_pairs, const pao_config *cfg, const uint8_t *pair_mode,
+                 int64_t *scores, int64_t *rows, int64_t *cols, int n_threads)
 {
     if (n_threads < 1) n_threads = 1;
     if (n_threads > 256) n_threads = 256;
@@ -180,7 +197,7 @@ int pao_score_batch(const uint8_t *seq1, const uint64_t *off1,
         int64_t b = t * chunk, e = b + chunk;
         if (b >= n_pairs) break;
         if (e > n_pairs) e = n_pairs;
-        jobs[t] = (batch_job){seq1, off1, seq2, off2, cfg, pair_mode, scores, b, e, 0};
+        jobs[t] = (batch_job){seq1, off1, seq2, off2, cfg, pair_mode, scores, rows, cols, b, e, 0};
         if (n_threads == 1) { batch_worker(&jobs[t]); started = 1; break; }
         pthread_create(&tids[t], NULL, batch_worker, &jobs[t]);
         ++started;
@@ -191,4 +208,12 @@ int pao_score_batch(const uint8_t *seq1, const uint64_t *off1,
         if (jobs[t].status) status = -1;
     }
     return status;
+}
+
+int pao_score_batch(const uint8_t *seq1, const uint64_t *off1,
+                    const uint8_t *seq2, const uint64_t *off2,
+                    int64_t n_pairs, const pao_config *cfg, const uint8_t *pair_mode,
+                    int64_t *scores, int n_threads)
+{
+    return pao_dp_batch(seq1, off1, seq2, off2, n_pairs, cfg, pair_mode, scores, NULL, NULL, n_threads);
 }

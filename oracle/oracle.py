@@ -74,6 +74,11 @@ def _lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
     lib = ctypes.CDLL(LIB_PATH)
+    lib.pao_dp_batch.restype = ctypes.c_int
+    lib.pao_dp_batch.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_int64, ctypes.POINTER(_CConfig), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_void_p, ctypes.c_int]
     lib.pao_score_batch.restype = ctypes.c_int
     lib.pao_score_batch.argtypes = [
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -116,6 +121,35 @@ def score_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_method=None, th
     if rc != 0:
         raise ValueError("symbol outside the substitution matrix alphabet")
     return scores[:n]
+
+
+def dp_vectors_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_method=None, threads: int = 1):
+    """Scores plus the final DP row and column of every pair (int64): rows[(off2[p]-off2[0]) + p + j]
+    = H(m_p, j), j = 0..n_p, and cols[(off1[p]-off1[0]) + p + i] = H(i, n_p), i = 0..m_p."""
+    n = len(off1) - 1
+    scores = np.zeros(max(n, 1), dtype=np.int64)
+    off1 = np.ascontiguousarray(off1, dtype=np.uint64)
+    off2 = np.ascontiguousarray(off2, dtype=np.uint64)
+    rows = np.zeros(max(int(off2[-1] - off2[0]) + n, 1), dtype=np.int64)
+    cols = np.zeros(max(int(off1[-1] - off1[0]) + n, 1), dtype=np.int64)
+    c = _CConfig(cfg.mode, int(cfg.first_column_free), int(cfg.first_row_free),
+                 int(cfg.last_column_free), int(cfg.last_row_free), cfg.gap_open, cfg.gap_extension,
+                 cfg.score_kind, cfg.match, cfg.mismatch, len(cfg.symbols), None, None)
+    mat = sym = None
+    if cfg.score_kind == MATRIX:
+        mat = (ctypes.c_int32 * len(cfg.matrix))(*cfg.matrix)
+        sym = (ctypes.c_uint8 * len(cfg.symbols))(*cfg.symbols)
+        c.matrix = mat
+        c.symbols = sym
+    pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
+    buf1 = np.ascontiguousarray(buf1, dtype=np.uint8)
+    buf2 = np.ascontiguousarray(buf2, dtype=np.uint8)
+    rc = _lib().pao_dp_batch(buf1.ctypes.data, off1.ctypes.data, buf2.ctypes.data, off2.ctypes.data, n,
+                             ctypes.byref(c), None if pm is None else pm.ctypes.data, scores.ctypes.data,
+                             rows.ctypes.data, cols.ctypes.data, threads)
+    if rc != 0:
+        raise ValueError("symbol outside the substitution matrix alphabet")
+    return scores[:n], rows[:int(off2[-1] - off2[0]) + n], cols[:int(off1[-1] - off1[0]) + n]
 
 
 SIMD_LIB_PATH = os.path.join(_HERE, "libsimdport.so")

@@ -275,3 +275,86 @@ def test_simd_baseline_matches_oracle_blosum_mixed_methods_and_int32_bulks():
     got = O.simd_score_packed(b1, o1, b2, o2, cfg, threads=4)
     assert np.array_equal(want, got.astype(np.int64))
     assert want.max() > 32767
+
+
+# ---- the final DP row and column (oracle.dp_vectors_packed) ------------------------------------
+# H(i, j) of the global method is the best score of aligning s1[:i] with s2[:j] under the leading
+# end-gap rules and penalised trailing gaps, so each exported entry is pinned by enumeration.  For
+# the local method H(i, j) = max(0, best fully penalised alignment of s1[k:i] with s2[l:j] over all
+# k <= i, l <= j): every cell can start an alignment at value 0 (the zero clamp and the free
+# leading gaps).
+
+def _prefix_global(a, b, cfg):
+    pen = O.OracleConfig(mode=O.GLOBAL, first_column_free=cfg.first_column_free,
+                         first_row_free=cfg.first_row_free, last_column_free=False, last_row_free=False,
+                         gap_open=cfg.gap_open, gap_extension=cfg.gap_extension, match=cfg.match,
+                         mismatch=cfg.mismatch)
+    return O.brute_force_score(a, b, pen)
+
+
+def _cell_local(a, b, cfg):
+    pen = O.OracleConfig(mode=O.GLOBAL, gap_open=cfg.gap_open, gap_extension=cfg.gap_extension,
+                         match=cfg.match, mismatch=cfg.mismatch)
+    best = 0
+    for k in range(len(a) + 1):
+        for l in range(len(b) + 1):
+            if k < len(a) or l < len(b):
+                best = max(best, O.brute_force_score(a[k:], b[l:], pen))
+    return best
+
+
+def _dp_vectors(a, b, cfg):
+    b1, o1 = O.pack([a])
+    b2, o2 = O.pack([b])
+    s, rows, cols = O.dp_vectors_packed(b1, o1, b2, o2, cfg)
+    return int(s[0]), rows.tolist(), cols.tolist()
+
+
+@pytest.mark.parametrize("flags", FLAG_COMBOS, ids=lambda f: "".join("F" if x else "p" for x in f))
+def test_dp_vectors_global_by_enumeration(flags):
+    rng = np.random.default_rng(1000 + (hash(flags) & 0xFFF))
+    fc, fr, lc, lr = flags
+    for trial in range(8):
+        cfg = O.OracleConfig(mode=O.GLOBAL, first_column_free=fc, first_row_free=fr, last_column_free=lc,
+                             last_row_free=lr, gap_open=int(rng.integers(-6, 1)), gap_extension=int(rng.integers(-3, 0)),
+                             match=int(rng.integers(1, 5)), mismatch=int(rng.integers(-5, 0)))
+        a = _random_seq(rng, "ACG", 0, 4)
+        b = _random_seq(rng, "ACG", 0, 4)
+        score, rows, cols = _dp_vectors(a, b, cfg)
+        assert rows == [_prefix_global(a, b[:j], cfg) for j in range(len(b) + 1)], (a, b, cfg)
+        assert cols == [_prefix_global(a[:i], b, cfg) for i in range(len(a) + 1)], (a, b, cfg)
+        # the tracker's reading of the two vectors (tracker_global_scalar.hpp:36-61)
+        want = rows[-1]
+        if lr:
+            want = max([want] + rows)
+        if lc:
+            want = max([want] + cols)
+        assert score == want
+
+
+def test_dp_vectors_local_by_enumeration():
+    rng = np.random.default_rng(77)
+    for trial in range(12):
+        cfg = O.OracleConfig(mode=O.LOCAL, gap_open=int(rng.integers(-6, 0)), gap_extension=int(rng.integers(-3, 0)),
+                             match=int(rng.integers(1, 5)), mismatch=int(rng.integers(-5, 0)))
+        a = _random_seq(rng, "ACGT", 0, 4)
+        b = _random_seq(rng, "ACGT", 0, 4)
+        _, rows, cols = _dp_vectors(a, b, cfg)
+        assert rows == [_cell_local(a, b[:j], cfg) for j in range(len(b) + 1)], (a, b, cfg)
+        assert cols == [_cell_local(a[:i], b, cfg) for i in range(len(a) + 1)], (a, b, cfg)
+
+
+def test_dp_vectors_transpose_and_batch_layout():
+    rng = np.random.default_rng(78)
+    seqs1 = [_random_seq(rng, "ACGT", 0, 30) for _ in range(40)]
+    seqs2 = [_random_seq(rng, "ACGT", 0, 30) for _ in range(40)]
+    cfg = O.OracleConfig(mode=O.GLOBAL, gap_open=-5, gap_extension=-2, match=2, mismatch=-3)
+    b1, o1 = O.pack(seqs1)
+    b2, o2 = O.pack(seqs2)
+    s, rows, cols = O.dp_vectors_packed(b1, o1, b2, o2, cfg, threads=4)
+    st, rows_t, cols_t = O.dp_vectors_packed(b2, o2, b1, o1, cfg, threads=4)
+    assert np.array_equal(s, st) and np.array_equal(rows, cols_t) and np.array_equal(cols, rows_t)
+    for p in range(40):
+        r0 = int(o2[p]) + p
+        c0 = int(o1[p]) + p
+        assert rows[r0 + len(seqs2[p])] == s[p] == cols[c0 + len(seqs1[p])]
