@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PA_ABI_VERSION 3
+#define PA_ABI_VERSION 4
 
 typedef enum pa_status {
     PA_OK = 0,
@@ -113,6 +113,19 @@ pa_status pa_score_batch(pa_ctx *ctx, const pa_config *cfg,
                          const uint8_t *seq2, const uint64_t *off2,
                          uint64_t n_pairs, const uint8_t *pair_method,
                          int32_t *scores, uint8_t *flags);
+
+/* The same plus the final DP column and row of every pair: the scores of the reference's
+ * result.dp_column() / dp_row() (result/aligner_result.hpp:41-59, filled by
+ * dp_algorithm_template_standard.hpp:43-100; the vectors the global tracker reads,
+ * tracker_global_scalar.hpp:36-61).  dp_column receives H(i, len2), i = 0..len1, of pair p at
+ * (off1[p] - off1[0]) + p; dp_row receives H(len1, j), j = 0..len2, at (off2[p] - off2[0]) + p.
+ * The gap-state halves of the reference's cells are not exported.  Runs every pair in int32
+ * lanes of the striped kernel. */
+pa_status pa_score_batch_dp(pa_ctx *ctx, const pa_config *cfg,
+                            const uint8_t *seq1, const uint64_t *off1,
+                            const uint8_t *seq2, const uint64_t *off2,
+                            uint64_t n_pairs, const uint8_t *pair_method,
+                            int32_t *scores, uint8_t *flags, int32_t *dp_column, int32_t *dp_row);
 
 /* Device-resident batch: the same work split into plan/upload, repeated runs, and download,
  * so a caller can time runs with inputs already in HBM. */

@@ -278,11 +278,17 @@ struct aligner_result {
     sequence2_t _sequence2;
     int32_t _score;
     uint8_t _flags;
+    // scores of the final DP column (H(i, n), i = 0..m) and row (H(m, j), j = 0..n); filled by
+    // compute(seq1, seq2, first_dp_column, first_dp_row), empty otherwise
+    std::vector<int32_t> _dp_column{};
+    std::vector<int32_t> _dp_row{};
 
     sequence1_t const &sequence1() const noexcept { return _sequence1; }
     sequence2_t const &sequence2() const noexcept { return _sequence2; }
     int32_t const &score() const noexcept { return _score; }
     uint8_t flags() const noexcept { return _flags; }   // PA_PAIR_* bits
+    std::vector<int32_t> const &dp_column() const noexcept { return _dp_column; }
+    std::vector<int32_t> const &dp_row() const noexcept { return _dp_row; }
 };
 
 // aligner_result_bulk.hpp:24-70: a view of one pair of a bulk result shared by all its views.
@@ -395,6 +401,31 @@ public:
         run(b1, o1, b2, o2, scores, flags);
         return aligner_result<sequence1_t, sequence2_t>{std::forward<sequence1_t>(sequence1),
                                                         std::forward<sequence2_t>(sequence2), scores[0], flags[0]};
+    }
+
+    // one pair with the initial DP column and row (interface_one_to_one_single.hpp:52-67).  Like the
+    // reference, which re-initialises both from the gap model and the end-gap rules before the
+    // recursion (dp_vector_single.hpp:67-80, affine_dp_algorithm.hpp:48-72), the device takes the
+    // boundary from the configuration; the result carries the scores of the final DP column and
+    // row (aligner_result.hpp:41-59) through pa_score_batch_dp.
+    template <detail::sequence sequence1_t, detail::sequence sequence2_t, typename dp_column_t, typename dp_row_t>
+    auto compute(sequence1_t &&sequence1, sequence2_t &&sequence2, dp_column_t &&, dp_row_t &&)
+    {
+        std::vector<uint8_t> b1, b2;
+        std::vector<uint64_t> o1{0}, o2{0};
+        detail::append(b1, o1, sequence1);
+        detail::append(b2, o2, sequence2);
+        std::vector<int32_t> col(b1.size() + 1), row(b2.size() + 1);
+        int32_t score = 0;
+        uint8_t flags = 0;
+        const pa_config cfg = device_config();
+        static const uint8_t empty = 0;
+        check(pa_score_batch_dp(_ctx.get(), &cfg, b1.empty() ? &empty : b1.data(), o1.data(),
+                                b2.empty() ? &empty : b2.data(), o2.data(), 1, nullptr, &score, &flags, col.data(),
+                                row.data()));
+        return aligner_result<sequence1_t, sequence2_t>{std::forward<sequence1_t>(sequence1),
+                                                        std::forward<sequence2_t>(sequence2), score, flags,
+                                                        std::move(col), std::move(row)};
     }
 
     // one-to-one bulk (interface_one_to_one_bulk.hpp:47-100), any bulk size

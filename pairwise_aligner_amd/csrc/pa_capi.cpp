@@ -28,7 +28,7 @@ struct DeviceBuffer {
 
 struct PlanDevice {
     pa::Plan plan;
-    DeviceBuffer tasks, rowcodes, colcodes, bnd, subset, enc, codes;
+    DeviceBuffer tasks, rowcodes, colcodes, bnd, subset, enc, codes, dp_base;
     std::vector<pa::CodeTables> code_host;   // per launch (kept until the encoder has read them)
     DeviceBuffer table[pa::kScoreShifts], ptable[pa::kScoreShifts], profile[pa::kScoreShifts];
     std::vector<uint32_t> subset_host;
@@ -57,7 +57,9 @@ struct pa_batch {
     std::vector<uint64_t> off1, off2;
     uint64_t n_pairs = 0;
     bool one_query = false;   // seq1 is one query scored against every seq2 (off1 = {0, length})
+    bool export_dp = false;   // the kernels also write the final DP rows and columns into dp_out
     DeviceBuffer residues;    // seq1 then seq2 bytes on the device (upload_residues)
+    DeviceBuffer dp_out;
     PlanDevice main;
     std::unique_ptr<PlanDevice> recompute;   // int32 re-plan of the pairs that saturated
     std::vector<uint32_t> recompute_set;
@@ -134,6 +136,7 @@ pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd, const DeviceBuffer &residues)
     const pa::Plan &p = pd.plan;
     if ((st = upload(ctx, pd.tasks, p.tasks)) != PA_OK) return st;
     if ((st = upload(ctx, pd.enc, p.enc)) != PA_OK) return st;
+    if ((st = upload(ctx, pd.dp_base, p.dp_base)) != PA_OK) return st;
     PA_HIP(hipMalloc(&pd.rowcodes.ptr, std::max<size_t>(p.n_rowcodes * 4, 16)), "hipMalloc");
     pd.rowcodes.bytes = std::max<size_t>(p.n_rowcodes * 4, 16);
     PA_HIP(hipMalloc(&pd.colcodes.ptr, std::max<size_t>(p.n_colcodes * 4, 16)), "hipMalloc");
@@ -202,6 +205,10 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     a.tbl_lo = p.tbl_lo[v];
     a.tbl_hi = p.tbl_hi[v];
     a.fc = p.fc; a.fr = p.fr; a.lc = p.lc; a.lr = p.lr;
+    if (L.key.export_dp) {
+        a.dp_out = static_cast<int32_t *>(b->dp_out.ptr);
+        a.dp_base = static_cast<const uint64_t *>(pd.dp_base.ptr) + 2ull * L.task_begin;
+    }
     return a;
 }
 
@@ -471,7 +478,7 @@ const char *pa_ctx_last_error(const pa_ctx *ctx) { return ctx ? ctx->last_error.
 // seq2[p]) for every p.
 static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
                               bool one_query, const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs,
-                              const uint8_t *pair_method, pa_batch **out)
+                              const uint8_t *pair_method, pa_batch **out, bool export_dp = false)
 {
     *out = nullptr;
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PA_ERR_DEVICE, "hipSetDevice");
@@ -503,7 +510,9 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
     if (cfg->lane_policy == PA_LANES_LEARNED) {
         if ((st = learned_widths(ctx, b->cfg, view, forced)) != PA_OK) return st;
     }
-    st = pa::make_plan(b->cfg, view, forced.empty() ? nullptr : forced.data(), nullptr, b->main.plan, err);
+    b->export_dp = export_dp;
+    st = pa::make_plan(b->cfg, view, forced.empty() ? nullptr : forced.data(), nullptr, b->main.plan, err, false,
+                       export_dp);
     if (st != PA_OK) return fail(ctx, st, err);
     const pa::Plan &p = b->main.plan;
     // Only int16 pairs whose bound does not exclude saturation need the flag read-back.
@@ -511,6 +520,10 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
 
     if ((st = upload_residues(ctx, view, b->residues)) != PA_OK) return st;
     if ((st = upload_plan(ctx, b->main, b->residues)) != PA_OK) return st;
+    if (export_dp) {
+        b->dp_out.bytes = std::max<size_t>(p.dp_entries * 4, 16);
+        PA_HIP(hipMalloc(&b->dp_out.ptr, b->dp_out.bytes), "hipMalloc");
+    }
     if ((st = upload(ctx, b->pairs, p.pairs)) != PA_OK) return st;
     PA_HIP(hipMalloc(&b->scores.ptr, std::max<size_t>(n_pairs * 4, 16)), "hipMalloc");
     PA_HIP(hipMalloc(&b->flags.ptr, (n_pairs + 1) * 4), "hipMalloc");   // + error slot
@@ -933,6 +946,48 @@ pa_status pa_score_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1,
     if (st == PA_OK) st = pa_batch_results(ctx, b, scores, flags);
     pa_batch_destroy(ctx, b);
     return st;
+}
+
+pa_status pa_score_batch_dp(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                            const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, const uint8_t *pair_method,
+                            int32_t *scores, uint8_t *flags, int32_t *dp_column, int32_t *dp_row)
+{
+    if (!ctx || !cfg || !off1 || !off2 || (n_pairs && (!seq1 || !seq2 || !dp_column || !dp_row)))
+        return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PA_ERR_DEVICE, "hipSetDevice");
+    pa_batch *b = nullptr;
+    pa_status st = create_batch(ctx, cfg, seq1, off1, false, seq2, off2, n_pairs, pair_method, &b, true);
+    if (st != PA_OK) return st;
+    std::unique_ptr<pa_batch> hold(b);
+    if ((st = pa_batch_run(ctx, b)) != PA_OK) return st;
+    if ((st = pa_batch_results(ctx, b, scores, flags)) != PA_OK) return st;
+    const uint64_t n1 = off1[n_pairs] - off1[0] + n_pairs, n2 = off2[n_pairs] - off2[0] + n_pairs;
+    if (n_pairs) {
+        PA_HIP(hipMemcpy(dp_column, b->dp_out.ptr, n1 * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        PA_HIP(hipMemcpy(dp_row, static_cast<char *>(b->dp_out.ptr) + n1 * 4, n2 * 4, hipMemcpyDeviceToHost),
+               "hipMemcpy");
+    }
+    // pairs with an empty sequence have no task: their vectors are boundary values
+    // (affine_initialisation_strategy.hpp:38-51; local: every end free)
+    const bool local_all = cfg->method == PA_METHOD_LOCAL;
+    for (uint64_t p = 0; p < n_pairs; ++p) {
+        const uint64_t m = off1[p + 1] - off1[p], n = off2[p + 1] - off2[p];
+        if (m && n) continue;
+        const bool local = pair_method ? pair_method[p] == PA_METHOD_LOCAL : local_all;
+        const bool fc = local || cfg->first_column_free, fr = local || cfg->first_row_free;
+        auto h = [&](uint64_t k, bool free_end) -> int32_t {
+            return (k == 0 || free_end) ? 0 : int32_t(cfg->gap_open + int64_t(cfg->gap_extension) * int64_t(k));
+        };
+        int32_t *col = dp_column + (off1[p] - off1[0]) + p, *row = dp_row + (off2[p] - off2[0]) + p;
+        if (m == 0) {
+            for (uint64_t j = 0; j <= n; ++j) row[j] = h(j, fr);
+            col[0] = h(n, fr);
+        } else {
+            for (uint64_t i = 0; i <= m; ++i) col[i] = h(i, fc);
+            row[0] = h(m, fc);
+        }
+    }
+    return PA_OK;
 }
 
 pa_status pa_score_one_to_many(pa_ctx *ctx, const pa_config *cfg, const uint8_t *query, uint64_t query_len,

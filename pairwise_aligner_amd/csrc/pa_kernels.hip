@@ -61,7 +61,8 @@ __device__ __forceinline__ typename Lane<W>::V row_reduce_max(typename Lane<W>::
     return L::from(m);
 }
 
-template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
+// EXPORT: also write the task's final DP column and row (int32 lanes; pa_plan.hpp, dp_base).
+template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false>
 __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 {
     using L = Lane<W>;
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
     constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
     constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
-    constexpr bool CDRIFT = DRIFT && !ACC;   // column drift on top
+    constexpr bool CDRIFT = DRIFT && !ACC && !EXPORT;   // column drift on top
     const int dge = DRIFT ? ge : 0;
     const int cg = CDRIFT ? -ge : 0;         // column drift per step
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
@@ -97,6 +98,16 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
     const int N = int(tk.N);
     constexpr int ROWS = G * R;
+    static_assert(!EXPORT || W == 32, "the DP export runs in int32 lanes");
+    int32_t *dp_col = nullptr, *dp_row = nullptr;   // H(0, n) and H(m, 0) of this task's vectors
+    if constexpr (EXPORT) {
+        dp_col = a.dp_out + a.dp_base[2 * gid];
+        dp_row = a.dp_out + a.dp_base[2 * gid + 1];
+        if (t == 0) {
+            dp_col[0] = h0_value(int(tk.n[0]), fr, go, ge);
+            dp_row[0] = h0_value(int(tk.m[0]), fc, go, ge);
+        }
+    }
 
     V runmax = LOCAL ? L::splat(0) : L::lowest();   // exact
     V satmax = L::lowest();                         // drifted (int16 saturation test)
@@ -181,6 +192,23 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 Hdiag = L::from(Hin);
                 Hout = L::bits(DRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
                 Eout = L::bits(DRIFT ? L::add(Eup, vout) : Eup);
+                if constexpr (EXPORT) {
+                    // the last row: H(m, c+1) from the lane that holds row m; the last column:
+                    // H(i, n) of every row of this lane at column n
+                    const int m = int(tk.m[0]), n = int(tk.n[0]);
+                    const int er = m - 1 - row0;
+                    if (er >= 0 && er < R && c < n) {
+                        int v = 0;
+#pragma unroll
+                        for (int r = 0; r < R; ++r) if (r == er) v = L::half(st.H[r], 0);
+                        dp_row[c + 1] = v + dge * (er + 1);
+                    }
+                    if (c == n - 1) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            if (row0 + r + 1 <= m) dp_col[row0 + r + 1] = L::half(st.H[r], 0) + dge * (r + 1);
+                    }
+                }
             }
             if constexpr (!LOCAL) {
                 // capture events are rare: test the whole wave first
@@ -226,7 +254,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 
         // whole chunks of G steps; the first is masked (lanes start one step apart)
         const int nsteps = ((N + 2 * G - 2) / G) * G;
-        constexpr bool kMaskAll = ROWMAX || TRACK;
+        constexpr bool kMaskAll = ROWMAX || TRACK || EXPORT;
 #pragma unroll 1
         for (int u = 0; u < G; ++u) step.template operator()<true>(u);
         chunk_end(0);
@@ -560,13 +588,18 @@ __global__ void init_scores_kernel(InitArgs a)
 // Launch dispatch
 // ---------------------------------------------------------------------------------------------
 
-template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
+template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false>
 static hipError_t launch_variant(const VariantKey &k, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st)
 {
     const uint32_t lds = SCORE != kScoreDna ? lds_bytes : 0;
     const uint32_t threads = 256;
     const uint32_t blocks = uint32_t((uint64_t(a.ntasks) * 16 + threads - 1) / threads);
-    if constexpr (SCORE == kScoreProfile) {
+    if constexpr (EXPORT) {
+        if (k.G != 16) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX, true>), dim3(blocks), dim3(threads),
+                           lds, st, a);
+        return hipGetLastError();
+    } else if constexpr (SCORE == kScoreProfile) {
         // striped kernel only, at rows per lane whose profile reads are free of bank conflicts
         if constexpr (R == 8 || R == 16) {
             return hipErrorInvalidValue;
@@ -591,6 +624,15 @@ static hipError_t launch_variant(const VariantKey &k, const KernelArgs &a, uint3
 template <int W, int R, int SCORE, bool LOCAL>
 static hipError_t dispatch_flags(const VariantKey &k, const KernelArgs &a, uint32_t lds, hipStream_t st)
 {
+    if (k.export_dp) {
+        if constexpr (W == 32 && R == kExportRows && SCORE != kScoreProfile && SCORE != kScorePairs) {
+            if (LOCAL) return launch_variant<W, R, SCORE, LOCAL, false, false, true>(k, a, lds, st);
+            if (k.rowmax) return launch_variant<W, R, SCORE, LOCAL, false, true, true>(k, a, lds, st);
+            return launch_variant<W, R, SCORE, LOCAL, false, false, true>(k, a, lds, st);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
     if (LOCAL) return launch_variant<W, R, SCORE, LOCAL, false, false>(k, a, lds, st);
     if (k.rowmax) {
         if (k.track) return launch_variant<W, R, SCORE, LOCAL, true, true>(k, a, lds, st);

@@ -26,6 +26,8 @@ struct KernelArgs {
     int32_t go, ge;
     uint32_t tbl_lo, tbl_hi;
     int32_t fc, fr, lc, lr;
+    int32_t *dp_out;            // DP export variants: the final rows and columns
+    const uint64_t *dp_base;    // 2 per task of this launch (pa_plan.hpp)
 };
 
 struct InitArgs {

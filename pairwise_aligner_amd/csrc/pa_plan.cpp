@@ -15,13 +15,13 @@ namespace pa {
 
 bool VariantKey::operator<(const VariantKey &o) const
 {
-    return std::tie(width, G, R, score, local, track, rowmax) <
-           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax);
+    return std::tie(width, G, R, score, local, track, rowmax, export_dp) <
+           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, o.export_dp);
 }
 bool VariantKey::operator==(const VariantKey &o) const
 {
-    return std::tie(width, G, R, score, local, track, rowmax) ==
-           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax);
+    return std::tie(width, G, R, score, local, track, rowmax, export_dp) ==
+           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, o.export_dp);
 }
 
 // Rows per lane that kernels are instantiated for (lanes per group G = 16).
@@ -314,7 +314,8 @@ int32_t max_substitution(const pa_config &cfg)
 }
 
 pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *forced_width,
-                    const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all)
+                    const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all,
+                    bool export_dp)
 {
     if (cfg.method != PA_METHOD_GLOBAL && cfg.method != PA_METHOD_LOCAL) { err = "invalid method"; return PA_ERR_INVALID_ARGUMENT; }
     // The cell update drops V + go + ge from the vertical gap state (it never exceeds V + ge
@@ -426,7 +427,8 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         const bool lower16 = rb.lower > -32700;
         const bool upper16 = rb.upper < 32767;
         uint8_t width;
-        if (forced_width && forced_width[p]) width = forced_width[p];
+        if (export_dp) width = 32;   // exported rows and columns are exact int32
+        else if (forced_width && forced_width[p]) width = forced_width[p];
         else if (cfg.lane_policy == PA_LANES_INT32) width = 32;
         else if (cfg.lane_policy == PA_LANES_INT16 || cfg.lane_policy == PA_LANES_LEARNED) width = lower16 ? 16 : 32;
         else width = (lower16 && upper16) ? 16 : 32;
@@ -436,7 +438,13 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         VariantKey &key = pair_key[p];
         key.width = width;
         const bool profiled = use_profile && width == 16;
-        choose_shape(pm.m, pm.n, profiled ? profile_tuning : plan.tuning, key.G, key.R);
+        if (export_dp) {
+            key.G = 16;
+            key.R = kExportRows;
+        } else {
+            choose_shape(pm.m, pm.n, profiled ? profile_tuning : plan.tuning, key.G, key.R);
+        }
+        key.export_dp = export_dp ? 1 : 0;
         if (profiled) key.score = kScoreProfile;
         else key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable[0].empty()) ? kScorePairs : plan.score;
         key.local = local;
@@ -466,6 +474,12 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     for (int c = 0; c < 256; ++c) plan.code[c] = alpha.code[c];
     plan.pad_rank = plan.alphabet ? plan.alphabet - 1 : 0;
     const uint64_t seq2_base = b.end1();   // the residue buffer holds seq1 bytes, then seq2 bytes
+    // DP export output: dp_column values of every pair (seq1 side), then the dp_row values
+    const uint64_t dp_total1 = b.one_query ? 0 : (b.off1[N] - b.off1[0]) + N;
+    if (export_dp) {
+        if (b.one_query) { err = "DP export needs one seq1 per pair"; return PA_ERR_INVALID_ARGUMENT; }
+        plan.dp_entries = dp_total1 + (b.off2[N] - b.off2[0]) + N;
+    }
 
     uint64_t n_rowcodes = 0, n_colcodes = 0;
     // Sorting and packing of the groups are independent: one thread per group.
@@ -550,6 +564,14 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 e.col_src[q] = plan.pairs[p].swapped ? s1 : s2;
             }
             plan.enc.push_back(e);
+            if (export_dp) {
+                // our rows are seq1 unless the pair is exchanged (then seq2)
+                const uint32_t p = h[0]->pair;
+                const uint64_t c1 = b.off1[p] - b.off1[0] + p, c2 = dp_total1 + (b.off2[p] - b.off2[0]) + p;
+                const bool sw = plan.pairs[p].swapped;
+                plan.dp_base.push_back(sw ? c2 : c1);   // H(0, n): the task's last column
+                plan.dp_base.push_back(sw ? c1 : c2);   // H(m, 0): the task's last row
+            }
         }
         L.task_end = uint32_t(plan.tasks.size());
         if (L.task_end > L.task_begin) plan.launches.push_back(L);

@@ -106,6 +106,7 @@ struct VariantKey {
     uint8_t local;   // method_local
     uint8_t track;   // int16 global: keep the running maximum to detect saturation
     uint8_t rowmax;  // global with trailing gaps free in both dimensions: per-row maxima
+    uint8_t export_dp;   // also write the final DP row and column of every pair (int32, R = 8)
     bool operator<(const VariantKey &o) const;
     bool operator==(const VariantKey &o) const;
 };
@@ -114,7 +115,13 @@ struct VariantKey {
 // method keeps exact values (0); the global method drifts along the rows (1) and, unless it
 // keeps per-row maxima, along the columns as well (2).
 inline constexpr int kScoreShifts = 3;
-inline int score_shift(const VariantKey &key) { return key.local ? 0 : ((key.track || key.rowmax) ? 1 : 2); }
+inline int score_shift(const VariantKey &key)
+{
+    return key.local ? 0 : ((key.track || key.rowmax || key.export_dp) ? 1 : 2);
+}
+
+// Rows per lane of the DP-export variants (int32 lanes, striped kernel only).
+inline constexpr uint8_t kExportRows = 8;
 
 struct Launch {
     VariantKey key;
@@ -170,6 +177,11 @@ struct Plan {
     uint64_t n_speculative = 0;               // int16 pairs whose bound does not exclude saturation
     uint64_t n_profiled = 0;                  // int16 pairs scored through the query profile
     std::vector<EncodeTask> enc;              // per task: residue sources for the device encoder
+    // DP export: per task, the index in the output of H(0, n) (the task's last column, m+1
+    // entries) and of H(m, 0) (its last row, n+1 entries).  The output holds the caller's
+    // dp_column values (seq1 side, pair p at off1[p]-off1[0]+p) followed by the dp_row values.
+    std::vector<uint64_t> dp_base;            // 2 per task
+    uint64_t dp_entries = 0;
     uint64_t n_rowcodes = 0, n_colcodes = 0;  // words of the encoded residue streams
     int16_t code[256] = {};                   // residue -> DNA code or matrix rank (-1: absent)
     uint32_t pad_rank = 0;                    // matrix rank of the pad symbol
@@ -198,7 +210,8 @@ struct BatchView {
 // `track_all` keeps the running maximum of H in every global variant (labels for the learned
 // range predictor).
 pa_status make_plan(const pa_config &cfg, const BatchView &batch, const uint8_t *forced_width,
-                    const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all = false);
+                    const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all = false,
+                    bool export_dp = false);
 
 // Modelled sweep cost of one task (arbitrary units): stripes x steps x per-step work.
 double task_cost(const VariantKey &key, const Task &t, const Tuning &tuning);

@@ -29,7 +29,8 @@ EXPORTS = ["pa_abi_version", "pa_ctx_create", "pa_ctx_destroy", "pa_ctx_last_err
            "pa_batch_create", "pa_batch_run", "pa_batch_results", "pa_batch_stats_get", "pa_batch_time",
            "pa_batch_destroy", "pa_model_create", "pa_model_destroy", "pa_model_param_count",
            "pa_model_set_params", "pa_model_get_params", "pa_features", "pa_model_forward", "pa_model_loss_grad",
-           "pa_model_train_step", "pa_dp_labels", "pa_tune", "pa_score_one_to_many", "pa_batch_create_one_to_many"]
+           "pa_model_train_step", "pa_dp_labels", "pa_tune", "pa_score_one_to_many", "pa_batch_create_one_to_many",
+           "pa_score_batch_dp"]
 N_FEATURES, N_OUTPUTS, N_HIDDEN = 8, 2, 64
 
 
@@ -133,6 +134,7 @@ def lib() -> ctypes.CDLL:
     L.pa_score_one_to_many.argtypes = [vp, ctypes.POINTER(CConfig), vp, u64, vp, vp, u64, vp, vp, vp]
     L.pa_batch_create_one_to_many.argtypes = [vp, ctypes.POINTER(CConfig), vp, u64, vp, vp, u64, vp,
                                               ctypes.POINTER(vp)]
+    L.pa_score_batch_dp.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pa_batch_run.argtypes = [vp, vp]
     L.pa_batch_results.argtypes = [vp, vp, vp, vp]
     L.pa_batch_stats_get.argtypes = [vp, ctypes.POINTER(CStats)]
@@ -153,7 +155,7 @@ def lib() -> ctypes.CDLL:
                  "pa_batch_results", "pa_batch_stats_get", "pa_batch_time", "pa_batch_destroy", "pa_model_create",
                  "pa_model_destroy", "pa_model_set_params", "pa_model_get_params", "pa_features",
                  "pa_model_forward", "pa_model_loss_grad", "pa_model_train_step", "pa_dp_labels", "pa_tune",
-                 "pa_score_one_to_many", "pa_batch_create_one_to_many"]:
+                 "pa_score_one_to_many", "pa_batch_create_one_to_many", "pa_score_batch_dp"]:
         getattr(L, name).restype = i32
     return L
 
@@ -231,6 +233,25 @@ class Context:
                                       scores.ctypes.data, flags.ctypes.data)
         self.check(st)
         return scores[:n], flags[:n]
+
+    def score_packed_dp(self, b1, o1, b2, o2, cfg: DeviceConfig, pair_method=None):
+        """Scores, flags and the final DP column and row of every pair (pa_score_batch_dp):
+        dp_column[(o1[p]-o1[0]) + p + i] = H(i, len2), dp_row[(o2[p]-o2[0]) + p + j] = H(len1, j)."""
+        n = len(o1) - 1
+        holder = _CConfigHolder(cfg)
+        scores = np.zeros(max(n, 1), dtype=np.int32)
+        flags = np.zeros(max(n, 1), dtype=np.uint8)
+        n1 = int(o1[-1] - o1[0]) + n
+        n2 = int(o2[-1] - o2[0]) + n
+        dp_column = np.zeros(max(n1, 1), dtype=np.int32)
+        dp_row = np.zeros(max(n2, 1), dtype=np.int32)
+        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
+        st = self._lib.pa_score_batch_dp(self.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
+                                         b2.ctypes.data, o2.ctypes.data, n, None if pm is None else pm.ctypes.data,
+                                         scores.ctypes.data, flags.ctypes.data, dp_column.ctypes.data,
+                                         dp_row.ctypes.data)
+        self.check(st)
+        return scores[:n], flags[:n], dp_column[:n1], dp_row[:n2]
 
     def score_one_to_many(self, query, b2, o2, cfg: DeviceConfig, pair_method=None):
         """Scores of (query, target p) for every packed target (pa_score_one_to_many)."""

@@ -170,12 +170,29 @@ static void bulk_calls()
     }
 }
 
+// compute(seq1, seq2, first_dp_column, first_dp_row): the final DP column and row.  Expected
+// values worked by hand from the recurrence (+4/-5, gap (-10,-1), global): H(4, j) of ACGT x AGT is
+// -14 -9 -13 1 and H(i, 3) is -13 -8 -12 -6 1.
+void dp_vectors()
+{
+    auto a = aligner::cfg::configure_aligner(aligner::cfg::score_model_unitary(standard, 4, -5));
+    std::vector<int> unused_column, unused_row;
+    auto r = a.compute(std::string{"ACGT"}, std::string{"AGT"}, unused_column, unused_row);
+    EXPECT_EQ(r.score(), 1);
+    EXPECT_EQ(r.dp_row() == (std::vector<int32_t>{-14, -9, -13, 1}), true);
+    EXPECT_EQ(r.dp_column() == (std::vector<int32_t>{-13, -8, -12, -6, 1}), true);
+    auto e = a.compute(std::string{""}, std::string{"CC"}, unused_column, unused_row);
+    EXPECT_EQ(e.dp_row() == (std::vector<int32_t>{0, -11, -12}), true);
+    EXPECT_EQ(e.dp_column() == (std::vector<int32_t>{-12}), true);
+}
+
 int main(int argc, char **argv)
 {
     config_checks();
     if (!(argc > 1 && std::strcmp(argv[1], "--config") == 0)) {
         known_answers();
         bulk_calls();
+        dp_vectors();
     }
     std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "OK", failures);
     return failures ? 1 : 0;

@@ -1,0 +1,102 @@
+"""GPU: the final DP column and row of every pair (pa_score_batch_dp), the device counterpart of
+the reference's result.dp_column() / dp_row() (result/aligner_result.hpp:41-59).
+
+Every exported entry is compared with the oracle's (oracle.dp_vectors_packed, pinned by
+enumeration in tests/test_oracle.py), across the end-gap combinations, the local method, per-pair
+methods, BLOSUM62 scoring, exchanged orientations (the planner puts the shorter sequence on the
+rows where the configuration allows), empty sequences and pairs longer than the wavefront
+kernel's threshold.
+"""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pairwise_aligner_amd import device, tables, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    sys.stdout.write(msg + "\n")
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return device.Context(0)
+
+
+def oracle_cfg(dc):
+    return O.OracleConfig(mode=dc.method, first_column_free=dc.first_column_free, first_row_free=dc.first_row_free,
+                          last_column_free=dc.last_column_free, last_row_free=dc.last_row_free,
+                          gap_open=dc.gap_open, gap_extension=dc.gap_extension, score_kind=dc.score_kind,
+                          match=dc.match, mismatch=dc.mismatch, symbols=dc.symbols, matrix=dc.matrix)
+
+
+def check(ctx, s1, s2, dc, pair_method=None, label=""):
+    log(f"dp export {label}: {len(s1)} pairs")
+    b1, o1 = workloads.pack(s1)
+    b2, o2 = workloads.pack(s2)
+    want_s, want_rows, want_cols = O.dp_vectors_packed(b1, o1, b2, o2, oracle_cfg(dc), pair_method=pair_method,
+                                                       threads=8)
+    got_s, flags, cols, rows = ctx.score_packed_dp(b1, o1, b2, o2, dc, pair_method)
+    assert np.array_equal(got_s.astype(np.int64), want_s), f"{label}: scores"
+    bad_c = np.nonzero(cols.astype(np.int64) != want_cols)[0]
+    bad_r = np.nonzero(rows.astype(np.int64) != want_rows)[0]
+    assert len(bad_c) == 0, f"{label}: {len(bad_c)} column entries differ, first at {bad_c[:5].tolist()}"
+    assert len(bad_r) == 0, f"{label}: {len(bad_r)} row entries differ, first at {bad_r[:5].tolist()}"
+    plain, _ = ctx.score_packed(b1, o1, b2, o2, dc, pair_method)
+    assert np.array_equal(plain, got_s)
+
+
+def dna_batch(rng, count, hi):
+    s1, s2 = workloads.dna_pairs(rng, count, 0, hi, divergence=0.15, indel_fraction=0.4)
+    # unrelated lengths too, so that both orientations occur
+    s2 = [b if k % 3 else bytes(workloads.random_residues(rng, workloads.DNA, int(rng.integers(0, hi))))
+          for k, b in enumerate(s2)]
+    return s1, s2
+
+
+@pytest.mark.parametrize("flags", list(itertools.product([False, True], repeat=4)),
+                         ids=lambda f: "".join("F" if x else "p" for x in f))
+def test_end_gaps(ctx, flags):
+    fc, fr, lc, lr = flags
+    rng = np.random.default_rng(900 + sum(int(f) << k for k, f in enumerate(flags)))
+    s1, s2 = dna_batch(rng, 150, 260)
+    dc = device.DeviceConfig(match=4, mismatch=-5, gap_open=-10, gap_extension=-1, first_column_free=fc,
+                             first_row_free=fr, last_column_free=lc, last_row_free=lr)
+    check(ctx, s1, s2, dc, label=f"ends {flags}")
+
+
+def test_local_and_mixed_methods(ctx):
+    rng = np.random.default_rng(950)
+    s1, s2 = dna_batch(rng, 300, 400)
+    dc = device.DeviceConfig(method=device.METHOD_LOCAL, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
+    check(ctx, s1, s2, dc, label="local")
+    pm = (np.arange(len(s1)) % 2).astype(np.uint8)
+    check(ctx, s1, s2, device.DeviceConfig(match=2, mismatch=-3, gap_open=-5, gap_extension=-2), pair_method=pm,
+          label="mixed methods")
+
+
+def test_blosum62(ctx):
+    rng = np.random.default_rng(960)
+    s1, s2 = workloads.protein_pairs(rng, 200, 0, 300)
+    t = tables.blosum62_standard()
+    for method in (device.METHOD_GLOBAL, device.METHOD_LOCAL):
+        dc = device.DeviceConfig(method=method, score_kind=device.SCORE_MATRIX,
+                                 symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
+                                 gap_open=-11, gap_extension=-1, last_row_free=True)
+        check(ctx, s1, s2, dc, label=f"blosum method {method}")
+
+
+def test_long_and_empty(ctx):
+    rng = np.random.default_rng(970)
+    s1, s2 = workloads.dna_pairs(rng, 4, 3500, 5000, divergence=0.1)
+    s1 += [b"", b"ACGT", b"", b"G"]
+    s2 += [b"", b"", b"ACGTA", b"G"]
+    for lr in (False, True):
+        dc = device.DeviceConfig(match=1, mismatch=-4, gap_open=-6, gap_extension=-1, last_row_free=lr)
+        check(ctx, s1, s2, dc, label=f"long and empty, last row free {lr}")
