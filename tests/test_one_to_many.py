@@ -137,3 +137,15 @@ def test_edge_cases(ctx):
     b2, o2 = workloads.pack([])
     got, _ = ctx.score_one_to_many(b"ACGT", b2, o2, device.DeviceConfig())
     assert len(got) == 0
+
+
+def test_learned_policy(ctx):
+    """The learned lane dispatcher reads one first sequence per pair: with one query the planner
+    repeats it for the feature kernel.  An untrained network makes arbitrary int16 guesses; the
+    saturation flag and the int32 recompute keep the scores exact."""
+    rng = np.random.default_rng(707)
+    q = workloads.protein_pairs(rng, 1, 400, 400)[0][0]
+    targets = workloads.protein_pairs(rng, 800, 20, 900)[1]
+    model = device.Model(ctx, seed=11)
+    check(ctx, q, targets, blosum(device.METHOD_LOCAL, lane_policy=device.LANES_LEARNED, model=model),
+          label="learned policy")
