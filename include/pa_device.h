@@ -128,7 +128,8 @@ pa_status pa_score_batch_dp(pa_ctx *ctx, const pa_config *cfg,
                             int32_t *scores, uint8_t *flags, int32_t *dp_column, int32_t *dp_row);
 
 /* Device-resident batch: the same work split into plan/upload, repeated runs, and download,
- * so a caller can time runs with inputs already in HBM. */
+ * so a caller can time runs with inputs already in HBM.  The input buffers are read only during
+ * the call: the batch keeps the residues on the device and no host copy of them. */
 pa_status pa_batch_create(pa_ctx *ctx, const pa_config *cfg,
                           const uint8_t *seq1, const uint64_t *off1,
                           const uint8_t *seq2, const uint64_t *off2,

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pa_device.h"
@@ -52,8 +53,10 @@ struct pa_batch {
     pa_config cfg{};
     std::vector<int32_t> matrix;
     std::vector<uint8_t> symbols;
-    // caller's inputs (host copies, kept for re-planning saturated pairs)
-    std::vector<uint8_t> seq1, seq2, pair_method;
+    // What re-planning the saturated pairs needs of the caller's inputs: the offsets, per-pair
+    // methods and, for one query, the query bytes (the other residues stay on the device only;
+    // the re-plan takes the alphabet from the main plan's residue codes)
+    std::vector<uint8_t> query, pair_method;
     std::vector<uint64_t> off1, off2;
     uint64_t n_pairs = 0;
     bool one_query = false;   // seq1 is one query scored against every seq2 (off1 = {0, length})
@@ -103,8 +106,8 @@ pa_status hip_fail(pa_ctx *ctx, hipError_t e, const char *what)
         if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
     } while (0)
 
-template <typename T>
-pa_status upload(pa_ctx *ctx, DeviceBuffer &buf, const std::vector<T> &host)
+template <typename T, typename A>
+pa_status upload(pa_ctx *ctx, DeviceBuffer &buf, const std::vector<T, A> &host)
 {
     const size_t bytes = std::max<size_t>(host.size() * sizeof(T), 16);
     PA_HIP(hipMalloc(&buf.ptr, bytes), "hipMalloc");
@@ -115,18 +118,27 @@ pa_status upload(pa_ctx *ctx, DeviceBuffer &buf, const std::vector<T> &host)
     return PA_OK;
 }
 
-// Uploads the residue bytes of a batch as one device buffer: seq1 bytes [0, end1), then the seq2
-// bytes (the layout the plan's EncodeTask offsets refer to).
-pa_status upload_residues(pa_ctx *ctx, const pa::BatchView &v, DeviceBuffer &buf)
+// Copies the residue bytes of a batch into one device buffer: seq1 bytes [0, end1), then the
+// seq2 bytes (the layout the plan's EncodeTask offsets refer to).  Touches nothing of the context
+// but its device and stream, so that it can run on a helper thread.
+hipError_t copy_residues(int device, hipStream_t stream, const pa::BatchView &v, DeviceBuffer &buf)
 {
-    const uint64_t n1 = v.end1(), n2 = v.off2[v.n_pairs];
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return e;
+    const uint64_t n1 = v.n_pairs ? v.end1() : 0, n2 = v.n_pairs ? v.off2[v.n_pairs] : 0;
     const size_t bytes = std::max<size_t>(n1 + n2, 16);
-    PA_HIP(hipMalloc(&buf.ptr, bytes), "hipMalloc");
+    if ((e = hipMalloc(&buf.ptr, bytes)) != hipSuccess) return e;
     buf.bytes = bytes;
     char *dst = static_cast<char *>(buf.ptr);
-    if (n1) PA_HIP(hipMemcpyAsync(dst, v.seq1, n1, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
-    if (n2) PA_HIP(hipMemcpyAsync(dst + n1, v.seq2, n2, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
-    return PA_OK;
+    if (n1 && (e = hipMemcpyAsync(dst, v.seq1, n1, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
+    if (n2 && (e = hipMemcpyAsync(dst + n1, v.seq2, n2, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+pa_status upload_residues(pa_ctx *ctx, const pa::BatchView &v, DeviceBuffer &buf)
+{
+    hipError_t e = copy_residues(ctx->device, ctx->stream, v, buf);
+    return e == hipSuccess ? PA_OK : hip_fail(ctx, e, "residue upload");
 }
 
 // Uploads a plan and encodes its residue streams on the device from `residues` (upload_residues).
@@ -313,8 +325,9 @@ pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
         auto pd = std::make_unique<PlanDevice>();
         std::vector<uint8_t> forced(b->n_pairs, 0);
         for (uint32_t p : sat) forced[p] = 32;
-        pa::BatchView view{b->seq1.data(), b->off1.data(), b->seq2.data(), b->off2.data(), b->n_pairs,
-                           b->pair_method.empty() ? nullptr : b->pair_method.data(), b->one_query};
+        pa::BatchView view{b->one_query ? b->query.data() : nullptr, b->off1.data(), nullptr, b->off2.data(),
+                           b->n_pairs, b->pair_method.empty() ? nullptr : b->pair_method.data(), b->one_query,
+                           b->main.plan.code};
         std::string err;
         pa_config cfg = b->cfg;
         if ((st = pa::make_plan(cfg, view, forced.data(), &sat, pd->plan, err)) != PA_OK) return fail(ctx, st, err);
@@ -496,13 +509,12 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
     const uint64_t n1 = one_query ? 1 : n_pairs;
     b->off1.assign(off1, off1 + n1 + 1);
     b->off2.assign(off2, off2 + n_pairs + 1);
-    b->seq1.assign(seq1 ? seq1 : nullptr, seq1 ? seq1 + off1[n1] : nullptr);
-    b->seq2.assign(seq2 ? seq2 : nullptr, seq2 ? seq2 + off2[n_pairs] : nullptr);
+    if (one_query) b->query.assign(seq1, seq1 + off1[1]);
+    if (b->query.empty()) b->query.push_back(0);
     if (pair_method) b->pair_method.assign(pair_method, pair_method + n_pairs);
-    if (b->seq1.empty()) b->seq1.push_back(0);
-    if (b->seq2.empty()) b->seq2.push_back(0);
 
-    pa::BatchView view{b->seq1.data(), b->off1.data(), b->seq2.data(), b->off2.data(), n_pairs,
+    static const uint8_t kNoResidues[1] = {0};
+    pa::BatchView view{seq1 ? seq1 : kNoResidues, b->off1.data(), seq2 ? seq2 : kNoResidues, b->off2.data(), n_pairs,
                        pair_method ? b->pair_method.data() : nullptr, one_query};
     std::string err;
     std::vector<uint8_t> forced;
@@ -511,14 +523,19 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
         if ((st = learned_widths(ctx, b->cfg, view, forced)) != PA_OK) return st;
     }
     b->export_dp = export_dp;
+    // The residue bytes go to the device while the host plans (the upload needs nothing from
+    // the plan); the helper thread is joined before anything else uses the stream.
+    hipError_t upload_err = hipSuccess;
+    std::thread uploader([&] { upload_err = copy_residues(ctx->device, ctx->stream, view, b->residues); });
     st = pa::make_plan(b->cfg, view, forced.empty() ? nullptr : forced.data(), nullptr, b->main.plan, err, false,
                        export_dp);
+    uploader.join();
+    if (upload_err != hipSuccess) return hip_fail(ctx, upload_err, "residue upload");
     if (st != PA_OK) return fail(ctx, st, err);
     const pa::Plan &p = b->main.plan;
     // Only int16 pairs whose bound does not exclude saturation need the flag read-back.
     b->can_saturate = p.n_speculative > 0;
 
-    if ((st = upload_residues(ctx, view, b->residues)) != PA_OK) return st;
     if ((st = upload_plan(ctx, b->main, b->residues)) != PA_OK) return st;
     if (export_dp) {
         b->dp_out.bytes = std::max<size_t>(p.dp_entries * 4, 16);

@@ -6,7 +6,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
-#include <map>
+#include <immintrin.h>
 #include <numeric>
 #include <thread>
 #include <tuple>
@@ -40,13 +40,15 @@ Tuning resolve_tuning(const pa_tuning *t)
     return r;
 }
 
-void choose_shape(uint32_t m, uint32_t n, const Tuning &tuning, uint8_t &G, uint8_t &R)
+void choose_shape(uint32_t m, const Tuning &tuning, uint8_t &G, uint8_t &R)
 {
     if (m > tuning.wavefront_min_rows) {   // long pair: one workgroup of 8 waves, stripes of 64*16 rows
         G = 64;
         R = 16;
         return;
     }
+    // Modelled cost S * (n + G - 1) * (cell_cost * R + step_cost) for S stripes of G*R rows over
+    // n columns; n + G - 1 is the same for every R, so the best R depends on m alone.
     G = 16;
     double best = 1e300;
     R = 16;
@@ -55,7 +57,7 @@ void choose_shape(uint32_t m, uint32_t n, const Tuning &tuning, uint8_t &G, uint
         const uint8_t r = kRowsPerLane[i];
         const uint32_t rows = uint32_t(G) * r;
         const uint32_t S = (m + rows - 1) / rows;
-        const double cost = double(S) * (double(n) + G - 1) * (tuning.cell_cost * r + tuning.step_cost);
+        const double cost = double(S) * (tuning.cell_cost * r + tuning.step_cost);
         if (cost < best * 0.999) { best = cost; R = r; }
     }
 }
@@ -77,13 +79,16 @@ RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n)
 
 namespace {
 
-// Runs body(i) for i in [0, n) on up to 16 threads, in blocks of 256 indices.
-template <typename F>
-void parallel_for(size_t n, F &&body)
+unsigned planner_threads()
 {
-    const size_t block = 256;
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const unsigned nt = unsigned(std::min<size_t>(hw, (n + block - 1) / block));
+    return std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+
+// Runs body(i) for i in [0, n) on up to 16 threads, in blocks of `grain` indices.
+template <typename F>
+void parallel_for(size_t n, F &&body, size_t grain = 256)
+{
+    const unsigned nt = unsigned(std::min<size_t>(planner_threads(), (n + grain - 1) / grain));
     if (nt <= 1) {
         for (size_t i = 0; i < n; ++i) body(i);
         return;
@@ -91,9 +96,9 @@ void parallel_for(size_t n, F &&body)
     std::atomic<size_t> next{0};
     auto worker = [&]() {
         for (;;) {
-            const size_t b = next.fetch_add(block);
+            const size_t b = next.fetch_add(grain);
             if (b >= n) return;
-            for (size_t i = b; i < std::min(n, b + block); ++i) body(i);
+            for (size_t i = b; i < std::min(n, b + grain); ++i) body(i);
         }
     };
     std::vector<std::thread> pool;
@@ -102,6 +107,40 @@ void parallel_for(size_t n, F &&body)
     for (auto &th : pool) th.join();
 }
 
+// Sorts every vector of `vs` by cmp on up to 16 threads: each vector is cut into up to 16 runs
+// (one per 16Ki elements, a power of two), all runs of all vectors are sorted concurrently and
+// then merged pairwise.  For a strict total order (or with `stable`) the result is the one
+// std::sort (std::stable_sort) gives.
+template <typename T, typename Cmp>
+void parallel_sort(std::vector<std::vector<T>> &vs, Cmp cmp, bool stable)
+{
+    struct Run { size_t v, lo, hi; };
+    std::vector<std::vector<size_t>> cuts(vs.size());
+    std::vector<Run> runs;
+    for (size_t v = 0; v < vs.size(); ++v) {
+        const size_t n = vs[v].size();
+        size_t parts = 1;
+        while (parts < 16 && n / (parts * 2) >= 16384) parts *= 2;
+        for (size_t i = 0; i <= parts; ++i) cuts[v].push_back(n * i / parts);
+        for (size_t i = 0; i < parts; ++i) runs.push_back(Run{v, cuts[v][i], cuts[v][i + 1]});
+    }
+    parallel_for(runs.size(), [&](size_t r) {
+        auto first = vs[runs[r].v].begin();
+        if (stable) std::stable_sort(first + runs[r].lo, first + runs[r].hi, cmp);
+        else std::sort(first + runs[r].lo, first + runs[r].hi, cmp);
+    }, 1);
+    for (size_t w = 1; w < 16; w *= 2) {
+        std::vector<std::pair<size_t, size_t>> merges;   // (vector, first run)
+        for (size_t v = 0; v < vs.size(); ++v)
+            for (size_t i = 0; i + w < cuts[v].size() - 1; i += 2 * w) merges.push_back({v, i});
+        parallel_for(merges.size(), [&](size_t k) {
+            const auto [v, i] = merges[k];
+            const std::vector<size_t> &c = cuts[v];
+            auto first = vs[v].begin();
+            std::inplace_merge(first + c[i], first + c[i + w], first + c[std::min(i + 2 * w, c.size() - 1)], cmp);
+        }, 1);
+    }
+}
 
 struct Alphabet {
     bool dna = false;
@@ -109,21 +148,66 @@ struct Alphabet {
     uint32_t size = 0;
 };
 
+// Marks the byte values in [p, p + n).  Bytes already marked are recognised 32 at a time by a
+// nibble-table membership test (AVX2 byte shuffles: lo[l] has bit h set when byte h*16 + l is
+// marked, for h < 8; hi[h] = 1 << h), the rest one at a time.
+__attribute__((target("avx2"))) void mark_bytes_avx2(const uint8_t *p, uint64_t n, std::array<uint8_t, 256> &m)
+{
+    alignas(32) uint8_t lo[32], hi[32];
+    auto tables = [&]() {
+        for (int l = 0; l < 16; ++l) {
+            uint8_t bits = 0;
+            for (int h = 0; h < 8; ++h) bits |= m[h * 16 + l] ? uint8_t(1u << h) : 0;
+            lo[l] = lo[l + 16] = bits;
+            hi[l] = hi[l + 16] = l < 8 ? uint8_t(1u << l) : 0;
+        }
+    };
+    tables();
+    __m256i vlo = _mm256_load_si256(reinterpret_cast<const __m256i *>(lo));
+    __m256i vhi = _mm256_load_si256(reinterpret_cast<const __m256i *>(hi));
+    const __m256i nib = _mm256_set1_epi8(0x0F), zero = _mm256_setzero_si256();
+    uint64_t i = 0;
+    for (; i + 32 <= n; i += 32) {
+        const __m256i x = _mm256_loadu_si256(reinterpret_cast<const __m256i *>(p + i));
+        const __m256i a = _mm256_shuffle_epi8(vlo, _mm256_and_si256(x, nib));
+        const __m256i b = _mm256_shuffle_epi8(vhi, _mm256_and_si256(_mm256_srli_epi16(x, 4), nib));
+        uint32_t miss = uint32_t(_mm256_movemask_epi8(_mm256_cmpeq_epi8(_mm256_and_si256(a, b), zero)));
+        if (!miss) continue;
+        bool added = false;
+        for (; miss; miss &= miss - 1) {
+            const uint8_t c = p[i + uint32_t(__builtin_ctz(miss))];
+            added |= !m[c];
+            m[c] = 1;
+        }
+        if (added) {
+            tables();
+            vlo = _mm256_load_si256(reinterpret_cast<const __m256i *>(lo));
+            vhi = _mm256_load_si256(reinterpret_cast<const __m256i *>(hi));
+        }
+    }
+    for (; i < n; ++i) m[p[i]] = 1;
+}
+
 // Marks the byte values that occur in the batch's sequences (in parallel over blocks of bytes).
 void scan_residues(const BatchView &b, bool (&seen)[256])
 {
     const uint8_t *src[2] = {b.seq1 + b.off1[0], b.seq2 + b.off2[0]};
     const uint64_t len[2] = {b.end1() - b.off1[0], b.off2[b.n_pairs] - b.off2[0]};
-    constexpr uint64_t kBlock = uint64_t(1) << 16;
+    constexpr uint64_t kBlock = uint64_t(1) << 18;
     const uint64_t nb0 = (len[0] + kBlock - 1) / kBlock, nb1 = (len[1] + kBlock - 1) / kBlock;
+    const bool avx2 = __builtin_cpu_supports("avx2");
     std::vector<std::array<uint8_t, 256>> marks(nb0 + nb1);
     parallel_for(nb0 + nb1, [&](size_t k) {
         const int side = k < nb0 ? 0 : 1;
         const uint64_t lo = (side ? k - nb0 : k) * kBlock, hi = std::min(len[side], lo + kBlock);
         std::array<uint8_t, 256> &m = marks[k];
         m.fill(0);
+        if (avx2) {
+            mark_bytes_avx2(src[side] + lo, hi - lo, m);
+            return;
+        }
         for (uint64_t i = lo; i < hi; ++i) m[src[side][i]] = 1;
-    });
+    }, 1);
     for (const auto &m : marks)
         for (int c = 0; c < 256; ++c) seen[c] = seen[c] || m[c];
 }
@@ -133,7 +217,10 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
     for (int c = 0; c < 256; ++c) alpha.code[c] = -1;
     if (cfg.score_kind == PA_SCORE_UNITARY) {
         bool seen[256] = {false};
-        scan_residues(b, seen);
+        if (b.known_codes)
+            for (int c = 0; c < 256; ++c) seen[c] = b.known_codes[c] >= 0;
+        else
+            scan_residues(b, seen);
         uint32_t k = 0;
         for (int c = 0; c < 256; ++c) if (seen[c]) alpha.code[c] = int16_t(k++);
         alpha.size = k;
@@ -207,6 +294,7 @@ pa_status build_alphabet(const pa_config &cfg, const BatchView &b, Plan &plan, A
             plan.smin = std::min(plan.smin, v);
         }
     // every residue must be in the alphabet (alphabet_rank_map_scalar.hpp:59-63 asserts it)
+    if (b.known_codes) return PA_OK;
     bool seen[256] = {false};
     scan_residues(b, seen);
     for (int c = 0; c < 256; ++c)
@@ -404,7 +492,18 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     std::vector<uint8_t> selected(N, only ? 0 : 1);
     if (only) for (uint32_t p : *only) selected[p] = 1;
 
-    std::map<VariantKey, std::vector<Item>> groups;
+    // Rows per lane of the striped kernel by DP rows m, tabulated once for the common lengths.
+    auto rows_table = [&](const Tuning &t) {
+        std::vector<uint8_t> tab(std::min<uint64_t>(uint64_t(t.wavefront_min_rows) + 1, 16384));
+        parallel_for(tab.size(), [&](size_t m) {
+            uint8_t G, R;
+            choose_shape(uint32_t(m), t, G, R);
+            tab[m] = R;
+        });
+        return tab;
+    };
+    const std::vector<uint8_t> rows_for_m = rows_table(plan.tuning);
+    const std::vector<uint8_t> profile_rows_for = use_profile ? rows_table(profile_tuning) : std::vector<uint8_t>{};
     // Per pair, in parallel: orientation, lane width and kernel variant.
     std::vector<VariantKey> pair_key(N);
     std::vector<uint8_t> speculative(N, 0);   // int16 without a proof against saturation
@@ -442,7 +541,13 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
             key.G = 16;
             key.R = kExportRows;
         } else {
-            choose_shape(pm.m, pm.n, profiled ? profile_tuning : plan.tuning, key.G, key.R);
+            const std::vector<uint8_t> &rows_for = profiled ? profile_rows_for : rows_for_m;
+            if (pm.m < rows_for.size()) {
+                key.G = 16;
+                key.R = rows_for[pm.m];
+            } else {
+                choose_shape(pm.m, profiled ? profile_tuning : plan.tuning, key.G, key.R);
+            }
         }
         key.export_dp = export_dp ? 1 : 0;
         if (profiled) key.score = kScoreProfile;
@@ -452,23 +557,76 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         key.rowmax = (!local && lc && lr) ? 1 : 0;
     });
     if (out_of_range) { err = "scores may leave the int32 range"; return PA_ERR_RANGE; }
-    // Group the pairs by variant (in pair order, so that planning is deterministic).
-    const VariantKey *last_key = nullptr;
-    std::vector<Item> *last_items = nullptr;
-    for (uint64_t p = 0; p < N; ++p) {
-        const PairMeta &pm = plan.pairs[p];
-        if (pm.width == 0) continue;
-        const VariantKey &key = pair_key[p];
-        if (!last_key || !(key == *last_key)) {
-            last_items = &groups[key];
-            last_key = &key;
+    // Group the pairs by variant: groups in VariantKey order, pairs in pair order within a group
+    // (planning is deterministic).  Blocks of pairs are scanned in parallel: distinct keys, then
+    // counts per group, then the items scattered to their places.
+    constexpr size_t kPairBlock = 1 << 16;
+    auto packed = [](const VariantKey &k) { uint64_t v; std::memcpy(&v, &k, sizeof v); return v; };
+    const size_t nblk = (N + kPairBlock - 1) / kPairBlock;
+    std::vector<std::vector<VariantKey>> blk_keys(nblk);
+    parallel_for(nblk, [&](size_t k) {
+        std::vector<uint64_t> seen_keys;
+        for (size_t p = k * kPairBlock; p < std::min<size_t>(N, (k + 1) * kPairBlock); ++p) {
+            if (plan.pairs[p].width == 0) continue;
+            const uint64_t pk = packed(pair_key[p]);
+            if (std::find(seen_keys.begin(), seen_keys.end(), pk) != seen_keys.end()) continue;
+            seen_keys.push_back(pk);
+            blk_keys[k].push_back(pair_key[p]);
         }
-        const uint32_t rows = uint32_t(key.G) * key.R;
-        last_items->push_back(Item{uint32_t(p), pm.m, pm.n, (pm.m + rows - 1) / rows});
-        plan.cells += uint64_t(pm.m) * pm.n;
-        if (key.width == 16) ++plan.n16; else ++plan.n32;
-        if (speculative[p]) ++plan.n_speculative;
-        if (key.score == kScoreProfile) ++plan.n_profiled;
+    }, 1);
+    std::vector<VariantKey> keys;
+    for (const auto &ks : blk_keys) keys.insert(keys.end(), ks.begin(), ks.end());
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const size_t ng = keys.size();
+    std::vector<uint64_t> packed_keys(ng);
+    for (size_t g = 0; g < ng; ++g) packed_keys[g] = packed(keys[g]);
+    std::vector<uint32_t> gid(N);
+    std::vector<uint64_t> blk_pos(nblk * ng, 0);   // per block and group: count, then start
+    parallel_for(nblk, [&](size_t k) {
+        uint32_t g = 0;
+        for (size_t p = k * kPairBlock; p < std::min<size_t>(N, (k + 1) * kPairBlock); ++p) {
+            if (plan.pairs[p].width == 0) continue;
+            const uint64_t pk = packed(pair_key[p]);
+            if (packed_keys[g] != pk)
+                g = uint32_t(std::find(packed_keys.begin(), packed_keys.end(), pk) - packed_keys.begin());
+            gid[p] = g;
+            ++blk_pos[k * ng + g];
+        }
+    }, 1);
+    std::vector<std::vector<Item>> items(ng);
+    for (size_t g = 0; g < ng; ++g) {
+        uint64_t total = 0;
+        for (size_t k = 0; k < nblk; ++k) {
+            const uint64_t c = blk_pos[k * ng + g];
+            blk_pos[k * ng + g] = total;
+            total += c;
+        }
+        items[g].resize(total);
+    }
+    struct Counts { uint64_t cells = 0, n16 = 0, n32 = 0, spec = 0, prof = 0; };
+    std::vector<Counts> blk_counts(nblk);
+    parallel_for(nblk, [&](size_t k) {
+        Counts &c = blk_counts[k];
+        for (size_t p = k * kPairBlock; p < std::min<size_t>(N, (k + 1) * kPairBlock); ++p) {
+            const PairMeta &pm = plan.pairs[p];
+            if (pm.width == 0) continue;
+            const uint32_t g = gid[p];
+            const VariantKey &key = keys[g];
+            const uint32_t rows = uint32_t(key.G) * key.R;
+            items[g][blk_pos[k * ng + g]++] = Item{uint32_t(p), pm.m, pm.n, (pm.m + rows - 1) / rows};
+            c.cells += uint64_t(pm.m) * pm.n;
+            if (key.width == 16) ++c.n16; else ++c.n32;
+            if (speculative[p]) ++c.spec;
+            if (key.score == kScoreProfile) ++c.prof;
+        }
+    }, 1);
+    for (const Counts &c : blk_counts) {
+        plan.cells += c.cells;
+        plan.n16 += c.n16;
+        plan.n32 += c.n32;
+        plan.n_speculative += c.spec;
+        plan.n_profiled += c.prof;
     }
 
     for (int c = 0; c < 256; ++c) plan.code[c] = alpha.code[c];
@@ -481,23 +639,21 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         plan.dp_entries = dp_total1 + (b.off2[N] - b.off2[0]) + N;
     }
 
-    uint64_t n_rowcodes = 0, n_colcodes = 0;
-    // Sorting and packing of the groups are independent: one thread per group.
-    std::vector<std::pair<const VariantKey *, std::vector<Item> *>> group_list;
-    for (auto &[key, items] : groups) group_list.push_back({&key, &items});
-    std::vector<std::vector<std::pair<int64_t, int64_t>>> group_packs(group_list.size());
-    auto pack_group = [&](size_t gi) {
-        const VariantKey &key = *group_list[gi].first;
-        std::vector<Item> &items = *group_list[gi].second;
-        // Pack int16 pairs two per task: similar stripe count and column count side by side.
-        std::sort(items.begin(), items.end(), [](const Item &a, const Item &c) {
-            return std::tie(a.S, a.n, a.m, a.pair) > std::tie(c.S, c.n, c.m, c.pair);
-        });
-        std::vector<std::pair<int64_t, int64_t>> &packs = group_packs[gi];  // indices into items
-        if (key.width == 16) {
+    // Tasks of each group: int16 pairs packed two per task (similar stripe count and column
+    // count side by side), longest tasks first so that the tail of a launch is made of short
+    // tasks.
+    using Pack = std::pair<int64_t, int64_t>;   // indices into the group's items; -1: no second half
+    parallel_sort(items, [](const Item &a, const Item &c) {
+        return std::tie(a.S, a.n, a.m, a.pair) > std::tie(c.S, c.n, c.m, c.pair);
+    }, false);
+    std::vector<std::vector<Pack>> group_packs(ng);
+    parallel_for(ng, [&](size_t g) {
+        const std::vector<Item> &it = items[g];
+        std::vector<Pack> &packs = group_packs[g];
+        if (keys[g].width == 16) {
             size_t i = 0;
-            while (i < items.size()) {
-                if (i + 1 < items.size() && (!key.rowmax || items[i].n == items[i + 1].n)) {
+            while (i < it.size()) {
+                if (i + 1 < it.size() && (!keys[g].rowmax || it[i].n == it[i + 1].n)) {
                     packs.push_back({int64_t(i), int64_t(i + 1)});
                     i += 2;
                 } else {
@@ -506,74 +662,129 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 }
             }
         } else {
-            for (size_t i = 0; i < items.size(); ++i) packs.push_back({int64_t(i), -1});
+            packs.resize(it.size());
+            for (size_t i = 0; i < it.size(); ++i) packs[i] = {int64_t(i), -1};
         }
-        const uint32_t G = key.G;
-        auto work = [&](const std::pair<int64_t, int64_t> &pk) {
-            const Item &a = items[pk.first];
-            uint32_t S = a.S, n = a.n;
-            if (pk.second >= 0) { S = std::max(S, items[pk.second].S); n = std::max(n, items[pk.second].n); }
-            return uint64_t(S) * (n + G - 1);
-        };
-        // Longest tasks first so that the tail of the launch is made of short tasks.
-        std::stable_sort(packs.begin(), packs.end(), [&](auto &x, auto &y) { return work(x) > work(y); });
-    };
-    {
-        std::vector<std::thread> pool;
-        for (size_t gi = 1; gi < group_list.size(); ++gi) pool.emplace_back(pack_group, gi);
-        if (!group_list.empty()) pack_group(0);
-        for (auto &th : pool) th.join();
-    }
+    }, 1);
+    // work of a pack: stripes x (columns + G - 1), kept with the pack for the sort
+    struct Weighed { uint64_t work; Pack pk; };
+    std::vector<std::vector<Weighed>> weighed(ng);
+    parallel_for(ng, [&](size_t g) {
+        const std::vector<Item> &it = items[g];
+        const uint32_t G = keys[g].G;
+        weighed[g].resize(group_packs[g].size());
+        for (size_t i = 0; i < group_packs[g].size(); ++i) {
+            const Pack &pk = group_packs[g][i];
+            uint32_t S = it[pk.first].S, n = it[pk.first].n;
+            if (pk.second >= 0) { S = std::max(S, it[pk.second].S); n = std::max(n, it[pk.second].n); }
+            weighed[g][i] = Weighed{uint64_t(S) * (n + G - 1), pk};
+        }
+    }, 1);
+    parallel_sort(weighed, [](const Weighed &x, const Weighed &y) { return x.work > y.work; }, true);
+    parallel_for(ng, [&](size_t g) {
+        for (size_t i = 0; i < weighed[g].size(); ++i) group_packs[g][i] = weighed[g][i].pk;
+    }, 1);
 
-    for (size_t gi = 0; gi < group_list.size(); ++gi) {
-        const VariantKey &key = *group_list[gi].first;
-        const std::vector<Item> &items = *group_list[gi].second;
-        const std::vector<std::pair<int64_t, int64_t>> &packs = group_packs[gi];
-        const uint32_t G = key.G, R = key.R, rows = G * R;
-        Launch L{key, uint32_t(plan.tasks.size()), 0, 0};
-        for (auto &pk : packs) {
-            const Item *h[2] = {&items[pk.first], pk.second >= 0 ? &items[pk.second] : nullptr};
-            Task t{};
-            t.S = std::max(h[0]->S, h[1] ? h[1]->S : 0u);
-            t.N = std::max(h[0]->n, h[1] ? h[1]->n : 0u);
+    // Task records, in group order.  Chunks of packs are measured in parallel (code words,
+    // boundary entries, lane-cells), their offsets are prefix sums in task order, and the
+    // chunks are then filled in parallel.
+    constexpr size_t kPackChunk = 1 << 13;
+    struct Chunk { size_t g, lo, hi; uint64_t task, rowcodes, colcodes, bnd, padded; };
+    std::vector<Chunk> chunks;
+    for (size_t g = 0; g < ng; ++g)
+        for (size_t lo = 0; lo < group_packs[g].size(); lo += kPackChunk)
+            chunks.push_back(Chunk{g, lo, std::min(group_packs[g].size(), lo + kPackChunk), 0, 0, 0, 0, 0});
+    auto task_shape = [&](size_t g, const Pack &pk, uint32_t &S, uint32_t &Nc) {
+        const Item &a = items[g][pk.first];
+        S = a.S;
+        Nc = a.n;
+        if (pk.second >= 0) { S = std::max(S, items[g][pk.second].S); Nc = std::max(Nc, items[g][pk.second].n); }
+    };
+    parallel_for(chunks.size(), [&](size_t c) {
+        Chunk &ch = chunks[c];
+        const VariantKey &key = keys[ch.g];
+        const uint32_t G = key.G, rows = G * key.R;
+        for (size_t i = ch.lo; i < ch.hi; ++i) {
+            uint32_t S, Nc;
+            task_shape(ch.g, group_packs[ch.g][i], S, Nc);
+            const uint32_t cw = col_words(Nc, G);
+            if (S > 1) ch.bnd += cw;
+            if (key.score != kScoreProfile) ch.rowcodes += uint64_t(S) * rows / (key.score == kScorePairs ? 2 : 1);
+            ch.colcodes += cw;
+            ch.padded += uint64_t(S) * rows * Nc * (key.width == 16 ? 2 : 1);
+        }
+    }, 1);
+    uint64_t n_rowcodes = 0, n_colcodes = 0, n_tasks = 0;
+    for (Chunk &ch : chunks) {
+        const uint64_t r = ch.rowcodes, cc = ch.colcodes, bn = ch.bnd;
+        ch.task = n_tasks;
+        ch.rowcodes = n_rowcodes;
+        ch.colcodes = n_colcodes;
+        ch.bnd = plan.bnd_entries;
+        n_tasks += ch.hi - ch.lo;
+        n_rowcodes += r;
+        n_colcodes += cc;
+        plan.bnd_entries += bn;
+        plan.padded_cells += ch.padded;
+    }
+    plan.tasks.resize(n_tasks);
+    plan.enc.resize(n_tasks);
+    if (export_dp) plan.dp_base.resize(2 * n_tasks);
+    parallel_for(chunks.size(), [&](size_t c) {
+        const Chunk &ch = chunks[c];
+        const VariantKey &key = keys[ch.g];
+        const uint32_t G = key.G, rows = G * key.R;
+        uint64_t rc = ch.rowcodes, cc = ch.colcodes, bn = ch.bnd, ti = ch.task;
+        for (size_t i = ch.lo; i < ch.hi; ++i, ++ti) {
+            const Pack &pk = group_packs[ch.g][i];
+            const Item *h[2] = {&items[ch.g][pk.first], pk.second >= 0 ? &items[ch.g][pk.second] : nullptr};
+            Task &t = plan.tasks[ti];
+            t = Task{};
+            task_shape(ch.g, pk, t.S, t.N);
             for (int q = 0; q < 2; ++q) {
                 t.pair[q] = h[q] ? h[q]->pair : kNone;
                 t.m[q] = h[q] ? h[q]->m : 0;
                 t.n[q] = h[q] ? h[q]->n : 0;
             }
-            t.row_off = uint32_t(std::min<uint64_t>(n_rowcodes, 0xFFFFFFFFull));
-            t.col_off = uint32_t(std::min<uint64_t>(n_colcodes, 0xFFFFFFFFull));
+            t.row_off = uint32_t(std::min<uint64_t>(rc, 0xFFFFFFFFull));
+            t.col_off = uint32_t(std::min<uint64_t>(cc, 0xFFFFFFFFull));
             t.bnd_off = kNone;
             const uint32_t cw = col_words(t.N, G);
             if (t.S > 1) {
-                t.bnd_off = uint32_t(std::min<uint64_t>(plan.bnd_entries, 0xFFFFFFFFull));
-                plan.bnd_entries += cw;
+                t.bnd_off = uint32_t(std::min<uint64_t>(bn, 0xFFFFFFFFull));
+                bn += cw;
             }
-            if (key.score != kScoreProfile) n_rowcodes += uint64_t(t.S) * rows / (key.score == kScorePairs ? 2 : 1);
-            n_colcodes += cw;
-            const uint64_t pc = uint64_t(t.S) * rows * t.N * (key.width == 16 ? 2 : 1);
-            L.padded_cells += pc;
-            plan.padded_cells += pc;
-            plan.tasks.push_back(t);
-            EncodeTask e{};
+            if (key.score != kScoreProfile) rc += uint64_t(t.S) * rows / (key.score == kScorePairs ? 2 : 1);
+            cc += cw;
+            EncodeTask &e = plan.enc[ti];
+            e = EncodeTask{};
             for (int q = 0; q < 2; ++q) {
                 if (!h[q]) continue;
                 const uint32_t p = h[q]->pair;
-                const uint64_t s1 = uint64_t(b.s1(p) - b.seq1), s2 = seq2_base + b.off2[p];
+                const uint64_t s1 = b.src1(p), s2 = seq2_base + b.off2[p];
                 e.row_src[q] = plan.pairs[p].swapped ? s2 : s1;
                 e.col_src[q] = plan.pairs[p].swapped ? s1 : s2;
             }
-            plan.enc.push_back(e);
             if (export_dp) {
                 // our rows are seq1 unless the pair is exchanged (then seq2)
                 const uint32_t p = h[0]->pair;
                 const uint64_t c1 = b.off1[p] - b.off1[0] + p, c2 = dp_total1 + (b.off2[p] - b.off2[0]) + p;
                 const bool sw = plan.pairs[p].swapped;
-                plan.dp_base.push_back(sw ? c2 : c1);   // H(0, n): the task's last column
-                plan.dp_base.push_back(sw ? c1 : c2);   // H(m, 0): the task's last row
+                plan.dp_base[2 * ti] = sw ? c2 : c1;       // H(0, n): the task's last column
+                plan.dp_base[2 * ti + 1] = sw ? c1 : c2;   // H(m, 0): the task's last row
             }
         }
-        L.task_end = uint32_t(plan.tasks.size());
+    }, 1);
+    for (size_t g = 0; g < ng; ++g) {
+        Launch L{keys[g], 0, 0, 0};
+        bool first = true;
+        for (const Chunk &ch : chunks) {
+            if (ch.g != g) continue;
+            if (first) L.task_begin = uint32_t(ch.task);
+            first = false;
+            L.task_end = uint32_t(ch.task + ch.hi - ch.lo);
+            L.padded_cells += ch.padded;
+        }
         if (L.task_end > L.task_begin) plan.launches.push_back(L);
     }
     if (n_rowcodes > 0xFFFFFFF0ull || n_colcodes > 0xFFFFFFF0ull || plan.bnd_entries > 0x7FFFFFF0ull) {

@@ -20,7 +20,10 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
+#include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "pa_device.h"
@@ -77,6 +80,22 @@ inline constexpr uint32_t kProfileMaxBytes = 65536;
 inline constexpr uint32_t kProfileRowsMask = 0x7F & ~0x44u;   // R = 4, 6, 10, 12, 14
 inline uint32_t profile_col_offset(uint32_t b, uint32_t rows) { return b * rows * 2; }
 
+// Allocator whose value-less construct leaves trivially constructible elements uninitialised:
+// the planner's large per-task arrays are sized first and then filled (and paged in) by
+// parallel passes, instead of being zeroed on one thread.
+template <typename T>
+struct UninitAllocator : std::allocator<T> {
+    template <typename U> struct rebind { using other = UninitAllocator<U>; };
+    UninitAllocator() noexcept = default;
+    template <typename U> UninitAllocator(const UninitAllocator<U> &) noexcept {}
+    template <typename U> void construct(U *p) noexcept { ::new (static_cast<void *>(p)) U; }
+    template <typename U, typename... A> void construct(U *p, A &&...a)
+    {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <typename T> using FillVector = std::vector<T, UninitAllocator<T>>;
+
 struct Task {
     uint32_t row_off;
     uint32_t col_off;
@@ -110,6 +129,7 @@ struct VariantKey {
     bool operator<(const VariantKey &o) const;
     bool operator==(const VariantKey &o) const;
 };
+static_assert(sizeof(VariantKey) == 8, "VariantKey is compared as one 64-bit word by the planner");
 
 // Substitution-score shift of a kernel variant, in units of -ge (pa_cells.hpp): the local
 // method keeps exact values (0); the global method drifts along the rows (1) and, unless it
@@ -169,18 +189,18 @@ struct Plan {
     int32_t smax = 0, smin = 0;               // extreme substitution scores
 
     std::vector<PairMeta> pairs;
-    std::vector<Task> tasks;
+    FillVector<Task> tasks;
     std::vector<Launch> launches;
     uint64_t bnd_entries = 0;                 // (H,E) entries of the boundary buffer
     uint64_t cells = 0, padded_cells = 0;
     uint64_t n16 = 0, n32 = 0;
     uint64_t n_speculative = 0;               // int16 pairs whose bound does not exclude saturation
     uint64_t n_profiled = 0;                  // int16 pairs scored through the query profile
-    std::vector<EncodeTask> enc;              // per task: residue sources for the device encoder
+    FillVector<EncodeTask> enc;               // per task: residue sources for the device encoder
     // DP export: per task, the index in the output of H(0, n) (the task's last column, m+1
     // entries) and of H(m, 0) (its last row, n+1 entries).  The output holds the caller's
     // dp_column values (seq1 side, pair p at off1[p]-off1[0]+p) followed by the dp_row values.
-    std::vector<uint64_t> dp_base;            // 2 per task
+    FillVector<uint64_t> dp_base;             // 2 per task
     uint64_t dp_entries = 0;
     uint64_t n_rowcodes = 0, n_colcodes = 0;  // words of the encoded residue streams
     int16_t code[256] = {};                   // residue -> DNA code or matrix rank (-1: absent)
@@ -199,8 +219,12 @@ struct BatchView {
     // One query against many targets: every pair's first sequence is seq1[off1[0], off1[1]) (off1
     // holds two entries).
     bool one_query = false;
+    // Residue codes of an earlier plan of the same batch (Plan::code): the alphabet is taken from
+    // them and the residue bytes are not read (seq1 only for a one-query profile, seq2 not at all).
+    const int16_t *known_codes = nullptr;
 
     const uint8_t *s1(uint64_t p) const { return seq1 + off1[one_query ? 0 : p]; }
+    uint64_t src1(uint64_t p) const { return off1[one_query ? 0 : p]; }   // offset of s1(p) in seq1
     uint64_t len1(uint64_t p) const { return one_query ? off1[1] - off1[0] : off1[p + 1] - off1[p]; }
     uint64_t end1() const { return off1[one_query ? 1 : n_pairs]; }   // end of the first sequences
 };
@@ -233,10 +257,10 @@ RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n);
 // Pairs with more rows than this go to the wavefront kernel (G = 64, a workgroup per task).
 inline constexpr uint32_t kWavefrontMinRows = 3072;
 
-// Choice of (G, R) for a pair with m rows and n columns: the wavefront kernel for pairs with
-// more than tuning.wavefront_min_rows rows, else the allowed rows per lane with the lowest
-// modelled sweep cost (stripes x steps x per-step work).
-void choose_shape(uint32_t m, uint32_t n, const Tuning &tuning, uint8_t &G, uint8_t &R);
+// Choice of (G, R) for a pair with m rows: the wavefront kernel for pairs with more than
+// tuning.wavefront_min_rows rows, else the allowed rows per lane with the lowest modelled sweep
+// cost (stripes x steps x per-step work; the step count is the same for every R).
+void choose_shape(uint32_t m, const Tuning &tuning, uint8_t &G, uint8_t &R);
 
 // LDS bytes a launch of variant `key` needs for its substitution table.
 uint32_t lds_bytes(const Plan &plan, const VariantKey &key);
