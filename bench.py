@@ -203,6 +203,8 @@ def run(name, steps, warmup, with_cpu=True, autotune=False, pairs=None, e2e_reps
     batch.close()
     if e2e_reps:
         res["end_to_end"] = end_to_end(ctx, w, dc, cells, scores, e2e_reps)
+        if "query" not in w:
+            res["end_to_end"]["from_fasta_files"] = from_files(ctx, w, dc, cells, scores, min(e2e_reps, 5))
     return res
 
 
@@ -229,6 +231,53 @@ def end_to_end(ctx, w, dc, cells, want_scores, reps):
     log(f"[bench] end to end: median {med * 1e3:.1f} ms over {reps} calls, planning {plan_ms:.1f} ms")
     return dict(median_ms=round(med * 1e3, 3), gcups=round(cells / med / 1e9, 2), reps=reps,
                 plan_and_upload_ms=round(plan_ms, 2), scores_identical=bool(np.array_equal(got, want_scores)))
+
+
+def from_files(ctx, w, dc, cells, want_scores, reps):
+    """Sequences from FASTA files to scores: the two sides are written as FASTA files (60-column
+    lines), then read back with pairwise_aligner_amd.fastx (host, parallel) and scored with the
+    one-shot entry point; the median of `reps` runs, and the parse rate of the files alone."""
+    import shutil
+    import tempfile
+    from pairwise_aligner_amd import fastx
+    d = tempfile.mkdtemp(prefix="pa_fastx_")
+    try:
+        paths, nbytes = [], 0
+        for side in ("1", "2"):
+            b, o = w["b" + side], w["o" + side]
+            raw = b.tobytes()
+            parts = []
+            for r in range(len(o) - 1):
+                seq = raw[o[r]:o[r + 1]]
+                parts.append(b">s%d\n" % r)
+                parts.extend(seq[k:k + 60] + b"\n" for k in range(0, len(seq), 60))
+            path = os.path.join(d, f"seq{side}.fa")
+            with open(path, "wb") as f:
+                f.write(b"".join(parts))
+            paths.append(path)
+            nbytes += os.path.getsize(path)
+        parse_s, total_s = [], []
+        for k in range(reps + 1):
+            t0 = time.perf_counter()
+            r1, r2 = fastx.read(paths[0]), fastx.read(paths[1])
+            t1 = time.perf_counter()
+            got, _ = ctx.score_packed(r1.residues, r1.offsets, r2.residues, r2.offsets, dc, w.get("pair_method"))
+            t2 = time.perf_counter()
+            if k == 0:
+                same_input = bool(np.array_equal(r1.residues, w["b1"]) and np.array_equal(r2.offsets, w["o2"]))
+                same_scores = bool(np.array_equal(got, want_scores))
+                continue
+            parse_s.append(t1 - t0)
+            total_s.append(t2 - t0)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    parse_med, total_med = float(np.median(parse_s)), float(np.median(total_s))
+    log(f"[bench] from FASTA files: {nbytes / 1e6:.0f} MB parsed in {parse_med * 1e3:.1f} ms, "
+        f"files to scores {total_med * 1e3:.1f} ms")
+    return dict(fasta_bytes=nbytes, parse_ms=round(parse_med * 1e3, 2),
+                parse_gb_per_s=round(nbytes / parse_med / 1e9, 2), files_to_scores_ms=round(total_med * 1e3, 2),
+                gcups=round(cells / total_med / 1e9, 2), reps=reps, input_identical=same_input,
+                scores_identical=same_scores)
 
 
 def run_mixed(count, steps, warmup, with_oracle=True):

@@ -25,7 +25,8 @@
 //   * score() is int32_t and exact for every score model, including the *_simd ones whose
 //     reference lanes wrap on overflow (simd_score_type.hpp:164-192); saturated int16 lanes
 //     are recomputed in int32;
-//   * dp_column()/dp_row() of a result are not produced (score-only device path).
+//   * dp_column()/dp_row() hold the scores of the final DP column and row only after the
+//     four-argument compute(seq1, seq2, first_dp_column, first_dp_row); they are empty otherwise.
 #pragma once
 
 #include <array>

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpa_device.so")
-SOURCES = ["pa_kernels.hip", "pa_encode.hip", "pa_learned.hip", "pa_capi.cpp", "pa_plan.cpp"]
+SOURCES = ["pa_kernels.hip", "pa_encode.hip", "pa_learned.hip", "pa_capi.cpp", "pa_plan.cpp", "pa_fastx.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
