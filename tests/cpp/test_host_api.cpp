@@ -1,9 +1,9 @@
 // C++ host API test: the reference's scalar known-answer tests
 // (test/unit/affine/*_scalar*_test.cpp, test/unit/configuration/*.cpp), written the way the
 // reference's tests configure and call the aligner, plus one-to-one bulk and one-to-many calls
-// checked against the single-pair call.
+// checked against the single-pair call, and FASTA / FASTQ records read for compute().
 //
-//   test_host_api --config   checks the configuration layer only (no GPU needed)
+//   test_host_api --config   checks the configuration layer and the record reader (no GPU needed)
 //   test_host_api            runs everything on device 0
 #include <cstdio>
 #include <cstring>
@@ -12,6 +12,7 @@
 #include <string_view>
 #include <vector>
 
+#include <pairwise_aligner/amd/fastx.hpp>
 #include <pairwise_aligner/configuration/configure_aligner.hpp>
 #include <pairwise_aligner/configuration/gap_model_affine.hpp>
 #include <pairwise_aligner/configuration/method_global.hpp>
@@ -186,13 +187,51 @@ void dp_vectors()
     EXPECT_EQ(e.dp_column() == (std::vector<int32_t>{-12}), true);
 }
 
+// The reference's FASTQ fixture (test/data/in.fastq), as text and as tests/golden/in.fastq.
+static constexpr std::string_view in_fastq =
+    "@seq1\nACGTTTGATTCGCG\n+\n!**55CCF>>>>>C\n@seq2\nTCGGGGGATTCGCG\n+\n!**65CCF>CF>>C\n";
+
+static void fastx_checks(std::string const &golden_dir)
+{
+    auto r = aligner::fastx_records::parse(in_fastq);
+    EXPECT_EQ(r.size(), 2u);
+    EXPECT_EQ(r.is_fastq(), true);
+    EXPECT_EQ(r[0] == "ACGTTTGATTCGCG"sv && r[1] == "TCGGGGGATTCGCG"sv, true);
+    EXPECT_EQ(r.name(0) == "seq1"sv && r.name(1) == "seq2"sv, true);
+    EXPECT_EQ(r.quality(1) == "!**65CCF>CF>>C"sv, true);
+    auto f = aligner::fastx_records::read(golden_dir + "in.fastq");
+    EXPECT_EQ(f.size() == 2 && f[0] == r[0] && f[1] == r[1], true);
+    auto fa = aligner::fastx_records::parse(">a desc\nacg\ntt\n\n>b\n", true);
+    EXPECT_EQ(fa.size() == 2 && fa[0] == "ACGTT"sv && fa[1].empty() && !fa.is_fastq(), true);
+    bool threw = false;
+    try {
+        aligner::fastx_records::parse("@r\nACGT\n+\nII\n");
+    } catch (std::runtime_error const &) {
+        threw = true;
+    }
+    EXPECT_EQ(threw, true);
+}
+
+// Records read with fastx_records go straight into compute().
+static void fastx_compute()
+{
+    auto r = aligner::fastx_records::parse(in_fastq);
+    auto a = aligner::cfg::configure_aligner(aligner::cfg::score_model_unitary(standard, 4, -5));
+    auto bulk = a.compute(std::vector<std::string_view>{r[0]}, std::vector<std::string_view>{r[1]});
+    EXPECT_EQ(bulk[0].score(), a.compute(std::string{r[0]}, std::string{r[1]}).score());
+}
+
 int main(int argc, char **argv)
 {
+    std::string golden_dir = argv[0];
+    golden_dir = golden_dir.substr(0, golden_dir.rfind('/') + 1) + "../golden/";
     config_checks();
+    fastx_checks(golden_dir);
     if (!(argc > 1 && std::strcmp(argv[1], "--config") == 0)) {
         known_answers();
         bulk_calls();
         dp_vectors();
+        fastx_compute();
     }
     std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "OK", failures);
     return failures ? 1 : 0;
