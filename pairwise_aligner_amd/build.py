@@ -59,7 +59,8 @@ def build_probe() -> str:
 
 
 def build_cpp_tests() -> str:
-    """Compiles tests/cpp/test_host_api (the header-only C++ host API against libpa_device.so)."""
+    """Compiles tests/cpp/test_host_api (the header-only C++ host API against libpa_device.so) and
+    tests/cpp/test_planner (the host planner on its own, no device code)."""
     src = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
     out = os.path.join(ROOT, "tests", "cpp", "test_host_api")
     cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), src,
@@ -69,6 +70,13 @@ def build_cpp_tests() -> str:
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("g++ failed building tests/cpp/test_host_api")
+    planner = os.path.join(ROOT, "tests", "cpp", "test_planner")
+    cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", "-pthread", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+           os.path.join(ROOT, "tests", "cpp", "test_planner.cpp"), os.path.join(CSRC, "pa_plan.cpp"), "-o", planner]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building tests/cpp/test_planner")
     return out
 
 

@@ -551,7 +551,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         }
         key.export_dp = export_dp ? 1 : 0;
         if (profiled) key.score = kScoreProfile;
-        else key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable[0].empty()) ? kScorePairs : plan.score;
+        else key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable[0].empty()) ? uint8_t(kScorePairs) : plan.score;
         key.local = local;
         key.track = (!local && ((width == 16 && !upper16) || track_all)) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;

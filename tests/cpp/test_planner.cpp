@@ -1,0 +1,239 @@
+// Host planner checks (no GPU): make_plan on seeded random batches, against what a plan must
+// satisfy whatever its packing choices:
+//   * every pair with two non-empty sequences sits in exactly one task half, empty pairs in none;
+//   * each half fits its task (m <= S*G*R, n <= N), int32 tasks have one half, keys agree with the
+//     pair's width and method, launches partition the tasks into runs of one key;
+//   * code-word, boundary and DP-export offsets are the running sums the kernels assume;
+//   * the encoder's residue sources point at the pair's sequences in the uploaded buffer;
+//   * cells and the int16/int32 counts add up; the residue codes equal a byte-by-byte scan
+//     (the planner's vectorised alphabet scan, bytes >= 0x80 and ragged block ends included);
+//   * planning twice gives the same plan.
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "pa_plan.hpp"
+
+using namespace pa;
+
+static int failures = 0;
+#define CHECK(cond, ...)                                          \
+    do {                                                          \
+        if (!(cond)) {                                            \
+            if (failures < 20) {                                  \
+                std::printf("FAIL %s:%d: %s: ", __FILE__, __LINE__, #cond); \
+                std::printf(__VA_ARGS__);                         \
+                std::printf("\n");                                \
+            }                                                     \
+            ++failures;                                           \
+        }                                                         \
+    } while (0)
+
+struct Batch {
+    std::vector<uint8_t> s1, s2, method;
+    std::vector<uint64_t> o1{0}, o2{0};
+    bool one_query = false;
+    BatchView view() const
+    {
+        return BatchView{s1.data(), o1.data(), s2.data(), o2.data(), o2.size() - 1, method.empty() ? nullptr : method.data(),
+                         one_query};
+    }
+};
+
+static Batch random_batch(std::mt19937_64 &rng, int n, const char *alphabet, int max_len, bool one_query, bool long_pairs,
+                          bool odd_bytes)
+{
+    Batch b;
+    b.one_query = one_query;
+    const size_t k = std::strlen(alphabet);
+    auto seq = [&](std::vector<uint8_t> &out, int len) {
+        for (int i = 0; i < len; ++i) out.push_back(uint8_t(alphabet[rng() % k]));
+        if (odd_bytes && len > 3 && rng() % 50 == 0) out[out.size() - 1 - rng() % 3] = uint8_t(0x80 + rng() % 8);
+    };
+    if (one_query) {
+        seq(b.s1, 1 + int(rng() % max_len));
+        b.o1.push_back(b.s1.size());
+    }
+    for (int p = 0; p < n; ++p) {
+        int l1 = int(rng() % (max_len + 1)), l2 = int(rng() % (max_len + 1));
+        if (p % 37 == 0) l1 = 0;
+        if (p % 41 == 0) l2 = 0;
+        if (long_pairs && p % 97 == 5) l1 = 3100 + int(rng() % 2000);
+        if (!one_query) {
+            seq(b.s1, l1);
+            b.o1.push_back(b.s1.size());
+        }
+        seq(b.s2, l2);
+        b.o2.push_back(b.s2.size());
+        b.method.push_back(uint8_t(rng() % 2));
+    }
+    if (b.s1.empty()) b.s1.push_back('A');
+    if (b.s2.empty()) b.s2.push_back('A');
+    return b;
+}
+
+static uint64_t fnv(const void *p, size_t n, uint64_t h)
+{
+    const uint8_t *c = static_cast<const uint8_t *>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    return h;
+}
+
+static uint64_t digest(const Plan &p)
+{
+    uint64_t h = 1469598103934665603ull;
+    h = fnv(p.pairs.data(), p.pairs.size() * sizeof(PairMeta), h);
+    h = fnv(p.tasks.data(), p.tasks.size() * sizeof(Task), h);
+    h = fnv(p.enc.data(), p.enc.size() * sizeof(EncodeTask), h);
+    h = fnv(p.dp_base.data(), p.dp_base.size() * 8, h);
+    for (const Launch &L : p.launches) h = fnv(&L, sizeof L, h);
+    h = fnv(p.launch_order.data(), p.launch_order.size() * 4, h);
+    return fnv(p.launch_stream.data(), p.launch_stream.size(), h);
+}
+
+static void check_plan(const char *label, const pa_config &cfg, const Batch &b, const uint8_t *forced = nullptr,
+                       const std::vector<uint32_t> *only = nullptr, bool export_dp = false)
+{
+    const BatchView v = b.view();
+    Plan p;
+    std::string err;
+    const pa_status st = make_plan(cfg, v, forced, only, p, err, false, export_dp);
+    if (st == PA_ERR_INVALID_SYMBOL) {   // a byte outside the matrix alphabet: expected for odd bytes
+        std::printf("  %s: rejected (%s)\n", label, err.c_str());
+        return;
+    }
+    CHECK(st == PA_OK, "%s: status %d %s", label, st, err.c_str());
+    if (st != PA_OK) return;
+    const uint64_t N = v.n_pairs;
+    std::vector<bool> selected(N, only == nullptr);
+    if (only)
+        for (uint32_t q : *only) selected[q] = true;
+
+    // residue codes against a byte-by-byte scan
+    bool seen[256] = {};
+    for (uint64_t i = v.off1[0]; i < v.end1(); ++i) seen[b.s1[i]] = true;
+    for (uint64_t i = 0; i < v.off2[N]; ++i) seen[b.s2[i]] = true;
+    if (cfg.score_kind == PA_SCORE_UNITARY) {
+        int k = 0;
+        for (int c = 0; c < 256; ++c) CHECK(p.code[c] == (seen[c] ? k++ : -1), "%s: code of byte %d", label, c);
+    }
+
+    // every non-empty selected pair in exactly one half
+    std::vector<int> count(N, 0);
+    uint64_t rowcodes = 0, colcodes = 0, bnd = 0, cells = 0, n16 = 0, n32 = 0;
+    for (const Launch &L : p.launches) {
+        const VariantKey &key = L.key;
+        const uint32_t rows = uint32_t(key.G) * key.R;
+        CHECK(L.task_begin < L.task_end && L.task_end <= p.tasks.size(), "%s: launch range", label);
+        for (uint32_t t = L.task_begin; t < L.task_end; ++t) {
+            const Task &tk = p.tasks[t];
+            CHECK(tk.row_off == uint32_t(rowcodes) && tk.col_off == uint32_t(colcodes), "%s: code offsets of task %u", label, t);
+            const uint32_t cw = col_words(tk.N, key.G);
+            if (tk.S > 1) {
+                CHECK(tk.bnd_off == uint32_t(bnd), "%s: boundary offset of task %u", label, t);
+                bnd += cw;
+            } else {
+                CHECK(tk.bnd_off == kNone, "%s: single-stripe task %u has a boundary", label, t);
+            }
+            if (key.score != kScoreProfile) rowcodes += uint64_t(tk.S) * rows / (key.score == kScorePairs ? 2 : 1);
+            colcodes += cw;
+            CHECK(key.width == 16 || tk.pair[1] == kNone, "%s: int32 task %u with two halves", label, t);
+            for (int q = 0; q < 2; ++q) {
+                if (tk.pair[q] == kNone) continue;
+                const uint32_t pr = tk.pair[q];
+                CHECK(pr < N, "%s: pair index", label);
+                if (pr >= N) continue;
+                ++count[pr];
+                const PairMeta &pm = p.pairs[pr];
+                CHECK(pm.width == key.width, "%s: width of pair %u", label, pr);
+                CHECK(pm.local == key.local, "%s: method of pair %u", label, pr);
+                CHECK(tk.m[q] == pm.m && tk.n[q] == pm.n && pm.m <= uint64_t(tk.S) * rows && pm.n <= tk.N,
+                      "%s: shape of pair %u", label, pr);
+                const uint64_t l1 = v.len1(pr), l2 = v.off2[pr + 1] - v.off2[pr];
+                CHECK((pm.swapped ? l2 : l1) == pm.m && (pm.swapped ? l1 : l2) == pm.n, "%s: orientation of pair %u", label, pr);
+                const uint64_t s1 = v.src1(pr), s2 = v.end1() + v.off2[pr];
+                const EncodeTask &e = p.enc[t];
+                CHECK(e.row_src[q] == (pm.swapped ? s2 : s1) && e.col_src[q] == (pm.swapped ? s1 : s2),
+                      "%s: residue sources of pair %u", label, pr);
+                if (export_dp && q == 0) {
+                    const uint64_t c1 = v.off1[pr] - v.off1[0] + pr;
+                    const uint64_t c2 = (v.off1[N] - v.off1[0]) + N + (v.off2[pr] - v.off2[0]) + pr;
+                    CHECK(p.dp_base[2 * t] == (pm.swapped ? c2 : c1) && p.dp_base[2 * t + 1] == (pm.swapped ? c1 : c2),
+                          "%s: export offsets of pair %u", label, pr);
+                }
+                cells += uint64_t(pm.m) * pm.n;
+                if (key.width == 16) ++n16; else ++n32;
+            }
+        }
+    }
+    CHECK(rowcodes == p.n_rowcodes && colcodes == p.n_colcodes && bnd == p.bnd_entries, "%s: code totals", label);
+    CHECK(cells == p.cells && n16 == p.n16 && n32 == p.n32, "%s: cell and width counts", label);
+    for (uint64_t q = 0; q < N; ++q) {
+        const bool nonempty = v.len1(q) > 0 && v.off2[q + 1] > v.off2[q];
+        CHECK(count[q] == ((selected[q] && nonempty) ? 1 : 0), "%s: pair %llu in %d halves", label,
+              (unsigned long long)q, count[q]);
+        if (forced && selected[q] && nonempty && forced[q]) CHECK(p.pairs[q].width == forced[q], "%s: forced width", label);
+    }
+    std::set<uint32_t> order(p.launch_order.begin(), p.launch_order.end());
+    CHECK(order.size() == p.launches.size() && p.launch_stream.size() == p.launches.size(), "%s: launch order", label);
+
+    Plan again;
+    make_plan(cfg, v, forced, only, again, err, false, export_dp);
+    CHECK(digest(again) == digest(p), "%s: planning is not deterministic", label);
+    std::printf("  %s: %llu pairs, %zu tasks, %zu launches\n", label, (unsigned long long)N, p.tasks.size(), p.launches.size());
+}
+
+int main()
+{
+    std::mt19937_64 rng(20261204);
+    pa_config dna{};
+    dna.gap_open = -10;
+    dna.gap_extension = -1;
+    dna.match = 4;
+    dna.mismatch = -5;
+    static const char *kAmino = "ARNDCQEGHILKMFPSTWYVX";
+    std::vector<int32_t> matrix(21 * 21);
+    for (int i = 0; i < 21; ++i)
+        for (int j = 0; j < 21; ++j) matrix[i * 21 + j] = i == j ? 5 : -1 - (i + j) % 3;
+    pa_config prot{};
+    prot.gap_open = -11;
+    prot.gap_extension = -1;
+    prot.score_kind = PA_SCORE_MATRIX;
+    prot.matrix_dim = 21;
+    prot.matrix = matrix.data();
+    prot.matrix_symbols = reinterpret_cast<const uint8_t *>(kAmino);
+
+    for (int rep = 0; rep < 2; ++rep) {
+        // large enough for several blocks of pairs and several sorted runs per group
+        Batch big = random_batch(rng, 150000, "ACGT", 400, false, rep == 1, false);
+        check_plan("dna global+local", dna, big);
+        pa_config lr = dna;
+        lr.last_row_free = 1;
+        check_plan("dna free last row", lr, big);
+        Batch odd = random_batch(rng, 20000, "ACGTN", 300, false, false, true);
+        check_plan("dna with odd bytes", dna, odd);
+        Batch pb = random_batch(rng, 40000, kAmino, 500, false, rep == 1, false);
+        prot.method = PA_METHOD_LOCAL;
+        check_plan("protein local", prot, pb);
+        prot.method = PA_METHOD_GLOBAL;
+        check_plan("protein per-pair methods", prot, pb);
+        check_plan("protein export", prot, pb, nullptr, nullptr, true);
+        std::vector<uint8_t> forced(pb.o2.size() - 1, 0);
+        std::vector<uint32_t> only;
+        for (size_t q = 0; q < forced.size(); q += 3) {
+            forced[q] = 32;
+            only.push_back(uint32_t(q));
+        }
+        check_plan("protein int32 subset", prot, pb, forced.data(), &only);
+        Batch oq = random_batch(rng, 30000, kAmino, 500, true, false, false);
+        prot.method = PA_METHOD_LOCAL;
+        check_plan("protein one query", prot, oq);
+        Batch odd_p = random_batch(rng, 2000, kAmino, 200, false, false, true);
+        check_plan("protein with odd bytes", prot, odd_p);
+    }
+    std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "OK", failures);
+    return failures ? 1 : 0;
+}
