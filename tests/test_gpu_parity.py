@@ -273,3 +273,34 @@ def test_long_protein_local(ctx):
                              symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
                              gap_open=-11, gap_extension=-1)
     check(ctx, s1, s2, dc, label="long protein local")
+
+
+def test_large_one_shot_batch(ctx):
+    """A one-shot call on 528,609 pairs: scores equal the oracle's and flags equal those of the
+    same pairs run as a device-resident batch, with per-pair methods, empty sequences and int16
+    pairs that saturate and are recomputed, spread over the batch."""
+    rng = np.random.default_rng(41)
+    n = 2 * 262144 + 4321
+    lens1 = rng.integers(0, 48, size=n)
+    lens2 = rng.integers(0, 48, size=n)
+    sat = [5, 300000, n - 7]            # identical 1,200-nt pairs: 36,000 at match +30
+    for p in sat:
+        lens1[p] = lens2[p] = 1200
+    b1, o1 = workloads.random_packed(rng, lens1, workloads.DNA)
+    b2, o2 = workloads.random_packed(rng, lens2, workloads.DNA)
+    b2 = b2.copy()
+    for p in sat:
+        b2[o2[p]:o2[p + 1]] = b1[o1[p]:o1[p + 1]]
+    pm = (rng.random(n) < 0.5).astype(np.uint8)
+    dc = device.DeviceConfig(match=30, mismatch=-5, gap_open=-10, gap_extension=-1, lane_policy=device.LANES_INT16)
+    log(f"case large one-shot batch: {n} pairs")
+    got, flags = ctx.score_packed(b1, o1, b2, o2, dc, pm)
+    want = O.score_packed(b1, o1, b2, o2, oracle_cfg(dc), pair_method=pm, threads=8)
+    bad = np.nonzero(got.astype(np.int64) != want)[0]
+    assert len(bad) == 0, f"{len(bad)} mismatches, first {bad[:5].tolist()}"
+    assert np.all(flags[sat] & device.PAIR_RECOMPUTED) and np.all(got[sat] == 36000)
+    one = device.Batch(ctx, b1, o1, b2, o2, dc, pm)
+    one.run()
+    same, same_flags = one.results()
+    one.close()
+    assert np.array_equal(same, got) and np.array_equal(same_flags, flags)
