@@ -269,8 +269,11 @@ pa::InitArgs init_args(pa_batch *b, const pa::Plan &p)
 }
 
 // The main pass: initial scores, then every launch of the main plan.  Unless the tuning asks
-// for eager launches it is captured once into a hipGraph and replayed (one launch call per pass,
-// no per-kernel launch gaps).  The graph holds kernels and event dependencies only.
+// for eager launches, a pass with two or more striped launches is captured once into a hipGraph
+// and replayed (one launch call per pass, no per-kernel launch gaps, the launches' streams
+// overlapping).  A pass with one striped launch runs eagerly: its two kernels run back to back
+// either way, and launching them costs less than launching a graph (10,000 DNA pairs of 150 nt:
+// 0.089 against 0.096 ms per pass).  The graph holds kernels and event dependencies only.
 pa_status enqueue_main(pa_ctx *ctx, pa_batch *b)
 {
     hipError_t e = pa::launch_init(init_args(b, b->main.plan), ctx->stream);
@@ -280,7 +283,7 @@ pa_status enqueue_main(pa_ctx *ctx, pa_batch *b)
 
 pa_status launch_main(pa_ctx *ctx, pa_batch *b)
 {
-    if (b->main.plan.tuning.eager) return enqueue_main(ctx, b);
+    if (b->main.plan.tuning.eager || b->main.plan.launches.size() <= 1) return enqueue_main(ctx, b);
     if (!b->graph_exec) {
         PA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         pa_status st = enqueue_main(ctx, b);
