@@ -304,3 +304,38 @@ def test_large_one_shot_batch(ctx):
     same, same_flags = one.results()
     one.close()
     assert np.array_equal(same, got) and np.array_equal(same_flags, flags)
+
+
+@pytest.mark.parametrize("one_query", [False, True])
+def test_batch_does_not_read_host_inputs_after_creation(ctx, one_query):
+    """A device-resident batch keeps no host copy of the residues: after creation the caller's
+    buffers are overwritten, and repeated runs -- including the int32 re-plan and recompute of
+    saturated int16 pairs, which takes the alphabet from the main plan -- still give the oracle's
+    scores."""
+    rng = np.random.default_rng(51)
+    s1, s2 = workloads.dna_pairs(rng, 600, 50, 400, divergence=0.2, indel_fraction=0.3)
+    if one_query:
+        s1 = [b"ACGT" * 400] * len(s2)          # one 1,600-nt query
+        s2[:4] = [b"ACGT" * 400] * 4               # identical targets saturate int16 at match +30
+    else:
+        s1[:4] = [b"ACGT" * 400] * 4
+        s2[:4] = list(s1[:4])
+    dc = device.DeviceConfig(match=30, mismatch=-5, gap_open=-10, gap_extension=-1, lane_policy=device.LANES_INT16)
+    want = O.score_pairs(s1, s2, oracle_cfg(dc), threads=8)
+    b2, o2 = workloads.pack(s2)
+    if one_query:
+        q = np.frombuffer(s1[0], dtype=np.uint8).copy()
+        batch = device.Batch.one_to_many(ctx, q.tobytes(), b2, o2, dc)
+        batch._keep[0][:] = ord("T")   # the query buffer the library was given
+    else:
+        b1, o1 = workloads.pack(s1)
+        batch = device.Batch(ctx, b1, o1, b2, o2, dc)
+        b1[:] = ord("T")
+    b2[:] = ord("G")
+    for rep in range(3):
+        log(f"host-input independence, one_query={one_query}, run {rep}")
+        batch.run()
+        got, flags = batch.results()
+        assert np.array_equal(got.astype(np.int64), want), f"run {rep}"
+        assert np.all(flags[:4] & device.PAIR_RECOMPUTED)
+    batch.close()
