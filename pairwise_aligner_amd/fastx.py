@@ -89,7 +89,10 @@ class _Handle:
 
     def __del__(self):
         if self.handle:
-            _lib().pa_fastx_destroy(self.handle)
+            try:
+                _lib().pa_fastx_destroy(self.handle)
+            except Exception:   # interpreter shutdown: the library may already be unloaded
+                pass
             self.handle = None
 
 
