@@ -241,7 +241,8 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
                                              typename Lane<W>::V diag, typename Lane<W>::V &runmax,
                                              const typename Lane<W>::V (&sc)[R], uint32_t Bin, uint32_t tbl_lo,
                                              uint32_t tbl_hi, typename Lane<W>::V vgo,
-                                             typename Lane<W>::V vgogo, typename Lane<W>::V vge)
+                                             typename Lane<W>::V vgogo, typename Lane<W>::V vge,
+                                             typename Lane<W>::V vfloor = typename Lane<W>::V{})
 {
     using L = Lane<W>;
     using V = typename L::V;
@@ -254,12 +255,27 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
         if constexpr (SCORE == kScoreDna) s = L::from(__builtin_amdgcn_perm(tbl_hi, tbl_lo, st.A[r] ^ Bin));
         else s = sc[r];
         D[r] = L::add(r == 0 ? diag : st.H[r - 1], s);
-        if constexpr (LOCAL) D[r] = L::vmax(D[r], L::splat(0));
+        // the local clamp at 0; with the column drift, 0 of this step's frame (vfloor)
+        if constexpr (LOCAL) D[r] = L::vmax(D[r], CDRIFT ? vfloor : L::splat(0));
     }
     V DF[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) DF[r] = L::vmax(D[r], st.F[r]);
-    if constexpr (CDRIFT) {
+    if constexpr (CDRIFT && !DRIFT) {
+        // Column drift only (local method): every value of a step carries the step's drift, so
+        // the horizontal gap state loses its extension term, W'(j+1) = max(W'(j), H'(j) + go),
+        // and the vertical one keeps it: V(r+1) = max(V(r) + ge, H(r) + go + ge)
+        // = max(V(r), H(r) + go) + ge, sharing the opening term H + go with W.
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const V best = L::vmax(DF[r], Eup);
+            const V open = L::add(best, vgo);
+            Eup = L::add(L::vmax(Eup, open), vge);
+            st.H[r] = best;
+            if constexpr (LOCAL) runmax = L::vmax(runmax, best);
+            st.F[r] = L::vmax(st.F[r], open);
+        }
+    } else if constexpr (CDRIFT) {
         // Row and column drift: both gap states lose their extension term,
         //   V'(r+1) = max(V'(r), H'(r) + go),   W'(j+1) = max(W'(j), H'(j) + go),
         // and share the opening term.

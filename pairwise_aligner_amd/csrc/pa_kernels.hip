@@ -87,13 +87,15 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
     constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
     constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
-    constexpr bool CDRIFT = DRIFT && !ACC && !EXPORT;   // column drift on top
+    // column drift: on top of the row drift for the global method, alone for the local method
+    // (whose zero clamp and running maximum then work in the step's frame)
+    constexpr bool CDRIFT = !ACC && !EXPORT;
     const int dge = DRIFT ? ge : 0;
     const int cg = CDRIFT ? -ge : 0;         // column drift per step
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
     // shifts the last row's drift back when H, V leave the lane (and, with the column drift,
     // forward by one step: the next lane uses them one step later)
-    const V vout = L::splat(CDRIFT ? ge * (R - 1) : ge * R);
+    const V vout = L::splat((DRIFT ? ge * R : 0) + (CDRIFT ? -ge : 0));
     const V vrebase = L::splat(ge * G);      // end of a chunk: the column drift drops by G|ge|
     const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
     const int N = int(tk.N);
@@ -109,8 +111,11 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         }
     }
 
-    V runmax = LOCAL ? L::splat(0) : L::lowest();   // exact
+    // exact; for the local method with the column drift, 0 in the frame of step -1 (it moves to
+    // the frame of each step as the step begins and back by G steps at each chunk end)
+    V runmax = LOCAL ? L::splat(CDRIFT ? -cg : 0) : L::lowest();
     V satmax = L::lowest();                         // drifted (int16 saturation test)
+    const V vcg = L::splat(cg);
 
     for (uint32_t s = 0; s < tk.S; ++s) {
         const int row0 = int(s) * ROWS + t * R;   // DP row above this lane's block
@@ -185,13 +190,14 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             const uint32_t Ein = dpp<kRowShr1>(convE, Eout);
             convH = dpp<kRowShl1>(convH, convH);
             convE = dpp<kRowShl1>(convE, convE);
+            if constexpr (LOCAL && CDRIFT) runmax = L::add(runmax, vcg);   // into this step's frame
             if (!MASKED || uint32_t(c) < uint32_t(N)) {
                 V Eup = L::from(Ein);
                 update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo, tbl_hi,
-                                                                     vgo, vgogo, vge);
+                                                                     vgo, vgogo, vge, L::splat(cg * (k % G)));
                 Hdiag = L::from(Hin);
-                Hout = L::bits(DRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
-                Eout = L::bits(DRIFT ? L::add(Eup, vout) : Eup);
+                Hout = L::bits(DRIFT || CDRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
+                Eout = L::bits(DRIFT || CDRIFT ? L::add(Eup, vout) : Eup);
                 if constexpr (EXPORT) {
                     // the last row: H(m, c+1) from the lane that holds row m; the last column:
                     // H(i, n) of every row of this lane at column n
@@ -249,6 +255,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 Hdiag = L::add(Hdiag, vrebase);
                 Hout = L::bits(L::add(L::from(Hout), vrebase));
                 Eout = L::bits(L::add(L::from(Eout), vrebase));
+                if constexpr (LOCAL) runmax = L::add(runmax, vrebase);
             }
         };
 
@@ -283,7 +290,8 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 #pragma unroll
             for (int q = 0; q < L::NH; ++q) {
                 if (tk.pair[q] == kNone) continue;
-                const int v = L::half(mv, q);
+                // the local maximum is in the frame of step -1 with the column drift
+                const int v = L::half(mv, q) + (LOCAL && CDRIFT ? cg : 0);
                 if constexpr (LOCAL) a.scores[tk.pair[q]] = v;
                 if (W == 16 && L::half(sv, q) >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
                 if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
@@ -362,7 +370,7 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
     // shifts the last row's drift back when H, V leave the lane (and, with the column drift,
     // forward by one step: the next lane uses them one step later)
-    const V vout = L::splat(CDRIFT ? ge * (R - 1) : ge * R);
+    const V vout = L::splat((DRIFT ? ge * R : 0) + (CDRIFT ? -ge : 0));
     const V vrebase = L::splat(ge * G);      // end of a chunk: the column drift drops by G|ge|
     const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
     const int N = int(tk.N);
@@ -370,8 +378,11 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     const uint32_t *colp = a.colcodes + tk.col_off;
     uint32_t *gbnd = a.bnd + 2ull * tk.bnd_off;   // last wave -> first wave, one full row
 
-    V runmax = LOCAL ? L::splat(0) : L::lowest();   // exact
+    // exact; for the local method with the column drift, 0 in the frame of step -1 (it moves to
+    // the frame of each step as the step begins and back by G steps at each chunk end)
+    V runmax = LOCAL ? L::splat(CDRIFT ? -cg : 0) : L::lowest();
     V satmax = L::lowest();                         // drifted (int16 saturation test)
+    const V vcg = L::splat(cg);
 
     for (int s = wave; s < S; s += kWfWaves) {
         const int round = s / kWfWaves;

@@ -131,13 +131,15 @@ struct VariantKey {
 };
 static_assert(sizeof(VariantKey) == 8, "VariantKey is compared as one 64-bit word by the planner");
 
-// Substitution-score shift of a kernel variant, in units of -ge (pa_cells.hpp): the local
-// method keeps exact values (0); the global method drifts along the rows (1) and, unless it
-// keeps per-row maxima, along the columns as well (2).
+// Substitution-score shift of a kernel variant, in units of -ge (pa_cells.hpp): the global
+// method drifts along the rows (1) and, unless it keeps per-row maxima, along the columns as
+// well (2); the local method drifts along the columns only in the striped kernel (1) and keeps
+// exact values in the wavefront kernel and the DP export (0).
 inline constexpr int kScoreShifts = 3;
 inline int score_shift(const VariantKey &key)
 {
-    return key.local ? 0 : ((key.track || key.rowmax || key.export_dp) ? 1 : 2);
+    if (key.local) return (key.G == 16 && !key.export_dp) ? 1 : 0;   // striped: column drift
+    return (key.track || key.rowmax || key.export_dp) ? 1 : 2;
 }
 
 // Rows per lane of the DP-export variants (int32 lanes, striped kernel only).
