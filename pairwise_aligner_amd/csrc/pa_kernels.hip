@@ -378,11 +378,8 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     const uint32_t *colp = a.colcodes + tk.col_off;
     uint32_t *gbnd = a.bnd + 2ull * tk.bnd_off;   // last wave -> first wave, one full row
 
-    // exact; for the local method with the column drift, 0 in the frame of step -1 (it moves to
-    // the frame of each step as the step begins and back by G steps at each chunk end)
-    V runmax = LOCAL ? L::splat(CDRIFT ? -cg : 0) : L::lowest();
+    V runmax = LOCAL ? L::splat(0) : L::lowest();   // exact (no column drift for the local method here)
     V satmax = L::lowest();                         // drifted (int16 saturation test)
-    const V vcg = L::splat(cg);
 
     for (int s = wave; s < S; s += kWfWaves) {
         const int round = s / kWfWaves;
