@@ -148,6 +148,10 @@ def test_empty_and_blank_inputs():
     assert recs.sequences() == [b"", b"AC", b""]
     recs = fastx.parse(b"@a\n\n+\n\n@b\nA\n+\nI")   # empty record, no final newline
     assert recs.sequences() == [b"", b"A"] and recs.qualities.tobytes() == b"I"
+    recs = fastx.parse(b">x\r\nAC GT\r\nTT\r")   # CRLF, a blank inside a line, no final newline
+    assert recs.names == ["x"] and recs.sequences() == [b"ACGTTT"]
+    recs = fastx.parse(b">only header")
+    assert recs.names == ["only header"] and recs.sequences() == [b""]
 
 
 @pytest.mark.parametrize("text, what", [
