@@ -79,10 +79,12 @@ def make_workload(name, pairs=None):
         w = workloads.config_uneven()
         dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
         return w, dc, "mixed"
-    if name == "long":
+    if name in ("long", "long_int16"):
+        # the configuration asks for int32 scores; long_int16 runs the same pairs in re-centred
+        # int16 lanes (the planner's default for long global pairs within the spread bound)
         w = workloads.config_long()
         dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=1, mismatch=-4, gap_open=-6, gap_extension=-1,
-                                 lane_policy=device.LANES_INT32)
+                                 lane_policy=device.LANES_INT32 if name == "long" else device.LANES_AUTO)
         return w, dc, "global"
     raise SystemExit(f"unknown workload {name}")
 
@@ -428,6 +430,7 @@ def main():
                                 "dna150": "DNA global, 10,000 pairs x 150 nt, +4/-5, gap (-10,-1)",
                                 "uneven": "1,000,000 DNA pairs lognormal lengths, half global half local",
                                 "long": "2,000 DNA pairs 10-100 kbp, int32",
+                                "long_int16": "2,000 DNA pairs 10-100 kbp, re-centred int16 lanes",
                                 "protein_1xn": "one protein query (300 aa) against 200,000 targets 50-500, "
                                                "local BLOSUM62, gap (-11,-1), query profile"}.get(args.workload,
                                                                                                    args.workload)

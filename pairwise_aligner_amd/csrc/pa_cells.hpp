@@ -33,6 +33,7 @@ template <> struct Lane<16> {
     __device__ static V from(uint32_t u) { return __builtin_bit_cast(V, u); }
     __device__ static int half(V v, int q) { return q == 0 ? int(v.x) : int(v.y); }
     __device__ static V lowest() { return V{short(-32768), short(-32768)}; }
+    __device__ static V pair(int a, int b) { return V{short(clamp(a)), short(clamp(b))}; }   // halves a, b
     // packed top-boundary word of an int value (<= 0): both halves, clamped at -32768
     __device__ static uint32_t top_word(int x) { return (uint32_t(x < -32768 ? -32768 : x) & 0xFFFFu) * 0x10001u; }
 };
@@ -48,6 +49,7 @@ template <> struct Lane<32> {
     __device__ static V from(uint32_t u) { return int(u); }
     __device__ static int half(V v, int) { return v; }
     __device__ static V lowest() { return INT32_MIN; }
+    __device__ static V pair(int a, int) { return a; }
     __device__ static uint32_t top_word(int x) { return uint32_t(x); }
 };
 
@@ -371,25 +373,27 @@ struct Capture {
         ev = next(-1);
     }
 
+    // off0, off1: offsets of the halves' values (int16 re-centring in the wavefront kernel)
     template <bool ACC>
     __device__ __forceinline__ void at_column(int c, const Rows<W, R, ACC> &st, const Task &tk, int32_t *scores,
-                                              int dge, int dcol = 0)
+                                              int dge, int dcol = 0, int off0 = 0, int off1 = 0)
     {
         if (c != ev) return;
 #pragma unroll
         for (int q = 0; q < Lane<W>::NH; ++q) {
+            const int off = q == 0 ? off0 : off1;
             if (c == cap[q]) {
                 int v = 0;
 #pragma unroll
                 for (int r = 0; r < R; ++r) if (r == capr[q]) v = Lane<W>::half(st.H[r], q);
-                atomicMax(scores + tk.pair[q], v + dge * (capr[q] + 1) + dcol);
+                atomicMax(scores + tk.pair[q], v + dge * (capr[q] + 1) + dcol + off);
             }
             if (c == lcap[q]) {
                 int v = INT32_MIN;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
                     if (r < lvr[q]) { const int h = Lane<W>::half(st.H[r], q) + dge * (r + 1) + dcol; v = h > v ? h : v; }
-                atomicMax(scores + tk.pair[q], v);
+                atomicMax(scores + tk.pair[q], v + off);
             }
         }
         ev = next(c);

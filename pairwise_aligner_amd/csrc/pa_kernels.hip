@@ -306,6 +306,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 
 constexpr int kWfWaves = 4;     // waves per workgroup
 constexpr int kWfRing = 256;    // columns per LDS ring between consecutive waves
+constexpr int kWfOffSlots = 8;  // chunks of ring offsets kept per wave (the ring spans 4 chunks)
 // Bounded waits: after this many polls a wait gives up and flags the batch, so a protocol
 // error ends as a reported failure instead of a hang.
 constexpr uint32_t kWfSpinLimit = 1u << 24;
@@ -315,6 +316,9 @@ struct WfShared {
     uint32_t cons[kWfWaves];   // positions consumed by wave w from its predecessor's ring
     uint32_t failed;           // a wait gave up: later waits return at once
     uint32_t ring[kWfWaves - 1][kWfRing][2];   // (H, V) from wave w to wave w+1
+    // int16 re-centring: the offsets of each half of wave w's ring values, per chunk of G
+    // columns (chunk c = columns [cG - G + 1, cG], slot c % kWfOffSlots)
+    int ring_off[kWfWaves - 1][kWfOffSlots][2];
 };
 
 __device__ __forceinline__ uint32_t lds_load_acquire(uint32_t *p)
@@ -365,6 +369,13 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
     constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
     constexpr bool CDRIFT = DRIFT && !ACC;   // column drift on top
+    // int16 global pairs are re-centred: each wave keeps its values relative to a per-half
+    // offset (int32, wave-uniform), moved at every chunk end to the value of lane 0's first row,
+    // so that the band of cells in flight (within about 1,200 steps of each other) stays far
+    // inside the int16 range whatever the absolute scores; the planner uses int16 here only
+    // when that spread bound holds.  Values crossing to another wave carry their writer's
+    // offsets (the LDS ring per chunk, the HBM row as exact int32).
+    constexpr bool RC = W == 16 && CDRIFT;
     const int dge = DRIFT ? ge : 0;
     const int cg = CDRIFT ? -ge : 0;         // column drift per step
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
@@ -377,6 +388,7 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     const int S = int(tk.S);
     const uint32_t *colp = a.colcodes + tk.col_off;
     uint32_t *gbnd = a.bnd + 2ull * tk.bnd_off;   // last wave -> first wave, one full row
+    // (re-centred: four words per column, the exact H and V of each half)
 
     V runmax = LOCAL ? L::splat(0) : L::lowest();   // exact (no column drift for the local method here)
     V satmax = L::lowest();                         // drifted (int16 saturation test)
@@ -386,10 +398,13 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
         const uint32_t pos0 = uint32_t(round) * uint32_t(N);
         const int row0 = s * ROWS + t * R;
         Rows<W, R, ACC> st;
+        // re-centring offsets of the two halves: the exact H(first row of the stripe, 0) to start
+        int off0 = 0, off1 = 0;
+        if constexpr (RC) off0 = off1 = h0_value(s * ROWS, fc, go, ge);
         // lane t starts at step t: its initial column (j = 0) counts as computed at step t-1
         init_rows<W, R, SCORE, ACC, DRIFT, CDRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank,
-                                                   cg * (t - 1));
-        V Hdiag = L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1));
+                                                   cg * (t - 1) - off0);
+        V Hdiag = L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1) - off0);
         Capture<W, R> cap;
         if constexpr (!LOCAL) cap.init(tk, row0, lc);
 
@@ -411,6 +426,17 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                 const uint32_t slot = (pos0 + uint32_t(col)) % kWfRing;
                 h = sh.ring[wave - 1][slot][0];
                 e = sh.ring[wave - 1][slot][1];
+                if constexpr (RC) {   // from the writer's offsets of that chunk to ours
+                    const int c = (col + G - 1) / G;
+                    const int *wo = sh.ring_off[wave - 1][c % kWfOffSlots];
+                    const V d = L::pair(wo[0] - off0, wo[1] - off1);
+                    h = L::bits(L::add(L::from(h), d));
+                    e = L::bits(L::add(L::from(e), d));
+                }
+            } else if constexpr (RC) {   // exact values of each half
+                const int *x = reinterpret_cast<const int *>(gbnd) + 4 * col;
+                h = L::bits(L::pair(x[0] - off0, x[1] - off1));
+                e = L::bits(L::pair(x[2] - off0, x[3] - off1));
             } else {
                 h = gbnd[2 * col];
                 e = gbnd[2 * col + 1];
@@ -459,6 +485,11 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                 if (src != 0) {
                     topH = convH;
                     topE = convE;
+                } else if constexpr (RC) {
+                    const int h = hb_top + cg * u, e = hb_top + go + ge - dge + cg * u;
+                    topH = L::bits(L::pair(h - off0, h - off1));
+                    topE = L::bits(L::pair(e - off0, e - off1));
+                    hb_top = fr ? 0 : hb_top + ge;
                 } else {
                     topH = L::top_word(hb_top + cg * u);
                     topE = L::top_word(hb_top + go + ge - dge + cg * u);   // drifted by |ge| for row 0
@@ -477,7 +508,7 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                     Hdiag = L::from(Hin);
                     Hout = L::bits(DRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
                     Eout = L::bits(DRIFT ? L::add(Eup, vout) : Eup);
-                    if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores, dge, -cg * u);
+                    if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores, dge, -cg * u, off0, off1);
                 }
                 if (dst != 0) {
                     qH = dpp<kWaveShl1>(Hout, qH);
@@ -515,9 +546,23 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                         sh.ring[wave][slot][0] = qH;
                         sh.ring[wave][slot][1] = qE;
                     }
+                    if constexpr (RC) {   // this chunk's offsets, published with its columns
+                        if (t == 0) {
+                            sh.ring_off[wave][(k0 / G) % kWfOffSlots][0] = off0;
+                            sh.ring_off[wave][(k0 / G) % kWfOffSlots][1] = off1;
+                        }
+                    }
                 } else if (col >= 0 && col < N) {
-                    gbnd[2 * col] = qH;
-                    gbnd[2 * col + 1] = qE;
+                    if constexpr (RC) {
+                        int *x = reinterpret_cast<int *>(gbnd) + 4 * col;
+                        x[0] = L::half(L::from(qH), 0) + off0;
+                        x[1] = L::half(L::from(qH), 1) + off1;
+                        x[2] = L::half(L::from(qE), 0) + off0;
+                        x[3] = L::half(L::from(qE), 1) + off1;
+                    } else {
+                        gbnd[2 * col] = qH;
+                        gbnd[2 * col + 1] = qE;
+                    }
                 }
                 if (hi > k0 + 1 - G) {   // this chunk produced columns: publish them
                     if (dst == 2) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -526,7 +571,28 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
-            if constexpr (CDRIFT) {
+            if constexpr (RC) {
+                // re-centre on lane 0's first row, then drop the column drift as below
+                const uint32_t ref = __builtin_amdgcn_readlane(L::bits(st.H[0]), 0);
+                const int d0 = int(int16_t(ref & 0xFFFFu)), d1 = int(int16_t(ref >> 16));
+                off0 += d0;
+                off1 += d1;
+                const V vd = L::pair(-d0, -d1);
+                const V vt = L::pair(ge * G - d0, ge * G - d1);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    st.H[r] = L::add(st.H[r], vt);
+                    st.F[r] = L::add(st.F[r], vt);
+                }
+                Hdiag = L::add(Hdiag, vt);
+                Hout = L::bits(L::add(L::from(Hout), vt));
+                Eout = L::bits(L::add(L::from(Eout), vt));
+                // top-boundary values already converted to this wave's offsets
+                convH = L::bits(L::add(L::from(convH), vd));
+                convE = L::bits(L::add(L::from(convE), vd));
+                nextH = L::bits(L::add(L::from(nextH), vd));
+                nextE = L::bits(L::add(L::from(nextE), vd));
+            } else if constexpr (CDRIFT) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     st.H[r] = L::add(st.H[r], vrebase);

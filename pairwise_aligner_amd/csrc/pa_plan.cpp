@@ -5,6 +5,7 @@
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <immintrin.h>
 #include <numeric>
@@ -504,6 +505,11 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     };
     const std::vector<uint8_t> rows_for_m = rows_table(plan.tuning);
     const std::vector<uint8_t> profile_rows_for = use_profile ? rows_table(profile_tuning) : std::vector<uint8_t>{};
+    // Spread bound of the re-centred int16 wavefront tasks: one step (row or column) changes H
+    // by at most smax + |go| + |ge| (+ the pad's 1 and |smin| for the gap and pad states).
+    const int64_t recentre_step = std::max<int64_t>(plan.smax, 0) + std::max<int64_t>(-int64_t(plan.smin), 1) +
+                                  std::abs(int64_t(plan.go)) + std::abs(int64_t(plan.ge));
+    const bool recentre_ok = recentre_step * int64_t(kRecentreReach) < 16000;
     // Per pair, in parallel: orientation, lane width and kernel variant.
     std::vector<VariantKey> pair_key(N);
     std::vector<uint8_t> speculative(N, 0);   // int16 without a proof against saturation
@@ -523,8 +529,14 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         if (!selected[p] || pm.m == 0 || pm.n == 0) return;
         const RangeBound rb = range_bound(plan, local, pm.m, pm.n);
         if (rb.lower < INT32_MIN / 2 || rb.upper > INT32_MAX / 2) { out_of_range = true; return; }
-        const bool lower16 = rb.lower > -32700;
-        const bool upper16 = rb.upper < 32767;
+        bool lower16 = rb.lower > -32700;
+        bool upper16 = rb.upper < 32767;
+        // long global pairs without per-row maxima run re-centred in the wavefront kernel: int16
+        // whatever the absolute scores when the spread bound holds; otherwise int16 only with
+        // the absolute bounds and the saturation tracking (which keeps absolute values)
+        const bool wavefront = !export_dp && pm.m > plan.tuning.wavefront_min_rows;
+        const bool rc_candidate = wavefront && !local && !(lc && lr);
+        if (rc_candidate && recentre_ok) lower16 = upper16 = true;
         uint8_t width;
         if (export_dp) width = 32;   // exported rows and columns are exact int32
         else if (forced_width && forced_width[p]) width = forced_width[p];
@@ -553,7 +565,8 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         if (profiled) key.score = kScoreProfile;
         else key.score = (plan.score == kScoreMatrix && width == 16 && !plan.ptable[0].empty()) ? uint8_t(kScorePairs) : plan.score;
         key.local = local;
-        key.track = (!local && ((width == 16 && !upper16) || track_all)) ? 1 : 0;
+        key.track = (!local && ((width == 16 && !upper16) || track_all ||
+                                (width == 16 && rc_candidate && !recentre_ok))) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;
     });
     if (out_of_range) { err = "scores may leave the int32 range"; return PA_ERR_RANGE; }
@@ -708,7 +721,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
             uint32_t S, Nc;
             task_shape(ch.g, group_packs[ch.g][i], S, Nc);
             const uint32_t cw = col_words(Nc, G);
-            if (S > 1) ch.bnd += cw;
+            if (S > 1) ch.bnd += recentred(key) ? 2 * uint64_t(cw) : cw;
             if (key.score != kScoreProfile) ch.rowcodes += uint64_t(S) * rows / (key.score == kScorePairs ? 2 : 1);
             ch.colcodes += cw;
             ch.padded += uint64_t(S) * rows * Nc * (key.width == 16 ? 2 : 1);
@@ -752,7 +765,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
             const uint32_t cw = col_words(t.N, G);
             if (t.S > 1) {
                 t.bnd_off = uint32_t(std::min<uint64_t>(bn, 0xFFFFFFFFull));
-                bn += cw;
+                bn += recentred(key) ? 2 * uint64_t(cw) : cw;
             }
             if (key.score != kScoreProfile) rc += uint64_t(t.S) * rows / (key.score == kScorePairs ? 2 : 1);
             cc += cw;

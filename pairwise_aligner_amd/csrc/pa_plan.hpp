@@ -249,6 +249,17 @@ int32_t max_substitution(const pa_config &cfg);
 // conveyor chunks of G words past the last column.
 inline uint32_t col_words(uint32_t N, uint32_t G) { return ((N + G - 1) / G) * G + 4 * G; }
 
+// Re-centred int16 wavefront tasks (global, no per-row maxima): the kernel keeps each wave's
+// values relative to an offset moved every chunk, so int16 needs only the spread of the cells in
+// flight to fit: adjacent cells differ by at most kRecentreStep(...) and the cells of a wave lie
+// within kRecentreReach steps of each other.  Their HBM boundary rows hold exact int32 values
+// (two entries per column).
+inline constexpr uint32_t kRecentreReach = 64 * 16 + 3 * 64;
+inline bool recentred(const VariantKey &key)
+{
+    return key.G == 64 && key.width == 16 && !key.local && !key.track && !key.rowmax;
+}
+
 // Conservative int16 range analysis of one pair (rows m, columns n).
 struct RangeBound {
     int64_t lower;   // every real H and gap state is >= lower

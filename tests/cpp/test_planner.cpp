@@ -134,7 +134,7 @@ static void check_plan(const char *label, const pa_config &cfg, const Batch &b, 
             const uint32_t cw = col_words(tk.N, key.G);
             if (tk.S > 1) {
                 CHECK(tk.bnd_off == uint32_t(bnd), "%s: boundary offset of task %u", label, t);
-                bnd += cw;
+                bnd += recentred(key) ? 2 * uint64_t(cw) : cw;   // exact int32 rows when re-centred
             } else {
                 CHECK(tk.bnd_off == kNone, "%s: single-stripe task %u has a boundary", label, t);
             }

@@ -339,3 +339,28 @@ def test_batch_does_not_read_host_inputs_after_creation(ctx, one_query):
         assert np.array_equal(got.astype(np.int64), want), f"run {rep}"
         assert np.all(flags[:4] & device.PAIR_RECOMPUTED)
     batch.close()
+
+
+@pytest.mark.parametrize("ends", [(False, False, False), (True, True, False), (False, False, True)],
+                         ids=["penalised", "leading-free", "last-column-free"])
+def test_recentred_int16_long_pairs(ctx, ends):
+    """Long global pairs in re-centred int16 lanes (wavefront kernel): scores far outside the
+    int16 range (40,000 for identical 40 kbp sequences, about -100,000 along a penalised first
+    row), exact and without a recompute; several rounds of stripes (the HBM boundary row), two
+    pairs of different lengths per task."""
+    fc, fr, lc = ends
+    rng = np.random.default_rng(61)
+    same = bytes(workloads.random_residues(rng, workloads.DNA, 40000))
+    s1 = [same]
+    s2 = [same]
+    a, b = workloads.dna_pairs(rng, 3, 12000, 30000, divergence=0.15, indel_fraction=0.6)
+    s1 += a
+    s2 += b
+    s1.append(bytes(workloads.random_residues(rng, workloads.DNA, 20000)))
+    s2.append(bytes(workloads.random_residues(rng, workloads.DNA, 4000)))
+    dc = device.DeviceConfig(match=1, mismatch=-4, gap_open=-6, gap_extension=-1, first_column_free=fc,
+                             first_row_free=fr, last_column_free=lc)
+    got, flags = check(ctx, s1, s2, dc, label=f"re-centred int16 long pairs ends={ends}")
+    assert np.all(flags & device.PAIR_INT16) and not np.any(flags & device.PAIR_RECOMPUTED), flags
+    if not (fc or fr or lc):
+        assert got[0] == 40000
