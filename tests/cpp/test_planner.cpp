@@ -186,8 +186,50 @@ static void check_plan(const char *label, const pa_config &cfg, const Batch &b, 
     std::printf("  %s: %llu pairs, %zu tasks, %zu launches\n", label, (unsigned long long)N, p.tasks.size(), p.launches.size());
 }
 
+// Long global pairs: re-centred int16 exactly when the spread bound holds (no saturation
+// tracking), otherwise int32 or int16 on the absolute bounds, then with the saturation tracking.
+static void check_recentring()
+{
+    std::mt19937_64 rng(7);
+    Batch b;
+    for (int p = 0; p < 40; ++p) {
+        const int l1 = 3200 + int(rng() % 4000), l2 = 3200 + int(rng() % 4000);
+        for (int i = 0; i < l1; ++i) b.s1.push_back("ACGT"[rng() % 4]);
+        for (int i = 0; i < l2; ++i) b.s2.push_back("ACGT"[rng() % 4]);
+        b.o1.push_back(b.s1.size());
+        b.o2.push_back(b.s2.size());
+    }
+    struct Case { int match, mismatch, go, ge, policy; bool want_rc; uint8_t want_width; };
+    const Case cases[] = {
+        {1, -4, -6, -1, PA_LANES_AUTO, true, 16},     // C = 12: 14,592 < 16,000
+        {1, -4, -6, -1, PA_LANES_INT32, false, 32},
+        {4, -5, -10, -1, PA_LANES_AUTO, false, 16},   // C = 20: beyond the bound; absolute bounds hold
+        {4, -5, -10, -1, PA_LANES_INT16, false, 16},  // tracked int16 (absolute values)
+    };
+    for (const Case &c : cases) {
+        pa_config cfg{};
+        cfg.match = c.match;
+        cfg.mismatch = c.mismatch;
+        cfg.gap_open = c.go;
+        cfg.gap_extension = c.ge;
+        cfg.lane_policy = c.policy;
+        Plan p;
+        std::string err;
+        const BatchView v = b.view();
+        CHECK(make_plan(cfg, v, nullptr, nullptr, p, err) == PA_OK, "recentring plan: %s", err.c_str());
+        for (const Launch &L : p.launches) {
+            CHECK(L.key.G == 64, "long pairs in the wavefront kernel");
+            CHECK(L.key.width == c.want_width, "width %d, expected %d (policy %d)", L.key.width, c.want_width, c.policy);
+            CHECK(recentred(L.key) == c.want_rc, "re-centred %d, expected %d", int(recentred(L.key)), int(c.want_rc));
+            if (L.key.width == 16 && !c.want_rc) CHECK(L.key.track == 1, "non-re-centred int16 keeps the tracking");
+        }
+    }
+    std::printf("  re-centring choices: checked\n");
+}
+
 int main()
 {
+    check_recentring();
     std::mt19937_64 rng(20261204);
     pa_config dna{};
     dna.gap_open = -10;
