@@ -293,7 +293,10 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 // the local maximum is in the frame of step -1 with the column drift
                 const int v = L::half(mv, q) + (LOCAL && CDRIFT ? cg : 0);
                 if constexpr (LOCAL) a.scores[tk.pair[q]] = v;
-                if (W == 16 && L::half(sv, q) >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
+                // with the local column drift a value clamped at kMax in the frame of step u is
+                // kMax - u|ge| in exact terms and at least kMax - G|ge| in the frame of step -1
+                if (W == 16 && L::half(sv, q) >= L::kMax - (LOCAL && CDRIFT ? G * cg : 0))
+                    atomicOr(a.flags + tk.pair[q], kFlagSaturated);
                 if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
             }
         }
@@ -368,14 +371,15 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
     constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
     constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
-    constexpr bool CDRIFT = DRIFT && !ACC;   // column drift on top
+    // column drift: on top of the row drift for the global method, alone for the local method
+    constexpr bool CDRIFT = !ACC;
     // int16 global pairs are re-centred: each wave keeps its values relative to a per-half
     // offset (int32, wave-uniform), moved at every chunk end to the value of lane 0's first row,
     // so that the band of cells in flight (within about 1,200 steps of each other) stays far
     // inside the int16 range whatever the absolute scores; the planner uses int16 here only
     // when that spread bound holds.  Values crossing to another wave carry their writer's
     // offsets (the LDS ring per chunk, the HBM row as exact int32).
-    constexpr bool RC = W == 16 && CDRIFT;
+    constexpr bool RC = W == 16 && CDRIFT && !LOCAL;
     const int dge = DRIFT ? ge : 0;
     const int cg = CDRIFT ? -ge : 0;         // column drift per step
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
@@ -390,8 +394,11 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     uint32_t *gbnd = a.bnd + 2ull * tk.bnd_off;   // last wave -> first wave, one full row
     // (re-centred: four words per column, the exact H and V of each half)
 
-    V runmax = LOCAL ? L::splat(0) : L::lowest();   // exact (no column drift for the local method here)
+    // exact; for the local method (column drift) 0 in the frame of step -1, moved into each
+    // step's frame as the step begins and back by G steps at each chunk end
+    V runmax = LOCAL ? L::splat(-cg) : L::lowest();
     V satmax = L::lowest();                         // drifted (int16 saturation test)
+    const V vcg = L::splat(cg);
 
     for (int s = wave; s < S; s += kWfWaves) {
         const int round = s / kWfWaves;
@@ -501,13 +508,14 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                     convH = dpp<kWaveShl1>(convH, convH);
                     convE = dpp<kWaveShl1>(convE, convE);
                 }
+                if constexpr (LOCAL) runmax = L::add(runmax, vcg);   // into this step's frame
                 if (uint32_t(c) < uint32_t(N)) {
                     V Eup = L::from(Ein);
                     update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo,
-                                                                         tbl_hi, vgo, vgogo, vge);
+                                                                         tbl_hi, vgo, vgogo, vge, L::splat(cg * u));
                     Hdiag = L::from(Hin);
-                    Hout = L::bits(DRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
-                    Eout = L::bits(DRIFT ? L::add(Eup, vout) : Eup);
+                    Hout = L::bits(DRIFT || CDRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
+                    Eout = L::bits(DRIFT || CDRIFT ? L::add(Eup, vout) : Eup);
                     if constexpr (!LOCAL) cap.at_column(c, st, tk, a.scores, dge, -cg * u, off0, off1);
                 }
                 if (dst != 0) {
@@ -601,6 +609,7 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                 Hdiag = L::add(Hdiag, vrebase);
                 Hout = L::bits(L::add(L::from(Hout), vrebase));
                 Eout = L::bits(L::add(L::from(Eout), vrebase));
+                if constexpr (LOCAL) runmax = L::add(runmax, vrebase);
             }
         }
         // free last row only: the int16 saturation tracking (TRACK) keeps per-row maxima too, but
@@ -616,9 +625,13 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
 #pragma unroll
             for (int q = 0; q < L::NH; ++q) {
                 if (tk.pair[q] == kNone) continue;
-                const int v = L::half(mv, q);
+                // the local maximum is in the frame of step -1 (column drift)
+                const int v = L::half(mv, q) + (LOCAL ? cg : 0);
                 if constexpr (LOCAL) atomicMax(a.scores + tk.pair[q], v);
-                if (W == 16 && L::half(sv, q) >= L::kMax) atomicOr(a.flags + tk.pair[q], kFlagSaturated);
+                // with the local column drift a value clamped at kMax in the frame of step u is
+                // kMax - u|ge| in exact terms and at least kMax - G|ge| in the frame of step -1
+                if (W == 16 && L::half(sv, q) >= L::kMax - (LOCAL && CDRIFT ? G * cg : 0))
+                    atomicOr(a.flags + tk.pair[q], kFlagSaturated);
                 if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
             }
         }
@@ -707,13 +720,16 @@ static hipError_t dispatch_flags(const VariantKey &k, const KernelArgs &a, uint3
             return hipErrorInvalidValue;
         }
     }
-    if (LOCAL) return launch_variant<W, R, SCORE, LOCAL, false, false>(k, a, lds, st);
-    if (k.rowmax) {
-        if (k.track) return launch_variant<W, R, SCORE, LOCAL, true, true>(k, a, lds, st);
-        return launch_variant<W, R, SCORE, LOCAL, false, true>(k, a, lds, st);
+    if constexpr (LOCAL) {   // local variants keep no per-row maxima
+        return launch_variant<W, R, SCORE, LOCAL, false, false>(k, a, lds, st);
+    } else {
+        if (k.rowmax) {
+            if (k.track) return launch_variant<W, R, SCORE, LOCAL, true, true>(k, a, lds, st);
+            return launch_variant<W, R, SCORE, LOCAL, false, true>(k, a, lds, st);
+        }
+        if (k.track) return launch_variant<W, R, SCORE, LOCAL, true, false>(k, a, lds, st);
+        return launch_variant<W, R, SCORE, LOCAL, false, false>(k, a, lds, st);
     }
-    if (k.track) return launch_variant<W, R, SCORE, LOCAL, true, false>(k, a, lds, st);
-    return launch_variant<W, R, SCORE, LOCAL, false, false>(k, a, lds, st);
 }
 
 template <int W, int R>

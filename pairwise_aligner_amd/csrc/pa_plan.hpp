@@ -133,12 +133,12 @@ static_assert(sizeof(VariantKey) == 8, "VariantKey is compared as one 64-bit wor
 
 // Substitution-score shift of a kernel variant, in units of -ge (pa_cells.hpp): the global
 // method drifts along the rows (1) and, unless it keeps per-row maxima, along the columns as
-// well (2); the local method drifts along the columns only in the striped kernel (1) and keeps
-// exact values in the wavefront kernel and the DP export (0).
+// well (2); the local method drifts along the columns only (1), except in the DP export, which
+// keeps exact values (0).
 inline constexpr int kScoreShifts = 3;
 inline int score_shift(const VariantKey &key)
 {
-    if (key.local) return (key.G == 16 && !key.export_dp) ? 1 : 0;   // striped: column drift
+    if (key.local) return key.export_dp ? 0 : 1;   // column drift (not in the DP export)
     return (key.track || key.rowmax || key.export_dp) ? 1 : 2;
 }
 

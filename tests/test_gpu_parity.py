@@ -156,20 +156,23 @@ def test_lane_policies(ctx, policy):
         assert np.all(flags & device.PAIR_INT32)
 
 
-def test_saturation_recompute(ctx):
-    """Scores beyond int16: identical 9 kbp sequences with match +4 (score 36,000) in int16 lanes
-    must be flagged, recomputed in int32 and come out exact."""
+@pytest.mark.parametrize("length, match", [(9000, 4), (2500, 15)], ids=["wavefront", "striped"])
+def test_saturation_recompute(ctx, length, match):
+    """Scores beyond int16: identical sequences (9 kbp at +4, 36,000, in the wavefront kernel;
+    2.5 kbp at +15, 37,500, in the striped kernel) in int16 lanes must be flagged, recomputed in
+    int32 and come out exact, for the global method (saturation tracking) and the local one
+    (running maximum in the column drift's frames)."""
     rng = np.random.default_rng(21)
-    base = workloads.unpack(*workloads.random_packed(rng, np.full(6, 9000), workloads.DNA))
+    base = workloads.unpack(*workloads.random_packed(rng, np.full(6, length), workloads.DNA))
     short1, short2 = workloads.dna_pairs(rng, 40, 100, 200, divergence=0.05)
     s1 = base + short1
     s2 = list(base) + short2
     for method in (device.METHOD_GLOBAL, device.METHOD_LOCAL):
-        dc = device.DeviceConfig(method=method, match=4, mismatch=-5, gap_open=-10, gap_extension=-1,
+        dc = device.DeviceConfig(method=method, match=match, mismatch=-5, gap_open=-10, gap_extension=-1,
                                  lane_policy=device.LANES_INT16)
-        got, flags = check(ctx, s1, s2, dc, label=f"saturation method={method}")
+        got, flags = check(ctx, s1, s2, dc, label=f"saturation method={method} length={length}")
         assert np.all(flags[:6] & device.PAIR_RECOMPUTED), flags[:6]
-        assert np.all(got[:6] == 36000)
+        assert np.all(got[:6] == length * match)
         assert not np.any(flags[6:] & device.PAIR_RECOMPUTED)
 
 
