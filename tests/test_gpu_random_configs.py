@@ -8,6 +8,7 @@ and unrelated pairs.  It runs through the one-to-one entry point and, for one ca
 one-to-many entry point with the first sequence as the query.  The reference's differential tests
 (alignment_simd_test_template.hpp:195-227) check SIMD against scalar results the same way.
 """
+import os
 import sys
 
 import numpy as np
@@ -18,7 +19,9 @@ from pairwise_aligner_amd import device, tables, workloads
 
 pytestmark = pytest.mark.gpu
 
-N_CASES = 64
+# 64 seeded cases by default; PA_RANDOM_CASES widens the sweep (the seeds of the default run stay
+# the first ones).
+N_CASES = int(os.environ.get("PA_RANDOM_CASES", "64"))
 
 
 def log(msg):
