@@ -81,10 +81,12 @@ __device__ __forceinline__ int h0_value(int i, int free_flag, int go, int ge)
 // the diagonal term gains |ge| (D' = H'(r-1) + s - ge, folded into the substitution scores by
 // the planner), and values leaving the lane are shifted back: H exact, V + |ge| (the drift of
 // the next lane's first row).  Per-row maxima are kept drifted and shifted back when read.  The
-// local method keeps exact values: its zero clamp would need a different bound per row.
+// local method has no row drift: its zero clamp would need a different bound per row.
 //
-// Column drift (global method without per-row maxima, on top of the row drift).  A value
-// computed at step k of a chunk of G steps (u = k mod G) also carries |ge|*u.  Consecutive
+// Column drift (global method without per-row maxima, on top of the row drift; local method
+// without per-row maxima or the DP export, alone, with the zero clamp raised to |ge|*u and the
+// running maximum moved into each step's frame).  A value computed at step k of a chunk of G
+// steps (u = k mod G) also carries |ge|*u.  Consecutive
 // columns of a row are consecutive steps, so the horizontal gap state loses its extension term
 // too, W'(j+1) = max(W'(j), H'(j) + go), and the diagonal term gains another |ge| (scores shifted
 // by -2ge).  The drift is the same for every lane at a step, so values passed between lanes are
