@@ -4,8 +4,9 @@ Each case draws a configuration the way a user could write one -- method (or per
 the four end-gap rules, gap open and extension in [-15, 0] and [-5, 0] (zero included), unitary
 scores or BLOSUM62 (standard or extended), a lane policy -- and a batch whose lengths mix empty,
 short, medium and long sequences (beyond the wavefront kernel's threshold now and then), related
-and unrelated pairs.  It runs through the one-to-one entry point and, for one case in three, the
-one-to-many entry point with the first sequence as the query.  The reference's differential tests
+and unrelated pairs.  It runs through the one-to-one entry point, for one case in three the
+one-to-many entry point with the first sequence as the query, and for one seed in four the DP
+export (final DP column and row of every pair).  The reference's differential tests
 (alignment_simd_test_template.hpp:195-227) check SIMD against scalar results the same way.
 """
 import os
@@ -99,3 +100,9 @@ def test_random_configuration(ctx, seed):
         want_q = O.score_packed(rep1, rep_o1, b2, o2, oracle_cfg(dc), pair_method=pm, threads=8)
         got_q, _ = ctx.score_one_to_many(q, b2, o2, dc, pm)
         assert np.array_equal(got_q.astype(np.int64), want_q), f"case {seed}: one-to-many"
+    if seed % 4 == 1:
+        want_s, want_rows, want_cols = O.dp_vectors_packed(b1, o1, b2, o2, oracle_cfg(dc), pair_method=pm, threads=8)
+        got_s, _, cols, rows = ctx.score_packed_dp(b1, o1, b2, o2, dc, pm)
+        assert np.array_equal(got_s.astype(np.int64), want_s), f"case {seed}: DP export scores"
+        assert np.array_equal(cols.astype(np.int64), want_cols), f"case {seed}: DP export columns"
+        assert np.array_equal(rows.astype(np.int64), want_rows), f"case {seed}: DP export rows"
