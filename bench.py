@@ -14,7 +14,6 @@ import argparse
 import dataclasses
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -354,9 +353,6 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy three times (scales the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters for roofline.traffic)")
-    ap.add_argument("--valu-probe", action="store_true",
-                    help="only run the VALU issue-rate probe (pairwise_aligner_amd/valu_probe): cycles per "
-                         "instruction of one wave and the shader clock for the kernels' instructions")
     ap.add_argument("--e2e-reps", type=int, default=10,
                     help="calls of the one-shot entry point timed end to end (0: skip)")
     ap.add_argument("--pairs", type=int, default=None,
@@ -372,16 +368,6 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copies": 3, "check": int(b[-1].item())}))
-        return
-    if args.valu_probe:
-        probe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pairwise_aligner_amd", "valu_probe")
-        if not os.path.exists(probe):
-            raise SystemExit("pairwise_aligner_amd/valu_probe is not built (python -m pairwise_aligner_amd.build)")
-        res = subprocess.run([probe], capture_output=True, text=True, timeout=120)
-        sys.stdout.write(res.stdout)
-        if res.returncode != 0:
-            sys.stderr.write(res.stderr)
-            raise SystemExit(res.returncode)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (pairs are independent; multi-GPU sharding is not built); "
