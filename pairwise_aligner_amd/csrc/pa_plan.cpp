@@ -537,6 +537,10 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         const bool wavefront = !export_dp && pm.m > plan.tuning.wavefront_min_rows;
         const bool rc_candidate = wavefront && !local && !(lc && lr);
         if (rc_candidate && recentre_ok) lower16 = upper16 = true;
+        // The int16 kernels shift values by up to |ge| per row (17 rows) and per column step (64
+        // steps); with a gap extension that large the shifted frames alone would leave int16,
+        // so such configurations run in int32 under every policy.
+        if (-int64_t(plan.ge) * 81 > 16000) lower16 = upper16 = false;
         uint8_t width;
         if (export_dp) width = 32;   // exported rows and columns are exact int32
         else if (forced_width && forced_width[p]) width = forced_width[p];

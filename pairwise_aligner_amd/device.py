@@ -160,6 +160,50 @@ def lib() -> ctypes.CDLL:
     return L
 
 
+def _residues(b, name: str) -> np.ndarray:
+    if isinstance(b, (bytes, bytearray, memoryview)):
+        b = np.frombuffer(bytes(b), dtype=np.uint8)
+    b = np.asarray(b)
+    if b.dtype != np.uint8:
+        raise TypeError(f"{name}: residues must be uint8 (got {b.dtype})")
+    b = np.ascontiguousarray(b)
+    return b if b.size else np.zeros(1, dtype=np.uint8)
+
+
+def _offsets(o, buf: np.ndarray, name: str) -> np.ndarray:
+    o = np.asarray(o)
+    if o.ndim != 1 or o.size < 1 or not np.issubdtype(o.dtype, np.integer):
+        raise TypeError(f"{name}: offsets must be a non-empty 1-d integer array")
+    if o.dtype != np.uint64:
+        if np.any(o < 0):
+            raise ValueError(f"{name}: negative offset")
+        o = o.astype(np.uint64)
+    o = np.ascontiguousarray(o)
+    if o.size > 1 and np.any(o[1:] < o[:-1]):
+        raise ValueError(f"{name}: offsets decrease")
+    if int(o[-1]) > buf.size:
+        raise ValueError(f"{name}: last offset {int(o[-1])} beyond the {buf.size} residues")
+    return o
+
+
+def checked_batch(b1, o1, b2, o2, pair_method=None, one_query=False):
+    """Inputs of a batch as the C ABI reads them: contiguous uint8 residues, uint64 offsets that
+    stay inside their buffers, equally many pairs on both sides (one_query: o1 holds the query's
+    two offsets), a uint8 method per pair.  Raises instead of letting the library read out of
+    bounds."""
+    b1, b2 = _residues(b1, "seq1"), _residues(b2, "seq2")
+    o1, o2 = _offsets(o1, b1, "off1"), _offsets(o2, b2, "off2")
+    n = len(o2) - 1
+    if not one_query and len(o1) != len(o2):
+        raise ValueError(f"{len(o1) - 1} first sequences against {n} second sequences")
+    pm = None
+    if pair_method is not None:
+        pm = np.ascontiguousarray(pair_method, dtype=np.uint8)
+        if pm.shape != (n,):
+            raise ValueError(f"pair_method holds {pm.size} entries for {n} pairs")
+    return b1, o1, b2, o2, pm
+
+
 def pack_sequences(seqs: Sequence[bytes]):
     lens = np.fromiter((len(s) for s in seqs), dtype=np.uint64, count=len(seqs))
     off = np.zeros(len(seqs) + 1, dtype=np.uint64)
@@ -223,11 +267,11 @@ class Context:
         return self.score_packed(b1, o1, b2, o2, cfg, pair_method)
 
     def score_packed(self, b1, o1, b2, o2, cfg: DeviceConfig, pair_method=None):
+        b1, o1, b2, o2, pm = checked_batch(b1, o1, b2, o2, pair_method)
         n = len(o1) - 1
         holder = _CConfigHolder(cfg)
         scores = np.zeros(max(n, 1), dtype=np.int32)
         flags = np.zeros(max(n, 1), dtype=np.uint8)
-        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
         st = self._lib.pa_score_batch(self.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
                                       b2.ctypes.data, o2.ctypes.data, n, None if pm is None else pm.ctypes.data,
                                       scores.ctypes.data, flags.ctypes.data)
@@ -237,6 +281,7 @@ class Context:
     def score_packed_dp(self, b1, o1, b2, o2, cfg: DeviceConfig, pair_method=None):
         """Scores, flags and the final DP column and row of every pair (pa_score_batch_dp):
         dp_column[(o1[p]-o1[0]) + p + i] = H(i, len2), dp_row[(o2[p]-o2[0]) + p + j] = H(len1, j)."""
+        b1, o1, b2, o2, pm = checked_batch(b1, o1, b2, o2, pair_method)
         n = len(o1) - 1
         holder = _CConfigHolder(cfg)
         scores = np.zeros(max(n, 1), dtype=np.int32)
@@ -245,7 +290,6 @@ class Context:
         n2 = int(o2[-1] - o2[0]) + n
         dp_column = np.zeros(max(n1, 1), dtype=np.int32)
         dp_row = np.zeros(max(n2, 1), dtype=np.int32)
-        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
         st = self._lib.pa_score_batch_dp(self.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
                                          b2.ctypes.data, o2.ctypes.data, n, None if pm is None else pm.ctypes.data,
                                          scores.ctypes.data, flags.ctypes.data, dp_column.ctypes.data,
@@ -256,11 +300,12 @@ class Context:
     def score_one_to_many(self, query, b2, o2, cfg: DeviceConfig, pair_method=None):
         """Scores of (query, target p) for every packed target (pa_score_one_to_many)."""
         q = np.frombuffer(bytes(query), dtype=np.uint8).copy() if len(query) else np.zeros(1, dtype=np.uint8)
+        _, _, b2, o2, pm = checked_batch(q, np.array([0, len(query)], dtype=np.uint64), b2, o2, pair_method,
+                                         one_query=True)
         n = len(o2) - 1
         holder = _CConfigHolder(cfg)
         scores = np.zeros(max(n, 1), dtype=np.int32)
         flags = np.zeros(max(n, 1), dtype=np.uint8)
-        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
         st = self._lib.pa_score_one_to_many(self.handle, ctypes.byref(holder.c), q.ctypes.data, len(query),
                                             b2.ctypes.data, o2.ctypes.data, n,
                                             None if pm is None else pm.ctypes.data, scores.ctypes.data,
@@ -274,10 +319,11 @@ class Batch:
 
     def __init__(self, ctx: Context, b1, o1, b2, o2, cfg: DeviceConfig, pair_method=None):
         self.ctx = ctx
+        self.handle = None
+        b1, o1, b2, o2, pm = checked_batch(b1, o1, b2, o2, pair_method)
         self.n = len(o1) - 1
         self._keep = (b1, o1, b2, o2)
         holder = _CConfigHolder(cfg)
-        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
         h = ctypes.c_void_p()
         st = ctx._lib.pa_batch_create(ctx.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
                                       b2.ctypes.data, o2.ctypes.data, self.n,
@@ -293,10 +339,11 @@ class Batch:
         self.ctx = ctx
         self.n = len(o2) - 1
         q = np.frombuffer(bytes(query), dtype=np.uint8).copy() if len(query) else np.zeros(1, dtype=np.uint8)
-        self._keep = (q, b2, o2)
         self.handle = None
+        _, _, b2, o2, pm = checked_batch(q, np.array([0, len(query)], dtype=np.uint64), b2, o2, pair_method,
+                                         one_query=True)
+        self._keep = (q, b2, o2)
         holder = _CConfigHolder(cfg)
-        pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
         h = ctypes.c_void_p()
         st = ctx._lib.pa_batch_create_one_to_many(ctx.handle, ctypes.byref(holder.c), q.ctypes.data, len(query),
                                                   b2.ctypes.data, o2.ctypes.data, self.n,
@@ -399,6 +446,7 @@ class Model:
 
 
 def features(ctx: Context, cfg: DeviceConfig, b1, o1, b2, o2) -> np.ndarray:
+    b1, o1, b2, o2, _ = checked_batch(b1, o1, b2, o2)
     holder = _CConfigHolder(cfg)
     n = len(o1) - 1
     out = np.zeros((max(n, 1), N_FEATURES), dtype=np.float32)
@@ -409,6 +457,7 @@ def features(ctx: Context, cfg: DeviceConfig, b1, o1, b2, o2) -> np.ndarray:
 
 def dp_labels(ctx: Context, cfg: DeviceConfig, b1, o1, b2, o2):
     """(max of H over each pair's DP matrix, modelled+measured cost in ns) from an int32 DP pass."""
+    b1, o1, b2, o2, _ = checked_batch(b1, o1, b2, o2)
     holder = _CConfigHolder(cfg)
     n = len(o1) - 1
     max_h = np.zeros(max(n, 1), dtype=np.int32)
@@ -421,9 +470,9 @@ def dp_labels(ctx: Context, cfg: DeviceConfig, b1, o1, b2, o2):
 def tune(ctx: Context, cfg: DeviceConfig, b1, o1, b2, o2, pair_method=None, sample_pairs: int = 0,
          iters: int = 0) -> Tuning:
     """Runs the library's autotuner (pa_tune) on a sample of the batch; returns the fastest tuning."""
+    b1, o1, b2, o2, pm = checked_batch(b1, o1, b2, o2, pair_method)
     holder = _CConfigHolder(cfg)
     n = len(o1) - 1
-    pm = None if pair_method is None else np.ascontiguousarray(pair_method, dtype=np.uint8)
     out = CTuning()
     ctx.check(ctx._lib.pa_tune(ctx.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data, b2.ctypes.data,
                                o2.ctypes.data, n, None if pm is None else pm.ctypes.data, sample_pairs, iters,

@@ -177,42 +177,102 @@ def config_protein_1xn(seed: int = 6, count: int = 200_000, query_len: int = 300
                 scoring=dict(kind="blosum62"), gap=(-11, -1), method="local")
 
 
-def config_uneven(seed: int = 3, count: int = 1_000_000):
-    rng = np.random.default_rng(seed)
-    L = np.clip(np.round(rng.lognormal(np.log(250.0), 0.8, size=count)), 50, 10_000).astype(np.int64)
-    b1, o1 = random_packed(rng, L, DNA)
-    div = rng.uniform(0.01, 0.20, size=count)
-    b2, o2 = mutate_packed(rng, b1, o1, div, 0.1, 0.7, DNA)
-    pm = (np.arange(count) % 2).astype(np.uint8)   # 0 global, 1 local
+def _chunks(count: int, chunk: int):
+    return [(c, start, min(chunk, count - start)) for c, start in enumerate(range(0, count, chunk))]
+
+
+def take(buf: np.ndarray, off: np.ndarray, n: int):
+    """The first n sequences of a packed batch."""
+    off = off[:n + 1]
+    end = int(off[-1])
+    return (buf[:end] if end else np.zeros(1, dtype=np.uint8)), off
+
+
+def pick(buf: np.ndarray, off: np.ndarray, idx):
+    """Sequences idx (in that order) of a packed batch, packed."""
+    return pack([buf[int(off[i]):int(off[i + 1])].tobytes() for i in idx])
+
+
+def config_uneven(seed: int = 3, count: int = 1_000_000, limit: int = None, chunk: int = 50_000):
+    """Chunk c of `chunk` pairs comes from its own stream (seed, c), so the first `limit` pairs
+    are the same pairs as in the whole batch."""
+    n = count if limit is None else min(limit, count)
+    parts = []
+    for c, start, k in _chunks(count, chunk):
+        if start >= n:
+            break
+        rng = np.random.default_rng([seed, c])
+        L = np.clip(np.round(rng.lognormal(np.log(250.0), 0.8, size=k)), 50, 10_000).astype(np.int64)
+        b1, o1 = random_packed(rng, L, DNA)
+        div = rng.uniform(0.01, 0.20, size=k)
+        b2, o2 = mutate_packed(rng, b1, o1, div, 0.1, 0.7, DNA)
+        parts.append((b1[:int(o1[-1])], o1, b2[:int(o2[-1])], o2))
+    b1, o1, b2, o2 = concat_packed(parts)
+    b1, o1 = take(b1, o1, n)
+    b2, o2 = take(b2, o2, n)
+    pm = (np.arange(n) % 2).astype(np.uint8)   # 0 global, 1 local
     return dict(name="uneven", b1=b1, o1=o1, b2=b2, o2=o2, pair_method=pm,
                 scoring=dict(kind="unitary", match=2, mismatch=-3), gap=(-5, -2), method="mixed")
 
 
-def config_long(seed: int = 4, count: int = 2_000):
-    rng = np.random.default_rng(seed)
-    b1, o1 = random_packed(rng, rng.integers(10_000, 100_001, size=count), DNA)
-    b2, o2 = mutate_packed(rng, b1, o1, 0.15, 0.6, 0.5, DNA)
+def config_long(seed: int = 4, count: int = 2_000, select=None, chunk: int = 50):
+    """Target lengths come from stream (seed, 0); the residues of chunk c of `chunk` pairs from
+    stream (seed, 1, c), so any selection of pairs (`select`: pair indices, in that order) is
+    generated without the rest.  `long_lengths` gives the target lengths alone."""
+    lengths = long_lengths(seed, count)
+    want = range(count) if select is None else [int(i) for i in select]
+    need = sorted({i // chunk for i in want})
+    made = {}
+    for c in need:
+        start = c * chunk
+        k = min(chunk, count - start)
+        rng = np.random.default_rng([seed, 1, c])
+        b1, o1 = random_packed(rng, lengths[start:start + k], DNA)
+        b2, o2 = mutate_packed(rng, b1, o1, 0.15, 0.6, 0.5, DNA)
+        made[c] = (b1, o1, b2, o2)
+    if select is None:
+        b1, o1, b2, o2 = concat_packed([(m[0][:int(m[1][-1])], m[1], m[2][:int(m[3][-1])], m[3])
+                                        for _, m in sorted(made.items())])
+    else:
+        s1, s2 = [], []
+        for i in want:
+            mb1, mo1, mb2, mo2 = made[i // chunk]
+            j = i % chunk
+            s1.append(mb1[int(mo1[j]):int(mo1[j + 1])].tobytes())
+            s2.append(mb2[int(mo2[j]):int(mo2[j + 1])].tobytes())
+        b1, o1 = pack(s1)
+        b2, o2 = pack(s2)
     return dict(name="long", b1=b1, o1=o1, b2=b2, o2=o2, pair_method=None,
                 scoring=dict(kind="unitary", match=1, mismatch=-4), gap=(-6, -1), method="global")
 
 
-def config_mixed(seed: int = 5, count: int = 500_000, chunk: int = 25_000):
+def long_lengths(seed: int = 4, count: int = 2_000) -> np.ndarray:
+    return np.random.default_rng([seed, 0]).integers(10_000, 100_001, size=count)
+
+
+def config_mixed(seed: int = 5, count: int = 500_000, limit: int = None, chunk: int = 25_000):
     """Half DNA, half protein pairs; lengths uniform 100-5,000; identity uniform 50-100%
     (divergence 0-50% of positions, 90% of events substitutions).  Returns the DNA and protein
-    halves separately because they use different scoring.  Generated in chunks of `chunk` pairs
-    to bound the temporaries."""
-    rng = np.random.default_rng(seed)
+    halves separately because they use different scoring.  Chunk c of each half comes from its
+    own stream (seed, half, c), so `limit` (pairs per half) gives the first pairs of each half of
+    the whole batch."""
     half = count // 2
     out = {}
-    for kind, alpha, p, n in (("dna", DNA, None, half), ("protein", AMINO, AMINO_FREQ, count - half)):
+    for h, (kind, alpha, p, n) in enumerate((("dna", DNA, None, half), ("protein", AMINO, AMINO_FREQ, count - half))):
+        keep = n if limit is None else min(limit, n)
         parts = []
-        for start in range(0, n, chunk):
-            k = min(chunk, n - start)
+        for c, start, k in _chunks(n, chunk):
+            if start >= keep:
+                break
+            rng = np.random.default_rng([seed, h, c])
             b1, o1 = random_packed(rng, rng.integers(100, 5_001, size=k), alpha, p)
             div = 1.0 - rng.uniform(0.5, 1.0, size=k)
             b2, o2 = mutate_packed(rng, b1, o1, div, 0.1, 0.7, alpha, p)
             parts.append((b1[:int(o1[-1])], o1, b2[:int(o2[-1])], o2))
-        out[kind] = dict(zip(("b1", "o1", "b2", "o2"), concat_packed(parts)))
+        b1, o1, b2, o2 = concat_packed(parts)
+        b1, o1 = take(b1, o1, keep)
+        b2, o2 = take(b2, o2, keep)
+        out[kind] = dict(b1=b1, o1=o1, b2=b2, o2=o2)
     out["name"] = "mixed"
     return out
 

@@ -227,9 +227,45 @@ static void check_recentring()
     std::printf("  re-centring choices: checked\n");
 }
 
+// A gap extension so large that the int16 kernels' drift frames (|ge| per row and per column
+// step) alone would leave int16: every pair runs in int32, under every lane policy.
+static void check_large_gap_extension()
+{
+    std::mt19937_64 rng(11);
+    Batch b;
+    for (int p = 0; p < 30; ++p) {
+        const int l1 = 20 + int(rng() % (p == 0 ? 4000 : 300)), l2 = 20 + int(rng() % 300);
+        for (int i = 0; i < l1; ++i) b.s1.push_back("ACGT"[rng() % 4]);
+        for (int i = 0; i < l2; ++i) b.s2.push_back("ACGT"[rng() % 4]);
+        b.o1.push_back(b.s1.size());
+        b.o2.push_back(b.s2.size());
+    }
+    for (int ge : {-600, -3000}) {
+        for (int method : {PA_METHOD_GLOBAL, PA_METHOD_LOCAL}) {
+            for (int policy : {PA_LANES_AUTO, PA_LANES_INT16}) {
+                pa_config cfg{};
+                cfg.method = method;
+                cfg.match = 4;
+                cfg.mismatch = -5;
+                cfg.gap_open = -10;
+                cfg.gap_extension = ge;
+                cfg.lane_policy = policy;
+                Plan p;
+                std::string err;
+                const BatchView v = b.view();
+                CHECK(make_plan(cfg, v, nullptr, nullptr, p, err) == PA_OK, "large ge plan: %s", err.c_str());
+                for (const Launch &L : p.launches)
+                    CHECK(L.key.width == 32, "ge %d method %d policy %d: width %d", ge, method, policy, L.key.width);
+            }
+        }
+    }
+    std::printf("  large gap extension: int32 only\n");
+}
+
 int main()
 {
     check_recentring();
+    check_large_gap_extension();
     std::mt19937_64 rng(20261204);
     pa_config dna{};
     dna.gap_open = -10;

@@ -192,6 +192,24 @@ def test_tracked_int16_global_scores(ctx, ends):
     assert np.count_nonzero(stayed) > 30   # scored by the tracked int16 kernels
 
 
+@pytest.mark.parametrize("ge", [-600, -3000])
+def test_large_gap_extension_runs_int32(ctx, ge):
+    """A gap extension so large that the int16 kernels' drift frames (|ge| per row and per
+    column step) would leave int16 by themselves: the planner runs every pair in int32, also under
+    int16 speculation, and the scores equal the oracle's (short pairs and one pair longer than
+    3,072 rows, for the wavefront kernel)."""
+    rng = np.random.default_rng(90 - ge)
+    s1, s2 = workloads.dna_pairs(rng, 80, 20, 300, divergence=0.2)
+    l1, l2 = workloads.dna_pairs(rng, 1, 3500, 3600, divergence=0.1)
+    s1, s2 = s1 + l1, s2 + l2
+    for method in (device.METHOD_GLOBAL, device.METHOD_LOCAL):
+        for policy in (device.LANES_INT16, device.LANES_AUTO):
+            dc = device.DeviceConfig(method=method, match=4, mismatch=-5, gap_open=-10, gap_extension=ge,
+                                     lane_policy=policy)
+            got, flags = check(ctx, s1, s2, dc, label=f"ge={ge} method={method} policy={policy}")
+            assert np.all(flags & device.PAIR_INT32), flags
+
+
 def test_empty_and_tiny(ctx):
     s1 = [b"", b"A", b"", b"ACGT", b"G", b"", b"AC"]
     s2 = [b"", b"", b"A", b"", b"G", b"ACGTT", b"CA"]
