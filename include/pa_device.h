@@ -127,6 +127,25 @@ pa_status pa_score_batch_dp(pa_ctx *ctx, const pa_config *cfg,
                             uint64_t n_pairs, const uint8_t *pair_method,
                             int32_t *scores, uint8_t *flags, int32_t *dp_column, int32_t *dp_row);
 
+/* One-shot scoring that allocates no device memory.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Arguments, results and errors as pa_score_batch, except:
+ *   - every device buffer of the call is a slice of one static array per device of
+ *     PA_NOALLOC_SCRATCH_BYTES (64 MiB), which the library holds from the first pa_ctx_create on
+ *     that device until the process ends; a batch whose buffers do not fit returns
+ *     PA_ERR_OUT_OF_MEMORY (the message says how much was needed);
+ *   - PA_LANES_LEARNED returns PA_ERR_UNSUPPORTED; launches are eager (no graph is instantiated);
+ *   - calls on one device share its array, which keeps the last call's contents: calls that
+ *     overlapped on one device would race, so the library serialises them with a per-device lock
+ *     (other processes hold their own copy).  The call is synchronous; it is not capturable into
+ *     a HIP graph. */
+#define PA_NOALLOC_SCRATCH_BYTES (64u << 20)
+pa_status pa_score_batch_noalloc(pa_ctx *ctx, const pa_config *cfg,
+                                 const uint8_t *seq1, const uint64_t *off1,
+                                 const uint8_t *seq2, const uint64_t *off2,
+                                 uint64_t n_pairs, const uint8_t *pair_method,
+                                 int32_t *scores, uint8_t *flags);
+
 /* Device-resident batch: the same work split into plan/upload, repeated runs, and download,
  * so a caller can time runs with inputs already in HBM.  The input buffers are read only during
  * the call: the batch keeps the residues on the device and no host copy of them. */

@@ -19,13 +19,44 @@
 #include "pa_device.h"
 #include "pa_kernels.hpp"
 #include "pa_learned.hpp"
+#include "pa_noalloc.hpp"
 #include "pa_plan.hpp"
 
 struct DeviceBuffer {
     void *ptr = nullptr;
     size_t bytes = 0;
-    ~DeviceBuffer() { if (ptr) (void)hipFree(ptr); }
+    bool owned = true;   // false: a slice of a caller's arena (pa_score_batch_noalloc)
+    ~DeviceBuffer() { if (ptr && owned) (void)hipFree(ptr); }
 };
+
+// Device memory handed out by bumping an offset through one preallocated region instead of
+// hipMalloc (pa_score_batch_noalloc's scratch).  Slices are 256-byte aligned.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+    size_t refused = 0;   // bytes of the first request that did not fit
+};
+
+// hipMalloc, or a slice of `arena` when one is given.
+hipError_t dev_alloc(DeviceBuffer &buf, size_t bytes, Arena *arena)
+{
+    bytes = std::max<size_t>(bytes, 16);
+    if (!arena) {
+        hipError_t e = hipMalloc(&buf.ptr, bytes);
+        if (e == hipSuccess) buf.bytes = bytes;
+        return e;
+    }
+    const size_t off = (arena->used + 255) & ~size_t(255);
+    if (off > arena->size || bytes > arena->size - off) {
+        if (!arena->refused) arena->refused = bytes;
+        return hipErrorOutOfMemory;
+    }
+    buf.ptr = arena->base + off;
+    buf.bytes = bytes;
+    buf.owned = false;
+    arena->used = off + bytes;
+    return hipSuccess;
+}
 
 struct PlanDevice {
     pa::Plan plan;
@@ -68,6 +99,7 @@ struct pa_batch {
     std::vector<uint32_t> recompute_set;
     DeviceBuffer pairs, scores, flags;
     bool can_saturate = false;
+    Arena *arena = nullptr;                // device buffers from this arena instead of hipMalloc
     hipGraph_t graph = nullptr;            // captured main pass (see launch_main)
     hipGraphExec_t graph_exec = nullptr;
     pa_batch_stats stats{};
@@ -107,11 +139,9 @@ pa_status hip_fail(pa_ctx *ctx, hipError_t e, const char *what)
     } while (0)
 
 template <typename T, typename A>
-pa_status upload(pa_ctx *ctx, DeviceBuffer &buf, const std::vector<T, A> &host)
+pa_status upload(pa_ctx *ctx, DeviceBuffer &buf, const std::vector<T, A> &host, Arena *arena = nullptr)
 {
-    const size_t bytes = std::max<size_t>(host.size() * sizeof(T), 16);
-    PA_HIP(hipMalloc(&buf.ptr, bytes), "hipMalloc");
-    buf.bytes = bytes;
+    PA_HIP(dev_alloc(buf, host.size() * sizeof(T), arena), "device allocation");
     if (!host.empty())
         PA_HIP(hipMemcpyAsync(buf.ptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream),
                "hipMemcpyAsync");
@@ -121,14 +151,13 @@ pa_status upload(pa_ctx *ctx, DeviceBuffer &buf, const std::vector<T, A> &host)
 // Copies the residue bytes of a batch into one device buffer: seq1 bytes [0, end1), then the
 // seq2 bytes (the layout the plan's EncodeTask offsets refer to).  Touches nothing of the context
 // but its device and stream, so that it can run on a helper thread.
-hipError_t copy_residues(int device, hipStream_t stream, const pa::BatchView &v, DeviceBuffer &buf)
+hipError_t copy_residues(int device, hipStream_t stream, const pa::BatchView &v, DeviceBuffer &buf,
+                         Arena *arena = nullptr)
 {
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return e;
     const uint64_t n1 = v.n_pairs ? v.end1() : 0, n2 = v.n_pairs ? v.off2[v.n_pairs] : 0;
-    const size_t bytes = std::max<size_t>(n1 + n2, 16);
-    if ((e = hipMalloc(&buf.ptr, bytes)) != hipSuccess) return e;
-    buf.bytes = bytes;
+    if ((e = dev_alloc(buf, n1 + n2, arena)) != hipSuccess) return e;
     char *dst = static_cast<char *>(buf.ptr);
     if (n1 && (e = hipMemcpyAsync(dst, v.seq1, n1, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
     if (n2 && (e = hipMemcpyAsync(dst + n1, v.seq2, n2, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
@@ -142,20 +171,18 @@ pa_status upload_residues(pa_ctx *ctx, const pa::BatchView &v, DeviceBuffer &buf
 }
 
 // Uploads a plan and encodes its residue streams on the device from `residues` (upload_residues).
-pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd, const DeviceBuffer &residues)
+pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd, const DeviceBuffer &residues, Arena *arena = nullptr)
 {
     pa_status st;
     const pa::Plan &p = pd.plan;
-    if ((st = upload(ctx, pd.tasks, p.tasks)) != PA_OK) return st;
-    if ((st = upload(ctx, pd.enc, p.enc)) != PA_OK) return st;
-    if ((st = upload(ctx, pd.dp_base, p.dp_base)) != PA_OK) return st;
-    PA_HIP(hipMalloc(&pd.rowcodes.ptr, std::max<size_t>(p.n_rowcodes * 4, 16)), "hipMalloc");
-    pd.rowcodes.bytes = std::max<size_t>(p.n_rowcodes * 4, 16);
-    PA_HIP(hipMalloc(&pd.colcodes.ptr, std::max<size_t>(p.n_colcodes * 4, 16)), "hipMalloc");
-    pd.colcodes.bytes = std::max<size_t>(p.n_colcodes * 4, 16);
+    if ((st = upload(ctx, pd.tasks, p.tasks, arena)) != PA_OK) return st;
+    if ((st = upload(ctx, pd.enc, p.enc, arena)) != PA_OK) return st;
+    if ((st = upload(ctx, pd.dp_base, p.dp_base, arena)) != PA_OK) return st;
+    PA_HIP(dev_alloc(pd.rowcodes, p.n_rowcodes * 4, arena), "device allocation");
+    PA_HIP(dev_alloc(pd.colcodes, p.n_colcodes * 4, arena), "device allocation");
     pd.code_host.resize(p.launches.size());
     for (size_t l = 0; l < p.launches.size(); ++l) pa::code_tables(p, p.launches[l].key, pd.code_host[l]);
-    if ((st = upload(ctx, pd.codes, pd.code_host)) != PA_OK) return st;
+    if ((st = upload(ctx, pd.codes, pd.code_host, arena)) != PA_OK) return st;
     for (size_t l = 0; l < p.launches.size(); ++l) {
         const pa::Launch &L = p.launches[l];
         pa::EncodeArgs ea{};
@@ -176,15 +203,13 @@ pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd, const DeviceBuffer &residues)
         if (e != hipSuccess) return hip_fail(ctx, e, "encode launch");
     }
     for (int v = 0; v < pa::kScoreShifts; ++v) {
-        if ((st = upload(ctx, pd.table[v], pd.plan.table[v])) != PA_OK) return st;
-        if ((st = upload(ctx, pd.ptable[v], pd.plan.ptable[v])) != PA_OK) return st;
-        if ((st = upload(ctx, pd.profile[v], pd.plan.profile[v])) != PA_OK) return st;
+        if ((st = upload(ctx, pd.table[v], pd.plan.table[v], arena)) != PA_OK) return st;
+        if ((st = upload(ctx, pd.ptable[v], pd.plan.ptable[v], arena)) != PA_OK) return st;
+        if ((st = upload(ctx, pd.profile[v], pd.plan.profile[v], arena)) != PA_OK) return st;
     }
-    const size_t bnd_bytes = std::max<size_t>(pd.plan.bnd_entries * 8, 16);
-    PA_HIP(hipMalloc(&pd.bnd.ptr, bnd_bytes), "hipMalloc");
-    pd.bnd.bytes = bnd_bytes;
+    PA_HIP(dev_alloc(pd.bnd, pd.plan.bnd_entries * 8, arena), "device allocation");
     if (!pd.subset_host.empty())
-        if ((st = upload(ctx, pd.subset, pd.subset_host)) != PA_OK) return st;
+        if ((st = upload(ctx, pd.subset, pd.subset_host, arena)) != PA_OK) return st;
     return PA_OK;
 }
 
@@ -335,7 +360,7 @@ pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
         pa_config cfg = b->cfg;
         if ((st = pa::make_plan(cfg, view, forced.data(), &sat, pd->plan, err)) != PA_OK) return fail(ctx, st, err);
         pd->subset_host = sat;
-        if ((st = upload_plan(ctx, *pd, b->residues)) != PA_OK) return st;
+        if ((st = upload_plan(ctx, *pd, b->residues, b->arena)) != PA_OK) return st;
         b->recompute = std::move(pd);
         b->recompute_set = sat;
     }
@@ -471,6 +496,8 @@ pa_status pa_ctx_create(pa_ctx **out, int device)
     if (hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess) return PA_ERR_DEVICE;
     for (auto &ev : ctx->join)
         if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return PA_ERR_DEVICE;
+    // the address of this device's copy of pa_score_batch_noalloc's scratch (pa_noalloc.hip)
+    if (pa::noalloc_setup(device) != hipSuccess) return PA_ERR_DEVICE;
     *out = ctx.release();
     return PA_OK;
 }
@@ -494,12 +521,14 @@ const char *pa_ctx_last_error(const pa_ctx *ctx) { return ctx ? ctx->last_error.
 // seq2[p]) for every p.
 static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
                               bool one_query, const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs,
-                              const uint8_t *pair_method, pa_batch **out, bool export_dp = false)
+                              const uint8_t *pair_method, pa_batch **out, bool export_dp = false,
+                              Arena *arena = nullptr)
 {
     *out = nullptr;
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PA_ERR_DEVICE, "hipSetDevice");
     auto t0 = std::chrono::steady_clock::now();
     auto b = std::make_unique<pa_batch>();
+    b->arena = arena;
     b->cfg = *cfg;
     if (cfg->score_kind == PA_SCORE_MATRIX && cfg->matrix && cfg->matrix_symbols && cfg->matrix_dim > 0) {
         b->matrix.assign(cfg->matrix, cfg->matrix + size_t(cfg->matrix_dim) * cfg->matrix_dim);
@@ -529,7 +558,7 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
     // The residue bytes go to the device while the host plans (the upload needs nothing from
     // the plan); the helper thread is joined before anything else uses the stream.
     hipError_t upload_err = hipSuccess;
-    std::thread uploader([&] { upload_err = copy_residues(ctx->device, ctx->stream, view, b->residues); });
+    std::thread uploader([&] { upload_err = copy_residues(ctx->device, ctx->stream, view, b->residues, arena); });
     st = pa::make_plan(b->cfg, view, forced.empty() ? nullptr : forced.data(), nullptr, b->main.plan, err, false,
                        export_dp);
     uploader.join();
@@ -539,17 +568,24 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
     // Only int16 pairs whose bound does not exclude saturation need the flag read-back.
     b->can_saturate = p.n_speculative > 0;
 
-    if ((st = upload_plan(ctx, b->main, b->residues)) != PA_OK) return st;
+    if ((st = upload_plan(ctx, b->main, b->residues, arena)) != PA_OK) return st;
     if (export_dp) {
         b->dp_out.bytes = std::max<size_t>(p.dp_entries * 4, 16);
         PA_HIP(hipMalloc(&b->dp_out.ptr, b->dp_out.bytes), "hipMalloc");
     }
-    if ((st = upload(ctx, b->pairs, p.pairs)) != PA_OK) return st;
-    PA_HIP(hipMalloc(&b->scores.ptr, std::max<size_t>(n_pairs * 4, 16)), "hipMalloc");
-    PA_HIP(hipMalloc(&b->flags.ptr, (n_pairs + 1) * 4), "hipMalloc");   // + error slot
+    if ((st = upload(ctx, b->pairs, p.pairs, arena)) != PA_OK) return st;
+    PA_HIP(dev_alloc(b->scores, n_pairs * 4, arena), "device allocation");
+    PA_HIP(dev_alloc(b->flags, (n_pairs + 1) * 4, arena), "device allocation");   // + error slot
     // the initial-score kernel clears the flags and the error slot on every pass; a batch without
     // pairs launches nothing, so its error slot starts cleared here (outside any graph capture)
-    PA_HIP(hipMemsetAsync(b->flags.ptr, 0, (n_pairs + 1) * 4, ctx->stream), "hipMemsetAsync");
+    if (arena) {
+        static const uint32_t kZero = 0;
+        PA_HIP(hipMemcpyAsync(static_cast<uint32_t *>(b->flags.ptr) + n_pairs, &kZero, 4, hipMemcpyHostToDevice,
+                              ctx->stream),
+               "hipMemcpyAsync");
+    } else {
+        PA_HIP(hipMemsetAsync(b->flags.ptr, 0, (n_pairs + 1) * 4, ctx->stream), "hipMemsetAsync");
+    }
     PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 
     b->stats.n_pairs = n_pairs;
@@ -966,6 +1002,44 @@ pa_status pa_score_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1,
     if (st == PA_OK) st = pa_batch_results(ctx, b, scores, flags);
     pa_batch_destroy(ctx, b);
     return st;
+}
+
+pa_status pa_score_batch_noalloc(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                                 const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs,
+                                 const uint8_t *pair_method, int32_t *scores, uint8_t *flags)
+{
+    if (!ctx || !cfg || !off1 || !off2 || (n_pairs && (!seq1 || !seq2 || !scores)))
+        return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
+    if (cfg->lane_policy == PA_LANES_LEARNED)
+        return fail(ctx, PA_ERR_UNSUPPORTED, "pa_score_batch_noalloc: PA_LANES_LEARNED allocates its features; "
+                                             "use another lane policy");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PA_ERR_DEVICE, "hipSetDevice");
+    void *base = pa::noalloc_scratch(ctx->device);
+    if (!base) return fail(ctx, PA_ERR_DEVICE, "pa_score_batch_noalloc: no scratch on this device");
+    // calls on one device share its scratch: one at a time
+    std::lock_guard<std::mutex> lock(pa::noalloc_call_mutex(ctx->device));
+    Arena arena{static_cast<char *>(base), pa::kNoallocScratchBytes, 0, 0};
+    // eager launches: instantiating a graph is left to callers that may allocate
+    pa_config c = *cfg;
+    pa_tuning t = cfg->tuning ? *cfg->tuning : pa_tuning{};
+    t.eager = 1;
+    c.tuning = &t;
+    auto too_big = [&](pa_status st) {
+        if (st == PA_ERR_OUT_OF_MEMORY && arena.refused)
+            return fail(ctx, st, "pa_score_batch_noalloc: the batch needs more than the " +
+                                     std::to_string(pa::kNoallocScratchBytes >> 20) + " MiB scratch (" +
+                                     std::to_string(arena.used) + " bytes in use, " + std::to_string(arena.refused) +
+                                     " more requested)");
+        return st;
+    };
+    pa_batch *b = nullptr;
+    pa_status st = create_batch(ctx, &c, seq1, off1, false, seq2, off2, n_pairs, pair_method, &b, false, &arena);
+    if (st != PA_OK) return too_big(st);
+    std::unique_ptr<pa_batch> hold(b);
+    if ((st = run_pass(ctx, b, false)) != PA_OK) return too_big(st);
+    PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if ((st = check_error_slot(ctx, b)) != PA_OK) return st;
+    return pa_batch_results(ctx, b, scores, flags);
 }
 
 pa_status pa_score_batch_dp(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,

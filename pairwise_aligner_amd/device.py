@@ -30,7 +30,7 @@ EXPORTS = ["pa_abi_version", "pa_ctx_create", "pa_ctx_destroy", "pa_ctx_last_err
            "pa_batch_destroy", "pa_model_create", "pa_model_destroy", "pa_model_param_count",
            "pa_model_set_params", "pa_model_get_params", "pa_features", "pa_model_forward", "pa_model_loss_grad",
            "pa_model_train_step", "pa_dp_labels", "pa_tune", "pa_score_one_to_many", "pa_batch_create_one_to_many",
-           "pa_score_batch_dp"]
+           "pa_score_batch_dp", "pa_score_batch_noalloc"]
 N_FEATURES, N_OUTPUTS, N_HIDDEN = 8, 2, 64
 
 
@@ -134,6 +134,7 @@ def lib() -> ctypes.CDLL:
     L.pa_score_one_to_many.argtypes = [vp, ctypes.POINTER(CConfig), vp, u64, vp, vp, u64, vp, vp, vp]
     L.pa_batch_create_one_to_many.argtypes = [vp, ctypes.POINTER(CConfig), vp, u64, vp, vp, u64, vp,
                                               ctypes.POINTER(vp)]
+    L.pa_score_batch_noalloc.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp, vp]
     L.pa_score_batch_dp.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pa_batch_run.argtypes = [vp, vp]
     L.pa_batch_results.argtypes = [vp, vp, vp, vp]
@@ -155,7 +156,8 @@ def lib() -> ctypes.CDLL:
                  "pa_batch_results", "pa_batch_stats_get", "pa_batch_time", "pa_batch_destroy", "pa_model_create",
                  "pa_model_destroy", "pa_model_set_params", "pa_model_get_params", "pa_features",
                  "pa_model_forward", "pa_model_loss_grad", "pa_model_train_step", "pa_dp_labels", "pa_tune",
-                 "pa_score_one_to_many", "pa_batch_create_one_to_many", "pa_score_batch_dp"]:
+                 "pa_score_one_to_many", "pa_batch_create_one_to_many", "pa_score_batch_dp",
+                 "pa_score_batch_noalloc"]:
         getattr(L, name).restype = i32
     return L
 
@@ -275,6 +277,21 @@ class Context:
         st = self._lib.pa_score_batch(self.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
                                       b2.ctypes.data, o2.ctypes.data, n, None if pm is None else pm.ctypes.data,
                                       scores.ctypes.data, flags.ctypes.data)
+        self.check(st)
+        return scores[:n], flags[:n]
+
+    def score_packed_noalloc(self, b1, o1, b2, o2, cfg: DeviceConfig, pair_method=None):
+        """As score_packed, through pa_score_batch_noalloc: no device memory is allocated; every
+        device buffer is a slice of the library's 64 MiB per-device scratch."""
+        b1, o1, b2, o2, pm = checked_batch(b1, o1, b2, o2, pair_method)
+        n = len(o1) - 1
+        holder = _CConfigHolder(cfg)
+        scores = np.zeros(max(n, 1), dtype=np.int32)
+        flags = np.zeros(max(n, 1), dtype=np.uint8)
+        st = self._lib.pa_score_batch_noalloc(self.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
+                                              b2.ctypes.data, o2.ctypes.data, n,
+                                              None if pm is None else pm.ctypes.data, scores.ctypes.data,
+                                              flags.ctypes.data)
         self.check(st)
         return scores[:n], flags[:n]
 
