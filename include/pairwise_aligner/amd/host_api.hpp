@@ -25,8 +25,9 @@
 //   * score() is int32_t and exact for every score model, including the *_simd ones whose
 //     reference lanes wrap on overflow (simd_score_type.hpp:164-192); saturated int16 lanes
 //     are recomputed in int32;
-//   * dp_column()/dp_row() hold the scores of the final DP column and row only after the
-//     four-argument compute(seq1, seq2, first_dp_column, first_dp_row); they are empty otherwise.
+//   * dp_column()/dp_row() hold the scores of the final DP column and row only after a
+//     four-argument compute (one pair, one-to-one bulk or one-to-many); they are empty otherwise;
+//     a bulk result's dp_column()/dp_row() is its own pair's, not the whole bulk's.
 #pragma once
 
 #include <array>
@@ -127,6 +128,11 @@ concept substitution_table = requires(t const &m) {
 
 } // namespace detail
 
+// configuration/initial.hpp:74: the rule every chain starts from.  Every rule below can take it
+// (or no predecessor at all) in place of a predecessor rule.
+using initial_t = detail::initial_rule;
+inline constexpr initial_t initial{};
+
 // score_model_unitary.hpp:209, score_model_unitary_simd.hpp:230,
 // score_model_unitary_simd_saturated.hpp:284
 struct score_model_unitary_fn {
@@ -204,6 +210,12 @@ struct method_global_fn {
         return detail::rule<detail::rule_category::method, payload_t, std::remove_cvref_t<pred_t>>{
             payload_t{false, leading, trailing}, pred};
     }
+
+    // method_global.hpp:137-140: the method as the first rule of a chain
+    constexpr auto operator()(leading_end_gap leading, trailing_end_gap trailing) const
+    {
+        return (*this)(initial, leading, trailing);
+    }
 };
 
 inline constexpr method_global_fn method_global{};
@@ -219,6 +231,9 @@ struct method_local_fn {
                       trailing_end_gap{end_gap::free, end_gap::free}},
             pred};
     }
+
+    // method_local.hpp:129-132: the method as the first rule of a chain
+    constexpr auto operator()() const { return (*this)(initial); }
 };
 
 inline constexpr method_local_fn method_local{};
@@ -299,6 +314,9 @@ struct bulk_result {
     bulk2_t _sequences2;
     std::vector<int32_t> _scores;
     std::vector<uint8_t> _flags;
+    // per pair: the final DP column and row (four-argument compute only)
+    std::vector<std::vector<int32_t>> _dp_columns{};
+    std::vector<std::vector<int32_t>> _dp_rows{};
 };
 
 template <typename bulk_result_t, bool one_to_many>
@@ -324,6 +342,25 @@ public:
     }
     int32_t score() const noexcept { return _result->_scores[_index]; }
     uint8_t flags() const noexcept { return _result->_flags[_index]; }
+
+    // aligner_result_bulk.hpp:43-51.  The reference returns the bulk's whole DP column / row
+    // (one SIMD vector per position); here each pair's own final column H(i, n), i = 0..m, and
+    // row H(m, j), j = 0..n, filled by the four-argument compute and empty otherwise.
+    std::vector<int32_t> const &dp_column() const & noexcept
+    {
+        return _result->_dp_columns.empty() ? none() : _result->_dp_columns[_index];
+    }
+    std::vector<int32_t> const &dp_row() const & noexcept
+    {
+        return _result->_dp_rows.empty() ? none() : _result->_dp_rows[_index];
+    }
+
+private:
+    static std::vector<int32_t> const &none() noexcept
+    {
+        static const std::vector<int32_t> empty{};
+        return empty;
+    }
 };
 
 // The configured aligner: a pa_config plus a device context (interface_one_to_one_*.hpp).
@@ -362,6 +399,29 @@ class aligner {
         static const uint8_t empty = 0;
         check(pa_score_batch(_ctx.get(), &cfg, b1.empty() ? &empty : b1.data(), o1.data(),
                              b2.empty() ? &empty : b2.data(), o2.data(), n, nullptr, scores.data(), flags.data()));
+    }
+
+    // scores plus every pair's final DP column and row: pa_score_batch_dp, sliced per pair
+    void run_dp(std::vector<uint8_t> const &b1, std::vector<uint64_t> const &o1, std::vector<uint8_t> const &b2,
+                std::vector<uint64_t> const &o2, std::vector<int32_t> &scores, std::vector<uint8_t> &flags,
+                std::vector<std::vector<int32_t>> &columns, std::vector<std::vector<int32_t>> &rows)
+    {
+        const uint64_t n = o1.size() - 1;
+        scores.assign(n, 0);
+        flags.assign(n, 0);
+        std::vector<int32_t> col(b1.size() + n + 1), row(b2.size() + n + 1);
+        const pa_config cfg = device_config();
+        static const uint8_t empty = 0;
+        check(pa_score_batch_dp(_ctx.get(), &cfg, b1.empty() ? &empty : b1.data(), o1.data(),
+                                b2.empty() ? &empty : b2.data(), o2.data(), n, nullptr, scores.data(), flags.data(),
+                                col.data(), row.data()));
+        columns.resize(n);
+        rows.resize(n);
+        for (uint64_t p = 0; p < n; ++p) {
+            const int32_t *c = col.data() + o1[p] + p, *r = row.data() + o2[p] + p;
+            columns[p].assign(c, c + (o1[p + 1] - o1[p]) + 1);
+            rows[p].assign(r, r + (o2[p + 1] - o2[p]) + 1);
+        }
     }
 
     // one query against every target: pa_score_one_to_many (query profile on the device)
@@ -445,6 +505,50 @@ public:
         std::vector<aligner_result_bulk<result_t, false>> results;
         results.reserve(o1.size() - 1);
         for (std::size_t i = 0; i + 1 < o1.size(); ++i) results.emplace_back(shared, i);
+        return results;
+    }
+
+    // one-to-one bulk with the initial DP column and row (interface_one_to_one_bulk.hpp:73-100):
+    // as for one pair, the boundary is the configuration's; every result carries its pair's
+    // final DP column and row (dp_column() / dp_row())
+    template <detail::sequence_bulk bulk1_t, detail::sequence_bulk bulk2_t, typename dp_column_t, typename dp_row_t>
+    auto compute(bulk1_t &&bulk1, bulk2_t &&bulk2, dp_column_t &&, dp_row_t &&)
+    {
+        std::vector<uint8_t> b1, b2;
+        std::vector<uint64_t> o1{0}, o2{0};
+        for (auto &&s : bulk1) detail::append(b1, o1, s);
+        for (auto &&s : bulk2) detail::append(b2, o2, s);
+        if (o1.size() != o2.size()) throw std::invalid_argument("bulk compute needs equally many sequences");
+        using result_t = bulk_result<bulk1_t, bulk2_t>;
+        auto shared = std::make_shared<result_t>(result_t{std::forward<bulk1_t>(bulk1), std::forward<bulk2_t>(bulk2),
+                                                          {}, {}});
+        run_dp(b1, o1, b2, o2, shared->_scores, shared->_flags, shared->_dp_columns, shared->_dp_rows);
+        std::vector<aligner_result_bulk<result_t, false>> results;
+        results.reserve(o1.size() - 1);
+        for (std::size_t i = 0; i + 1 < o1.size(); ++i) results.emplace_back(shared, i);
+        return results;
+    }
+
+    // one-to-many with the initial DP column and row (interface_one_to_many_bulk.hpp:67-77): the
+    // query scored against every target, each result with its pair's final DP column and row
+    template <detail::sequence sequence1_t, detail::sequence_bulk bulk2_t, typename dp_column_t, typename dp_row_t>
+    auto compute(sequence1_t &&sequence1, bulk2_t &&bulk2, dp_column_t &&, dp_row_t &&)
+    {
+        std::vector<uint8_t> q, b1, b2;
+        std::vector<uint64_t> oq{0}, o1{0}, o2{0};
+        detail::append(q, oq, sequence1);
+        for (auto &&s : bulk2) {
+            detail::append(b2, o2, s);
+            b1.insert(b1.end(), q.begin(), q.end());
+            o1.push_back(b1.size());
+        }
+        using result_t = bulk_result<sequence1_t, bulk2_t>;
+        auto shared = std::make_shared<result_t>(result_t{std::forward<sequence1_t>(sequence1),
+                                                          std::forward<bulk2_t>(bulk2), {}, {}});
+        run_dp(b1, o1, b2, o2, shared->_scores, shared->_flags, shared->_dp_columns, shared->_dp_rows);
+        std::vector<aligner_result_bulk<result_t, true>> results;
+        results.reserve(o2.size() - 1);
+        for (std::size_t i = 0; i + 1 < o2.size(); ++i) results.emplace_back(shared, i);
         return results;
     }
 

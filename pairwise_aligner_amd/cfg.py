@@ -53,9 +53,21 @@ class _Rule:
             r = r.predecessor
 
 
+class _Initial:
+    """initial.hpp:74: the rule a chain starts from (passing it is the same as no predecessor)."""
+
+    def __repr__(self):
+        return "cfg.initial"
+
+
+initial = _Initial()
+
+
 def _split(args):
     if args and isinstance(args[0], _Rule):
         return args[0], args[1:]
+    if args and isinstance(args[0], _Initial):
+        return None, args[1:]
     return None, args
 
 
@@ -133,12 +145,20 @@ INT32_MIN = -(2 ** 31)
 
 
 class aligner_result:
-    """aligner_result.hpp:33-105: the two sequences and the score."""
+    """aligner_result.hpp:33-105: the two sequences, the score and, after the four-argument
+    compute, the final DP column H(i, n), i = 0..m, and row H(m, j), j = 0..n (empty otherwise)."""
 
-    __slots__ = ("_s1", "_s2", "_score", "_flags")
+    __slots__ = ("_s1", "_s2", "_score", "_flags", "_dp_column", "_dp_row")
 
-    def __init__(self, s1, s2, score, flags=0):
+    def __init__(self, s1, s2, score, flags=0, dp_column=(), dp_row=()):
         self._s1, self._s2, self._score, self._flags = s1, s2, int(score), int(flags)
+        self._dp_column, self._dp_row = list(dp_column), list(dp_row)
+
+    def dp_column(self) -> list:
+        return self._dp_column
+
+    def dp_row(self) -> list:
+        return self._dp_row
 
     def score(self) -> int:
         return self._score
@@ -169,11 +189,16 @@ class Aligner:
     def _ctx(self):
         return _dev.default_context(self.device)
 
-    def compute(self, seq1, seq2):
+    def compute(self, seq1, seq2, first_dp_column=None, first_dp_row=None):
         """compute(s1, s2) -> one result; compute(bulk1, bulk2) -> list; compute(s1, bulk2) -> list
-        (one against many)."""
+        (one against many).  With the initial DP column and row (interface_one_to_one_single.hpp:52-67,
+        interface_one_to_one_bulk.hpp:73-100, interface_one_to_many_bulk.hpp:67-77) every result also
+        carries its final DP column and row; like the reference, the boundary itself comes from the
+        configuration."""
         single1 = isinstance(seq1, (str, bytes, bytearray))
         single2 = isinstance(seq2, (str, bytes, bytearray))
+        if first_dp_column is not None or first_dp_row is not None:
+            return self._compute_dp(seq1, seq2, single1, single2)
         if single1 and single2:
             return self.compute([seq1], [seq2])[0]
         if single1:
@@ -188,6 +213,25 @@ class Aligner:
             raise ValueError("bulk compute needs the same number of sequences on both sides")
         scores, flags = self._ctx().score(s1, s2, self.config)
         return [aligner_result(a, b, sc, fl) for a, b, sc, fl in zip(seq1, seq2, scores, flags)]
+
+
+    def _compute_dp(self, seq1, seq2, single1, single2):
+        if single1 and single2:
+            return self._compute_dp([seq1], [seq2], False, False)[0]
+        firsts = [seq1] * len(seq2) if single1 else list(seq1)
+        s1 = [_as_bytes(s) for s in firsts]
+        s2 = [_as_bytes(s) for s in seq2]
+        if len(s1) != len(s2):
+            raise ValueError("bulk compute needs the same number of sequences on both sides")
+        b1, o1 = _dev.pack_sequences(s1)
+        b2, o2 = _dev.pack_sequences(s2)
+        scores, flags, col, row = self._ctx().score_packed_dp(b1, o1, b2, o2, self.config)
+        out = []
+        for p, (a, b) in enumerate(zip(firsts, seq2)):
+            c0, r0 = int(o1[p]) + p, int(o2[p]) + p
+            out.append(aligner_result(a, b, scores[p], flags[p], col[c0:c0 + len(s1[p]) + 1].tolist(),
+                                      row[r0:r0 + len(s2[p]) + 1].tolist()))
+        return out
 
 
 def configure_aligner(rule: _Rule, device: int = 0) -> Aligner:

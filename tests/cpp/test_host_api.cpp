@@ -15,6 +15,7 @@
 #include <pairwise_aligner/amd/fastx.hpp>
 #include <pairwise_aligner/configuration/configure_aligner.hpp>
 #include <pairwise_aligner/configuration/gap_model_affine.hpp>
+#include <pairwise_aligner/configuration/initial.hpp>
 #include <pairwise_aligner/configuration/method_global.hpp>
 #include <pairwise_aligner/configuration/method_local.hpp>
 #include <pairwise_aligner/configuration/score_model_matrix.hpp>
@@ -87,6 +88,30 @@ static void config_checks()
     EXPECT_EQ(s.config().lane_policy, PA_LANES_INT16);
     auto d = aligner::cfg::configure_aligner(aligner::cfg::gap_model_affine(aligner::cfg::score_model_unitary(1, -1), -2, -1));
     EXPECT_EQ(d.config().method, PA_METHOD_GLOBAL);   // default method
+
+    // chains that start from cfg::initial or from a method without predecessor
+    // (initial.hpp:74, method_global.hpp:137-140, method_local.hpp:129-132)
+    using aligner::cfg::end_gap;
+    auto l = aligner::cfg::configure_aligner(aligner::cfg::score_model_unitary(
+        aligner::cfg::gap_model_affine(aligner::cfg::method_local(), -10, -1), 4, -5));
+    EXPECT_EQ(l.config().method, PA_METHOD_LOCAL);
+    EXPECT_EQ(l.config().gap_open, -10);
+    EXPECT_EQ(l.config().match, 4);
+    auto g = aligner::cfg::configure_aligner(aligner::cfg::score_model_unitary(
+        aligner::cfg::gap_model_affine(aligner::cfg::method_global(aligner::cfg::leading_end_gap{end_gap::free, end_gap::penalised},
+                                                                   aligner::cfg::trailing_end_gap{end_gap::penalised, end_gap::free}),
+                                       -10, -1),
+        4, -5));
+    EXPECT_EQ(g.config().method, PA_METHOD_GLOBAL);
+    EXPECT_EQ(g.config().first_column_free, 1);
+    EXPECT_EQ(g.config().first_row_free, 0);
+    EXPECT_EQ(g.config().last_column_free, 0);
+    EXPECT_EQ(g.config().last_row_free, 1);
+    auto i = aligner::cfg::configure_aligner(
+        aligner::cfg::method_local(aligner::cfg::gap_model_affine(aligner::cfg::score_model_unitary(aligner::cfg::initial, 2, -1), -1, -1)));
+    EXPECT_EQ(i.config().method, PA_METHOD_LOCAL);
+    EXPECT_EQ(i.config().mismatch, -1);
+    static_assert(std::is_same_v<aligner::cfg::initial_t, std::remove_cvref_t<decltype(aligner::cfg::initial)>>);
 }
 
 static void known_answers()
@@ -126,6 +151,19 @@ static void known_answers()
     EXPECT_EQ(score_of(score_model_matrix(local, aligner::blosum62_extended<>), "ALIGATOR"sv, "GALORA"sv), 13);
     EXPECT_EQ(score_of(score_model_matrix(local, aligner::blosum62_extended<>), "ALIGATOR"sv, ""sv),
               std::numeric_limits<int32_t>::lowest());
+    // the same through chains that start from the method rule or from cfg::initial
+    EXPECT_EQ(score_of(aligner::cfg::score_model_unitary(aligner::cfg::gap_model_affine(aligner::cfg::method_local(), -10, -1), 4, -5),
+                       "TTTTTTTTTTTTTTT"sv, "TTTTTTTTTTTTTTT"sv), 60);
+    EXPECT_EQ(score_of(aligner::cfg::score_model_unitary(
+                           aligner::cfg::gap_model_affine(
+                               aligner::cfg::method_global(
+                                   aligner::cfg::leading_end_gap{aligner::cfg::end_gap::free, aligner::cfg::end_gap::free},
+                                   aligner::cfg::trailing_end_gap{aligner::cfg::end_gap::free, aligner::cfg::end_gap::free}),
+                               -10, -1),
+                           4, -5),
+                       "TTTTTCTGACACACT"sv, "CTGCACACGACGGGGGG"sv), 16);
+    EXPECT_EQ(score_of(aligner::cfg::gap_model_affine(aligner::cfg::score_model_unitary(aligner::cfg::initial, 4, -5), -10, -1),
+                       "ACGTGACTGACACTACGACT"sv, "ACGTGACTGACACTACGACT"sv), 80);
     // configuration_test.cpp:18-37
     auto proto = aligner::cfg::configure_aligner(aligner::cfg::method_global(
         aligner::cfg::gap_model_affine(aligner::cfg::score_model_unitary(4, -5), -10, -1),
@@ -185,6 +223,36 @@ void dp_vectors()
     auto e = a.compute(std::string{""}, std::string{"CC"}, unused_column, unused_row);
     EXPECT_EQ(e.dp_row() == (std::vector<int32_t>{0, -11, -12}), true);
     EXPECT_EQ(e.dp_column() == (std::vector<int32_t>{-12}), true);
+
+    // bulk and one-to-many forms (interface_one_to_one_bulk.hpp:73-100,
+    // interface_one_to_many_bulk.hpp:67-77; aligner_result_bulk.hpp:43-51): each result carries its
+    // own pair's vectors, equal to the single-pair call's
+    std::vector<std::string> s1{"ACGT", "", "ACGTACGGT", "TTAG", "GATTACA"}, s2{"AGT", "CC", "AGGT", "", "GATCA"};
+    for (auto const &cfg_a : {a, aligner::cfg::configure_aligner(aligner::cfg::score_model_unitary(local, 4, -5))}) {
+        auto b = cfg_a;
+        auto bulk = b.compute(s1, s2, unused_column, unused_row);
+        EXPECT_EQ(bulk.size(), s1.size());
+        for (std::size_t k = 0; k < s1.size(); ++k) {
+            auto one = b.compute(s1[k], s2[k], unused_column, unused_row);
+            EXPECT_EQ(bulk[k].score(), one.score());
+            EXPECT_EQ(bulk[k].dp_column() == one.dp_column(), true);
+            EXPECT_EQ(bulk[k].dp_row() == one.dp_row(), true);
+            EXPECT_EQ(bulk[k].dp_column().size(), s1[k].size() + 1);
+        }
+        auto many = b.compute(s1[0], s2, unused_column, unused_row);
+        EXPECT_EQ(many.size(), s2.size());
+        for (std::size_t k = 0; k < s2.size(); ++k) {
+            auto one = b.compute(s1[0], s2[k], unused_column, unused_row);
+            EXPECT_EQ(many[k].score(), one.score());
+            EXPECT_EQ(many[k].dp_column() == one.dp_column(), true);
+            EXPECT_EQ(many[k].dp_row() == one.dp_row(), true);
+        }
+    }
+    auto bulk = a.compute(s1, s2, unused_column, unused_row);
+    EXPECT_EQ(bulk[0].dp_row() == (std::vector<int32_t>{-14, -9, -13, 1}), true);
+    EXPECT_EQ(bulk[0].dp_column() == (std::vector<int32_t>{-13, -8, -12, -6, 1}), true);
+    auto plain = a.compute(s1, s2);
+    EXPECT_EQ(plain[0].dp_column().empty() && plain[0].dp_row().empty(), true);
 }
 
 // The reference's FASTQ fixture (test/data/in.fastq), as text and as tests/golden/in.fastq.

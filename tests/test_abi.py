@@ -49,6 +49,19 @@ def test_configure_aligner_translation():
     assert loc.symbols == b"ABCDEFGHIJKLMNOPQRSTUVWXYZ*" and len(loc.matrix) == 27 * 27
 
 
+def test_chains_from_initial_and_from_the_method():
+    """cfg.initial (initial.hpp:74) and methods without a predecessor (method_global.hpp:137-140,
+    method_local.hpp:129-132) configure like the chains with the rule in the middle."""
+    a = cfg.configure_aligner(cfg.score_model_unitary(cfg.gap_model_affine(cfg.method_local(), -10, -1), 4, -5)).config
+    b = cfg.configure_aligner(cfg.method_local(cfg.gap_model_affine(cfg.score_model_unitary(cfg.initial, 4, -5),
+                                                                    -10, -1))).config
+    assert a == b and a.method == device.METHOD_LOCAL
+    g = cfg.configure_aligner(cfg.score_model_unitary(cfg.gap_model_affine(
+        cfg.method_global(cfg.leading_end_gap(first_column=cfg.end_gap.free), cfg.trailing_end_gap()), -2, -1),
+        1, -1)).config
+    assert (g.method, g.first_column_free, g.first_row_free, g.gap_open) == (device.METHOD_GLOBAL, True, False, -2)
+
+
 def test_no_gpu_means_a_loud_error():
     if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU driver is present")

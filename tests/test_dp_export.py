@@ -100,3 +100,33 @@ def test_long_and_empty(ctx):
     for lr in (False, True):
         dc = device.DeviceConfig(match=1, mismatch=-4, gap_open=-6, gap_extension=-1, last_row_free=lr)
         check(ctx, s1, s2, dc, label=f"long and empty, last row free {lr}")
+
+
+def test_configured_aligner_bulk_and_one_to_many_vectors():
+    """The four-argument compute of the configured aligner (cfg mirror of the reference's
+    interface_one_to_one_bulk.hpp:73-100 and interface_one_to_many_bulk.hpp:67-77): every bulk
+    result carries its own pair's final DP column and row, equal to the oracle's; chains start
+    from the method rule and from cfg.initial."""
+    from pairwise_aligner_amd import cfg
+    rng = np.random.default_rng(77)
+    s1, s2 = dna_batch(rng, 60, 90)
+    rule = cfg.score_model_unitary(cfg.gap_model_affine(
+        cfg.method_global(cfg.leading_end_gap(first_row=cfg.end_gap.free),
+                          cfg.trailing_end_gap(last_column=cfg.end_gap.free)), -10, -1), 4, -5)
+    a = cfg.configure_aligner(rule)
+    dc = a.config
+    for firsts, results in ((s1, a.compute(s1, s2, [], [])), ([s1[0]] * len(s2), a.compute(s1[0], s2, [], []))):
+        b1, o1 = workloads.pack(firsts)
+        b2, o2 = workloads.pack(s2)
+        want_s, want_rows, want_cols = O.dp_vectors_packed(b1, o1, b2, o2, oracle_cfg(dc), threads=8)
+        for p, r in enumerate(results):
+            assert r.score() == want_s[p]
+            c0, r0 = int(o1[p]) + p, int(o2[p]) + p
+            assert r.dp_column() == want_cols[c0:c0 + len(firsts[p]) + 1].tolist(), p
+            assert r.dp_row() == want_rows[r0:r0 + len(s2[p]) + 1].tolist(), p
+    loc = cfg.configure_aligner(cfg.method_local(cfg.gap_model_affine(cfg.score_model_unitary(cfg.initial, 4, -5),
+                                                                      -10, -1)))
+    one = loc.compute(s1[3], s2[3], [], [])
+    want_s, want_rows, want_cols = O.dp_vectors_packed(*workloads.pack([s1[3]]), *workloads.pack([s2[3]]),
+                                                       oracle_cfg(loc.config))
+    assert one.score() == want_s[0] and one.dp_row() == want_rows.tolist() and one.dp_column() == want_cols.tolist()
