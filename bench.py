@@ -1,12 +1,21 @@
-"""Benchmark: GCUPS of batched affine-gap score-only DP on one MI355X, with the CPU oracle timed
+"""Benchmark: GCUPS of batched affine-gap score-only DP on one MI355X, with a CPU baseline timed
 beside it on the host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload protein|dna150|uneven|long|mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mixed|dna150|protein|uneven|long|
+                    long_int16|protein_1xn] [--mixed-pairs N] [--no-extra] [--e2e-reps R] [--pairs N]
 
-A step is one pass of the DP over the whole batch (all pairs of the workload), with inputs
-already resident in HBM.  `value` is GCUPS = sum over pairs of m*n / step time / 1e9 (interior
-cells, as the reference's benchmark fixture counts them).  The line printed last is the result:
-one JSON object.
+A step is one pass of the DP over the whole batch (every pair of the workload), inputs already
+resident in HBM.  GCUPS = sum over pairs of m*n / step time / 1e9 (interior cells, as the
+reference's benchmark fixture counts them).
+
+Default run: the headline is the mixed DNA/protein stream (500,000 pairs, lengths 100-5,000,
+identity 50-100%, both halves local) under the learned lane dispatcher -- the metric's "learned
+dispatch vs fixed int32" -- with the same pairs under fixed int32 lanes, int16 speculation and the
+length-bound policy beside it.  Every other configuration follows in `configs`: DNA 150 nt global
+(10,000 pairs), protein local (100,000 pairs), the uneven 1,000,000-pair batch (autotuned, graph
+replay), the 2,000 long pairs (int32 as configured, and re-centred int16), and one protein query
+against 200,000 targets; each with its roofline and a sampled CPU baseline.  The line printed
+last is the result: one JSON object.
 """
 from __future__ import annotations
 
@@ -24,15 +33,23 @@ sys.path.insert(0, ROOT)
 
 from pairwise_aligner_amd import device, tables, workloads  # noqa: E402
 
+METRIC = "GCUPS score-only affine DP on 1 GPU vs CPU ref; learned dispatch vs fixed int32"
 # MI355X peaks: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz; packed int16 issues two ops per lane.
 PEAK_INT16_TOPS = 256 * 4 * 16 * 2.4e9 * 2 / 1e12   # 78.6
 PEAK_INT32_TOPS = PEAK_INT16_TOPS / 2                # 39.3
-PEAK_HBM_GBS = 8000.0
 OPS_PER_CELL = {"global": 8, "local": 10}
 COPY_PROBE_BYTES = 256 << 20
+# CPU threads of the baseline: the host cores visible to this process, at most 16 (one GPU's
+# share of a GPU host's cores).
+MAX_CPU_THREADS = 16
 # HBM bytes per pass measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one run each),
 # scaled by the counters' reading for a plain copy of known size; see profiles/.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def log(msg):
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
 
 
 def measured_traffic(name, digest):
@@ -43,49 +60,28 @@ def measured_traffic(name, digest):
         return None
     if not t or t.get("inputs_sha256") != digest:
         return None
-    return t
+    return t["bytes_per_pass"]
 
 
-def log(msg):
-    sys.stderr.write(msg + "\n")
-    sys.stderr.flush()
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return dict(model=model, logical_cpus_visible=visible, os_cpu_count=os.cpu_count(),
+                threads_used=max(1, min(visible, MAX_CPU_THREADS)))
 
 
-def blosum_config(method, go, ge, policy):
+def blosum_config(method, go=-11, ge=-1, policy=device.LANES_AUTO):
     t = tables.blosum62_standard()
     return device.DeviceConfig(method=method, score_kind=device.SCORE_MATRIX,
                                symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
                                gap_open=go, gap_extension=ge, lane_policy=policy)
-
-
-def make_workload(name, pairs=None):
-    """The named workload; `pairs` overrides the pair count of the DNA 150 nt and protein workloads
-    (the batch-size sweep of the small DNA configuration)."""
-    count = {} if pairs is None else {"count": pairs}
-    if name == "dna150":
-        w = workloads.config_dna150(**count)
-        dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=4, mismatch=-5, gap_open=-10, gap_extension=-1)
-        return w, dc, "global"
-    if name == "protein":
-        w = workloads.config_protein(**count)
-        dc = blosum_config(device.METHOD_LOCAL, -11, -1, device.LANES_AUTO)
-        return w, dc, "local"
-    if name == "protein_1xn":
-        w = workloads.config_protein_1xn(**count)
-        dc = blosum_config(device.METHOD_LOCAL, -11, -1, device.LANES_AUTO)
-        return w, dc, "local"
-    if name == "uneven":
-        w = workloads.config_uneven()
-        dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
-        return w, dc, "mixed"
-    if name in ("long", "long_int16"):
-        # the configuration asks for int32 scores; long_int16 runs the same pairs in re-centred
-        # int16 lanes (the planner's default for long global pairs within the spread bound)
-        w = workloads.config_long()
-        dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=1, mismatch=-4, gap_open=-6, gap_extension=-1,
-                                 lane_policy=device.LANES_INT32 if name == "long" else device.LANES_AUTO)
-        return w, dc, "global"
-    raise SystemExit(f"unknown workload {name}")
 
 
 def oracle_cfg(dc):
@@ -96,117 +92,121 @@ def oracle_cfg(dc):
                           match=dc.match, mismatch=dc.mismatch, symbols=dc.symbols, matrix=dc.matrix)
 
 
-def cpu_baseline(w, dc, gpu_scores, budget_s=12.0):
+def cells_of(o1, o2):
+    return int(np.sum(np.diff(o1).astype(np.int64) * np.diff(o2).astype(np.int64)))
+
+
+def roofline(ops, cells, kernel_ms, peak, traffic=None):
+    achieved = ops * cells / (kernel_ms / 1e3) / 1e12 if kernel_ms > 0 else 0.0
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TOP/s",
+            "frac": round(achieved / peak, 4), "traffic": traffic, "ops_per_cell": ops, "kernel_ms": round(kernel_ms, 4)}
+
+
+# --------------------------------------------------------------------------------------------
+# CPU baseline
+# --------------------------------------------------------------------------------------------
+
+def cpu_baseline(halves, gpu_scores, budget_s=12.0, one_core=False, cell_cap=8e11):
     """Times the CPU baseline -- oracle/simd_port.cpp, the reference's inter-sequence SIMD scheme
-    (one pair per lane, int16 lanes where the bound allows) restated for AVX-512BW / AVX2 -- on
-    the host cores, on a bounded prefix of the same pairs, and compares the GPU scores of that
-    prefix with it.  A one-thread rate on a smaller prefix is reported beside it."""
+    (one pair per lane, 32 int16 / 16 int32 lanes per AVX-512 register, int16 where the bulk's
+    bound allows) restated for AVX-512BW / AVX2 -- on the host's cores, over a prefix of the same
+    pairs sized to about `budget_s` seconds, and compares the GPU scores of that prefix with it.
+    `halves`: [(b1, o1, b2, o2, device config, pair methods)], sampled alike (the mixed stream's
+    two halves).  The prefix has at least 32 pairs per thread (full bulks for every thread) unless
+    that exceeds `cell_cap` cells."""
     from oracle import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
-    b1, o1, b2, o2 = w["b1"], w["o1"], w["b2"], w["o2"]
-    n = len(o1) - 1
-    oc = oracle_cfg(dc)
-    pm_all = w.get("pair_method")
-    cells_cum = np.cumsum(np.diff(o1).astype(np.int64) * np.diff(o2).astype(np.int64))
+    cpu = host_cpu()
+    threads = cpu["threads_used"]
     isa = {512: "AVX-512BW", 256: "AVX2"}.get(O.simd_isa(), "none")
 
-    def timed(sample, nthreads):
-        pm = None if pm_all is None else pm_all[:sample]
-        t0 = time.perf_counter()
-        got = O.simd_score_packed(b1, o1[:sample + 1], b2, o2[:sample + 1], oc, pair_method=pm, threads=nthreads)
-        return got, time.perf_counter() - t0
+    def prefix(k_of_half, reps=1):
+        tot_cells, tot_s, got = 0, 0.0, []
+        for (b1, o1, b2, o2, dc, pm), k in zip(halves, k_of_half):
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                g = O.simd_score_packed(b1, o1[:k + 1], b2, o2[:k + 1], oracle_cfg(dc),
+                                        pair_method=None if pm is None else pm[:k], threads=nthreads)
+            got.append(g)
+            tot_s += time.perf_counter() - t0
+            tot_cells += reps * cells_of(o1[:k + 1], o2[:k + 1])
+        return got, tot_cells, tot_s
 
-    def sized(target_s, nthreads):
-        # calibrate on a prefix of about 1e8 cells, then size the sample to target_s seconds
-        cal = int(min(n, max(64 * nthreads, np.searchsorted(cells_cum, 1e8) + 1)))
-        got, dt = timed(cal, nthreads)
-        rate = cells_cum[cal - 1] / max(dt, 1e-6)
-        sample = int(min(n, max(cal, np.searchsorted(cells_cum, rate * target_s) + 1)))
-        if sample > cal:
-            got, dt = timed(sample, nthreads)
-        else:
-            sample = cal
-        return got, dt, sample
+    def first_k(o1, o2, cells):
+        cum = np.cumsum(np.diff(o1).astype(np.int64) * np.diff(o2).astype(np.int64))
+        return int(min(len(cum), max(1, np.searchsorted(cum, cells) + 1))), cum
 
-    got, dt, sample = sized(budget_s, threads)
-    cells = int(cells_cum[sample - 1])
-    _, dt1, sample1 = sized(budget_s / 4, 1)
-    cells1 = int(cells_cum[sample1 - 1])
-    mism = int(np.count_nonzero(gpu_scores[:sample].astype(np.int64) != got.astype(np.int64)))
-    return dict(value=cells / dt / 1e9, unit="GCUPS", cores=threads, kind="port",
-                sample=f"first {sample} of {n} pairs ({cells} cells) in {dt:.2f} s; inter-sequence SIMD port "
-                       f"of the reference (oracle/simd_port.cpp, {isa}, int16 lanes where the bound allows) "
-                       f"on {threads} threads",
-                value_1core=cells1 / dt1 / 1e9, sample_1core=f"first {sample1} pairs ({cells1} cells) in {dt1:.2f} s",
-                isa=isa, mismatches_vs_gpu=mism, sample_pairs=sample)
+    def sized(target_s):
+        # calibrate on a prefix of each half (about 1e8 cells, and full bulks for every thread
+        # unless that exceeds cell_cap), then scale every half's prefix to the time budget
+        ks = []
+        for b1, o1, b2, o2, _, _ in halves:
+            n = len(o1) - 1
+            k, cum = first_k(o1, o2, 1e8)
+            want = min(32 * nthreads, n)
+            k = max(k, want) if cum[want - 1] <= cell_cap else max(1, int(np.searchsorted(cum, cell_cap)))
+            ks.append(min(k, n))
+        got, cells, dt = prefix(ks)
+        if dt < target_s * 0.5:
+            scale = target_s / max(dt, 1e-6)
+            ks = [max(k, first_k(o1, o2, cells_of(o1[:k + 1], o2[:k + 1]) * scale)[0])
+                  for (b1, o1, b2, o2, _, _), k in zip(halves, ks)]
+            got, cells, dt = prefix(ks)
+        reps = 1
+        if dt < target_s * 0.5:   # the whole batch is sampled and takes less: repeat it
+            reps = int(min(1000, np.ceil(target_s * 0.5 / max(dt, 1e-6))))
+            got, cells, dt = prefix(ks, reps)
+        return got, cells, dt, ks, reps
+
+    nthreads = threads
+    got, cells, dt, ks, reps = sized(budget_s)
+    mism = 0
+    for g, s, k in zip(got, gpu_scores, ks):
+        mism += int(np.count_nonzero(s[:k].astype(np.int64) != g.astype(np.int64)))
+    n_total = sum(len(h[1]) - 1 for h in halves)
+    res = dict(value=round(cells / dt / 1e9, 3), unit="GCUPS", cores=threads, kind="port",
+               sample=(f"first {'+'.join(str(k) for k in ks)} of {n_total} pairs"
+                       f"{f', {reps} times' if reps > 1 else ''} ({cells} cells) in {dt:.2f} s; "
+                       f"inter-sequence SIMD port of the reference (oracle/simd_port.cpp, {isa}, int16 lanes where "
+                       f"the bound allows, int32 otherwise) on {threads} threads"),
+               isa=isa, host_cpu=cpu["model"], host_logical_cpus_visible=cpu["logical_cpus_visible"],
+               host_os_cpu_count=cpu["os_cpu_count"], mismatches_vs_gpu=mism)
+    if one_core:
+        nthreads = 1
+        _, cells1, dt1, ks1, reps1 = sized(budget_s / 4)
+        res["value_1core"] = round(cells1 / dt1 / 1e9, 3)
+        res["sample_1core"] = (f"first {'+'.join(str(k) for k in ks1)} pairs{f', {reps1} times' if reps1 > 1 else ''} "
+                               f"({cells1} cells) in {dt1:.2f} s")
+    return res
 
 
-def run(name, steps, warmup, with_cpu=True, autotune=False, pairs=None, e2e_reps=0):
-    t0 = time.perf_counter()
-    w, dc, method = make_workload(name, pairs)
-    n = len(w["o1"]) - 1
-    cells = int(np.sum(np.diff(w["o1"]).astype(np.int64) * np.diff(w["o2"]).astype(np.int64)))
-    log(f"[bench] {name}: {n} pairs, {cells:.3e} cells, generated in {time.perf_counter() - t0:.1f} s")
-    ctx = device.default_context(0)
-    tuned = None
-    if autotune:
-        # the library's autotuner times candidate planner/launch parameters on a 5% sample
-        t1 = time.perf_counter()
-        tuning = device.tune(ctx, dc, w["b1"], w["o1"], w["b2"], w["o2"], w.get("pair_method"),
-                             sample_pairs=max(2000, n // 20), iters=2)
-        dc = dataclasses.replace(dc, tuning=tuning)
-        tuned = dict(tuning=dataclasses.asdict(tuning), seconds=round(time.perf_counter() - t1, 2))
-        log(f"[bench] {name}: autotuned {tuned}")
-    one_to_one = None
-    if "query" in w:
-        # one query against many targets: the query-profile path, beside the one-to-one path
-        # on the same pairs (the query repeated)
-        ref = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
-        for _ in range(warmup):
-            ref.run()
-        ref_total, ref_main = ref.time(steps)
-        ref_scores, _ = ref.results()
-        ref.close()
-        one_to_one = dict(ms_per_step=ref_total / steps, main_kernel_ms=ref_main)
-        batch = device.Batch.one_to_many(ctx, w["query"], w["b2"], w["o2"], dc, w.get("pair_method"))
+# --------------------------------------------------------------------------------------------
+# GPU timing
+# --------------------------------------------------------------------------------------------
+
+def gpu_run(ctx, b1, o1, b2, o2, dc, pm, steps, warmup, query=None):
+    """Creates the device-resident batch, runs `warmup` untimed passes and times `steps` passes
+    (HIP events on the context stream: whole passes, and the main DP launches alone)."""
+    if query is not None:
+        batch = device.Batch.one_to_many(ctx, query, b2, o2, dc, pm)
     else:
-        batch = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
-    st = batch.stats()
-    log(f"[bench] plan: {st}")
+        batch = device.Batch(ctx, b1, o1, b2, o2, dc, pm)
     for _ in range(warmup):
         batch.run()
-    try:
-        import torch
-        torch.cuda.synchronize()
-    except Exception:
-        pass
-    t1 = time.perf_counter()
     total_ms, main_ms = batch.time(steps)
-    wall = time.perf_counter() - t1
     scores, flags = batch.results()
     st = batch.stats()
-    ms_step = total_ms / steps
-    gcups = cells / (ms_step / 1e3) / 1e9
-    ops = OPS_PER_CELL["local" if method == "local" else "global"] if method != "mixed" else 9
-    peak = PEAK_INT16_TOPS if dc.lane_policy != device.LANES_INT32 else PEAK_INT32_TOPS
-    achieved = ops * cells / (main_ms / 1e3) / 1e12 if main_ms > 0 else 0.0
-    res = dict(name=name, pairs=n, cells=cells, ms_per_step=ms_step, main_kernel_ms=main_ms, wall_s=wall,
-               gcups=gcups, achieved_tops=achieved, peak_tops=peak, ops_per_cell=ops, stats=st,
-               inputs_sha256=workloads.digest(w["b1"], w["o1"], w["b2"], w["o2"]), autotuned=tuned)
-    if one_to_one is not None:
-        one_to_one["scores_identical"] = bool(np.array_equal(ref_scores, scores))
-        one_to_one["speedup_of_profile"] = one_to_one["ms_per_step"] / ms_step
-        res["one_to_one"] = one_to_one
-    if with_cpu:
-        res["cpu"] = cpu_baseline(w, dc, scores)
-    log(f"[bench] {name}: {gcups:.1f} GCUPS ({ms_step:.3f} ms/step, main kernels {main_ms:.3f} ms), "
-        f"{achieved:.2f} of {peak:.1f} TOP/s")
     batch.close()
-    if e2e_reps:
-        res["end_to_end"] = end_to_end(ctx, w, dc, cells, scores, e2e_reps)
-        if "query" not in w:
-            res["end_to_end"]["from_fasta_files"] = from_files(ctx, w, dc, cells, scores, min(e2e_reps, 5))
-    return res
+    return dict(ms_per_step=total_ms / steps, kernel_ms=main_ms, scores=scores, flags=flags, stats=st)
+
+
+def plan_summary(st):
+    return {k: st[k] for k in ("n_int16", "n_int32", "n_recomputed", "padded_cells", "n_launches", "n_profiled")}
+
+
+def dtype_of(stats_list):
+    n16 = sum(s["n_int16"] for s in stats_list)
+    n32 = sum(s["n_int32"] for s in stats_list) + sum(s["n_recomputed"] for s in stats_list)
+    return "int16" if n32 == 0 else ("int32" if n16 == 0 else "int16+int32")
 
 
 def end_to_end(ctx, w, dc, cells, want_scores, reps):
@@ -223,10 +223,7 @@ def end_to_end(ctx, w, dc, cells, want_scores, reps):
     _, got = once()
     times = [once()[0] for _ in range(reps)]
     med = float(np.median(times))
-    if "query" in w:
-        b = device.Batch.one_to_many(ctx, w["query"], w["b2"], w["o2"], dc, w.get("pair_method"))
-    else:
-        b = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
+    b = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))
     plan_ms = b.stats()["plan_ms"]
     b.close()
     log(f"[bench] end to end: median {med * 1e3:.1f} ms over {reps} calls, planning {plan_ms:.1f} ms")
@@ -281,66 +278,198 @@ def from_files(ctx, w, dc, cells, want_scores, reps):
                 scores_identical=same_scores)
 
 
-def run_mixed(count, steps, warmup, with_oracle=True):
-    """Learned lane dispatch on the mixed workload: the same pairs under fixed int32 lanes, the
-    length-bound policy and the learned policy.  Both halves are aligned locally (DNA +2/-3,
-    gap (-5,-2); protein BLOSUM62, gap (-11,-1)).  The network is trained first, on labels the
-    DP produces for separately seeded training pairs."""
+# --------------------------------------------------------------------------------------------
+# The configurations
+# --------------------------------------------------------------------------------------------
+
+WORKLOAD_NAMES = {
+    "dna150": "DNA global, 10,000 pairs x 150 nt (10% substitutions, 1% indels), +4/-5, gap (-10,-1), int16",
+    "protein": "protein local BLOSUM62, 100,000 pairs, lengths 50-500, gap (-11,-1), int16 with int32 recompute",
+    "uneven": "1,000,000 DNA pairs, lognormal lengths (median 250, sigma 0.8) 50-10,000, half global half local, "
+              "+2/-3, gap (-5,-2), autotuned, graph replay",
+    "long": "2,000 DNA pairs 10-100 kbp, 15% divergence, global, +1/-4, gap (-6,-1), int32 wavefront kernel",
+    "long_int16": "the same 2,000 long pairs in re-centred int16 lanes (wavefront kernel)",
+    "protein_1xn": "one protein query (300 aa) against 200,000 targets 50-500, local BLOSUM62, gap (-11,-1), "
+                   "query profile",
+    "mixed": "mixed DNA/protein stream, {n} pairs, lengths 100-5,000, identity 50-100%, local (DNA +2/-3 gap "
+             "(-5,-2); protein BLOSUM62 gap (-11,-1)), learned lane dispatch",
+}
+
+
+def make_workload(name, pairs=None):
+    count = {} if pairs is None else {"count": pairs}
+    if name == "dna150":
+        w = workloads.config_dna150(**count)
+        return w, device.DeviceConfig(method=device.METHOD_GLOBAL, match=4, mismatch=-5, gap_open=-10,
+                                      gap_extension=-1), "global"
+    if name == "protein":
+        return workloads.config_protein(**count), blosum_config(device.METHOD_LOCAL), "local"
+    if name == "protein_1xn":
+        return workloads.config_protein_1xn(**count), blosum_config(device.METHOD_LOCAL), "local"
+    if name == "uneven":
+        w = workloads.config_uneven(**count)
+        return w, device.DeviceConfig(method=device.METHOD_GLOBAL, match=2, mismatch=-3, gap_open=-5,
+                                      gap_extension=-2), "mixed"
+    if name in ("long", "long_int16"):
+        w = workloads.config_long(**count)
+        return w, device.DeviceConfig(method=device.METHOD_GLOBAL, match=1, mismatch=-4, gap_open=-6, gap_extension=-1,
+                                      lane_policy=device.LANES_INT32 if name == "long" else device.LANES_AUTO), "global"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def bench_config(name, steps, warmup, pairs=None, with_cpu=True, e2e_reps=0, files=False, w=None, dc=None,
+                 method=None, cpu=None):
+    """One configuration: GPU GCUPS over `steps` passes, roofline of the pass's DP launches, and
+    the CPU baseline on a sample of the same pairs (or `cpu`, a result computed already)."""
+    t0 = time.perf_counter()
+    if w is None:
+        w, dc, method = make_workload(name, pairs)
+    n = len(w["o2"]) - 1
+    cells = cells_of(w["o1"], w["o2"])
+    digest = workloads.digest(w["b1"], w["o1"], w["b2"], w["o2"])
+    log(f"[bench] {name}: {n} pairs, {cells:.3e} cells, generated in {time.perf_counter() - t0:.1f} s")
+    ctx = device.default_context(0)
+    tuned = None
+    if name == "uneven":
+        # the library's autotuner times candidate planner/launch parameters on a 5% sample
+        t1 = time.perf_counter()
+        tuning = device.tune(ctx, dc, w["b1"], w["o1"], w["b2"], w["o2"], w.get("pair_method"),
+                             sample_pairs=max(2000, n // 20), iters=2)
+        dc = dataclasses.replace(dc, tuning=tuning)
+        tuned = dict(tuning=dataclasses.asdict(tuning), seconds=round(time.perf_counter() - t1, 2))
+        log(f"[bench] {name}: autotuned {tuned}")
+    one_to_one = None
+    if "query" in w:
+        ref = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, None, steps, warmup)
+        one_to_one = dict(ms_per_step=round(ref["ms_per_step"], 4))
+    r = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"), steps, warmup, query=w.get("query"))
+    gcups = cells / (r["ms_per_step"] / 1e3) / 1e9
+    ops = 9 if method == "mixed" else OPS_PER_CELL[method]
+    peak = PEAK_INT32_TOPS if r["stats"]["n_int16"] == 0 else PEAK_INT16_TOPS
+    rl = roofline(ops, cells, r["kernel_ms"], peak, measured_traffic(name, digest))
+    log(f"[bench] {name}: {gcups:.1f} GCUPS ({r['ms_per_step']:.3f} ms/step, DP launches {r['kernel_ms']:.3f} ms), "
+        f"{rl['achieved']:.2f} of {rl['peak']:.1f} TOP/s")
+    out = dict(workload=WORKLOAD_NAMES.get(name, name) if pairs is None else f"{name} with {pairs} pairs",
+               gcups=round(gcups, 2), ms_per_step=round(r["ms_per_step"], 4), steps=steps, pairs=n, cells=cells,
+               dtype=dtype_of([r["stats"]]), roofline=rl, plan=plan_summary(r["stats"]), inputs_sha256=digest)
+    if tuned:
+        out["autotuned"] = tuned
+    if one_to_one is not None:
+        one_to_one["speedup_of_profile"] = round(one_to_one["ms_per_step"] / r["ms_per_step"], 3)
+        one_to_one["scores_identical"] = bool(np.array_equal(ref["scores"], r["scores"]))
+        out["one_to_one_same_pairs"] = one_to_one
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    elif with_cpu:
+        out["cpu_baseline"] = cpu_baseline([(w["b1"], w["o1"], w["b2"], w["o2"], dc, w.get("pair_method"))],
+                                           [r["scores"]], one_core=(name == "protein"))
+        log(f"[bench] {name}: CPU baseline {out['cpu_baseline']['value']} GCUPS "
+            f"({out['cpu_baseline']['mismatches_vs_gpu']} mismatches)")
+    if e2e_reps:
+        out["end_to_end"] = end_to_end(ctx, w, dc, cells, r["scores"], e2e_reps)
+        if files:
+            out["end_to_end"]["from_fasta_files"] = from_files(ctx, w, dc, cells, r["scores"], min(e2e_reps, 5))
+    out["_scores"] = r["scores"]
+    return out
+
+
+def bench_long(steps, warmup):
+    """The 2,000 long pairs: int32 lanes as configured, and the same pairs in re-centred int16
+    lanes; one CPU baseline (int32, as the configuration asks) for both."""
+    w, dc32, method = make_workload("long")
+    res32 = bench_config("long", steps, warmup, w=w, dc=dc32, method=method)
+    dc16 = dataclasses.replace(dc32, lane_policy=device.LANES_AUTO)
+    res16 = bench_config("long_int16", steps, warmup, w=w, dc=dc16, method=method, cpu=res32["cpu_baseline"])
+    res16["scores_identical_to_int32"] = bool(np.array_equal(res16["_scores"], res32["_scores"]))
+    return res32, res16
+
+
+def bench_mixed(count, steps, warmup, side_steps=5):
+    """Learned lane dispatch on the mixed stream: the same pairs under the learned policy (timed
+    over `steps` passes: the headline), fixed int32 lanes, int16 speculation (every pair int16,
+    saturated pairs recomputed in int32) and the length-bound policy (timed over
+    min(steps, side_steps) passes each).  The network is trained first, on labels the DP produces
+    for separately seeded training pairs."""
     from pairwise_aligner_amd import learned
     t0 = time.perf_counter()
     w = workloads.config_mixed(count=count)
     log(f"[bench] mixed: {count} pairs generated in {time.perf_counter() - t0:.1f} s")
     ctx = device.default_context(0)
     dna = device.DeviceConfig(method=device.METHOD_LOCAL, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
-    prot = blosum_config(device.METHOD_LOCAL, -11, -1, device.LANES_AUTO)
+    prot = blosum_config(device.METHOD_LOCAL)
     t1 = time.perf_counter()
     sets = [(prot, learned.training_pairs("protein", 1001, 4000)), (dna, learned.training_pairs("dna", 1002, 2000))]
     x, y = learned.build_dataset(ctx, sets)
     model, losses = learned.train(ctx, x, y, steps=1500, batch=2048, log=log)
     train_s = time.perf_counter() - t1
     halves = [("dna", dna, w["dna"]), ("protein", prot, w["protein"])]
-    cells = {k: int(np.sum(np.diff(h["o1"]).astype(np.int64) * np.diff(h["o2"]).astype(np.int64)))
-             for k, _, h in halves}
-    out = {"pairs": count, "cells": sum(cells.values()), "train_seconds": round(train_s, 2),
-           "train_loss_first_last": [round(float(np.mean(losses[:20])), 5), round(float(np.mean(losses[-20:])), 5)]}
+    cells = sum(cells_of(h["o1"], h["o2"]) for _, _, h in halves)
+    digest = workloads.digest(*[h[k] for _, _, h in halves for k in ("b1", "o1", "b2", "o2")])
+    out = {"workload": WORKLOAD_NAMES["mixed"].format(n=f"{count:,}"), "pairs": count, "cells": cells,
+           "inputs_sha256": digest, "train_seconds": round(train_s, 2),
+           "train_loss_first_last": [round(float(np.mean(losses[:20])), 5), round(float(np.mean(losses[-20:])), 5)],
+           "policies": {}}
     ref_scores = {}
-    for name, policy in (("int32", device.LANES_INT32), ("bound", device.LANES_AUTO), ("learned", device.LANES_LEARNED)):
-        total_ms = 0.0
-        n16 = n32 = nrec = 0
-        mism = 0
+    legs = (("learned", device.LANES_LEARNED, steps, warmup), ("int32", device.LANES_INT32, min(steps, side_steps), 1),
+            ("int16_speculative", device.LANES_INT16, min(steps, side_steps), 1),
+            ("bound", device.LANES_AUTO, min(steps, side_steps), 1))
+    for name, policy, k, wu in legs:
+        total_ms = kernel_ms = 0.0
+        stats, mism = [], 0
         for kind, dc, h in halves:
-            dcp = device.DeviceConfig(**{**dc.__dict__, "lane_policy": policy,
-                                         "model": model if policy == device.LANES_LEARNED else None})
-            bt = device.Batch(ctx, h["b1"], h["o1"], h["b2"], h["o2"], dcp)
-            for _ in range(warmup):
-                bt.run()
-            ms, _ = bt.time(steps)
-            total_ms += ms / steps
-            sc, _ = bt.results()
-            st = bt.stats()
-            n16 += st["n_int16"]; n32 += st["n_int32"]; nrec += st["n_recomputed"]
+            dcp = dataclasses.replace(dc, lane_policy=policy, model=model if policy == device.LANES_LEARNED else None)
+            r = gpu_run(ctx, h["b1"], h["o1"], h["b2"], h["o2"], dcp, None, k, wu)
+            total_ms += r["ms_per_step"]
+            kernel_ms += r["kernel_ms"]
+            stats.append(r["stats"])
             if kind in ref_scores:
-                mism += int(np.count_nonzero(ref_scores[kind] != sc))
+                mism += int(np.count_nonzero(ref_scores[kind] != r["scores"]))
             else:
-                ref_scores[kind] = sc
-            bt.close()
-        out[name] = {"gcups": round(out["cells"] / (total_ms / 1e3) / 1e9, 2), "ms_per_step": round(total_ms, 3),
-                     "n_int16": n16, "n_int32": n32, "n_recomputed": nrec, "score_mismatches_vs_int32": mism}
-        log(f"[bench] mixed/{name}: {out[name]}")
-    out["speedup_learned_vs_int32"] = round(out["learned"]["gcups"] / out["int32"]["gcups"], 3)
-    out["speedup_bound_vs_int32"] = round(out["bound"]["gcups"] / out["int32"]["gcups"], 3)
-    if with_oracle:
-        from oracle import oracle as O
-        bad = 0
-        for kind, dc, h in halves:
-            k = 40
-            want = O.score_packed(h["b1"], h["o1"][:k + 1], h["b2"], h["o2"][:k + 1], oracle_cfg(dc),
-                                  threads=min(os.cpu_count() or 1, 16))
-            bad += int(np.count_nonzero(ref_scores[kind][:k].astype(np.int64) != want))
-        out["oracle_sample_mismatches"] = bad
-        out["oracle_sample_pairs"] = 80
+                ref_scores[kind] = r["scores"]
+        leg = {"gcups": round(cells / (total_ms / 1e3) / 1e9, 2), "ms_per_step": round(total_ms, 3), "steps": k,
+               "kernel_ms": round(kernel_ms, 3), "dtype": dtype_of(stats),
+               "n_int16": sum(s["n_int16"] for s in stats), "n_int32": sum(s["n_int32"] for s in stats),
+               "n_recomputed": sum(s["n_recomputed"] for s in stats),
+               "score_mismatches_vs_learned": mism,
+               "roofline": roofline(OPS_PER_CELL["local"], cells, kernel_ms,
+                                    PEAK_INT32_TOPS if policy == device.LANES_INT32 else PEAK_INT16_TOPS,
+                                    measured_traffic(f"mixed_{name}", digest))}
+        out["policies"][name] = leg
+        log(f"[bench] mixed/{name}: {leg}")
+    pol = out["policies"]
+    for name in ("int16_speculative", "bound"):
+        out[f"speedup_{name}_vs_int32"] = round(pol[name]["gcups"] / pol["int32"]["gcups"], 3)
+    out["speedup_learned_vs_int32"] = round(pol["learned"]["gcups"] / pol["int32"]["gcups"], 3)
+    out["speedup_learned_vs_int16_speculative"] = round(pol["learned"]["gcups"] / pol["int16_speculative"]["gcups"], 3)
+    # the CPU baseline on a prefix of each half (the same pairs), and an oracle spot check
+    out["cpu_baseline"] = cpu_baseline([(h["b1"], h["o1"], h["b2"], h["o2"], dc, None) for _, dc, h in halves],
+                                       [ref_scores[k] for k, _, _ in halves], one_core=True)
+    log(f"[bench] mixed: CPU baseline {out['cpu_baseline']['value']} GCUPS")
+    from oracle import oracle as O
+    bad = 0
+    for kind, dc, h in halves:
+        k = 40
+        want = O.score_packed(h["b1"], h["o1"][:k + 1], h["b2"], h["o2"][:k + 1], oracle_cfg(dc),
+                              threads=host_cpu()["threads_used"])
+        bad += int(np.count_nonzero(ref_scores[kind][:k].astype(np.int64) != want))
+    out["oracle_sample_mismatches"] = bad
+    out["oracle_sample_pairs"] = 80
     model.close()
     return out
+
+
+def copy_probe():
+    import torch
+    a = torch.arange(COPY_PROBE_BYTES // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copies": 3, "check": int(b[-1].item())}))
+
+
+def strip(d):
+    return {k: v for k, v in d.items() if not k.startswith("_")}
 
 
 def main():
@@ -348,118 +477,65 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="protein")
-    ap.add_argument("--no-extra", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--workload", default="mixed")
+    ap.add_argument("--no-extra", action="store_true", help="only the named workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy three times (scales the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters for roofline.traffic)")
     ap.add_argument("--e2e-reps", type=int, default=10,
-                    help="calls of the one-shot entry point timed end to end (0: skip)")
-    ap.add_argument("--pairs", type=int, default=None,
-                    help="pair count of the dna150 / protein workload (default: 10,000 / 100,000)")
-    ap.add_argument("--mixed-pairs", type=int, default=100_000,
-                    help="pairs of the mixed workload used for the learned-dispatch comparison (config: 500,000)")
+                    help="calls of the one-shot entry point timed end to end on the protein workload (0: skip)")
+    ap.add_argument("--pairs", type=int, default=None, help="pair count of the named workload (not mixed)")
+    ap.add_argument("--mixed-pairs", type=int, default=500_000, help="pairs of the mixed stream (config: 500,000)")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
-        a = torch.arange(COPY_PROBE_BYTES // 4, dtype=torch.int32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copies": 3, "check": int(b[-1].item())}))
+        copy_probe()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (pairs are independent; multi-GPU sharding is not built); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
+    line = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "data": "synthetic (seeded generators in pairwise_aligner_amd/workloads.py)"}
+    configs = {}
     if args.workload == "mixed":
-        res = run_mixed(args.mixed_pairs, max(1, args.steps), args.warmup)
-        print(json.dumps({"metric": "GCUPS score-only affine DP on 1 GPU vs CPU ref; learned dispatch vs fixed int32",
-                          "value": res["learned"]["gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
-                          "warmup": args.warmup, "ms_per_step": res["learned"]["ms_per_step"],
-                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int16+int32",
-                          "data": "synthetic", "config": {"workload": f"mixed DNA/protein, {args.mixed_pairs} pairs"},
-                          "mixed_learned_dispatch": res}))
-        return
-    main_res = run(args.workload, args.steps, args.warmup, autotune=args.workload == "uneven", pairs=args.pairs,
-                   e2e_reps=args.e2e_reps)
-    extra = mixed = uneven = None
-    if not args.no_extra and args.workload != "dna150":
-        extra = run("dna150", max(args.steps, 50), args.warmup, with_cpu=False)
-    if not args.no_extra and args.workload != "uneven":
-        uneven = run("uneven", 5, 1, with_cpu=False, autotune=True, e2e_reps=min(args.e2e_reps, 5))
-    otm = None
-    if not args.no_extra and args.workload != "protein_1xn":
-        otm = run("protein_1xn", max(args.steps, 10), args.warmup, with_cpu=False)
+        m = bench_mixed(args.mixed_pairs, args.steps, args.warmup)
+        lead = m["policies"]["learned"]
+        line.update({"value": lead["gcups"], "unit": "GCUPS", "ms_per_step": lead["ms_per_step"],
+                     "dtype": lead["dtype"],
+                     "config": {"workload": m["workload"], "pairs": m["pairs"], "cells": m["cells"],
+                                "inputs_sha256": m["inputs_sha256"]},
+                     "roofline": lead["roofline"], "cpu_baseline": m["cpu_baseline"],
+                     "learned_vs_int32": m["speedup_learned_vs_int32"],
+                     "learned_dispatch": {k: v for k, v in m.items()
+                                          if k not in ("workload", "pairs", "cells", "inputs_sha256", "cpu_baseline")}})
+    else:
+        r = bench_config(args.workload, args.steps, args.warmup, pairs=args.pairs,
+                         e2e_reps=args.e2e_reps if args.workload == "protein" else 0,
+                         files=args.workload == "protein")
+        line.update({"value": r["gcups"], "unit": "GCUPS", "ms_per_step": r["ms_per_step"], "dtype": r["dtype"],
+                     "config": {"workload": r["workload"], "pairs": r["pairs"], "cells": r["cells"],
+                                "inputs_sha256": r["inputs_sha256"]},
+                     "roofline": r["roofline"], "cpu_baseline": r.get("cpu_baseline")})
+        line["details"] = strip({k: v for k, v in r.items() if k not in ("roofline", "cpu_baseline")})
     if not args.no_extra:
-        mixed = run_mixed(args.mixed_pairs, 1, 1)
-
-    cpu = main_res.pop("cpu")
-    traffic = measured_traffic(args.workload, main_res["inputs_sha256"])
-    frac = main_res["achieved_tops"] / main_res["peak_tops"]
-    line = {
-        "metric": "GCUPS score-only affine DP on 1 GPU vs CPU ref; learned dispatch vs fixed int32",
-        "value": round(main_res["gcups"], 3),
-        "unit": "GCUPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(main_res["ms_per_step"], 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int16" if main_res["stats"]["n_int32"] == 0 else "int16+int32",
-        "data": "synthetic (seeded generators in pairwise_aligner_amd/workloads.py)",
-        "config": {"workload": {"protein": "protein local BLOSUM62, 100,000 pairs, lengths 50-500, gap (-11,-1)",
-                                "dna150": "DNA global, 10,000 pairs x 150 nt, +4/-5, gap (-10,-1)",
-                                "uneven": "1,000,000 DNA pairs lognormal lengths, half global half local",
-                                "long": "2,000 DNA pairs 10-100 kbp, int32",
-                                "long_int16": "2,000 DNA pairs 10-100 kbp, re-centred int16 lanes",
-                                "protein_1xn": "one protein query (300 aa) against 200,000 targets 50-500, "
-                                               "local BLOSUM62, gap (-11,-1), query profile"}.get(args.workload,
-                                                                                                   args.workload)
-                   if args.pairs is None else f"{args.workload} workload with {args.pairs} pairs",
-                   "pairs": main_res["pairs"], "cells": main_res["cells"],
-                   "inputs_sha256": main_res["inputs_sha256"]},
-        "roofline": {"bound": "valu", "achieved": round(main_res["achieved_tops"], 3),
-                     "peak": round(main_res["peak_tops"], 2), "unit": "TOP/s", "frac": round(frac, 4),
-                     "traffic": None if traffic is None else traffic["bytes_per_pass"],
-                     "ops_per_cell": main_res["ops_per_cell"], "kernel_ms": round(main_res["main_kernel_ms"], 4)},
-        "cpu_baseline": {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample", "value_1core",
-                                             "sample_1core", "isa")},
-        "mismatches_vs_oracle_sample": cpu["mismatches_vs_gpu"],
-        "plan": {k: main_res["stats"][k] for k in ("n_int16", "n_int32", "n_recomputed", "padded_cells", "n_launches",
-                                                   "n_profiled")},
-    }
-    if main_res.get("end_to_end"):
-        line["end_to_end"] = main_res["end_to_end"]
-    if main_res.get("one_to_one"):
-        line["one_to_one_same_pairs"] = {k: (round(v, 4) if isinstance(v, float) else v)
-                                         for k, v in main_res["one_to_one"].items()}
-    if main_res.get("autotuned"):
-        line["autotuned"] = main_res["autotuned"]
-    if mixed is not None:
-        line["mixed_learned_dispatch"] = mixed
-    if uneven is not None:
-        line["uneven_1m_autotuned_graph"] = {
-            "gcups": round(uneven["gcups"], 2), "ms_per_step": round(uneven["ms_per_step"], 3),
-            "pairs": uneven["pairs"], "cells": uneven["cells"], "autotuned": uneven["autotuned"],
-            "roofline_frac": round(uneven["achieved_tops"] / uneven["peak_tops"], 4),
-            "n_launches": uneven["stats"]["n_launches"], "plan_ms": round(uneven["stats"]["plan_ms"], 1),
-            "end_to_end": uneven.get("end_to_end")}
-    if otm is not None:
-        line["protein_1xn_query_profile"] = {
-            "gcups": round(otm["gcups"], 2), "ms_per_step": round(otm["ms_per_step"], 4),
-            "roofline_frac": round(otm["achieved_tops"] / otm["peak_tops"], 4),
-            "one_to_one_ms_per_step": round(otm["one_to_one"]["ms_per_step"], 4),
-            "speedup_of_profile": round(otm["one_to_one"]["speedup_of_profile"], 3),
-            "scores_identical": otm["one_to_one"]["scores_identical"], "n_profiled": otm["stats"]["n_profiled"]}
-    if extra is not None:
-        line["dna150_global_int16"] = {"gcups": round(extra["gcups"], 2), "ms_per_step": round(extra["ms_per_step"], 4),
-                                       "kernel_ms": round(extra["main_kernel_ms"], 4),
-                                       "roofline_frac": round(extra["achieved_tops"] / extra["peak_tops"], 4)}
+        side = min(args.steps, 5)
+        if args.workload != "dna150":
+            configs["dna150_global_int16"] = strip(bench_config("dna150", max(args.steps, 50), args.warmup))
+        if args.workload != "protein":
+            configs["protein_local_100k"] = strip(bench_config("protein", args.steps, args.warmup,
+                                                               e2e_reps=args.e2e_reps, files=args.e2e_reps > 0))
+        if args.workload != "uneven":
+            configs["uneven_1m_autotuned_graph"] = strip(bench_config("uneven", side, 1, e2e_reps=min(args.e2e_reps, 3)))
+        if args.workload not in ("long", "long_int16"):
+            r32, r16 = bench_long(min(args.steps, 3), 1)
+            configs["long_2000_int32"] = strip(r32)
+            configs["long_2000_recentred_int16"] = strip(r16)
+        if args.workload != "protein_1xn":
+            configs["protein_1xn_query_profile"] = strip(bench_config("protein_1xn", max(args.steps, 10), args.warmup,
+                                                                      with_cpu=False))
+        line["configs"] = configs
     print(json.dumps(line))
 
 

@@ -19,6 +19,8 @@ Configurations (sizes as stated; names are used in bench.py):
 from __future__ import annotations
 
 import hashlib
+import os
+from concurrent.futures import ThreadPoolExecutor
 from typing import Tuple
 
 import numpy as np
@@ -181,6 +183,16 @@ def _chunks(count: int, chunk: int):
     return [(c, start, min(chunk, count - start)) for c, start in enumerate(range(0, count, chunk))]
 
 
+def _map_chunks(fn, items):
+    """fn over independent chunks on a few threads (numpy releases the GIL in its bulk work);
+    results in item order."""
+    workers = max(1, min(len(items), 8, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1))
+    if workers == 1:
+        return [fn(it) for it in items]
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(fn, items))
+
+
 def take(buf: np.ndarray, off: np.ndarray, n: int):
     """The first n sequences of a packed batch."""
     off = off[:n + 1]
@@ -197,16 +209,17 @@ def config_uneven(seed: int = 3, count: int = 1_000_000, limit: int = None, chun
     """Chunk c of `chunk` pairs comes from its own stream (seed, c), so the first `limit` pairs
     are the same pairs as in the whole batch."""
     n = count if limit is None else min(limit, count)
-    parts = []
-    for c, start, k in _chunks(count, chunk):
-        if start >= n:
-            break
+
+    def make(item):
+        c, start, k = item
         rng = np.random.default_rng([seed, c])
         L = np.clip(np.round(rng.lognormal(np.log(250.0), 0.8, size=k)), 50, 10_000).astype(np.int64)
         b1, o1 = random_packed(rng, L, DNA)
         div = rng.uniform(0.01, 0.20, size=k)
         b2, o2 = mutate_packed(rng, b1, o1, div, 0.1, 0.7, DNA)
-        parts.append((b1[:int(o1[-1])], o1, b2[:int(o2[-1])], o2))
+        return (b1[:int(o1[-1])], o1, b2[:int(o2[-1])], o2)
+
+    parts = _map_chunks(make, [it for it in _chunks(count, chunk) if it[1] < n])
     b1, o1, b2, o2 = concat_packed(parts)
     b1, o1 = take(b1, o1, n)
     b2, o2 = take(b2, o2, n)
@@ -222,14 +235,16 @@ def config_long(seed: int = 4, count: int = 2_000, select=None, chunk: int = 50)
     lengths = long_lengths(seed, count)
     want = range(count) if select is None else [int(i) for i in select]
     need = sorted({i // chunk for i in want})
-    made = {}
-    for c in need:
+
+    def make(c):
         start = c * chunk
         k = min(chunk, count - start)
         rng = np.random.default_rng([seed, 1, c])
         b1, o1 = random_packed(rng, lengths[start:start + k], DNA)
         b2, o2 = mutate_packed(rng, b1, o1, 0.15, 0.6, 0.5, DNA)
-        made[c] = (b1, o1, b2, o2)
+        return (b1, o1, b2, o2)
+
+    made = dict(zip(need, _map_chunks(make, need)))
     if select is None:
         b1, o1, b2, o2 = concat_packed([(m[0][:int(m[1][-1])], m[1], m[2][:int(m[3][-1])], m[3])
                                         for _, m in sorted(made.items())])
@@ -260,15 +275,16 @@ def config_mixed(seed: int = 5, count: int = 500_000, limit: int = None, chunk: 
     out = {}
     for h, (kind, alpha, p, n) in enumerate((("dna", DNA, None, half), ("protein", AMINO, AMINO_FREQ, count - half))):
         keep = n if limit is None else min(limit, n)
-        parts = []
-        for c, start, k in _chunks(n, chunk):
-            if start >= keep:
-                break
+
+        def make(item, h=h, alpha=alpha, p=p):
+            c, start, k = item
             rng = np.random.default_rng([seed, h, c])
             b1, o1 = random_packed(rng, rng.integers(100, 5_001, size=k), alpha, p)
             div = 1.0 - rng.uniform(0.5, 1.0, size=k)
             b2, o2 = mutate_packed(rng, b1, o1, div, 0.1, 0.7, alpha, p)
-            parts.append((b1[:int(o1[-1])], o1, b2[:int(o2[-1])], o2))
+            return (b1[:int(o1[-1])], o1, b2[:int(o2[-1])], o2)
+
+        parts = _map_chunks(make, [it for it in _chunks(n, chunk) if it[1] < keep])
         b1, o1, b2, o2 = concat_packed(parts)
         b1, o1 = take(b1, o1, keep)
         b2, o2 = take(b2, o2, keep)
