@@ -106,7 +106,7 @@ def roofline(ops, cells, kernel_ms, peak, traffic=None):
 # CPU baseline
 # --------------------------------------------------------------------------------------------
 
-def cpu_baseline(halves, gpu_scores, budget_s=12.0, one_core=False, cell_cap=8e11):
+def cpu_baseline(halves, gpu_scores, budget_s=12.0, one_core=False, cell_cap=2e12):
     """Times the CPU baseline -- oracle/simd_port.cpp, the reference's inter-sequence SIMD scheme
     (one pair per lane, 32 int16 / 16 int32 lanes per AVX-512 register, int16 where the bulk's
     bound allows) restated for AVX-512BW / AVX2 -- on the host's cores, over a prefix of the same
@@ -137,7 +137,8 @@ def cpu_baseline(halves, gpu_scores, budget_s=12.0, one_core=False, cell_cap=8e1
 
     def sized(target_s):
         # calibrate on a prefix of each half (about 1e8 cells, and full bulks for every thread
-        # unless that exceeds cell_cap), then scale every half's prefix to the time budget
+        # unless that exceeds cell_cap), grow every half's prefix towards the time budget, and
+        # repeat the whole batch when even all of it takes less
         ks = []
         for b1, o1, b2, o2, _, _ in halves:
             n = len(o1) - 1
@@ -146,13 +147,16 @@ def cpu_baseline(halves, gpu_scores, budget_s=12.0, one_core=False, cell_cap=8e1
             k = max(k, want) if cum[want - 1] <= cell_cap else max(1, int(np.searchsorted(cum, cell_cap)))
             ks.append(min(k, n))
         got, cells, dt = prefix(ks)
-        if dt < target_s * 0.5:
-            scale = target_s / max(dt, 1e-6)
+        for _ in range(3):
+            full = all(k == len(h[1]) - 1 for h, k in zip(halves, ks))
+            if full or dt >= target_s * 0.5:
+                break
+            scale = min(64.0, target_s / max(dt, 1e-6))
             ks = [max(k, first_k(o1, o2, cells_of(o1[:k + 1], o2[:k + 1]) * scale)[0])
                   for (b1, o1, b2, o2, _, _), k in zip(halves, ks)]
             got, cells, dt = prefix(ks)
         reps = 1
-        if dt < target_s * 0.5:   # the whole batch is sampled and takes less: repeat it
+        if dt < target_s * 0.5 and all(k == len(h[1]) - 1 for h, k in zip(halves, ks)):
             reps = int(min(1000, np.ceil(target_s * 0.5 / max(dt, 1e-6))))
             got, cells, dt = prefix(ks, reps)
         return got, cells, dt, ks, reps
