@@ -250,8 +250,9 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
 }
 
 // Enqueues the launches of a plan: forked from the context stream onto the plan's streams in
-// the plan's order, joined back into the context stream.
-pa_status launch_plan(pa_ctx *ctx, pa_batch *b, PlanDevice &pd)
+// the plan's order, joined back into the context stream.  self_init: the DP kernels write their
+// pairs' initial scores and flags (the main pass); otherwise the initial-score kernel has.
+pa_status launch_plan(pa_ctx *ctx, pa_batch *b, PlanDevice &pd, bool self_init)
 {
     const pa::Plan &p = pd.plan;
     const size_t nl = p.launches.size();
@@ -264,7 +265,8 @@ pa_status launch_plan(pa_ctx *ctx, pa_batch *b, PlanDevice &pd)
     }
     for (uint32_t li : p.launch_order) {
         const pa::Launch &L = p.launches[li];
-        const pa::KernelArgs a = kernel_args(pd, b, L);
+        pa::KernelArgs a = kernel_args(pd, b, L);
+        a.self_init = self_init ? 1u : 0u;
         const uint32_t lds = pa::lds_bytes(p, L.key);
         hipStream_t st = fork ? ctx->side[p.launch_stream[li]] : ctx->stream;
         hipError_t e = pa::launch_stripe(L.key, a, lds, st);
@@ -293,7 +295,9 @@ pa::InitArgs init_args(pa_batch *b, const pa::Plan &p)
     return ia;
 }
 
-// The main pass: initial scores, then every launch of the main plan.  Unless the tuning asks
+// The main pass: every launch of the main plan, whose kernels initialise their own pairs; the
+// initial-score kernel runs first only for batches with pairs that have no task (an empty
+// sequence: a closed-form score).  Unless the tuning asks
 // for eager launches, a pass with two or more striped launches is captured once into a hipGraph
 // and replayed (one launch call per pass, no per-kernel launch gaps, the launches' streams
 // overlapping).  A pass with one striped launch runs eagerly: its two kernels run back to back
@@ -301,9 +305,11 @@ pa::InitArgs init_args(pa_batch *b, const pa::Plan &p)
 // 0.089 against 0.096 ms per pass).  The graph holds kernels and event dependencies only.
 pa_status enqueue_main(pa_ctx *ctx, pa_batch *b)
 {
-    hipError_t e = pa::launch_init(init_args(b, b->main.plan), ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "init launch");
-    return launch_plan(ctx, b, b->main);
+    if (b->main.plan.n_empty > 0) {
+        hipError_t e = pa::launch_init(init_args(b, b->main.plan), ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "init launch");
+    }
+    return launch_plan(ctx, b, b->main, true);
 }
 
 pa_status launch_main(pa_ctx *ctx, pa_batch *b)
@@ -373,7 +379,7 @@ pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
     // the per-pair width in the main plan says 16; the recomputed pairs report int32
     hipError_t e = pa::launch_init(ia, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "init launch");
-    return launch_plan(ctx, b, *b->recompute);
+    return launch_plan(ctx, b, *b->recompute, false);
 }
 
 // ---- learned dispatch helpers ---------------------------------------------------------------
@@ -463,12 +469,18 @@ pa_status learned_widths(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView 
     return PA_OK;
 }
 
-// The wavefront kernel's bounded waits set flags[n_pairs] when they give up.
+// The wavefront kernel's bounded waits set flags[n_pairs] when they give up; the slot is cleared
+// again here (passes do not clear it), so that one failed pass is reported once.
 pa_status check_error_slot(pa_ctx *ctx, pa_batch *b)
 {
     uint32_t err = 0;
-    PA_HIP(hipMemcpy(&err, static_cast<uint32_t *>(b->flags.ptr) + b->n_pairs, 4, hipMemcpyDeviceToHost), "hipMemcpy");
-    if (err) return fail(ctx, PA_ERR_DEVICE, "wavefront kernel: a boundary wait exceeded its limit");
+    uint32_t *slot = static_cast<uint32_t *>(b->flags.ptr) + b->n_pairs;
+    PA_HIP(hipMemcpy(&err, slot, 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (err) {
+        static const uint32_t kZero = 0;
+        PA_HIP(hipMemcpy(slot, &kZero, 4, hipMemcpyHostToDevice), "hipMemcpy");
+        return fail(ctx, PA_ERR_DEVICE, "wavefront kernel: a boundary wait exceeded its limit");
+    }
     return PA_OK;
 }
 
@@ -593,7 +605,7 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
     b->stats.padded_cells = p.padded_cells;
     b->stats.n_int16 = p.n16;
     b->stats.n_int32 = p.n32;
-    b->stats.n_launches = p.launches.size() + 1;
+    b->stats.n_launches = p.launches.size() + (p.n_empty > 0 ? 1 : 0);
     b->stats.n_profiled = p.n_profiled;
     b->stats.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = b.release();

@@ -66,8 +66,37 @@ __device__ __forceinline__ typename Lane<W>::V row_reduce_max(typename Lane<W>::
     return L::from(m);
 }
 
+// Initial score of a pair with m, n >= 1 (tracker_global_scalar.hpp:36-61): INT32_MIN, raised by
+// the free-end candidates that no DP cell visits, H(m, 0) with a free last row and H(0, n) with a
+// free last column (affine_initialisation_strategy.hpp:38-51).
+__device__ __forceinline__ int32_t initial_score(int m, int n, int go, int ge, int fc, int fr, int lc, int lr)
+{
+    int64_t s = INT32_MIN;
+    if (lr) { const int64_t h = fc ? 0 : go + int64_t(ge) * m; s = h > s ? h : s; }
+    if (lc) { const int64_t h = fr ? 0 : go + int64_t(ge) * n; s = h > s ? h : s; }
+    return int32_t(s);
+}
+
+// With a.self_init: the initial score and flags of a task's pairs, written by one lane before
+// any other lane of the wave (or, with the barrier that follows, of the workgroup) raises the
+// score with an atomic or flags a saturation.
+template <int W, bool LOCAL>
+__device__ __forceinline__ void init_task_pairs(const KernelArgs &a, const Task &tk, int fc, int fr, int lc, int lr)
+{
+#pragma unroll
+    for (int q = 0; q < Lane<W>::NH; ++q) {
+        if (tk.pair[q] == kNone) continue;
+        a.flags[tk.pair[q]] = W == 16 ? 0x01u : 0x02u;
+        // the local maximum starts at INT32_MIN (the wavefront kernel's waves raise it atomically)
+        a.scores[tk.pair[q]] = LOCAL ? INT32_MIN : initial_score(int(tk.m[q]), int(tk.n[q]), a.go, a.ge, fc, fr, lc, lr);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+}
+
 // EXPORT: also write the task's final DP column and row (int32 lanes; pa_plan.hpp, dp_base).
-template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false>
+// ONE: every task of the launch is one stripe (m <= G*R): no boundary rows through HBM, and lane
+// 0's top boundary (the initial DP row) kept in two registers instead of conveyors.
+template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false, bool ONE = false>
 __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 {
     using L = Lane<W>;
@@ -90,11 +119,13 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     const int go = a.go, ge = a.ge;
     // method_local.hpp:41-49: a local alignment has every end gap free
     const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
+    if (a.self_init && t == 0) init_task_pairs<W, LOCAL>(a, tk, fc, fr, lc, LOCAL ? 0 : a.lr);
     constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
     constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
     // column drift: on top of the row drift for the global method, alone for the local method
     // (whose zero clamp and running maximum then work in the step's frame)
     constexpr bool CDRIFT = !ACC && !EXPORT;
+    static_assert(!(ONE && EXPORT), "the DP export keeps its boundary conveyors");
     const int dge = DRIFT ? ge : 0;
     const int cg = CDRIFT ? -ge : 0;         // column drift per step
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
@@ -131,9 +162,12 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         V Hdiag = L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1));
         Capture<W, R> cap;
         if constexpr (!LOCAL) cap.init(tk, row0, lc);
+        // the step at which this lane's next capture event comes (events are rare: one wave
+        // test per step against it)
+        int kev = cap.ev == INT32_MAX ? INT32_MAX : cap.ev + t;
 
-        const bool has_bnd_in = s > 0;
-        const bool has_bnd_out = s + 1 < tk.S;
+        const bool has_bnd_in = !ONE && s > 0;
+        const bool has_bnd_out = !ONE && s + 1 < tk.S;
         const uint32_t *colp = a.colcodes + tk.col_off;
         uint32_t *bndp = has_bnd_in || has_bnd_out ? a.bnd + 2ull * tk.bnd_off : nullptr;
 
@@ -168,9 +202,19 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 e = L::top_word(hv + go + ge - dge + cg * t);
             }
         };
-        uint32_t convH, convE, nextH, nextE;
-        top_chunk(t, convH, convE);
-        top_chunk(G + t, nextH, nextE);
+        uint32_t convH = 0, convE = 0, nextH = 0, nextE = 0;
+        // ONE: lane 0's top boundary at the current step, H(0, c+1) and the vertical gap state
+        // below it; it changes by top_step per step (the free first row's column drift, or the
+        // penalised first row's extension without the column drift) and by the chunk rebase
+        uint32_t topH = 0, topE = 0;
+        const int top_step = CDRIFT ? (fr ? cg : 0) : (fr ? 0 : ge);
+        const V vtop_step = L::splat(top_step);
+        if constexpr (ONE) {
+            top_chunk(0, topH, topE);
+        } else {
+            top_chunk(t, convH, convE);
+            top_chunk(G + t, nextH, nextE);
+        }
         // outputs of this lane from the previous step (read by lane t+1)
         uint32_t Hout = 0, Eout = 0;
         // bottom-row outputs of lane G-1 over one chunk of G steps, shifted down one lane per
@@ -186,15 +230,29 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         // the first chunk, where lane t waits t steps, and by the ROWMAX/TRACK variants whose
         // per-row maxima must not see pad columns).  Unmasked, lanes past column N-1 sweep pad
         // columns: pad cells never exceed the real ones and never feed them.
-        auto step = [&]<bool MASKED>(int k) {
+        // TOPSTEP (ONE only): lane 0's top boundary changes from step to step (top_step != 0)
+        auto step = [&]<bool MASKED, bool TOPSTEP>(int k) {
             const int c = k - t;   // DP column (0-based) of this lane at this step
             const uint32_t Bnext = dpp<kRowShr1>(conv, Bin);
             conv = dpp<kRowShl1>(conv, conv);
             issue_scores<W, R, SCORE>(st, Bnext, lds_raw, rd);
-            const uint32_t Hin = dpp<kRowShr1>(convH, Hout);
-            const uint32_t Ein = dpp<kRowShr1>(convE, Eout);
-            convH = dpp<kRowShl1>(convH, convH);
-            convE = dpp<kRowShl1>(convE, convE);
+            uint32_t Hin, Ein;
+            if constexpr (ONE) {
+                // lanes 1.. take lane t-1's outputs, lane 0 keeps its top boundary
+                topH = dpp<kRowShr1>(topH, Hout);
+                topE = dpp<kRowShr1>(topE, Eout);
+                Hin = topH;
+                Ein = topE;
+                if constexpr (TOPSTEP) {
+                    topH = L::bits(L::add(L::from(topH), vtop_step));
+                    topE = L::bits(L::add(L::from(topE), vtop_step));
+                }
+            } else {
+                Hin = dpp<kRowShr1>(convH, Hout);
+                Ein = dpp<kRowShr1>(convE, Eout);
+                convH = dpp<kRowShl1>(convH, convH);
+                convE = dpp<kRowShl1>(convE, convE);
+            }
             if constexpr (LOCAL && CDRIFT) runmax = L::add(runmax, vcg);   // into this step's frame
             if (!MASKED || uint32_t(c) < uint32_t(N)) {
                 V Eup = L::from(Ein);
@@ -223,20 +281,26 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             }
             if constexpr (!LOCAL) {
                 // capture events are rare: test the whole wave first
-                if (__builtin_expect(__builtin_amdgcn_ballot_w64(c == cap.ev) != 0, 0))
+                if (__builtin_expect(__builtin_amdgcn_ballot_w64(k == kev) != 0, 0)) {
                     cap.at_column(c, st, tk, a.scores, dge, -cg * (k % G));
+                    kev = cap.ev == INT32_MAX ? INT32_MAX : cap.ev + t;
+                }
             }
-            qH = dpp<kRowShl1>(Hout, qH);
-            qE = dpp<kRowShl1>(Eout, qE);
+            if constexpr (!ONE) {
+                qH = dpp<kRowShl1>(Hout, qH);
+                qE = dpp<kRowShl1>(Eout, qE);
+            }
             finish_scores<W, R, SCORE>(st, Bnext, tbl_lo, tbl_hi, rd, sc);
             Bin = Bnext;
         };
         auto chunk_end = [&](int k0) {
             conv = nextB;
             nextB = colp[k0 + 2 * G + 1 + t];
-            convH = nextH;
-            convE = nextE;
-            top_chunk(k0 + 2 * G + t, nextH, nextE);
+            if constexpr (!ONE) {
+                convH = nextH;
+                convE = nextE;
+                top_chunk(k0 + 2 * G + t, nextH, nextE);
+            }
             if (has_bnd_out) {
                 // lane G-1 computed column k0 + j - (G-1) at the chunk's step j; stored without
                 // the column drift it was sent with (cg*(j+1))
@@ -261,22 +325,30 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 Hout = L::bits(L::add(L::from(Hout), vrebase));
                 Eout = L::bits(L::add(L::from(Eout), vrebase));
                 if constexpr (LOCAL) runmax = L::add(runmax, vrebase);
+                if constexpr (ONE) {
+                    topH = L::bits(L::add(L::from(topH), vrebase));
+                    topE = L::bits(L::add(L::from(topE), vrebase));
+                }
             }
         };
 
         // whole chunks of G steps; the first is masked (lanes start one step apart)
         const int nsteps = ((N + 2 * G - 2) / G) * G;
         constexpr bool kMaskAll = ROWMAX || TRACK || EXPORT;
+        auto sweep = [&]<bool TOPSTEP>() {
 #pragma unroll 1
-        for (int u = 0; u < G; ++u) step.template operator()<true>(u);
-        chunk_end(0);
-        for (int k0 = G; k0 < nsteps; k0 += G) {
-            // two steps per iteration: the register allocator then alternates the exchanged
-            // values between two sets of registers instead of copying them back every step
+            for (int u = 0; u < G; ++u) step.template operator()<true, TOPSTEP>(u);
+            chunk_end(0);
+            for (int k0 = G; k0 < nsteps; k0 += G) {
+                // two steps per iteration: the register allocator then alternates the exchanged
+                // values between two sets of registers instead of copying them back every step
 #pragma unroll 2
-            for (int u = 0; u < G; ++u) step.template operator()<kMaskAll>(k0 + u);
-            chunk_end(k0);
-        }
+                for (int u = 0; u < G; ++u) step.template operator()<kMaskAll, TOPSTEP>(k0 + u);
+                chunk_end(k0);
+            }
+        };
+        if (ONE && top_step != 0) sweep.template operator()<ONE>();
+        else sweep.template operator()<false>();
         // free last row only: the int16 saturation tracking (TRACK) keeps per-row maxima too, but
         // they are not candidates for the score
         if constexpr (ROWMAX) cap.row_maximum(st, tk, a.scores, dge);
@@ -374,6 +446,10 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     const uint32_t lane_bank = (threadIdx.x & 31u) * 4u;
     const int go = a.go, ge = a.ge;
     const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
+    if (a.self_init) {   // uniform: every wave reaches the barrier
+        if (threadIdx.x == 0) init_task_pairs<W, LOCAL>(a, tk, fc, fr, lc, LOCAL ? 0 : a.lr);
+        __syncthreads();
+    }
     constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
     constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
     // column drift: on top of the row drift for the global method, alone for the local method
@@ -647,6 +723,18 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
 // Launch dispatch
 // ---------------------------------------------------------------------------------------------
 
+// The striped kernel of a variant, with or without the single-stripe specialisation.
+template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
+static void launch_striped(const VariantKey &k, const KernelArgs &a, dim3 blocks, dim3 threads, uint32_t lds,
+                           hipStream_t st)
+{
+    if (k.single)
+        hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX, false, true>), blocks, threads, lds, st,
+                           a);
+    else
+        hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX>), blocks, threads, lds, st, a);
+}
+
 template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false>
 static hipError_t launch_variant(const VariantKey &k, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st)
 {
@@ -664,8 +752,7 @@ static hipError_t launch_variant(const VariantKey &k, const KernelArgs &a, uint3
             return hipErrorInvalidValue;
         } else {
             if (k.G != 16) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(blocks), dim3(threads),
-                               lds, st, a);
+            launch_striped<W, R, SCORE, LOCAL, TRACK, ROWMAX>(k, a, dim3(blocks), dim3(threads), lds, st);
             return hipGetLastError();
         }
     } else {
@@ -674,8 +761,7 @@ static hipError_t launch_variant(const VariantKey &k, const KernelArgs &a, uint3
                                dim3(64 * kWfWaves), lds, st, a);
             return hipGetLastError();
         }
-        hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX>), dim3(blocks), dim3(threads), lds,
-                           st, a);
+        launch_striped<W, R, SCORE, LOCAL, TRACK, ROWMAX>(k, a, dim3(blocks), dim3(threads), lds, st);
         return hipGetLastError();
     }
 }

@@ -20,6 +20,7 @@ struct KernelArgs {
     int32_t *scores;
     uint32_t *flags;            // per pair; flags[error_slot] is set when a bounded wait gives up
     uint32_t error_slot;
+    uint32_t self_init;         // the DP kernels write their pairs' initial scores and flags
     int32_t *maxh;              // nullable: per-pair maximum of H over the matrix (LOCAL or TRACK variants)
     const int16_t *table;
     uint32_t table_entries;

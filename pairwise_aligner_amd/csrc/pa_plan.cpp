@@ -16,13 +16,15 @@ namespace pa {
 
 bool VariantKey::operator<(const VariantKey &o) const
 {
-    return std::tie(width, G, R, score, local, track, rowmax, export_dp) <
-           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, o.export_dp);
+    return std::make_tuple(width, G, R, score, local, track, rowmax, uint8_t(export_dp), uint8_t(single)) <
+           std::make_tuple(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, uint8_t(o.export_dp),
+                           uint8_t(o.single));
 }
 bool VariantKey::operator==(const VariantKey &o) const
 {
-    return std::tie(width, G, R, score, local, track, rowmax, export_dp) ==
-           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, o.export_dp);
+    return std::make_tuple(width, G, R, score, local, track, rowmax, uint8_t(export_dp), uint8_t(single)) ==
+           std::make_tuple(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, uint8_t(o.export_dp),
+                           uint8_t(o.single));
 }
 
 // Rows per lane that kernels are instantiated for (lanes per group G = 16).
@@ -572,6 +574,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         key.track = (!local && ((width == 16 && !upper16) || track_all ||
                                 (width == 16 && rc_candidate && !recentre_ok))) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;
+        key.single = (key.G == 16 && !export_dp && pm.m <= uint32_t(key.G) * key.R) ? 1 : 0;
     });
     if (out_of_range) { err = "scores may leave the int32 range"; return PA_ERR_RANGE; }
     // Group the pairs by variant: groups in VariantKey order, pairs in pair order within a group
@@ -621,13 +624,16 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         }
         items[g].resize(total);
     }
-    struct Counts { uint64_t cells = 0, n16 = 0, n32 = 0, spec = 0, prof = 0; };
+    struct Counts { uint64_t cells = 0, n16 = 0, n32 = 0, spec = 0, prof = 0, empty = 0; };
     std::vector<Counts> blk_counts(nblk);
     parallel_for(nblk, [&](size_t k) {
         Counts &c = blk_counts[k];
         for (size_t p = k * kPairBlock; p < std::min<size_t>(N, (k + 1) * kPairBlock); ++p) {
             const PairMeta &pm = plan.pairs[p];
-            if (pm.width == 0) continue;
+            if (pm.width == 0) {
+                if (selected[p]) ++c.empty;
+                continue;
+            }
             const uint32_t g = gid[p];
             const VariantKey &key = keys[g];
             const uint32_t rows = uint32_t(key.G) * key.R;
@@ -644,6 +650,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         plan.n32 += c.n32;
         plan.n_speculative += c.spec;
         plan.n_profiled += c.prof;
+        plan.n_empty += c.empty;
     }
 
     for (int c = 0; c < 256; ++c) plan.code[c] = alpha.code[c];

@@ -125,7 +125,9 @@ struct VariantKey {
     uint8_t local;   // method_local
     uint8_t track;   // int16 global: keep the running maximum to detect saturation
     uint8_t rowmax;  // global with trailing gaps free in both dimensions: per-row maxima
-    uint8_t export_dp;   // also write the final DP row and column of every pair (int32, R = 8)
+    uint8_t export_dp : 1;   // also write the final DP row and column of every pair (int32, R = 8)
+    uint8_t single : 1;      // striped kernel, every task one stripe (m <= G*R): no boundary rows
+    uint8_t unused_ : 6;
     bool operator<(const VariantKey &o) const;
     bool operator==(const VariantKey &o) const;
 };
@@ -198,6 +200,7 @@ struct Plan {
     uint64_t n16 = 0, n32 = 0;
     uint64_t n_speculative = 0;               // int16 pairs whose bound does not exclude saturation
     uint64_t n_profiled = 0;                  // int16 pairs scored through the query profile
+    uint64_t n_empty = 0;                     // planned pairs with an empty sequence (no task)
     FillVector<EncodeTask> enc;               // per task: residue sources for the device encoder
     // DP export: per task, the index in the output of H(0, n) (the task's last column, m+1
     // entries) and of H(m, 0) (its last row, n+1 entries).  The output holds the caller's
