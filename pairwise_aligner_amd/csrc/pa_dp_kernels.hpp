@@ -94,9 +94,11 @@ __device__ __forceinline__ void init_task_pairs(const KernelArgs &a, const Task 
 }
 
 // EXPORT: also write the task's final DP column and row (int32 lanes; pa_plan.hpp, dp_base).
-// ONE: every task of the launch is one stripe (m <= G*R): no boundary rows through HBM, and lane
-// 0's top boundary (the initial DP row) kept in two registers instead of conveyors.
-template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false, bool ONE = false>
+// A task of one stripe (m <= G*R; the group's choice, uniform over its 16 lanes) takes a sweep
+// without boundary rows through HBM, whose lane 0 keeps its top boundary (the initial DP row) in
+// two registers instead of conveyors.  Tasks are ordered by decreasing work, so the one- and
+// multi-stripe tasks of a launch share at most one wave.
+template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false>
 __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 {
     using L = Lane<W>;
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     // column drift: on top of the row drift for the global method, alone for the local method
     // (whose zero clamp and running maximum then work in the step's frame)
     constexpr bool CDRIFT = !ACC && !EXPORT;
-    static_assert(!(ONE && EXPORT), "the DP export keeps its boundary conveyors");
+
     const int dge = DRIFT ? ge : 0;
     const int cg = CDRIFT ? -ge : 0;         // column drift per step
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
@@ -153,7 +155,8 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     V satmax = L::lowest();                         // drifted (int16 saturation test)
     const V vcg = L::splat(cg);
 
-    for (uint32_t s = 0; s < tk.S; ++s) {
+    // one stripe of the task; ONE: the task has no other
+    auto stripe = [&]<bool ONE>(uint32_t s) {
         const int row0 = int(s) * ROWS + t * R;   // DP row above this lane's block
         Rows<W, R, ACC> st;
         // lane t starts at step t: its initial column (j = 0) counts as computed at step t-1
@@ -161,10 +164,13 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                                                    cg * (t - 1));
         V Hdiag = L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1));
         Capture<W, R> cap;
-        if constexpr (!LOCAL) cap.init(tk, row0, lc);
         // the step at which this lane's next capture event comes (events are rare: one wave
         // test per step against it)
-        int kev = cap.ev == INT32_MAX ? INT32_MAX : cap.ev + t;
+        int kev = INT32_MAX;
+        if constexpr (!LOCAL) {
+            cap.init(tk, row0, lc);
+            kev = cap.ev == INT32_MAX ? INT32_MAX : cap.ev + t;
+        }
 
         const bool has_bnd_in = !ONE && s > 0;
         const bool has_bnd_out = !ONE && s + 1 < tk.S;
@@ -358,6 +364,11 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
+    };
+    if (!EXPORT && tk.S == 1) {
+        stripe.template operator()<!EXPORT>(0);
+    } else {
+        for (uint32_t s = 0; s < tk.S; ++s) stripe.template operator()<false>(s);
     }
 
     if constexpr (LOCAL || TRACK) {
@@ -723,16 +734,11 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
 // Launch dispatch
 // ---------------------------------------------------------------------------------------------
 
-// The striped kernel of a variant, with or without the single-stripe specialisation.
 template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>
-static void launch_striped(const VariantKey &k, const KernelArgs &a, dim3 blocks, dim3 threads, uint32_t lds,
+static void launch_striped(const VariantKey &, const KernelArgs &a, dim3 blocks, dim3 threads, uint32_t lds,
                            hipStream_t st)
 {
-    if (k.single)
-        hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX, false, true>), blocks, threads, lds, st,
-                           a);
-    else
-        hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX>), blocks, threads, lds, st, a);
+    hipLaunchKernelGGL((stripe_kernel<W, 16, R, SCORE, LOCAL, TRACK, ROWMAX>), blocks, threads, lds, st, a);
 }
 
 template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false>

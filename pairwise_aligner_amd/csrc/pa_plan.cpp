@@ -16,15 +16,13 @@ namespace pa {
 
 bool VariantKey::operator<(const VariantKey &o) const
 {
-    return std::make_tuple(width, G, R, score, local, track, rowmax, uint8_t(export_dp), uint8_t(single)) <
-           std::make_tuple(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, uint8_t(o.export_dp),
-                           uint8_t(o.single));
+    return std::tie(width, G, R, score, local, track, rowmax, export_dp) <
+           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, o.export_dp);
 }
 bool VariantKey::operator==(const VariantKey &o) const
 {
-    return std::make_tuple(width, G, R, score, local, track, rowmax, uint8_t(export_dp), uint8_t(single)) ==
-           std::make_tuple(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, uint8_t(o.export_dp),
-                           uint8_t(o.single));
+    return std::tie(width, G, R, score, local, track, rowmax, export_dp) ==
+           std::tie(o.width, o.G, o.R, o.score, o.local, o.track, o.rowmax, o.export_dp);
 }
 
 // Rows per lane that kernels are instantiated for (lanes per group G = 16).
@@ -361,14 +359,22 @@ void code_tables(const Plan &plan, const VariantKey &key, CodeTables &out)
 void order_launches(Plan &plan)
 {
     // Wavefront launches first: each of their workgroups is one long serial sweep, so they
-    // set the critical path (longest task first).  Then the striped launches by decreasing
-    // work, so that the tail of the pass is made of short tasks.  Streams round robin.
+    // set the critical path (longest task first).  Then the striped launches by the time each
+    // would take on its own: its longest task, or its whole work spread over the groups the GPU
+    // holds at once, whichever is larger -- so that neither a launch of a few long tasks nor a
+    // large launch of short ones ends up in the tail of the pass.  Streams round robin.
+    constexpr double kResidentGroups = 1024.0 * 3 * 4;   // SIMDs x waves per SIMD x groups per wave
     const size_t nl = plan.launches.size();
-    std::vector<double> longest(nl, 0.0);
+    std::vector<double> longest(nl, 0.0), alone(nl, 0.0);
     for (size_t i = 0; i < nl; ++i) {
         const Launch &L = plan.launches[i];
-        for (uint32_t t = L.task_begin; t < L.task_end; ++t)
-            longest[i] = std::max(longest[i], task_cost(L.key, plan.tasks[t], plan.tuning));
+        double total = 0;
+        for (uint32_t t = L.task_begin; t < L.task_end; ++t) {
+            const double c = task_cost(L.key, plan.tasks[t], plan.tuning);
+            longest[i] = std::max(longest[i], c);
+            total += c;
+        }
+        alone[i] = std::max(longest[i], total / kResidentGroups);
     }
     plan.launch_order.resize(nl);
     std::iota(plan.launch_order.begin(), plan.launch_order.end(), 0u);
@@ -377,6 +383,7 @@ void order_launches(Plan &plan)
         const bool wa = a.key.G == 64, wb = b.key.G == 64;
         if (wa != wb) return wa;
         if (wa) return longest[x] > longest[y];
+        if (alone[x] != alone[y]) return alone[x] > alone[y];
         return a.padded_cells > b.padded_cells;
     });
     // With two or more streams the wavefront launches share stream 0 and the striped launches
@@ -574,7 +581,6 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         key.track = (!local && ((width == 16 && !upper16) || track_all ||
                                 (width == 16 && rc_candidate && !recentre_ok))) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;
-        key.single = (key.G == 16 && !export_dp && pm.m <= uint32_t(key.G) * key.R) ? 1 : 0;
     });
     if (out_of_range) { err = "scores may leave the int32 range"; return PA_ERR_RANGE; }
     // Group the pairs by variant: groups in VariantKey order, pairs in pair order within a group

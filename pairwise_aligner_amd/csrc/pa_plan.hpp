@@ -125,9 +125,7 @@ struct VariantKey {
     uint8_t local;   // method_local
     uint8_t track;   // int16 global: keep the running maximum to detect saturation
     uint8_t rowmax;  // global with trailing gaps free in both dimensions: per-row maxima
-    uint8_t export_dp : 1;   // also write the final DP row and column of every pair (int32, R = 8)
-    uint8_t single : 1;      // striped kernel, every task one stripe (m <= G*R): no boundary rows
-    uint8_t unused_ : 6;
+    uint8_t export_dp;   // also write the final DP row and column of every pair (int32, R = 8)
     bool operator<(const VariantKey &o) const;
     bool operator==(const VariantKey &o) const;
 };
