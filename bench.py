@@ -402,7 +402,8 @@ def bench_mixed(count, steps, warmup, side_steps=5):
     dna = device.DeviceConfig(method=device.METHOD_LOCAL, match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
     prot = blosum_config(device.METHOD_LOCAL)
     t1 = time.perf_counter()
-    sets = [(prot, learned.training_pairs("protein", 1001, 4000)), (dna, learned.training_pairs("dna", 1002, 2000))]
+    sets = [(prot, learned.training_pairs("protein", 1001, 4000)), (dna, learned.training_pairs("dna", 1002, 2000)),
+            (prot, learned.training_pairs("protein", 1003, 1000, 5000, 8000))]
     x, y = learned.build_dataset(ctx, sets)
     model, losses = learned.train(ctx, x, y, steps=1500, batch=2048, log=log)
     train_s = time.perf_counter() - t1
@@ -414,14 +415,18 @@ def bench_mixed(count, steps, warmup, side_steps=5):
            "train_loss_first_last": [round(float(np.mean(losses[:20])), 5), round(float(np.mean(losses[-20:])), 5)],
            "policies": {}}
     ref_scores = {}
-    legs = (("learned", device.LANES_LEARNED, steps, warmup), ("int32", device.LANES_INT32, min(steps, side_steps), 1),
-            ("int16_speculative", device.LANES_INT16, min(steps, side_steps), 1),
-            ("bound", device.LANES_AUTO, min(steps, side_steps), 1))
-    for name, policy, k, wu in legs:
+    side = min(steps, side_steps)
+    # the last leg orders each launch's tasks by the network's predicted cost instead of the
+    # geometric model (an A/B of the cost output; the planner's default is the geometric order)
+    legs = (("learned", device.LANES_LEARNED, steps, warmup, None), ("int32", device.LANES_INT32, side, 1, None),
+            ("int16_speculative", device.LANES_INT16, side, 1, None), ("bound", device.LANES_AUTO, side, 1, None),
+            ("learned_cost_order", device.LANES_LEARNED, side, 1, device.Tuning(cost_order=1)))
+    for name, policy, k, wu, tuning in legs:
         total_ms = kernel_ms = 0.0
         stats, mism = [], 0
         for kind, dc, h in halves:
-            dcp = dataclasses.replace(dc, lane_policy=policy, model=model if policy == device.LANES_LEARNED else None)
+            dcp = dataclasses.replace(dc, lane_policy=policy, model=model if policy == device.LANES_LEARNED else None,
+                                      tuning=tuning)
             r = gpu_run(ctx, h["b1"], h["o1"], h["b2"], h["o2"], dcp, None, k, wu)
             total_ms += r["ms_per_step"]
             kernel_ms += r["kernel_ms"]
@@ -458,7 +463,40 @@ def bench_mixed(count, steps, warmup, side_steps=5):
         bad += int(np.count_nonzero(ref_scores[kind][:k].astype(np.int64) != want))
     out["oracle_sample_mismatches"] = bad
     out["oracle_sample_pairs"] = 80
+    out["saturating_slice"] = bench_saturating(model, prot, min(steps, side_steps))
     model.close()
+    return out
+
+
+def bench_saturating(model, prot, steps, count=20_000):
+    """Where the lane choice decides: 20,000 protein pairs of 5-8 kbp at 80-100% identity, about
+    a quarter of which score beyond the int16 range.  The same four policies with the same trained
+    network; int16 speculation pays an int32 recomputation for every saturated pair (the batch's
+    first run plans it, the timed runs replay it)."""
+    t0 = time.perf_counter()
+    w = workloads.config_saturating(count=count)
+    cells = cells_of(w["o1"], w["o2"])
+    log(f"[bench] saturating: {count} pairs, {cells:.3e} cells, generated in {time.perf_counter() - t0:.1f} s")
+    ctx = device.default_context(0)
+    out = {"workload": "20,000 protein pairs 5-8 kbp, identity 80-100%, local BLOSUM62 gap (-11,-1)",
+           "pairs": count, "cells": cells, "policies": {}}
+    ref = None
+    for name, policy in (("learned", device.LANES_LEARNED), ("int32", device.LANES_INT32),
+                         ("int16_speculative", device.LANES_INT16), ("bound", device.LANES_AUTO)):
+        dc = dataclasses.replace(prot, lane_policy=policy, model=model if policy == device.LANES_LEARNED else None)
+        r = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, None, steps, 1)
+        mism = 0 if ref is None else int(np.count_nonzero(ref != r["scores"]))
+        ref = r["scores"] if ref is None else ref
+        out["policies"][name] = {"gcups": round(cells / (r["ms_per_step"] / 1e3) / 1e9, 2),
+                                 "ms_per_step": round(r["ms_per_step"], 3), "steps": steps,
+                                 "n_int16": r["stats"]["n_int16"], "n_int32": r["stats"]["n_int32"],
+                                 "n_recomputed": r["stats"]["n_recomputed"], "score_mismatches_vs_learned": mism}
+        log(f"[bench] saturating/{name}: {out['policies'][name]}")
+    pol = out["policies"]
+    out["scores_at_int16_limit"] = int(np.count_nonzero(ref >= 32767))
+    for name in ("learned", "int16_speculative", "bound"):
+        out[f"speedup_{name}_vs_int32"] = round(pol[name]["gcups"] / pol["int32"]["gcups"], 3)
+    out["speedup_learned_vs_int16_speculative"] = round(pol["learned"]["gcups"] / pol["int16_speculative"]["gcups"], 3)
     return out
 
 

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PA_ABI_VERSION 4
+#define PA_ABI_VERSION 5
 
 typedef enum pa_status {
     PA_OK = 0,
@@ -56,6 +56,8 @@ typedef struct pa_tuning {
     uint32_t wavefront_min_rows;  /* pairs with more DP rows use the wavefront kernel [3072] */
     uint32_t streams;             /* concurrent launch streams, 1..4 [4] */
     uint32_t eager;               /* 1: launch kernels one by one; 0: replay a captured hipGraph [0] */
+    uint32_t cost_order;          /* PA_LANES_LEARNED: 1 orders the tasks of a launch by the network's predicted
+                                     cost instead of the geometric model's [0] */
 } pa_tuning;
 
 typedef struct pa_config {

@@ -443,8 +443,10 @@ pa_status device_features(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView
 
 // Lane widths from the learned range predictor: int16 where the bound proves it, or where the
 // predicted maximum of H with a 10% + 256 margin stays below the int16 limit (then the pair is
-// tracked and recomputed in int32 if it saturates after all); int32 otherwise.
-pa_status learned_widths(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView &v, std::vector<uint8_t> &forced)
+// tracked and recomputed in int32 if it saturates after all); int32 otherwise.  `cost` receives
+// the cost output, in ns (y1 = log2(cost + 1) / 32).
+pa_status learned_widths(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView &v, std::vector<uint8_t> &forced,
+                         std::vector<float> &cost)
 {
     if (!cfg.model) return fail(ctx, PA_ERR_INVALID_ARGUMENT, "PA_LANES_LEARNED needs pa_config.model");
     DeviceBuffer dfeat, dy;
@@ -461,10 +463,12 @@ pa_status learned_widths(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView 
     PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     const double smax = std::max(1, pa::max_substitution(cfg));
     forced.assign(v.n_pairs, 0);
+    cost.assign(v.n_pairs, 0.0f);
     for (uint64_t p = 0; p < v.n_pairs; ++p) {
         const double mn = double(std::min(v.len1(p), v.off2[p + 1] - v.off2[p]));
         const double pred = std::max(0.0, double(y[p * pa::kModelOutputs])) * smax * mn;
         forced[p] = (pred * 1.10 + 256.0 < 32767.0) ? 16 : 32;
+        cost[p] = float(std::exp2(std::min(40.0, std::max(0.0, double(y[p * pa::kModelOutputs + 1]) * 32.0))) - 1.0);
     }
     return PA_OK;
 }
@@ -562,9 +566,11 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
                        pair_method ? b->pair_method.data() : nullptr, one_query};
     std::string err;
     std::vector<uint8_t> forced;
+    std::vector<float> cost;
     pa_status st;
     if (cfg->lane_policy == PA_LANES_LEARNED) {
-        if ((st = learned_widths(ctx, b->cfg, view, forced)) != PA_OK) return st;
+        if ((st = learned_widths(ctx, b->cfg, view, forced, cost)) != PA_OK) return st;
+        if (!pa::resolve_tuning(cfg->tuning).cost_order) cost.clear();
     }
     b->export_dp = export_dp;
     // The residue bytes go to the device while the host plans (the upload needs nothing from
@@ -572,7 +578,7 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
     hipError_t upload_err = hipSuccess;
     std::thread uploader([&] { upload_err = copy_residues(ctx->device, ctx->stream, view, b->residues, arena); });
     st = pa::make_plan(b->cfg, view, forced.empty() ? nullptr : forced.data(), nullptr, b->main.plan, err, false,
-                       export_dp);
+                       export_dp, cost.empty() ? nullptr : cost.data());
     uploader.join();
     if (upload_err != hipSuccess) return hip_fail(ctx, upload_err, "residue upload");
     if (st != PA_OK) return fail(ctx, st, err);
@@ -929,6 +935,7 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
         c.wavefront_min_rows = t.wavefront_min_rows;
         c.streams = t.streams;
         c.eager = t.eager ? 1 : 0;
+        c.cost_order = t.cost_order ? 1 : 0;
         return c;
     };
     // time of `iters` passes with tuning t (after one untimed pass), on the sample or the batch

@@ -69,6 +69,15 @@ __device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src)
     return uint32_t(__builtin_amdgcn_update_dpp(int(old), int(src), CTRL, 0xF, 0xF, false));
 }
 
+// DPP move whose lanes without a source take an unspecified value: for a conveyor shifted towards
+// lane 0, whose vacated lane is refilled (at the chunk end) before anything reads it.  Leaves the
+// register allocator free to place the result (no copy of a tied old value).
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_any(uint32_t src)
+{
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(src), CTRL, 0xF, 0xF, false));
+}
+
 __device__ __forceinline__ int h0_value(int i, int free_flag, int go, int ge)
 {
     return (i == 0 || free_flag) ? 0 : go + ge * i;

@@ -135,7 +135,11 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
     // forward by one step: the next lane uses them one step later)
     const V vout = L::splat((DRIFT ? ge * R : 0) + (CDRIFT ? -ge : 0));
     const V vrebase = L::splat(ge * G);      // end of a chunk: the column drift drops by G|ge|
-    const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
+    uint32_t tbl_lo = a.tbl_lo;
+    const uint32_t tbl_hi = a.tbl_hi;
+    // the permute's second table word must be a VGPR (one scalar operand per instruction): keep
+    // it in one for the whole kernel instead of moving it from its scalar register every step
+    if constexpr (SCORE == kScoreDna) asm volatile("" : "+v"(tbl_lo));
     const int N = int(tk.N);
     constexpr int ROWS = G * R;
     static_assert(!EXPORT || W == 32, "the DP export runs in int32 lanes");
@@ -239,8 +243,10 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         // TOPSTEP (ONE only): lane 0's top boundary changes from step to step (top_step != 0)
         auto step = [&]<bool MASKED, bool TOPSTEP>(int k) {
             const int c = k - t;   // DP column (0-based) of this lane at this step
+            // the conveyor moves on first, so that the old register is free for lane 0's value
+            const uint32_t shifted = dpp_any<kRowShl1>(conv);
             const uint32_t Bnext = dpp<kRowShr1>(conv, Bin);
-            conv = dpp<kRowShl1>(conv, conv);
+            conv = shifted;
             issue_scores<W, R, SCORE>(st, Bnext, lds_raw, rd);
             uint32_t Hin, Ein;
             if constexpr (ONE) {
@@ -254,10 +260,11 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                     topE = L::bits(L::add(L::from(topE), vtop_step));
                 }
             } else {
+                const uint32_t nextH_ = dpp_any<kRowShl1>(convH), nextE_ = dpp_any<kRowShl1>(convE);
                 Hin = dpp<kRowShr1>(convH, Hout);
                 Ein = dpp<kRowShr1>(convE, Eout);
-                convH = dpp<kRowShl1>(convH, convH);
-                convE = dpp<kRowShl1>(convE, convE);
+                convH = nextH_;
+                convE = nextE_;
             }
             if constexpr (LOCAL && CDRIFT) runmax = L::add(runmax, vcg);   // into this step's frame
             if (!MASKED || uint32_t(c) < uint32_t(N)) {
@@ -479,7 +486,11 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     // forward by one step: the next lane uses them one step later)
     const V vout = L::splat((DRIFT ? ge * R : 0) + (CDRIFT ? -ge : 0));
     const V vrebase = L::splat(ge * G);      // end of a chunk: the column drift drops by G|ge|
-    const uint32_t tbl_lo = a.tbl_lo, tbl_hi = a.tbl_hi;
+    uint32_t tbl_lo = a.tbl_lo;
+    const uint32_t tbl_hi = a.tbl_hi;
+    // the permute's second table word must be a VGPR (one scalar operand per instruction): keep
+    // it in one for the whole kernel instead of moving it from its scalar register every step
+    if constexpr (SCORE == kScoreDna) asm volatile("" : "+v"(tbl_lo));
     const int N = int(tk.N);
     const int S = int(tk.S);
     const uint32_t *colp = a.colcodes + tk.col_off;
@@ -577,8 +588,9 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
 #pragma unroll 2
             for (int u = 0; u < G; ++u) {
                 const int c = k0 + u - t;
+                const uint32_t shifted = dpp_any<kWaveShl1>(conv);
                 const uint32_t Bnext = dpp<kWaveShr1>(conv, Bin);
-                conv = dpp<kWaveShl1>(conv, conv);
+                conv = shifted;
                 issue_scores<W, R, SCORE>(st, Bnext, lds_raw, rd);
                 uint32_t topH, topE;
                 if (src != 0) {
@@ -594,12 +606,12 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                     topE = L::top_word(hb_top + go + ge - dge + cg * u);   // drifted by |ge| for row 0
                     hb_top = fr ? 0 : hb_top + ge;
                 }
+                if (src != 0) {
+                    convH = dpp_any<kWaveShl1>(convH);
+                    convE = dpp_any<kWaveShl1>(convE);
+                }
                 const uint32_t Hin = dpp<kWaveShr1>(topH, Hout);
                 const uint32_t Ein = dpp<kWaveShr1>(topE, Eout);
-                if (src != 0) {
-                    convH = dpp<kWaveShl1>(convH, convH);
-                    convE = dpp<kWaveShl1>(convE, convE);
-                }
                 if constexpr (LOCAL) runmax = L::add(runmax, vcg);   // into this step's frame
                 if (uint32_t(c) < uint32_t(N)) {
                     V Eup = L::from(Ein);

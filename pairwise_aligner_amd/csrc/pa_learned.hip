@@ -5,7 +5,8 @@
 // Features of pair (s1, s2) with lengths (m, n), computed by one wave per pair:
 //   0 log2(m + 1) / 16           1 log2(n + 1) / 16          2 min(m, n) / max(m, n)
 //   3 hashed 3-mer composition distance: L1 distance of the normalised 64-bin histograms / 2
-//   4 k-mer containment: fraction of the k-mers of s2 found in a 32768-bit set of s1's k-mers
+//   4 minimizer containment: the fraction of s2's (w = 8, k)-minimizers -- the smallest hashed
+//     k-mer of every window of w consecutive k-mers -- found in a 32768-bit set of s1's
 //     (k = 8 over 2-bit codes for alphabets of at most four symbols, else k = 3 over 5-bit codes)
 //   5 identity estimate containment^(1/k)
 //   6 max substitution score / 16                        7 1
@@ -22,7 +23,8 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + F * HID, OFF_W2 = OFF_B1 + HID, OFF_
               OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID * OUT, NPARAM = OFF_B3 + OUT;
 static_assert(NPARAM == kModelParams, "parameter layout");
 
-constexpr int kSetWords = 1024;   // 32768-bit k-mer set per wave
+constexpr int kSetWords = 1024;   // 32768-bit minimizer set per wave
+constexpr int kMinWindow = 8;     // k-mers per minimizer window
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x)
 {
@@ -30,51 +32,84 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x)
     return x;
 }
 
+// The (kMinWindow, k)-minimizers of sequence s (length len), visited by one wave: for every
+// minimizer, in order, visit(hash) is called on one lane.  A tile of 64 consecutive k-mers is
+// loaded one per lane; window j (its first k-mer at tile position j <= 64 - kMinWindow) takes the
+// smallest (hash, position) key of its k-mers by three shuffle steps; a window whose minimizer
+// sits at another position than its predecessor's contributes a new minimizer (the positions of
+// sliding-window minimizers never decrease).  Ties go to the leftmost k-mer.  A sequence with
+// fewer k-mers than a window has one window over all of them.
+template <typename Visit>
+__device__ __forceinline__ void for_each_minimizer(const uint8_t *s, int len, const FeatureArgs &a, Visit visit)
+{
+    const int lane = threadIdx.x % 64;
+    const int k = a.k, bits = a.bits;
+    const uint32_t kmask = (1u << (k * bits)) - 1u, smask = (1u << bits) - 1u;
+    const int kmers = len - k + 1;
+    if (kmers <= 0) return;
+    const int windows = kmers > kMinWindow ? kmers - kMinWindow + 1 : 1;
+    constexpr int kStep = 64 - kMinWindow + 1;   // windows per tile
+    uint64_t prev = ~0ull;                        // key of the previous window's minimizer
+    for (int base = 0; base < windows; base += kStep) {
+        const int i = base + lane;                // this lane's k-mer
+        uint64_t key = ~0ull;
+        if (i < kmers) {
+            uint32_t v = 0;
+            for (int j = 0; j < k; ++j) v = (v << bits) | (uint32_t(a.code[s[i + j]]) & smask);
+            key = (uint64_t(mix32(v & kmask)) << 32) | uint32_t(i);
+        }
+        // minimum over the kMinWindow k-mers starting at this lane
+        for (int d = 1; d < kMinWindow; d <<= 1) {
+            const uint64_t o = __shfl_down(key, d, 64);
+            key = (lane + d < 64 && o < key) ? o : key;
+        }
+        const bool valid = lane < kStep && base + lane < windows;
+        uint64_t before = __shfl_up(key, 1, 64);
+        if (lane == 0) before = prev;
+        if (valid && key != before) visit(uint32_t(key >> 32));
+        prev = __shfl(key, kStep - 1, 64);
+    }
+}
+
 __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a)
 {
     __shared__ uint32_t comp[4][2][64];
     __shared__ uint32_t kset[4][kSetWords];
-    __shared__ uint32_t hits[4];
+    __shared__ uint32_t hits[4], counts[4];
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const uint64_t p = uint64_t(blockIdx.x) * 4 + w;
     for (int i = lane; i < 128; i += 64) comp[w][i / 64][i % 64] = 0;
     for (int i = lane; i < kSetWords; i += 64) kset[w][i] = 0;
-    if (lane == 0) hits[w] = 0;
+    if (lane == 0) hits[w] = counts[w] = 0;
     __syncthreads();
     const bool live = p < a.n_pairs;
     const uint8_t *s1 = live ? a.seq1 + a.off1[p] : nullptr;
     const uint8_t *s2 = live ? a.seq2 + a.off2[p] : nullptr;
     const int m = live ? int(a.off1[p + 1] - a.off1[p]) : 0;
     const int n = live ? int(a.off2[p + 1] - a.off2[p]) : 0;
-    const int k = a.k, bits = a.bits;
-    const uint32_t kmask = (1u << (k * bits)) - 1u;
-    const uint32_t smask = (1u << bits) - 1u;
-    // 3-mer histograms and the k-mer set of s1
-    for (int i = lane; i < m; i += 64) {
-        if (i + 2 < m) {
-            const uint32_t h = mix32((uint32_t(s1[i]) << 16) | (uint32_t(s1[i + 1]) << 8) | s1[i + 2]) >> 26;
-            atomicAdd(&comp[w][0][h], 1u);
-        }
-        if (i + k <= m) {
-            uint32_t v = 0;
-            for (int j = 0; j < k; ++j) v = (v << bits) | (uint32_t(a.code[s1[i + j]]) & smask);
-            const uint32_t h = mix32(v & kmask) & (kSetWords * 32 - 1);
-            atomicOr(&kset[w][h >> 5], 1u << (h & 31));
-        }
+    const int k = a.k;
+    // 3-mer histograms and the minimizer set of s1
+    for (int i = lane; i + 2 < m; i += 64) {
+        const uint32_t h = mix32((uint32_t(s1[i]) << 16) | (uint32_t(s1[i + 1]) << 8) | s1[i + 2]) >> 26;
+        atomicAdd(&comp[w][0][h], 1u);
     }
     for (int i = lane; i + 2 < n; i += 64) {
         const uint32_t h = mix32((uint32_t(s2[i]) << 16) | (uint32_t(s2[i + 1]) << 8) | s2[i + 2]) >> 26;
         atomicAdd(&comp[w][1][h], 1u);
     }
+    for_each_minimizer(s1, m, a, [&](uint32_t h) {
+        h &= kSetWords * 32 - 1;
+        atomicOr(&kset[w][h >> 5], 1u << (h & 31));
+    });
     __syncthreads();
-    uint32_t local_hits = 0;
-    for (int i = lane; i + k <= n; i += 64) {
-        uint32_t v = 0;
-        for (int j = 0; j < k; ++j) v = (v << bits) | (uint32_t(a.code[s2[i + j]]) & smask);
-        const uint32_t h = mix32(v & kmask) & (kSetWords * 32 - 1);
+    uint32_t local_hits = 0, local_count = 0;
+    for_each_minimizer(s2, n, a, [&](uint32_t h) {
+        h &= kSetWords * 32 - 1;
         local_hits += (kset[w][h >> 5] >> (h & 31)) & 1u;
-    }
+        local_count += 1;
+    });
     atomicAdd(&hits[w], local_hits);
+    atomicAdd(&counts[w], local_count);
     __syncthreads();
     if (!live) return;
     // L1 distance of the normalised histograms, reduced over the wave
@@ -88,7 +123,7 @@ __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a)
         f[1] = log2f(float(n) + 1.0f) / 16.0f;
         f[2] = mx > 0 ? mn / mx : 0.0f;
         f[3] = (m > 2 && n > 2) ? d * 0.5f : 1.0f;
-        const float cont = (m >= k && n >= k) ? float(hits[w]) / float(n - k + 1) : 0.0f;
+        const float cont = (m >= k && n >= k && counts[w] > 0) ? float(hits[w]) / float(counts[w]) : 0.0f;
         f[4] = cont;
         f[5] = cont > 0 ? powf(cont, 1.0f / float(k)) : 0.0f;
         f[6] = a.smax / 16.0f;

@@ -38,6 +38,7 @@ Tuning resolve_tuning(const pa_tuning *t)
     if (t->wavefront_min_rows) r.wavefront_min_rows = std::max<uint32_t>(t->wavefront_min_rows, 256);
     if (t->streams) r.streams = std::min<uint32_t>(t->streams, 4);
     r.eager = t->eager != 0;
+    r.cost_order = t->cost_order != 0;
     return r;
 }
 
@@ -413,7 +414,7 @@ int32_t max_substitution(const pa_config &cfg)
 
 pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *forced_width,
                     const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all,
-                    bool export_dp)
+                    bool export_dp, const float *pair_cost)
 {
     if (cfg.method != PA_METHOD_GLOBAL && cfg.method != PA_METHOD_LOCAL) { err = "invalid method"; return PA_ERR_INVALID_ARGUMENT; }
     // The cell update drops V + go + ge from the vertical gap state (it never exceeds V + ge
@@ -707,7 +708,13 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
             const Pack &pk = group_packs[g][i];
             uint32_t S = it[pk.first].S, n = it[pk.first].n;
             if (pk.second >= 0) { S = std::max(S, it[pk.second].S); n = std::max(n, it[pk.second].n); }
-            weighed[g][i] = Weighed{uint64_t(S) * (n + G - 1), pk};
+            if (pair_cost) {   // the predicted cost of the pack's costlier pair, in tenths of a ns
+                float c = pair_cost[it[pk.first].pair];
+                if (pk.second >= 0) c = std::max(c, pair_cost[it[pk.second].pair]);
+                weighed[g][i] = Weighed{uint64_t(std::max(c, 0.0f) * 10.0f), pk};
+            } else {
+                weighed[g][i] = Weighed{uint64_t(S) * (n + G - 1), pk};
+            }
         }
     }, 1);
     parallel_sort(weighed, [](const Weighed &x, const Weighed &y) { return x.work > y.work; }, true);

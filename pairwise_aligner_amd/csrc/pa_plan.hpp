@@ -167,6 +167,7 @@ struct Tuning {
     uint32_t wavefront_min_rows = 3072;
     uint32_t streams = 4;
     bool eager = false;
+    bool cost_order = false;
 };
 Tuning resolve_tuning(const pa_tuning *t);
 
@@ -236,9 +237,11 @@ struct BatchView {
 // `only` restricts planning to a subset of pairs (nullptr = all).
 // `track_all` keeps the running maximum of H in every global variant (labels for the learned
 // range predictor).
+// `pair_cost` (nullable): a predicted cost per pair (the learned dispatcher's cost output); with
+// it the tasks of a launch are ordered by it instead of by the geometric model.
 pa_status make_plan(const pa_config &cfg, const BatchView &batch, const uint8_t *forced_width,
                     const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all = false,
-                    bool export_dp = false);
+                    bool export_dp = false, const float *pair_cost = nullptr);
 
 // Modelled sweep cost of one task (arbitrary units): stripes x steps x per-step work.
 double task_cost(const VariantKey &key, const Task &t, const Tuning &tuning);

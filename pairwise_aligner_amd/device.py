@@ -48,6 +48,7 @@ class CTuning(ctypes.Structure):
         ("wavefront_min_rows", ctypes.c_uint32),
         ("streams", ctypes.c_uint32),
         ("eager", ctypes.c_uint32),
+        ("cost_order", ctypes.c_uint32),
     ]
 
 
@@ -60,15 +61,16 @@ class Tuning:
     wavefront_min_rows: int = 0
     streams: int = 0
     eager: int = 0
+    cost_order: int = 0   # learned policy: order tasks by the predicted cost (1) or the geometric model (0)
 
     def to_c(self) -> CTuning:
         return CTuning(self.cell_cost, self.step_cost, self.rows_mask, self.wavefront_min_rows, self.streams,
-                       self.eager)
+                       self.eager, self.cost_order)
 
     @staticmethod
     def from_c(c: CTuning) -> "Tuning":
         return Tuning(round(c.cell_cost, 4), round(c.step_cost, 4), c.rows_mask, c.wavefront_min_rows, c.streams,
-                      c.eager)
+                      c.eager, c.cost_order)
 
 
 class CConfig(ctypes.Structure):

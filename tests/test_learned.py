@@ -74,6 +74,66 @@ def test_gpu_features():
     assert np.all(f[:, 7] == 1.0) and np.allclose(f[:, 6], 2 / 16)
 
 
+def _mix32(x):
+    x = np.asarray(x, dtype=np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7feb352d) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846ca68b) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def _minimizers(seq, code, k, bits, w=8):
+    """Positions' hashes of the sliding-window (w, k)-minimizers of seq: the smallest (hash,
+    position) of every window of w consecutive k-mers (one window over all k-mers when there are
+    fewer than w), each distinct position once."""
+    c = np.array([code[b] for b in seq], dtype=np.uint64) & ((1 << bits) - 1)
+    K = len(seq) - k + 1
+    if K <= 0:
+        return []
+    v = np.zeros(K, dtype=np.uint64)
+    for j in range(k):
+        v = (v << np.uint64(bits)) | c[j:j + K]
+    h = _mix32(v & ((1 << (k * bits)) - 1))
+    keys = (h << np.uint64(32)) | np.arange(K, dtype=np.uint64)
+    nw = K - w + 1 if K > w else 1
+    mins = [int(keys[j:j + w].min()) for j in range(nw)]
+    out, last = [], None
+    for key in mins:
+        if key != last:
+            out.append(key >> 32)
+        last = key
+    return out
+
+
+@pytest.mark.gpu
+def test_gpu_minimizer_containment_matches_numpy():
+    """Features 4 and 5 (minimizer containment and the identity it implies) against a numpy
+    restatement of the same definition, for DNA (2-bit codes, k = 8) and protein (5-bit, k = 3)."""
+    ctx = device.default_context(0)
+    rng = np.random.default_rng(11)
+    for kind in ("dna", "protein"):
+        b1, o1, b2, o2 = learned.training_pairs(kind, 12, 40, 5, 700)
+        s1, s2 = workloads.unpack(b1, o1), workloads.unpack(b2, o2)
+        if kind == "dna":
+            dc = device.DeviceConfig(match=2, mismatch=-3)
+            seen = sorted(set(b1.tobytes()) | set(b2.tobytes()))
+            code, k, bits = {c: i for i, c in enumerate(seen)}, 8, 2
+        else:
+            t = tables.blosum62_standard()
+            dc = device.DeviceConfig(score_kind=device.SCORE_MATRIX, symbols="".join(x for x, _ in t).encode(),
+                                     matrix=tuple(v for _, r in t for v in r))
+            code, k, bits = {c: c & 31 for c in range(256)}, 3, 5
+        f = device.features(ctx, dc, b1, o1, b2, o2)
+        for p in range(len(s1)):
+            m1 = {h & 32767 for h in _minimizers(s1[p], code, k, bits)}
+            m2 = _minimizers(s2[p], code, k, bits)
+            cont = (sum((h & 32767) in m1 for h in m2) / len(m2)) if (m2 and len(s1[p]) >= k) else 0.0
+            assert abs(f[p, 4] - cont) < 1e-6, (kind, p, f[p, 4], cont)
+            assert abs(f[p, 5] - (cont ** (1 / k) if cont > 0 else 0.0)) < 1e-5
+
+
 @pytest.mark.gpu
 def test_learned_dispatch_gains_over_bound_policy():
     """Train on DP-generated labels, then score held-out protein pairs whose length bound does
@@ -107,3 +167,55 @@ def test_learned_dispatch_gains_over_bound_policy():
     want = O.score_packed(b1, o1[:41], b2, o2[:41], O.OracleConfig(mode=O.LOCAL, score_kind=O.MATRIX,
                           symbols=prot.symbols, matrix=prot.matrix, gap_open=-11, gap_extension=-1), threads=8)
     assert np.array_equal(s32[:40].astype(np.int64), want)
+
+
+def _timed_policies(ctx, w, base, model, iters=3):
+    out = {}
+    for name, policy, mdl in (("int32", device.LANES_INT32, None), ("bound", device.LANES_AUTO, None),
+                              ("int16", device.LANES_INT16, None), ("learned", device.LANES_LEARNED, model)):
+        dc = device.DeviceConfig(**{**base.__dict__, "lane_policy": policy, "model": mdl})
+        bt = device.Batch(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc)
+        bt.run()                      # plans (and caches) the int32 recomputation of saturated pairs
+        ms, _ = bt.time(iters)
+        scores, flags = bt.results()
+        out[name] = dict(ms=ms / iters, scores=scores, flags=flags, stats=bt.stats())
+        bt.close()
+    return out
+
+
+@pytest.mark.gpu
+def test_learned_dispatch_beats_speculation_where_int16_saturates():
+    """Protein pairs whose local scores straddle the int16 limit (5-8 kbp at 80-100% identity:
+    about a quarter score 32,767 or more).  The length bound runs them all in int32; int16
+    speculation runs them all in int16 and recomputes the saturated ones in int32; the learned
+    dispatcher sends the pairs it predicts high to int32 at once.  It recomputes far fewer pairs
+    and takes less time than speculation, and than the bound policy; every policy returns the
+    fixed-int32 scores (and the oracle's on a sample)."""
+    ctx = device.default_context(0)
+    t = tables.blosum62_standard()
+    prot = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
+                               symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
+                               gap_open=-11, gap_extension=-1)
+    sets = [(prot, learned.training_pairs("protein", 300, 2500, 100, 5000)),
+            (prot, learned.training_pairs("protein", 301, 1500, 5000, 8000))]
+    x, y = learned.build_dataset(ctx, sets)
+    model, _ = learned.train(ctx, x, y, steps=800, batch=1024)
+    w = workloads.config_saturating(count=2000)
+    r = _timed_policies(ctx, w, prot, model)
+    n = len(w["o1"]) - 1
+    for name in ("bound", "int16", "learned"):
+        assert np.array_equal(r[name]["scores"], r["int32"]["scores"]), name
+    want = O.score_packed(w["b1"], w["o1"][:13], w["b2"], w["o2"][:13], O.OracleConfig(
+        mode=O.LOCAL, score_kind=O.MATRIX, symbols=prot.symbols, matrix=prot.matrix, gap_open=-11, gap_extension=-1),
+        threads=8)
+    assert np.array_equal(r["int32"]["scores"][:12].astype(np.int64), want)
+    saturated = int(np.count_nonzero(r["int32"]["scores"] >= 32767))
+    assert saturated > 0.15 * n
+    spec, lrn, bound = r["int16"], r["learned"], r["bound"]
+    print({k: (round(v["ms"], 3), v["stats"]["n_int16"], v["stats"]["n_recomputed"]) for k, v in r.items()})
+    assert spec["stats"]["n_recomputed"] >= saturated
+    assert lrn["stats"]["n_recomputed"] < 0.25 * spec["stats"]["n_recomputed"]
+    assert lrn["stats"]["n_int16"] > 0.5 * n
+    assert lrn["ms"] < 0.95 * spec["ms"], (lrn["ms"], spec["ms"])
+    assert lrn["ms"] < 0.95 * bound["ms"], (lrn["ms"], bound["ms"])
+    model.close()
