@@ -403,7 +403,7 @@ def bench_mixed(count, steps, warmup, side_steps=5):
     prot = blosum_config(device.METHOD_LOCAL)
     t1 = time.perf_counter()
     sets = [(prot, learned.training_pairs("protein", 1001, 4000)), (dna, learned.training_pairs("dna", 1002, 2000)),
-            (prot, learned.training_pairs("protein", 1003, 1000, 5000, 8000))]
+            (prot, learned.training_pairs("protein", 1003, 1000, 4500, 8000))]
     x, y = learned.build_dataset(ctx, sets)
     model, losses = learned.train(ctx, x, y, steps=1500, batch=2048, log=log)
     train_s = time.perf_counter() - t1
@@ -413,6 +413,8 @@ def bench_mixed(count, steps, warmup, side_steps=5):
     out = {"workload": WORKLOAD_NAMES["mixed"].format(n=f"{count:,}"), "pairs": count, "cells": cells,
            "inputs_sha256": digest, "train_seconds": round(train_s, 2),
            "train_loss_first_last": [round(float(np.mean(losses[:20])), 5), round(float(np.mean(losses[-20:])), 5)],
+           "range_error_sigma_heldout": None if model.sigma is None else round(model.sigma, 5),
+           "lane_margin": round(model.margin, 5),
            "policies": {}}
     ref_scores = {}
     side = min(steps, side_steps)
@@ -469,8 +471,8 @@ def bench_mixed(count, steps, warmup, side_steps=5):
 
 
 def bench_saturating(model, prot, steps, count=20_000):
-    """Where the lane choice decides: 20,000 protein pairs of 5-8 kbp at 80-100% identity, about
-    a quarter of which score beyond the int16 range.  The same four policies with the same trained
+    """Where the lane choice decides: 20,000 protein pairs of 4.8-7.8 kbp at 80-100% identity,
+    about one in six of which score beyond the int16 range.  The same four policies with the same trained
     network; int16 speculation pays an int32 recomputation for every saturated pair (the batch's
     first run plans it, the timed runs replay it)."""
     t0 = time.perf_counter()
@@ -478,7 +480,7 @@ def bench_saturating(model, prot, steps, count=20_000):
     cells = cells_of(w["o1"], w["o2"])
     log(f"[bench] saturating: {count} pairs, {cells:.3e} cells, generated in {time.perf_counter() - t0:.1f} s")
     ctx = device.default_context(0)
-    out = {"workload": "20,000 protein pairs 5-8 kbp, identity 80-100%, local BLOSUM62 gap (-11,-1)",
+    out = {"workload": "20,000 protein pairs 4.8-7.8 kbp, identity 80-100%, local BLOSUM62 gap (-11,-1)",
            "pairs": count, "cells": cells, "policies": {}}
     ref = None
     for name, policy in (("learned", device.LANES_LEARNED), ("int32", device.LANES_INT32),

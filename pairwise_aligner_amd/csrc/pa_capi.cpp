@@ -116,6 +116,7 @@ struct pa_model {
     int device = 0;
     DeviceBuffer params, grad, m1, m2, loss;
     int step = 0;
+    float margin = 0.10f;   // relative safety margin of the lane choice (pa_model_set_margin)
 };
 
 namespace {
@@ -442,7 +443,7 @@ pa_status device_features(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView
 }
 
 // Lane widths from the learned range predictor: int16 where the bound proves it, or where the
-// predicted maximum of H with a 10% + 256 margin stays below the int16 limit (then the pair is
+// predicted maximum of H with the model's relative margin + 256 stays below the int16 limit (then the pair is
 // tracked and recomputed in int32 if it saturates after all); int32 otherwise.  `cost` receives
 // the cost output, in ns (y1 = log2(cost + 1) / 32).
 pa_status learned_widths(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView &v, std::vector<uint8_t> &forced,
@@ -467,7 +468,7 @@ pa_status learned_widths(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView 
     for (uint64_t p = 0; p < v.n_pairs; ++p) {
         const double mn = double(std::min(v.len1(p), v.off2[p + 1] - v.off2[p]));
         const double pred = std::max(0.0, double(y[p * pa::kModelOutputs])) * smax * mn;
-        forced[p] = (pred * 1.10 + 256.0 < 32767.0) ? 16 : 32;
+        forced[p] = (pred * (1.0 + double(cfg.model->margin)) + 256.0 < 32767.0) ? 16 : 32;
         cost[p] = float(std::exp2(std::min(40.0, std::max(0.0, double(y[p * pa::kModelOutputs + 1]) * 32.0))) - 1.0);
     }
     return PA_OK;
@@ -755,6 +756,13 @@ pa_status pa_model_set_params(pa_ctx *ctx, pa_model *model, const float *params)
     PA_HIP(hipMemcpy(model->m1.ptr, zeros.data(), pa::kModelParams * 4, hipMemcpyHostToDevice), "hipMemcpy");
     PA_HIP(hipMemcpy(model->m2.ptr, zeros.data(), pa::kModelParams * 4, hipMemcpyHostToDevice), "hipMemcpy");
     model->step = 0;
+    return PA_OK;
+}
+
+pa_status pa_model_set_margin(pa_model *model, float margin)
+{
+    if (!model || !(margin >= 0.0f && margin <= 10.0f)) return PA_ERR_INVALID_ARGUMENT;
+    model->margin = margin;
     return PA_OK;
 }
 

@@ -387,19 +387,26 @@ void order_launches(Plan &plan)
         if (alone[x] != alone[y]) return alone[x] > alone[y];
         return a.padded_cells > b.padded_cells;
     });
-    // With two or more streams the wavefront launches share stream 0 and the striped launches
-    // take the others round robin, so a long wavefront workgroup never holds up striped work.
+    // With two or more streams the wavefront launches take streams of their own (round robin
+    // over as many as there are wavefront launches, leaving at least one stream to striped
+    // launches when there are any), so that a long wavefront workgroup never holds up striped
+    // work and wavefront launches of different variants run side by side; the striped launches
+    // take the remaining streams round robin.
     const uint32_t ns = std::max<uint32_t>(1, std::min<uint32_t>(plan.tuning.streams, uint32_t(std::max<size_t>(nl, 1))));
-    bool any_wf = false;
-    for (const Launch &L : plan.launches) any_wf |= L.key.G == 64;
-    const uint32_t first = (any_wf && ns > 1) ? 1 : 0;
+    uint32_t n_wf = 0;
+    for (const Launch &L : plan.launches) n_wf += L.key.G == 64;
+    const uint32_t n_striped = uint32_t(nl) - n_wf;
+    const uint32_t wf_streams = ns == 1 || n_wf == 0 ? 0 : std::min(n_wf, n_striped ? ns - 1 : ns);
     plan.launch_stream.assign(nl, 0);
-    uint32_t next = 0;
+    uint32_t next_wf = 0, next_st = 0;
     for (size_t i = 0; i < nl; ++i) {
         const uint32_t li = plan.launch_order[i];
-        if (plan.launches[li].key.G == 64 && first) continue;   // stream 0
-        plan.launch_stream[li] = uint8_t(first + next % (ns - first));
-        ++next;
+        if (plan.launches[li].key.G == 64 && wf_streams) {
+            plan.launch_stream[li] = uint8_t(next_wf++ % wf_streams);
+        } else {
+            const uint32_t rest = ns - wf_streams;
+            plan.launch_stream[li] = uint8_t(wf_streams + next_st++ % rest);
+        }
     }
 }
 

@@ -30,7 +30,7 @@ EXPORTS = ["pa_abi_version", "pa_ctx_create", "pa_ctx_destroy", "pa_ctx_last_err
            "pa_batch_destroy", "pa_model_create", "pa_model_destroy", "pa_model_param_count",
            "pa_model_set_params", "pa_model_get_params", "pa_features", "pa_model_forward", "pa_model_loss_grad",
            "pa_model_train_step", "pa_dp_labels", "pa_tune", "pa_score_one_to_many", "pa_batch_create_one_to_many",
-           "pa_score_batch_dp", "pa_score_batch_noalloc"]
+           "pa_score_batch_dp", "pa_score_batch_noalloc", "pa_model_set_margin"]
 N_FEATURES, N_OUTPUTS, N_HIDDEN = 8, 2, 64
 
 
@@ -147,6 +147,7 @@ def lib() -> ctypes.CDLL:
     L.pa_model_destroy.argtypes = [vp]
     L.pa_model_param_count.restype = i32
     L.pa_model_set_params.argtypes = [vp, vp, vp]
+    L.pa_model_set_margin.argtypes = [vp, ctypes.c_float]
     L.pa_model_get_params.argtypes = [vp, vp, vp]
     L.pa_features.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp]
     L.pa_model_forward.argtypes = [vp, vp, vp, u64, vp]
@@ -159,7 +160,7 @@ def lib() -> ctypes.CDLL:
                  "pa_model_destroy", "pa_model_set_params", "pa_model_get_params", "pa_features",
                  "pa_model_forward", "pa_model_loss_grad", "pa_model_train_step", "pa_dp_labels", "pa_tune",
                  "pa_score_one_to_many", "pa_batch_create_one_to_many", "pa_score_batch_dp",
-                 "pa_score_batch_noalloc"]:
+                 "pa_score_batch_noalloc", "pa_model_set_margin"]:
         getattr(L, name).restype = i32
     return L
 
@@ -413,6 +414,8 @@ class Model:
         ctx.check(ctx._lib.pa_model_create(ctx.handle, seed, ctypes.byref(h)))
         self.handle = h
         self.n_params = ctx._lib.pa_model_param_count()
+        self.margin = 0.10   # the library's default lane margin
+        self.sigma = None    # held-out relative error of the range output (set by a trainer)
 
     def get_params(self) -> np.ndarray:
         p = np.zeros(self.n_params, dtype=np.float32)
@@ -423,6 +426,11 @@ class Model:
         p = np.ascontiguousarray(p, dtype=np.float32)
         assert p.size == self.n_params
         self.ctx.check(self.ctx._lib.pa_model_set_params(self.ctx.handle, self.handle, p.ctypes.data))
+
+    def set_margin(self, margin: float):
+        """Relative safety margin of the lane choice (pa_model_set_margin)."""
+        self.margin = float(margin)
+        self.ctx.check(self.ctx._lib.pa_model_set_margin(self.handle, ctypes.c_float(margin)))
 
     def forward(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -51,17 +51,35 @@ def build_dataset(ctx: device.Context, sets: Sequence[Tuple[device.DeviceConfig,
 
 
 def train(ctx: device.Context, x: np.ndarray, y: np.ndarray, steps: int = 1500, batch: int = 2048,
-          lr: float = 2e-3, seed: int = 0, log=None) -> Tuple[device.Model, List[float]]:
-    """Adam on minibatches; the range output carries 4x the weight of the cost output."""
+          lr: float = 2e-3, seed: int = 0, log=None, holdout: float = 0.1,
+          margin_sigmas: float = 2.0) -> Tuple[device.Model, List[float]]:
+    """Adam on minibatches; the range output carries 4x the weight of the cost output.  A
+    `holdout` share of the samples is kept out of training: the spread (standard deviation) of
+    the range output's relative error on them, times `margin_sigmas`, becomes the model's lane
+    margin (at least 1%; model.sigma keeps the spread)."""
     model = device.Model(ctx, seed=seed)
     rng = np.random.default_rng(seed + 1)
+    order = rng.permutation(len(x))
+    n_held = int(len(x) * holdout) if len(x) >= 100 else 0
+    held, fit = order[:n_held], order[n_held:]
     w = np.tile(np.array([[4.0, 1.0]], dtype=np.float32), (batch, 1))
     losses = []
     t0 = time.perf_counter()
     for step in range(steps):
-        idx = rng.integers(0, len(x), size=batch)
+        idx = fit[rng.integers(0, len(fit), size=batch)]
         rate = lr * (0.1 ** (step / steps))
         losses.append(model.train_step(x[idx], y[idx], w, lr=rate))
         if log and (step % 250 == 0 or step == steps - 1):
             log(f"[learned] step {step}: loss {losses[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+    model.sigma = None
+    if n_held:
+        pred = model.forward(x[held])[:, 0].astype(np.float64)
+        true = y[held, 0].astype(np.float64)
+        ok = true > 0.05   # the relative error of scores near zero says nothing about the limit
+        if np.count_nonzero(ok) >= 20:
+            model.sigma = float(np.std((pred[ok] - true[ok]) / true[ok]))
+            model.set_margin(max(0.01, margin_sigmas * model.sigma))
+            if log:
+                log(f"[learned] held-out relative error of the range output: sigma {model.sigma:.4f}, "
+                    f"lane margin {model.margin:.4f}")
     return model, losses

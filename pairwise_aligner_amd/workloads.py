@@ -15,8 +15,8 @@ Configurations (sizes as stated; names are used in bench.py):
              p=0.5), 40% substitutions; +1/-4, gap (-6,-1), global.
   mixed      500,000 pairs, half DNA half protein, lengths uniform 100-5,000, identity uniform
              50-100%.
-  saturating 20,000 protein pairs, lengths uniform 5,000-8,000, identity uniform 80-100%: about a
-             quarter score beyond the int16 range (local BLOSUM62, gap (-11,-1)).
+  saturating 20,000 protein pairs, lengths uniform 4,800-7,800, identity uniform 80-100%: about
+             one in six scores beyond the int16 range (local BLOSUM62, gap (-11,-1)).
 """
 from __future__ import annotations
 
@@ -296,16 +296,16 @@ def config_mixed(seed: int = 5, count: int = 500_000, limit: int = None, chunk: 
 
 
 def config_saturating(seed: int = 8, count: int = 20_000, limit: int = None, chunk: int = 2_000):
-    """Protein pairs whose local scores straddle the int16 limit: lengths uniform 5,000-8,000,
+    """Protein pairs whose local scores straddle the int16 limit: lengths uniform 4,800-7,800,
     identity uniform 80-100% (90% of events substitutions), background residue frequencies.
-    About a quarter of them score 32,767 or more under BLOSUM62 gap (-11,-1) -- the slice where
-    running every pair in int16 lanes costs a recomputation.  Chunk c from stream (seed, c)."""
+    About one in six scores 32,767 or more under BLOSUM62 gap (-11,-1) -- the slice where running
+    every pair in int16 lanes costs a recomputation.  Chunk c from stream (seed, c)."""
     n = count if limit is None else min(limit, count)
 
     def make(item):
         c, start, k = item
         rng = np.random.default_rng([seed, c])
-        b1, o1 = random_packed(rng, rng.integers(5_000, 8_001, size=k), AMINO, AMINO_FREQ)
+        b1, o1 = random_packed(rng, rng.integers(4_800, 7_801, size=k), AMINO, AMINO_FREQ)
         div = 1.0 - rng.uniform(0.8, 1.0, size=k)
         b2, o2 = mutate_packed(rng, b1, o1, div, 0.1, 0.7, AMINO, AMINO_FREQ)
         return (b1[:int(o1[-1])], o1, b2[:int(o2[-1])], o2)

@@ -185,22 +185,24 @@ def _timed_policies(ctx, w, base, model, iters=3):
 
 @pytest.mark.gpu
 def test_learned_dispatch_beats_speculation_where_int16_saturates():
-    """Protein pairs whose local scores straddle the int16 limit (5-8 kbp at 80-100% identity:
-    about a quarter score 32,767 or more).  The length bound runs them all in int32; int16
-    speculation runs them all in int16 and recomputes the saturated ones in int32; the learned
-    dispatcher sends the pairs it predicts high to int32 at once.  It recomputes far fewer pairs
-    and takes less time than speculation, and than the bound policy; every policy returns the
-    fixed-int32 scores (and the oracle's on a sample)."""
+    """Protein pairs whose local scores straddle the int16 limit (4.8-7.8 kbp at 80-100%
+    identity: about one in six scores 32,767 or more).  The length bound runs them all in int32;
+    int16 speculation runs them all in int16 and recomputes the saturated ones in int32; the
+    learned dispatcher (its lane margin calibrated on held-out pairs) sends the pairs it predicts
+    high to int32 at once.  Learned is faster than speculation, which is faster than the bound;
+    learned recomputes far fewer pairs; every policy returns the fixed-int32 scores (and the
+    oracle's on a sample)."""
     ctx = device.default_context(0)
     t = tables.blosum62_standard()
     prot = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
                                symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
                                gap_open=-11, gap_extension=-1)
     sets = [(prot, learned.training_pairs("protein", 300, 2500, 100, 5000)),
-            (prot, learned.training_pairs("protein", 301, 1500, 5000, 8000))]
+            (prot, learned.training_pairs("protein", 301, 1500, 4500, 8000))]
     x, y = learned.build_dataset(ctx, sets)
     model, _ = learned.train(ctx, x, y, steps=800, batch=1024)
-    w = workloads.config_saturating(count=2000)
+    assert model.sigma is not None and model.sigma < 0.05, model.sigma
+    w = workloads.config_saturating(count=6000)
     r = _timed_policies(ctx, w, prot, model)
     n = len(w["o1"]) - 1
     for name in ("bound", "int16", "learned"):
@@ -210,12 +212,12 @@ def test_learned_dispatch_beats_speculation_where_int16_saturates():
         threads=8)
     assert np.array_equal(r["int32"]["scores"][:12].astype(np.int64), want)
     saturated = int(np.count_nonzero(r["int32"]["scores"] >= 32767))
-    assert saturated > 0.15 * n
+    assert saturated > 0.08 * n
     spec, lrn, bound = r["int16"], r["learned"], r["bound"]
     print({k: (round(v["ms"], 3), v["stats"]["n_int16"], v["stats"]["n_recomputed"]) for k, v in r.items()})
     assert spec["stats"]["n_recomputed"] >= saturated
     assert lrn["stats"]["n_recomputed"] < 0.25 * spec["stats"]["n_recomputed"]
     assert lrn["stats"]["n_int16"] > 0.5 * n
     assert lrn["ms"] < 0.95 * spec["ms"], (lrn["ms"], spec["ms"])
-    assert lrn["ms"] < 0.95 * bound["ms"], (lrn["ms"], bound["ms"])
+    assert spec["ms"] < bound["ms"], (spec["ms"], bound["ms"])
     model.close()
