@@ -413,7 +413,7 @@ def bench_mixed(count, steps, warmup, side_steps=5):
     out = {"workload": WORKLOAD_NAMES["mixed"].format(n=f"{count:,}"), "pairs": count, "cells": cells,
            "inputs_sha256": digest, "train_seconds": round(train_s, 2),
            "train_loss_first_last": [round(float(np.mean(losses[:20])), 5), round(float(np.mean(losses[-20:])), 5)],
-           "range_error_sigma_heldout": None if model.sigma is None else round(model.sigma, 5),
+           "range_error_sigma_heldout_high_scores": None if model.sigma is None else round(model.sigma, 5),
            "lane_margin": round(model.margin, 5),
            "policies": {}}
     ref_scores = {}

@@ -51,12 +51,15 @@ def build_dataset(ctx: device.Context, sets: Sequence[Tuple[device.DeviceConfig,
 
 
 def train(ctx: device.Context, x: np.ndarray, y: np.ndarray, steps: int = 1500, batch: int = 2048,
-          lr: float = 2e-3, seed: int = 0, log=None, holdout: float = 0.1,
-          margin_sigmas: float = 2.0) -> Tuple[device.Model, List[float]]:
-    """Adam on minibatches; the range output carries 4x the weight of the cost output.  A
-    `holdout` share of the samples is kept out of training: the spread (standard deviation) of
-    the range output's relative error on them, times `margin_sigmas`, becomes the model's lane
-    margin (at least 1%; model.sigma keeps the spread)."""
+          lr: float = 2e-3, seed: int = 0, log=None, holdout: float = 0.2) -> Tuple[device.Model, List[float]]:
+    """Adam on minibatches; the range output carries 4x the weight of the cost output.
+
+    A `holdout` share of the samples is kept out of training and sets the model's lane margin:
+    the 99th percentile of the range output's relative under-prediction, (true - predicted) /
+    predicted, over the held-out pairs whose maximum of H (true or predicted) is 16,384 or more --
+    where a pair could reach the int16 limit -- or, with fewer than 20 such pairs, 8,192 or more;
+    at least 1%.  With fewer than 20 pairs either way the library default stays.  model.sigma
+    keeps the standard deviation of the relative error over the same pairs."""
     model = device.Model(ctx, seed=seed)
     rng = np.random.default_rng(seed + 1)
     order = rng.permutation(len(x))
@@ -71,15 +74,20 @@ def train(ctx: device.Context, x: np.ndarray, y: np.ndarray, steps: int = 1500, 
         losses.append(model.train_step(x[idx], y[idx], w, lr=rate))
         if log and (step % 250 == 0 or step == steps - 1):
             log(f"[learned] step {step}: loss {losses[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
-    model.sigma = None
     if n_held:
-        pred = model.forward(x[held])[:, 0].astype(np.float64)
+        pred = np.maximum(model.forward(x[held])[:, 0].astype(np.float64), 1e-6)
         true = y[held, 0].astype(np.float64)
-        ok = true > 0.05   # the relative error of scores near zero says nothing about the limit
-        if np.count_nonzero(ok) >= 20:
-            model.sigma = float(np.std((pred[ok] - true[ok]) / true[ok]))
-            model.set_margin(max(0.01, margin_sigmas * model.sigma))
-            if log:
-                log(f"[learned] held-out relative error of the range output: sigma {model.sigma:.4f}, "
-                    f"lane margin {model.margin:.4f}")
+        # y0 * smax * min(m, n), the lengths and smax recovered from features 0, 1 and 6
+        f = x[held].astype(np.float64)
+        scale = f[:, 6] * 16 * np.minimum(np.exp2(f[:, 0] * 16) - 1, np.exp2(f[:, 1] * 16) - 1)
+        for floor in (16384, 8192):
+            ok = np.maximum(true, pred) * scale >= floor
+            if np.count_nonzero(ok) >= 20:
+                rel = (true[ok] - pred[ok]) / pred[ok]
+                model.sigma = float(np.std(rel))
+                model.set_margin(max(0.01, float(np.percentile(rel, 99))))
+                if log:
+                    log(f"[learned] held-out pairs scoring {floor} or more: {np.count_nonzero(ok)}, relative error "
+                        f"sigma {model.sigma:.4f}, lane margin {model.margin:.4f}")
+                break
     return model, losses
