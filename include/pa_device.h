@@ -58,6 +58,8 @@ typedef struct pa_tuning {
     uint32_t eager;               /* 1: launch kernels one by one; 0: replay a captured hipGraph [0] */
     uint32_t cost_order;          /* PA_LANES_LEARNED: 1 orders the tasks of a launch by the network's predicted
                                      cost instead of the geometric model's [0] */
+    uint32_t persistent;          /* 1: striped launches run as many blocks as the GPU keeps resident, whose
+                                     waves take tasks from a work queue; 0: one block per 16 tasks [0] */
 } pa_tuning;
 
 typedef struct pa_config {
@@ -185,7 +187,7 @@ pa_status pa_batch_destroy(pa_ctx *ctx, pa_batch *batch);
 /* Autotuner.  No counterpart in the reference.  Plans and times candidate tunings and writes
  * the fastest to *out, in two stages of coordinate descent: the shape-model fields (step_cost,
  * rows_mask) on a sample of the batch (every k-th pair, about sample_pairs pairs; 0 = 5% of the
- * batch, at least 2,000), then the launch-level fields (wavefront_min_rows, streams, eager) and
+ * batch, at least 2,000), then the launch-level fields (wavefront_min_rows, streams, eager, persistent) and
  * the stage-1 result against the defaults on the whole batch.  `iters` passes are timed per
  * candidate (0 = 3). */
 pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,

@@ -61,6 +61,8 @@ hipError_t dev_alloc(DeviceBuffer &buf, size_t bytes, Arena *arena)
 struct PlanDevice {
     pa::Plan plan;
     DeviceBuffer tasks, rowcodes, colcodes, bnd, subset, enc, codes, dp_base;
+    DeviceBuffer queues;   // persistent launches: two counters per launch, zero between passes
+    std::vector<uint32_t> queues_host;   // their initial zeros (kept until the upload has read them)
     std::vector<pa::CodeTables> code_host;   // per launch (kept until the encoder has read them)
     DeviceBuffer table[pa::kScoreShifts], ptable[pa::kScoreShifts], profile[pa::kScoreShifts];
     std::vector<uint32_t> subset_host;
@@ -209,6 +211,8 @@ pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd, const DeviceBuffer &residues,
         if ((st = upload(ctx, pd.profile[v], pd.plan.profile[v], arena)) != PA_OK) return st;
     }
     PA_HIP(dev_alloc(pd.bnd, pd.plan.bnd_entries * 8, arena), "device allocation");
+    pd.queues_host.assign(2 * p.launches.size(), 0u);
+    if ((st = upload(ctx, pd.queues, pd.queues_host, arena)) != PA_OK) return st;
     if (!pd.subset_host.empty())
         if ((st = upload(ctx, pd.subset, pd.subset_host, arena)) != PA_OK) return st;
     return PA_OK;
@@ -243,6 +247,10 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     a.tbl_lo = p.tbl_lo[v];
     a.tbl_hi = p.tbl_hi[v];
     a.fc = p.fc; a.fr = p.fr; a.lc = p.lc; a.lr = p.lr;
+    if (p.tuning.persistent && L.key.G == 16) {
+        const size_t li = size_t(&L - p.launches.data());
+        a.queue = static_cast<uint32_t *>(pd.queues.ptr) + 2 * li;
+    }
     if (L.key.export_dp) {
         a.dp_out = static_cast<int32_t *>(b->dp_out.ptr);
         a.dp_base = static_cast<const uint64_t *>(pd.dp_base.ptr) + 2ull * L.task_begin;
@@ -811,8 +819,9 @@ static pa_status loss_grad_device(pa_ctx *ctx, const pa_model *model, const floa
     pa_status st;
     if ((st = upload(ctx, dx, hx)) != PA_OK) return st;
     if ((st = upload(ctx, dt, ht)) != PA_OK) return st;
+    std::vector<float> hw;
     if (weight) {
-        std::vector<float> hw(weight, weight + n * pa::kModelOutputs);
+        hw.assign(weight, weight + n * pa::kModelOutputs);
         if ((st = upload(ctx, dw, hw)) != PA_OK) return st;
     }
     hipError_t e = pa::launch_loss_grad(static_cast<const float *>(model->params.ptr),
@@ -821,6 +830,8 @@ static pa_status loss_grad_device(pa_ctx *ctx, const pa_model *model, const floa
                                         static_cast<float *>(model->grad.ptr), static_cast<float *>(model->loss.ptr),
                                         ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "loss/gradient launch");
+    // the uploads above read host vectors that end with this function
+    PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return PA_OK;
 }
 
@@ -944,6 +955,7 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
         c.streams = t.streams;
         c.eager = t.eager ? 1 : 0;
         c.cost_order = t.cost_order ? 1 : 0;
+        c.persistent = t.persistent ? 1 : 0;
         return c;
     };
     // time of `iters` passes with tuning t (after one untimed pass), on the sample or the batch
@@ -1000,13 +1012,15 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
     const uint32_t wf[] = {2048, 3072, 4096};
     const uint32_t streams[] = {2, 4};
     const uint32_t eager[] = {0, 1};
-    for (int param = 0; param < 3; ++param) {
+    const uint32_t persistent[] = {0, 1};
+    for (int param = 0; param < 4; ++param) {
         const int nv = param == 0 ? 3 : 2;
         for (int v = 0; v < nv; ++v) {
             pa_tuning t = full;
             if (param == 0) t.wavefront_min_rows = wf[v];
             else if (param == 1) t.streams = streams[v];
-            else t.eager = eager[v];
+            else if (param == 2) t.eager = eager[v];
+            else t.persistent = persistent[v];
             if (std::memcmp(&t, &full, sizeof t) == 0) continue;
             double ms = 0;
             if ((st = measure(t, true, ms)) != PA_OK) return st;

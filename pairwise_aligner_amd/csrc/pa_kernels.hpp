@@ -29,6 +29,8 @@ struct KernelArgs {
     int32_t fc, fr, lc, lr;
     int32_t *dp_out;            // DP export variants: the final rows and columns
     const uint64_t *dp_base;    // 2 per task of this launch (pa_plan.hpp)
+    uint32_t *queue;            // nullable: the launch's work queue (next task, finished blocks), zero
+                                // between passes; striped kernel only
 };
 
 struct InitArgs {

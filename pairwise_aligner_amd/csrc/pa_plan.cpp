@@ -39,6 +39,7 @@ Tuning resolve_tuning(const pa_tuning *t)
     if (t->streams) r.streams = std::min<uint32_t>(t->streams, 4);
     r.eager = t->eager != 0;
     r.cost_order = t->cost_order != 0;
+    r.persistent = t->persistent != 0;
     return r;
 }
 

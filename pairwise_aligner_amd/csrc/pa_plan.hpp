@@ -168,6 +168,7 @@ struct Tuning {
     uint32_t streams = 4;
     bool eager = false;
     bool cost_order = false;
+    bool persistent = false;
 };
 Tuning resolve_tuning(const pa_tuning *t);
 

@@ -49,6 +49,7 @@ class CTuning(ctypes.Structure):
         ("streams", ctypes.c_uint32),
         ("eager", ctypes.c_uint32),
         ("cost_order", ctypes.c_uint32),
+        ("persistent", ctypes.c_uint32),
     ]
 
 
@@ -62,15 +63,16 @@ class Tuning:
     streams: int = 0
     eager: int = 0
     cost_order: int = 0   # learned policy: order tasks by the predicted cost (1) or the geometric model (0)
+    persistent: int = 0   # striped launches: resident blocks taking tasks from a work queue (1)
 
     def to_c(self) -> CTuning:
         return CTuning(self.cell_cost, self.step_cost, self.rows_mask, self.wavefront_min_rows, self.streams,
-                       self.eager, self.cost_order)
+                       self.eager, self.cost_order, self.persistent)
 
     @staticmethod
     def from_c(c: CTuning) -> "Tuning":
         return Tuning(round(c.cell_cost, 4), round(c.step_cost, 4), c.rows_mask, c.wavefront_min_rows, c.streams,
-                      c.eager, c.cost_order)
+                      c.eager, c.cost_order, c.persistent)
 
 
 class CConfig(ctypes.Structure):

@@ -73,6 +73,58 @@ def test_graph_replays_match_eager(ctx):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+def test_persistent_work_queue_launches(ctx, graph):
+    """Striped launches as resident blocks taking tasks from a work queue (pa_tuning.persistent):
+    exact on every replay (the last block resets the queue for the next pass), with eager work in
+    between, int16 saturation recomputed, a matrix score model and a batch larger than the grid."""
+    b1, o1, b2, o2, pm = uneven_batch(47, 30000, hi=3000)
+    base = dict(match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
+    dc = device.DeviceConfig(**base, tuning=device.Tuning(persistent=1, eager=0 if graph else 1))
+    want = oracle_scores(b1, o1, b2, o2, dc, pm)
+    bt = device.Batch(ctx, b1, o1, b2, o2, dc, pm)
+    be = device.Batch(ctx, b1, o1, b2, o2, device.DeviceConfig(**base, tuning=device.Tuning(eager=1)), pm)
+    for rep in range(4):
+        log(f"persistent pass {rep}")
+        bt.run()
+        got, _ = bt.results()
+        assert np.array_equal(got.astype(np.int64), want), f"pass {rep}: {np.count_nonzero(got != want)} mismatches"
+        be.run()
+    bt.time(3)
+    got, _ = bt.results()
+    assert np.array_equal(got.astype(np.int64), want)
+    bt.close()
+    be.close()
+    rng = np.random.default_rng(48)
+    s1, s2 = workloads.protein_pairs(rng, 3000, 0, 600)
+    t = tables.blosum62_standard()
+    prot = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
+                               symbols="".join(x for x, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
+                               gap_open=-11, gap_extension=-1,
+                               tuning=device.Tuning(persistent=1, eager=0 if graph else 1))
+    want = O.score_pairs(s1, s2, oracle_cfg(prot), threads=8)
+    for rep in range(2):
+        got, _ = ctx.score(s1, s2, prot)
+        assert np.array_equal(got.astype(np.int64), want), f"protein pass {rep}"
+    # int16 saturation (identical 2.5 kbp sequences at +15 in the striped kernel) and its int32
+    # recomputation, also as a work-queue launch
+    d1, d2 = workloads.dna_pairs(rng, 400, 50, 400, divergence=0.1)
+    same = workloads.unpack(*workloads.random_packed(rng, np.full(3, 2500), workloads.DNA))
+    d1, d2 = d1 + same, d2 + list(same)
+    dna = device.DeviceConfig(match=15, mismatch=-5, gap_open=-10, gap_extension=-1, lane_policy=device.LANES_INT16,
+                              tuning=device.Tuning(persistent=1, eager=0 if graph else 1))
+    want = O.score_pairs(d1, d2, oracle_cfg(dna), threads=8)
+    p1, q1 = workloads.pack(d1)
+    p2, q2 = workloads.pack(d2)
+    bt = device.Batch(ctx, p1, q1, p2, q2, dna)
+    for rep in range(3):
+        bt.run()
+        got, flags = bt.results()
+        assert np.array_equal(got.astype(np.int64), want), f"saturating pass {rep}"
+    assert np.all(flags[-3:] & device.PAIR_RECOMPUTED) and np.all(got[-3:] == 2500 * 15)
+    bt.close()
+
+
 def test_graph_with_saturation_recompute(ctx):
     """The recompute pass (int32, eager) after a replayed int16 pass."""
     rng = np.random.default_rng(43)
