@@ -398,12 +398,13 @@ def bench_long(steps, warmup):
     return res32, res16
 
 
-def bench_mixed(count, steps, warmup, side_steps=5):
+def bench_mixed(count, steps, warmup, side_steps=5, only=None):
     """Learned lane dispatch on the mixed stream: the same pairs under the learned policy (timed
     over `steps` passes: the headline), fixed int32 lanes, int16 speculation (every pair int16,
     saturated pairs recomputed in int32) and the length-bound policy (timed over
     min(steps, side_steps) passes each).  The network is trained first, on labels the DP produces
-    for separately seeded training pairs."""
+    for separately seeded training pairs.  `only` (a set of leg names) runs just those legs, with
+    no CPU baseline and no saturating slice: a short run to profile one leg."""
     from pairwise_aligner_amd import learned
     t0 = time.perf_counter()
     w = workloads.config_mixed(count=count)
@@ -434,6 +435,8 @@ def bench_mixed(count, steps, warmup, side_steps=5):
             ("int16_speculative", device.LANES_INT16, side, 1, None), ("bound", device.LANES_AUTO, side, 1, None),
             ("learned_cost_order", device.LANES_LEARNED, side, 1, device.Tuning(cost_order=1)))
     for name, policy, k, wu, tuning in legs:
+        if only and name not in only:
+            continue
         total_ms = kernel_ms = 0.0
         stats, mism = [], 0
         for kind, dc, h in halves:
@@ -458,6 +461,9 @@ def bench_mixed(count, steps, warmup, side_steps=5):
         out["policies"][name] = leg
         log(f"[bench] mixed/{name}: {leg}")
     pol = out["policies"]
+    if only:
+        model.close()
+        return out
     for name in ("int16_speculative", "bound"):
         out[f"speedup_{name}_vs_int32"] = round(pol[name]["gcups"] / pol["int32"]["gcups"], 3)
     out["speedup_learned_vs_int32"] = round(pol["learned"]["gcups"] / pol["int32"]["gcups"], 3)
@@ -540,6 +546,10 @@ def main():
                     help="calls of the one-shot entry point timed end to end on the protein workload (0: skip)")
     ap.add_argument("--pairs", type=int, default=None, help="pair count of the named workload (not mixed)")
     ap.add_argument("--mixed-pairs", type=int, default=500_000, help="pairs of the mixed stream (config: 500,000)")
+    ap.add_argument("--mixed-legs", default=None,
+                    help="comma-separated policies of the mixed stream to run (learned, int32, int16_speculative, "
+                         "bound, learned_cost_order); the headline leg must be among them; no CPU baseline, "
+                         "no saturating slice (for profiling one leg)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -554,14 +564,17 @@ def main():
             "data": "synthetic (seeded generators in pairwise_aligner_amd/workloads.py)"}
     configs = {}
     if args.workload == "mixed":
-        m = bench_mixed(args.mixed_pairs, args.steps, args.warmup)
+        only = set(args.mixed_legs.split(",")) if args.mixed_legs else None
+        if only is not None and "learned" not in only:
+            sys.exit("--mixed-legs must include learned (the headline leg)")
+        m = bench_mixed(args.mixed_pairs, args.steps, args.warmup, only=only)
         lead = m["policies"]["learned"]
         line.update({"value": lead["gcups"], "unit": "GCUPS", "ms_per_step": lead["ms_per_step"],
                      "dtype": lead["dtype"],
                      "config": {"workload": m["workload"], "pairs": m["pairs"], "cells": m["cells"],
                                 "inputs_sha256": m["inputs_sha256"]},
-                     "roofline": lead["roofline"], "cpu_baseline": m["cpu_baseline"],
-                     "learned_vs_int32": m["speedup_learned_vs_int32"],
+                     "roofline": lead["roofline"], "cpu_baseline": m.get("cpu_baseline"),
+                     "learned_vs_int32": m.get("speedup_learned_vs_int32"),
                      "learned_dispatch": {k: v for k, v in m.items()
                                           if k not in ("workload", "pairs", "cells", "inputs_sha256", "cpu_baseline")}})
     else:

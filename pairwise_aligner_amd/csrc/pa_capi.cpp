@@ -417,13 +417,10 @@ pa_status device_features(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView
     std::vector<uint8_t> s1;
     std::vector<uint64_t> o1(v.n_pairs + 1, 0);
     if (v.one_query) {
-        // the feature kernel reads one first sequence per pair: repeat the query
-        const uint64_t q = v.len1(0);
-        s1.reserve(q * v.n_pairs);
-        for (uint64_t p = 0; p < v.n_pairs; ++p) {
-            s1.insert(s1.end(), v.s1(0), v.s1(0) + q);
-            o1[p + 1] = o1[p] + q;
-        }
+        // one query for every pair: uploaded once, read by every pair's wave
+        s1.assign(v.s1(0), v.s1(0) + v.len1(0));
+        o1.assign({0, v.len1(0)});
+        fa.one_query = 1;
     } else {
         s1.assign(v.seq1 + v.off1[0], v.seq1 + v.off1[v.n_pairs]);
         for (uint64_t p = 0; p <= v.n_pairs; ++p) o1[p] = v.off1[p] - v.off1[0];

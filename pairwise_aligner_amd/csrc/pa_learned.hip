@@ -83,9 +83,10 @@ __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a)
     if (lane == 0) hits[w] = counts[w] = 0;
     __syncthreads();
     const bool live = p < a.n_pairs;
-    const uint8_t *s1 = live ? a.seq1 + a.off1[p] : nullptr;
+    const uint64_t p1 = a.one_query ? 0 : p;
+    const uint8_t *s1 = live ? a.seq1 + a.off1[p1] : nullptr;
     const uint8_t *s2 = live ? a.seq2 + a.off2[p] : nullptr;
-    const int m = live ? int(a.off1[p + 1] - a.off1[p]) : 0;
+    const int m = live ? int(a.off1[p1 + 1] - a.off1[p1]) : 0;
     const int n = live ? int(a.off2[p + 1] - a.off2[p]) : 0;
     const int k = a.k;
     // 3-mer histograms and the minimizer set of s1

@@ -16,7 +16,8 @@ inline constexpr int kModelParams = kModelFeatures * kModelHidden + kModelHidden
 
 struct FeatureArgs {
     const uint8_t *seq1;
-    const uint64_t *off1;
+    const uint64_t *off1;       // n_pairs + 1 offsets, or with one_query the query's two
+    int32_t one_query;          // every pair's first sequence is seq1[off1[0], off1[1])
     const uint8_t *seq2;
     const uint64_t *off2;
     uint64_t n_pairs;

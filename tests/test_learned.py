@@ -221,3 +221,37 @@ def test_learned_dispatch_beats_speculation_where_int16_saturates():
     assert lrn["ms"] < 0.95 * spec["ms"], (lrn["ms"], spec["ms"])
     assert spec["ms"] < bound["ms"], (spec["ms"], bound["ms"])
     model.close()
+
+
+@pytest.mark.gpu
+def test_learned_one_to_many_reads_the_query_once():
+    """One query against many targets under the learned dispatcher: the feature kernel reads the
+    single uploaded query for every pair, so the lane widths it picks equal those of the same pairs
+    with the query written out per pair, and the scores equal fixed int32 (and the oracle's on a
+    sample)."""
+    ctx = device.default_context(0)
+    t = tables.blosum62_standard()
+    prot = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
+                               symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
+                               gap_open=-11, gap_extension=-1)
+    sets = [(prot, learned.training_pairs("protein", 400, 1500, 100, 5000))]
+    x, y = learned.build_dataset(ctx, sets)
+    model, _ = learned.train(ctx, x, y, steps=300, batch=1024)
+    b1, o1, b2, o2 = learned.training_pairs("protein", 401, 400, 200, 4500)
+    query = workloads.unpack(b1, o1)[0]
+    targets = workloads.unpack(b2, o2)
+    rb1, ro1 = workloads.pack([query] * len(targets))
+    lrn = device.DeviceConfig(**{**prot.__dict__, "lane_policy": device.LANES_LEARNED, "model": model})
+    i32 = device.DeviceConfig(**{**prot.__dict__, "lane_policy": device.LANES_INT32})
+    s_one, f_one = ctx.score_one_to_many(query, b2, o2, lrn)
+    s_rep, f_rep = ctx.score_packed(rb1, ro1, b2, o2, lrn)
+    s_32, _ = ctx.score_one_to_many(query, b2, o2, i32)
+    assert np.array_equal(f_one & (device.PAIR_INT16 | device.PAIR_INT32),
+                          f_rep & (device.PAIR_INT16 | device.PAIR_INT32))
+    assert np.array_equal(s_one, s_32) and np.array_equal(s_rep, s_32)
+    assert np.count_nonzero(f_one & device.PAIR_INT16) > 0
+    want = O.score_packed(rb1, ro1[:21], b2, o2[:21], O.OracleConfig(
+        mode=O.LOCAL, score_kind=O.MATRIX, symbols=prot.symbols, matrix=prot.matrix, gap_open=-11, gap_extension=-1),
+        threads=8)
+    assert np.array_equal(s_32[:20].astype(np.int64), want)
+    model.close()
