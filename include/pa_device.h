@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PA_ABI_VERSION 5
+#define PA_ABI_VERSION 6
 
 typedef enum pa_status {
     PA_OK = 0,
@@ -60,6 +60,10 @@ typedef struct pa_tuning {
                                      cost instead of the geometric model's [0] */
     uint32_t persistent;          /* 1: striped launches run as many blocks as the GPU keeps resident, whose
                                      waves take tasks from a work queue; 0: one block per 16 tasks [0] */
+    float wavefront_tail;         /* a pair beyond wavefront_min_rows runs in the wavefront kernel only if its
+                                     modelled striped sweep exceeds this multiple of the batch's striped work per
+                                     resident lane group (where the batch has less other work than that); < 0: every
+                                     such pair [0.75] */
 } pa_tuning;
 
 typedef struct pa_config {
@@ -187,9 +191,9 @@ pa_status pa_batch_destroy(pa_ctx *ctx, pa_batch *batch);
 /* Autotuner.  No counterpart in the reference.  Plans and times candidate tunings and writes
  * the fastest to *out, in two stages of coordinate descent: the shape-model fields (step_cost,
  * rows_mask) on a sample of the batch (every k-th pair, about sample_pairs pairs; 0 = 5% of the
- * batch, at least 2,000), then the launch-level fields (wavefront_min_rows, streams, eager, persistent) and
- * the stage-1 result against the defaults on the whole batch.  `iters` passes are timed per
- * candidate (0 = 3). */
+ * batch, at least 2,000), then the launch-level fields (wavefront_min_rows, streams, eager,
+ * persistent, wavefront_tail) and the stage-1 result against the defaults on the whole batch.
+ * `iters` passes are timed per candidate (0 = 3). */
 pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
                   const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, const uint8_t *pair_method,
                   uint64_t sample_pairs, int iters, pa_tuning *out);

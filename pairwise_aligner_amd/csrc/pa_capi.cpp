@@ -953,6 +953,7 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
         c.eager = t.eager ? 1 : 0;
         c.cost_order = t.cost_order ? 1 : 0;
         c.persistent = t.persistent ? 1 : 0;
+        c.wavefront_tail = t.wavefront_tail > 0 ? float(t.wavefront_tail) : -1.0f;
         return c;
     };
     // time of `iters` passes with tuning t (after one untimed pass), on the sample or the batch
@@ -1010,14 +1011,16 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
     const uint32_t streams[] = {2, 4};
     const uint32_t eager[] = {0, 1};
     const uint32_t persistent[] = {0, 1};
-    for (int param = 0; param < 4; ++param) {
-        const int nv = param == 0 ? 3 : 2;
+    const float tails[] = {-1.0f, 0.5f, 1.0f};
+    for (int param = 0; param < 5; ++param) {
+        const int nv = param == 0 || param == 4 ? 3 : 2;
         for (int v = 0; v < nv; ++v) {
             pa_tuning t = full;
             if (param == 0) t.wavefront_min_rows = wf[v];
             else if (param == 1) t.streams = streams[v];
             else if (param == 2) t.eager = eager[v];
-            else t.persistent = persistent[v];
+            else if (param == 3) t.persistent = persistent[v];
+            else t.wavefront_tail = tails[v];
             if (std::memcmp(&t, &full, sizeof t) == 0) continue;
             double ms = 0;
             if ((st = measure(t, true, ms)) != PA_OK) return st;

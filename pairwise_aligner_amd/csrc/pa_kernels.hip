@@ -49,7 +49,7 @@ __global__ void init_scores_kernel(InitArgs a)
 template <int W>
 static hipError_t dispatch_rows(const VariantKey &k, const KernelArgs &a, uint32_t lds, hipStream_t st)
 {
-    if (k.G == 64) {
+    if (k.G == 64) {   // the wavefront kernel runs stripes of 64 x 16 rows only
         if (k.R == 16) return dispatch_score<W, 16>(k, a, lds, st);
         return hipErrorInvalidValue;
     }

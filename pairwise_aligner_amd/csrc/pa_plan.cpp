@@ -36,6 +36,8 @@ Tuning resolve_tuning(const pa_tuning *t)
     if (t->step_cost > 0) r.step_cost = t->step_cost;
     if (t->rows_mask & 0x7F) r.rows_mask = t->rows_mask & 0x7F;
     if (t->wavefront_min_rows) r.wavefront_min_rows = std::max<uint32_t>(t->wavefront_min_rows, 256);
+    if (t->wavefront_tail > 0) r.wavefront_tail = t->wavefront_tail;
+    else if (t->wavefront_tail < 0) r.wavefront_tail = 0;
     if (t->streams) r.streams = std::min<uint32_t>(t->streams, 4);
     r.eager = t->eager != 0;
     r.cost_order = t->cost_order != 0;
@@ -43,9 +45,9 @@ Tuning resolve_tuning(const pa_tuning *t)
     return r;
 }
 
-void choose_shape(uint32_t m, const Tuning &tuning, uint8_t &G, uint8_t &R)
+void choose_shape(uint32_t m, const Tuning &tuning, bool wavefront, uint8_t &G, uint8_t &R)
 {
-    if (m > tuning.wavefront_min_rows) {   // long pair: one workgroup of 8 waves, stripes of 64*16 rows
+    if (wavefront) {   // long pair: one workgroup of four waves, stripes of 64*16 rows
         G = 64;
         R = 16;
         return;
@@ -63,6 +65,14 @@ void choose_shape(uint32_t m, const Tuning &tuning, uint8_t &G, uint8_t &R)
         const double cost = double(S) * (tuning.cell_cost * r + tuning.step_cost);
         if (cost < best * 0.999) { best = cost; R = r; }
     }
+}
+
+double striped_cost(uint32_t m, uint32_t n, const Tuning &tuning)
+{
+    uint8_t G, R;
+    choose_shape(m, tuning, false, G, R);
+    const uint32_t rows = uint32_t(G) * R;
+    return double((m + rows - 1) / rows) * (double(n) + G - 1) * (tuning.cell_cost * R + tuning.step_cost);
 }
 
 RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n)
@@ -365,7 +375,6 @@ void order_launches(Plan &plan)
     // would take on its own: its longest task, or its whole work spread over the groups the GPU
     // holds at once, whichever is larger -- so that neither a launch of a few long tasks nor a
     // large launch of short ones ends up in the tail of the pass.  Streams round robin.
-    constexpr double kResidentGroups = 1024.0 * 3 * 4;   // SIMDs x waves per SIMD x groups per wave
     const size_t nl = plan.launches.size();
     std::vector<double> longest(nl, 0.0), alone(nl, 0.0);
     for (size_t i = 0; i < nl; ++i) {
@@ -516,7 +525,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         std::vector<uint8_t> tab(std::min<uint64_t>(uint64_t(t.wavefront_min_rows) + 1, 16384));
         parallel_for(tab.size(), [&](size_t m) {
             uint8_t G, R;
-            choose_shape(uint32_t(m), t, G, R);
+            choose_shape(uint32_t(m), t, false, G, R);
             tab[m] = R;
         });
         return tab;
@@ -528,6 +537,34 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     const int64_t recentre_step = std::max<int64_t>(plan.smax, 0) + std::max<int64_t>(-int64_t(plan.smin), 1) +
                                   std::abs(int64_t(plan.go)) + std::abs(int64_t(plan.ge));
     const bool recentre_ok = recentre_step * int64_t(kRecentreReach) < 16000;
+    // Long pairs: the wavefront kernel spreads one pair over a workgroup of four waves, which
+    // pays only where the pair's striped sweep would outlast the rest of the batch; where the
+    // batch holds enough other work, a lane group of the striped kernel sweeps the pair faster.
+    // A pair beyond wavefront_min_rows goes to the wavefront kernel when its modelled striped
+    // cost exceeds wavefront_tail times the batch's striped work per resident lane group (int16
+    // tasks hold two pairs; the orientation is taken as rows = the shorter sequence).
+    double tail_cost = 0;
+    if (!export_dp && plan.tuning.wavefront_tail > 0) {
+        constexpr size_t kBlock = 1 << 14;
+        const size_t nblk = (N + kBlock - 1) / kBlock;
+        std::vector<double> part(nblk, 0.0);
+        std::vector<uint8_t> has_long(nblk, 0);
+        parallel_for(nblk, [&](size_t k) {
+            for (size_t p = k * kBlock; p < std::min<size_t>(N, (k + 1) * kBlock); ++p) {
+                if (!selected[p]) continue;
+                const uint32_t l1 = uint32_t(b.len1(p)), l2 = uint32_t(b.off2[p + 1] - b.off2[p]);
+                const uint32_t m = std::min(l1, l2), n = std::max(l1, l2);
+                if (m == 0) continue;
+                part[k] += striped_cost(m, n, plan.tuning);
+                has_long[k] |= m > plan.tuning.wavefront_min_rows;
+            }
+        }, 1);
+        if (std::find(has_long.begin(), has_long.end(), 1) != has_long.end()) {
+            const double total = std::accumulate(part.begin(), part.end(), 0.0);
+            const double per_task = cfg.lane_policy == PA_LANES_INT32 ? 1.0 : 0.5;
+            tail_cost = plan.tuning.wavefront_tail * total * per_task / kResidentGroups;
+        }
+    }
     // Per pair, in parallel: orientation, lane width and kernel variant.
     std::vector<VariantKey> pair_key(N);
     std::vector<uint8_t> speculative(N, 0);   // int16 without a proof against saturation
@@ -552,7 +589,8 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         // long global pairs without per-row maxima run re-centred in the wavefront kernel: int16
         // whatever the absolute scores when the spread bound holds; otherwise int16 only with
         // the absolute bounds and the saturation tracking (which keeps absolute values)
-        const bool wavefront = !export_dp && pm.m > plan.tuning.wavefront_min_rows;
+        const bool wavefront = !export_dp && pm.m > plan.tuning.wavefront_min_rows &&
+                               (tail_cost == 0 || striped_cost(pm.m, pm.n, plan.tuning) > tail_cost);
         const bool rc_candidate = wavefront && !local && !(lc && lr);
         if (rc_candidate && recentre_ok) lower16 = upper16 = true;
         // The int16 kernels shift values by up to |ge| per row (17 rows) and per column step (64
@@ -580,7 +618,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 key.G = 16;
                 key.R = rows_for[pm.m];
             } else {
-                choose_shape(pm.m, profiled ? profile_tuning : plan.tuning, key.G, key.R);
+                choose_shape(pm.m, profiled ? profile_tuning : plan.tuning, wavefront, key.G, key.R);
             }
         }
         key.export_dp = export_dp ? 1 : 0;

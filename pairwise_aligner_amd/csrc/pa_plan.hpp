@@ -165,6 +165,9 @@ struct Tuning {
     double step_cost = 24.0;     // per-step work outside the cells (lane exchange, conveyors, loop)
     uint32_t rows_mask = 0x7F;   // bit i: R = 4 + 2i allowed
     uint32_t wavefront_min_rows = 3072;
+    // a pair beyond wavefront_min_rows runs in the wavefront kernel only if its modelled striped
+    // sweep exceeds this multiple of the batch's striped work per resident lane group (0: always)
+    double wavefront_tail = 0.75;
     uint32_t streams = 4;
     bool eager = false;
     bool cost_order = false;
@@ -275,10 +278,17 @@ RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n);
 // Pairs with more rows than this go to the wavefront kernel (G = 64, a workgroup per task).
 inline constexpr uint32_t kWavefrontMinRows = 3072;
 
-// Choice of (G, R) for a pair with m rows: the wavefront kernel for pairs with more than
-// tuning.wavefront_min_rows rows, else the allowed rows per lane with the lowest modelled sweep
-// cost (stripes x steps x per-step work; the step count is the same for every R).
-void choose_shape(uint32_t m, const Tuning &tuning, uint8_t &G, uint8_t &R);
+// Choice of (G, R) for a pair with m rows: G = 64, R = 16 for the wavefront kernel, else G = 16
+// and the allowed rows per lane with the lowest modelled sweep cost (stripes x steps x per-step
+// work; the step count is the same for every R).
+void choose_shape(uint32_t m, const Tuning &tuning, bool wavefront, uint8_t &G, uint8_t &R);
+
+// Modelled cost of an m x n pair swept by one lane group of the striped kernel (G = 16, the rows
+// per lane choose_shape picks): S * (n + G - 1) * (cell_cost * R + step_cost).
+double striped_cost(uint32_t m, uint32_t n, const Tuning &tuning);
+
+// Lane groups the GPU keeps resident at once in the model: 1,024 SIMDs x 3 waves x 4 groups.
+inline constexpr double kResidentGroups = 1024.0 * 3 * 4;
 
 // LDS bytes a launch of variant `key` needs for its substitution table.
 uint32_t lds_bytes(const Plan &plan, const VariantKey &key);

@@ -50,6 +50,7 @@ class CTuning(ctypes.Structure):
         ("eager", ctypes.c_uint32),
         ("cost_order", ctypes.c_uint32),
         ("persistent", ctypes.c_uint32),
+        ("wavefront_tail", ctypes.c_float),
     ]
 
 
@@ -64,15 +65,18 @@ class Tuning:
     eager: int = 0
     cost_order: int = 0   # learned policy: order tasks by the predicted cost (1) or the geometric model (0)
     persistent: int = 0   # striped launches: resident blocks taking tasks from a work queue (1)
+    # long pairs go to the wavefront kernel only where their striped sweep would exceed this
+    # multiple of the batch's striped work per resident lane group (< 0: every long pair)
+    wavefront_tail: float = 0.0
 
     def to_c(self) -> CTuning:
         return CTuning(self.cell_cost, self.step_cost, self.rows_mask, self.wavefront_min_rows, self.streams,
-                       self.eager, self.cost_order, self.persistent)
+                       self.eager, self.cost_order, self.persistent, self.wavefront_tail)
 
     @staticmethod
     def from_c(c: CTuning) -> "Tuning":
         return Tuning(round(c.cell_cost, 4), round(c.step_cost, 4), c.rows_mask, c.wavefront_min_rows, c.streams,
-                      c.eager, c.cost_order, c.persistent)
+                      c.eager, c.cost_order, c.persistent, round(c.wavefront_tail, 4))
 
 
 class CConfig(ctypes.Structure):

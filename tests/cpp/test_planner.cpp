@@ -8,6 +8,7 @@
 //   * cells and the int16/int32 counts add up; the residue codes equal a byte-by-byte scan
 //     (the planner's vectorised alphabet scan, bytes >= 0x80 and ragged block ends included);
 //   * planning twice gives the same plan.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -262,9 +263,72 @@ static void check_large_gap_extension()
     std::printf("  large gap extension: int32 only\n");
 }
 
+// Long pairs go to the wavefront kernel only where their striped sweep would be the batch's
+// tail: exactly the pairs beyond wavefront_min_rows whose modelled striped cost exceeds
+// wavefront_tail x (the batch's striped work, halved for two-pair int16 tasks) / resident lane
+// groups.  Alone, every long pair is such a tail; with a tail multiple that puts the threshold at
+// the median long pair, half of them are; a negative multiple sends them all.
+static void check_wavefront_tail()
+{
+    std::mt19937_64 rng(13);
+    Batch b;
+    for (int p = 0; p < 60; ++p) {
+        const int l1 = p % 3 == 0 ? 200 + int(rng() % 800) : 3100 + int(rng() % 4000);
+        const int l2 = l1 + int(rng() % 200) - 100;
+        for (int i = 0; i < l1; ++i) b.s1.push_back("ACGT"[rng() % 4]);
+        for (int i = 0; i < l2; ++i) b.s2.push_back("ACGT"[rng() % 4]);
+        b.o1.push_back(b.s1.size());
+        b.o2.push_back(b.s2.size());
+    }
+    pa_config cfg{};
+    cfg.method = PA_METHOD_LOCAL;
+    cfg.match = 2;
+    cfg.mismatch = -3;
+    cfg.gap_open = -5;
+    cfg.gap_extension = -2;
+    const BatchView v = b.view();
+    const Tuning defaults;
+    std::vector<double> cost(60), long_cost;
+    double total = 0;
+    for (int p = 0; p < 60; ++p) {
+        const uint32_t l1 = uint32_t(b.o1[p + 1] - b.o1[p]), l2 = uint32_t(b.o2[p + 1] - b.o2[p]);
+        cost[p] = striped_cost(std::min(l1, l2), std::max(l1, l2), defaults);
+        total += cost[p];
+        if (std::min(l1, l2) > defaults.wavefront_min_rows) long_cost.push_back(cost[p]);
+    }
+    std::sort(long_cost.begin(), long_cost.end());
+    const double median = long_cost[long_cost.size() / 2];
+    // wavefront_tail 0 (the default), a multiple placing the threshold just below the median long
+    // pair, and -1 (every long pair)
+    const float tails[] = {0.0f, float(median * 0.999 * kResidentGroups / (total * 0.5)), -1.0f};
+    for (float tail : tails) {
+        pa_tuning t{};
+        t.wavefront_tail = tail;
+        cfg.tuning = &t;
+        Plan p;
+        std::string err;
+        CHECK(make_plan(cfg, v, nullptr, nullptr, p, err) == PA_OK, "tail plan: %s", err.c_str());
+        const double threshold = tail > 0 ? tail * total * 0.5 / kResidentGroups : 0.0;
+        size_t n_wf = 0;
+        for (const Launch &L : p.launches)
+            for (uint32_t k = L.task_begin; k < L.task_end; ++k)
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t q = p.tasks[k].pair[h];
+                    if (q == kNone) continue;
+                    const bool want = p.pairs[q].m > defaults.wavefront_min_rows && cost[q] > threshold;
+                    CHECK((L.key.G == 64) == want, "tail %g pair %u (m %u): G %d", tail, q, p.pairs[q].m, L.key.G);
+                    n_wf += L.key.G == 64;
+                }
+        if (tail > 0) CHECK(n_wf == long_cost.size() - long_cost.size() / 2, "tail %g: %zu wavefront pairs", tail, n_wf);
+        else CHECK(n_wf == long_cost.size(), "tail %g: %zu wavefront pairs of %zu", tail, n_wf, long_cost.size());
+    }
+    std::printf("  wavefront tail rule: checked\n");
+}
+
 int main()
 {
     check_recentring();
+    check_wavefront_tail();
     check_large_gap_extension();
     std::mt19937_64 rng(20261204);
     pa_config dna{};

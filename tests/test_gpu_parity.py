@@ -275,6 +275,36 @@ def test_long_pairs_wavefront(ctx, policy, method):
     check(ctx, s1, s2, dc, label=f"long policy={policy} method={method}")
 
 
+@pytest.mark.parametrize("policy", [device.LANES_INT32, device.LANES_AUTO])
+@pytest.mark.parametrize("method", [device.METHOD_GLOBAL, device.METHOD_LOCAL])
+def test_long_pairs_striped(ctx, policy, method):
+    """The same kind of pairs with the wavefront kernel's tail rule set so high that no long pair
+    counts as the batch's tail: every pair runs in the striped kernel, the longest (12,000 rows)
+    as 47 stripes of 256 rows with the boundary row passed through HBM between stripes."""
+    rng = np.random.default_rng(43 + policy + method)
+    s1, s2 = workloads.dna_pairs(rng, 6, 3100, 12000, divergence=0.15, indel_fraction=0.6, geom_p=0.5)
+    s1_, s2_ = workloads.dna_pairs(rng, 100, 50, 400, divergence=0.1)
+    dc = device.DeviceConfig(method=method, match=1, mismatch=-4, gap_open=-6, gap_extension=-1, lane_policy=policy,
+                             tuning=device.Tuning(wavefront_tail=1e9))
+    check(ctx, s1 + s1_, s2 + s2_, dc, label=f"long striped policy={policy} method={method}")
+
+
+def test_long_protein_local_and_free_ends_striped(ctx):
+    rng = np.random.default_rng(92)
+    s1, s2 = workloads.protein_pairs(rng, 4, 3200, 6000)
+    t = tables.blosum62_standard()
+    never = device.Tuning(wavefront_tail=1e9)
+    dc = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
+                             symbols="".join(s for s, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
+                             gap_open=-11, gap_extension=-1, tuning=never)
+    check(ctx, s1, s2, dc, label="long protein local striped")
+    d1, d2 = workloads.dna_pairs(rng, 4, 3500, 9000, divergence=0.3)
+    d2 = [d2[0], d2[1][: len(d2[1]) // 2], d2[2], d2[3]]
+    dc = device.DeviceConfig(first_column_free=True, first_row_free=False, last_column_free=True, last_row_free=False,
+                             match=2, mismatch=-3, gap_open=-5, gap_extension=-2, tuning=never)
+    check(ctx, d1, d2, dc, label="long free ends striped")
+
+
 @pytest.mark.parametrize("flags", [(True, False, True, False), (False, True, False, True), (True, True, True, True)])
 def test_long_pairs_free_ends(ctx, flags):
     fc, fr, lc, lr = flags

@@ -147,7 +147,7 @@ def test_graph_with_saturation_recompute(ctx):
 TUNINGS = [
     device.Tuning(rows_mask=0x55),                      # R = 4, 8, 12, 16 only
     device.Tuning(rows_mask=0x2A, streams=1),           # R = 6, 10, 14 on one stream
-    device.Tuning(wavefront_min_rows=512, streams=2),   # medium pairs in the wavefront kernel
+    device.Tuning(wavefront_min_rows=512, streams=2, wavefront_tail=-1),   # medium pairs in the wavefront kernel
     device.Tuning(step_cost=64.0, eager=1),
     device.Tuning(cell_cost=4.0, step_cost=8.0, rows_mask=0x01),   # R = 4 only
 ]
