@@ -171,8 +171,24 @@ __global__ __launch_bounds__(256) void forward_kernel(const float *params, const
     }
 }
 
-// Mean squared error (weighted per output) and its gradient.  Gradients of one block are summed
-// in LDS, then added to the global gradient once per block.
+// Sum of v over the wave's 64 lanes, returned in every lane: running sums along each row of 16
+// lanes by DPP row shifts (lane 15 of a row ends with its row's sum; lanes shifted in from outside
+// the row add 0), then the four rows' sums.  Every lane of the wave must be active.
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));   // row_shr:1
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, false));   // row_shr:2
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, false));   // row_shr:4
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, false));   // row_shr:8
+    const int b = __float_as_int(v);
+    return (__int_as_float(__builtin_amdgcn_readlane(b, 15)) + __int_as_float(__builtin_amdgcn_readlane(b, 31))) +
+           (__int_as_float(__builtin_amdgcn_readlane(b, 47)) + __int_as_float(__builtin_amdgcn_readlane(b, 63)));
+}
+
+// Mean squared error (weighted per output) and its gradient.  Each wave sums every gradient entry
+// over its 64 samples in registers (wave_sum), one lane adds the sum to the block's LDS copy, and
+// the block adds that to the global gradient once.  Threads past the batch carry zero weight, so
+// they add nothing but keep every wave whole for the lane sums.
 __global__ __launch_bounds__(256) void loss_grad_kernel(const float *params, const float *x, const float *target,
                                                         const float *weight, uint64_t n, float inv_n, float *grad,
                                                         float *loss)
@@ -182,72 +198,76 @@ __global__ __launch_bounds__(256) void loss_grad_kernel(const float *params, con
     __shared__ float sloss;
     for (int i = threadIdx.x; i < NPARAM; i += blockDim.x) sg[i] = 0.0f;
     if (threadIdx.x == 0) sloss = 0.0f;
-    load_params(sp, params);
+    load_params(sp, params);   // ends with a barrier: sg and sloss are cleared too
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) {
-        float in[F], h1[HID], h2[HID], dy[OUT], dh2[HID];
+    const bool live = i < n;
+    const bool lead = threadIdx.x % 64 == 0;
+    auto add = [&](float *dst, float v) {
+        v = wave_sum(v);
+        if (lead) atomicAdd(dst, v);
+    };
+    float in[F], h1[HID], h2[HID], dy[OUT], dh2[HID];
 #pragma unroll
-        for (int f = 0; f < F; ++f) in[f] = x[i * F + f];
+    for (int f = 0; f < F; ++f) in[f] = live ? x[i * F + f] : 0.0f;
 #pragma unroll
-        for (int h = 0; h < HID; ++h) {
-            float acc = sp[OFF_B1 + h];
+    for (int h = 0; h < HID; ++h) {
+        float acc = sp[OFF_B1 + h];
 #pragma unroll
-            for (int f = 0; f < F; ++f) acc += in[f] * sp[OFF_W1 + f * HID + h];
-            h1[h] = acc > 0.0f ? acc : 0.0f;
-        }
+        for (int f = 0; f < F; ++f) acc += in[f] * sp[OFF_W1 + f * HID + h];
+        h1[h] = acc > 0.0f ? acc : 0.0f;
+    }
 #pragma unroll
-        for (int h = 0; h < HID; ++h) {
-            float acc = sp[OFF_B2 + h];
+    for (int h = 0; h < HID; ++h) {
+        float acc = sp[OFF_B2 + h];
 #pragma unroll 8
-            for (int g = 0; g < HID; ++g) acc += h1[g] * sp[OFF_W2 + g * HID + h];
-            h2[h] = acc > 0.0f ? acc : 0.0f;
-        }
-        float l = 0.0f;
+        for (int g = 0; g < HID; ++g) acc += h1[g] * sp[OFF_W2 + g * HID + h];
+        h2[h] = acc > 0.0f ? acc : 0.0f;
+    }
+    float l = 0.0f;
+#pragma unroll
+    for (int o = 0; o < OUT; ++o) {
+        float acc = sp[OFF_B3 + o];
+#pragma unroll
+        for (int h = 0; h < HID; ++h) acc += h2[h] * sp[OFF_W3 + h * OUT + o];
+        const float wt = live ? (weight ? weight[i * OUT + o] : 1.0f) : 0.0f;
+        const float e = live ? acc - target[i * OUT + o] : 0.0f;
+        l += wt * e * e;
+        dy[o] = 2.0f * wt * e * inv_n;
+    }
+    add(&sloss, l * inv_n);
+    // layer 3
+#pragma unroll
+    for (int o = 0; o < OUT; ++o) add(&sg[OFF_B3 + o], dy[o]);
+#pragma unroll
+    for (int h = 0; h < HID; ++h) {
+        float g = 0.0f;
 #pragma unroll
         for (int o = 0; o < OUT; ++o) {
-            float acc = sp[OFF_B3 + o];
-#pragma unroll
-            for (int h = 0; h < HID; ++h) acc += h2[h] * sp[OFF_W3 + h * OUT + o];
-            const float wt = weight ? weight[i * OUT + o] : 1.0f;
-            const float e = acc - target[i * OUT + o];
-            l += wt * e * e;
-            dy[o] = 2.0f * wt * e * inv_n;
+            add(&sg[OFF_W3 + h * OUT + o], h2[h] * dy[o]);
+            g += sp[OFF_W3 + h * OUT + o] * dy[o];
         }
-        atomicAdd(&sloss, l * inv_n);
-        // layer 3
+        dh2[h] = h2[h] > 0.0f ? g : 0.0f;
+    }
+    // layer 2
 #pragma unroll
-        for (int o = 0; o < OUT; ++o) atomicAdd(&sg[OFF_B3 + o], dy[o]);
+    for (int h = 0; h < HID; ++h) add(&sg[OFF_B2 + h], dh2[h]);
+    float dh1[HID];
 #pragma unroll
-        for (int h = 0; h < HID; ++h) {
-            float g = 0.0f;
-#pragma unroll
-            for (int o = 0; o < OUT; ++o) {
-                atomicAdd(&sg[OFF_W3 + h * OUT + o], h2[h] * dy[o]);
-                g += sp[OFF_W3 + h * OUT + o] * dy[o];
-            }
-            dh2[h] = h2[h] > 0.0f ? g : 0.0f;
-        }
-        // layer 2
-#pragma unroll
-        for (int h = 0; h < HID; ++h) atomicAdd(&sg[OFF_B2 + h], dh2[h]);
-        float dh1[HID];
-#pragma unroll
-        for (int g = 0; g < HID; ++g) {
-            float acc = 0.0f;
+    for (int g = 0; g < HID; ++g) {
+        float acc = 0.0f;
 #pragma unroll 8
-            for (int h = 0; h < HID; ++h) {
-                atomicAdd(&sg[OFF_W2 + g * HID + h], h1[g] * dh2[h]);
-                acc += sp[OFF_W2 + g * HID + h] * dh2[h];
-            }
-            dh1[g] = h1[g] > 0.0f ? acc : 0.0f;
-        }
-        // layer 1
-#pragma unroll
         for (int h = 0; h < HID; ++h) {
-            atomicAdd(&sg[OFF_B1 + h], dh1[h]);
-#pragma unroll
-            for (int f = 0; f < F; ++f) atomicAdd(&sg[OFF_W1 + f * HID + h], in[f] * dh1[h]);
+            add(&sg[OFF_W2 + g * HID + h], h1[g] * dh2[h]);
+            acc += sp[OFF_W2 + g * HID + h] * dh2[h];
         }
+        dh1[g] = h1[g] > 0.0f ? acc : 0.0f;
+    }
+    // layer 1
+#pragma unroll
+    for (int h = 0; h < HID; ++h) {
+        add(&sg[OFF_B1 + h], dh1[h]);
+#pragma unroll
+        for (int f = 0; f < F; ++f) add(&sg[OFF_W1 + f * HID + h], in[f] * dh1[h]);
     }
     __syncthreads();
     for (int j = threadIdx.x; j < NPARAM; j += blockDim.x)
