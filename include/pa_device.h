@@ -60,10 +60,10 @@ typedef struct pa_tuning {
                                      cost instead of the geometric model's [0] */
     uint32_t persistent;          /* 1: striped launches run as many blocks as the GPU keeps resident, whose
                                      waves take tasks from a work queue; 0: one block per 16 tasks [0] */
-    float wavefront_tail;         /* a pair beyond wavefront_min_rows runs in the wavefront kernel only if its
-                                     modelled striped sweep exceeds this multiple of the batch's striped work per
-                                     resident lane group (where the batch has less other work than that); < 0: every
-                                     such pair [0.75] */
+    float wavefront_tail;         /* a pair beyond wavefront_min_rows (and within 8,192 rows) runs in the wavefront
+                                     kernel only if its modelled striped sweep exceeds this multiple of the batch's
+                                     striped work per resident lane group (where the batch has less other work than
+                                     that); < 0: every such pair [0.75] */
 } pa_tuning;
 
 typedef struct pa_config {

@@ -541,28 +541,39 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
     // pays only where the pair's striped sweep would outlast the rest of the batch; where the
     // batch holds enough other work, a lane group of the striped kernel sweeps the pair faster.
     // A pair beyond wavefront_min_rows goes to the wavefront kernel when its modelled striped
-    // cost exceeds wavefront_tail times the batch's striped work per resident lane group (int16
-    // tasks hold two pairs; the orientation is taken as rows = the shorter sequence).
+    // cost exceeds wavefront_tail times the batch's striped work per resident lane group (a pair
+    // expected in int16 lanes counts half, as int16 tasks hold two pairs; the orientation is taken
+    // as rows = the shorter sequence), and always beyond kWavefrontAlwaysRows rows, where the
+    // workgroup's four waves sweep two or more full rounds of stripes.
     double tail_cost = 0;
     if (!export_dp && plan.tuning.wavefront_tail > 0) {
         constexpr size_t kBlock = 1 << 14;
         const size_t nblk = (N + kBlock - 1) / kBlock;
         std::vector<double> part(nblk, 0.0);
         std::vector<uint8_t> has_long(nblk, 0);
+        // the lane width the pair is expected to get below (re-centred long global pairs as int16)
+        auto expect16 = [&](size_t p, bool local, uint32_t m, uint32_t n) {
+            if (forced_width && forced_width[p]) return forced_width[p] == 16;
+            if (cfg.lane_policy == PA_LANES_INT32 || -int64_t(plan.ge) * 81 > 16000) return false;
+            if (!local && recentre_ok && m > plan.tuning.wavefront_min_rows) return true;
+            const RangeBound rb = range_bound(plan, local, m, n);
+            const bool lower16 = rb.lower > -32700;
+            return cfg.lane_policy == PA_LANES_AUTO ? lower16 && rb.upper < 32767 : lower16;
+        };
         parallel_for(nblk, [&](size_t k) {
             for (size_t p = k * kBlock; p < std::min<size_t>(N, (k + 1) * kBlock); ++p) {
                 if (!selected[p]) continue;
+                const bool local = b.pair_method ? b.pair_method[p] == PA_METHOD_LOCAL : cfg.method == PA_METHOD_LOCAL;
                 const uint32_t l1 = uint32_t(b.len1(p)), l2 = uint32_t(b.off2[p + 1] - b.off2[p]);
                 const uint32_t m = std::min(l1, l2), n = std::max(l1, l2);
                 if (m == 0) continue;
-                part[k] += striped_cost(m, n, plan.tuning);
+                part[k] += striped_cost(m, n, plan.tuning) * (expect16(p, local, m, n) ? 0.5 : 1.0);
                 has_long[k] |= m > plan.tuning.wavefront_min_rows;
             }
         }, 1);
         if (std::find(has_long.begin(), has_long.end(), 1) != has_long.end()) {
             const double total = std::accumulate(part.begin(), part.end(), 0.0);
-            const double per_task = cfg.lane_policy == PA_LANES_INT32 ? 1.0 : 0.5;
-            tail_cost = plan.tuning.wavefront_tail * total * per_task / kResidentGroups;
+            tail_cost = plan.tuning.wavefront_tail * total / kResidentGroups;
         }
     }
     // Per pair, in parallel: orientation, lane width and kernel variant.
@@ -590,7 +601,8 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         // whatever the absolute scores when the spread bound holds; otherwise int16 only with
         // the absolute bounds and the saturation tracking (which keeps absolute values)
         const bool wavefront = !export_dp && pm.m > plan.tuning.wavefront_min_rows &&
-                               (tail_cost == 0 || striped_cost(pm.m, pm.n, plan.tuning) > tail_cost);
+                               (tail_cost == 0 || pm.m > kWavefrontAlwaysRows ||
+                                striped_cost(pm.m, pm.n, plan.tuning) > tail_cost);
         const bool rc_candidate = wavefront && !local && !(lc && lr);
         if (rc_candidate && recentre_ok) lower16 = upper16 = true;
         // The int16 kernels shift values by up to |ge| per row (17 rows) and per column step (64

@@ -287,6 +287,11 @@ void choose_shape(uint32_t m, const Tuning &tuning, bool wavefront, uint8_t &G, 
 // per lane choose_shape picks): S * (n + G - 1) * (cell_cost * R + step_cost).
 double striped_cost(uint32_t m, uint32_t n, const Tuning &tuning);
 
+// Pairs with more rows than this (two rounds of the wavefront kernel's four 1,024-row stripes)
+// go to the wavefront kernel whatever the rest of the batch: its waves are then busy for all but
+// the last round's remainder, and (int32) it sweeps faster than a lane group of the striped kernel.
+inline constexpr uint32_t kWavefrontAlwaysRows = 8 * 1024;
+
 // Lane groups the GPU keeps resident at once in the model: 1,024 SIMDs x 3 waves x 4 groups.
 inline constexpr double kResidentGroups = 1024.0 * 3 * 4;
 

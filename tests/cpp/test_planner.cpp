@@ -265,9 +265,10 @@ static void check_large_gap_extension()
 
 // Long pairs go to the wavefront kernel only where their striped sweep would be the batch's
 // tail: exactly the pairs beyond wavefront_min_rows whose modelled striped cost exceeds
-// wavefront_tail x (the batch's striped work, halved for two-pair int16 tasks) / resident lane
-// groups.  Alone, every long pair is such a tail; with a tail multiple that puts the threshold at
-// the median long pair, half of them are; a negative multiple sends them all.
+// wavefront_tail x (the batch's striped work, pairs expected in int16 lanes counting half) /
+// resident lane groups (every pair here is int16).  Alone, every long pair is such a tail; with a
+// tail multiple that puts the threshold at the median long pair, half of them are; a negative
+// multiple sends them all.
 static void check_wavefront_tail()
 {
     std::mt19937_64 rng(13);
@@ -315,13 +316,28 @@ static void check_wavefront_tail()
                 for (int h = 0; h < 2; ++h) {
                     const uint32_t q = p.tasks[k].pair[h];
                     if (q == kNone) continue;
-                    const bool want = p.pairs[q].m > defaults.wavefront_min_rows && cost[q] > threshold;
+                    const bool want = p.pairs[q].m > defaults.wavefront_min_rows &&
+                                      (p.pairs[q].m > kWavefrontAlwaysRows || cost[q] > threshold);
                     CHECK((L.key.G == 64) == want, "tail %g pair %u (m %u): G %d", tail, q, p.pairs[q].m, L.key.G);
                     n_wf += L.key.G == 64;
                 }
         if (tail > 0) CHECK(n_wf == long_cost.size() - long_cost.size() / 2, "tail %g: %zu wavefront pairs", tail, n_wf);
         else CHECK(n_wf == long_cost.size(), "tail %g: %zu wavefront pairs of %zu", tail, n_wf, long_cost.size());
     }
+    // beyond kWavefrontAlwaysRows rows a pair goes to the wavefront kernel whatever the tail rule
+    Batch big;
+    for (int i = 0; i < int(kWavefrontAlwaysRows) + 50; ++i) big.s1.push_back("ACGT"[rng() % 4]);
+    big.s2 = big.s1;
+    big.o1.push_back(big.s1.size());
+    big.o2.push_back(big.s2.size());
+    pa_tuning never{};
+    never.wavefront_tail = 1e9f;
+    cfg.tuning = &never;
+    Plan pb;
+    std::string err;
+    CHECK(make_plan(cfg, big.view(), nullptr, nullptr, pb, err) == PA_OK, "always plan: %s", err.c_str());
+    CHECK(pb.launches.size() == 1 && pb.launches[0].key.G == 64, "a pair of %u rows in the wavefront kernel",
+          kWavefrontAlwaysRows + 50);
     std::printf("  wavefront tail rule: checked\n");
 }
 
