@@ -399,9 +399,12 @@ void order_launches(Plan &plan)
     });
     // With two or more streams the wavefront launches take streams of their own (round robin
     // over as many as there are wavefront launches, leaving at least one stream to striped
-    // launches when there are any), so that a long wavefront workgroup never holds up striped
-    // work and wavefront launches of different variants run side by side; the striped launches
-    // take the remaining streams round robin.
+    // launches when there are any), so that a long wavefront workgroup never holds up the first
+    // striped launches and wavefront launches of different variants run side by side; the
+    // striped launches go round robin over the remaining streams first and then over all of them,
+    // so that the later (smaller) ones queue behind the wavefront launches, which end well before
+    // the large striped ones, instead of forming the pass's tail behind those (uneven 1M batch,
+    // default tuning: 51 -> 37 ms per pass).
     const uint32_t ns = std::max<uint32_t>(1, std::min<uint32_t>(plan.tuning.streams, uint32_t(std::max<size_t>(nl, 1))));
     uint32_t n_wf = 0;
     for (const Launch &L : plan.launches) n_wf += L.key.G == 64;
@@ -414,8 +417,7 @@ void order_launches(Plan &plan)
         if (plan.launches[li].key.G == 64 && wf_streams) {
             plan.launch_stream[li] = uint8_t(next_wf++ % wf_streams);
         } else {
-            const uint32_t rest = ns - wf_streams;
-            plan.launch_stream[li] = uint8_t(wf_streams + next_st++ % rest);
+            plan.launch_stream[li] = uint8_t((wf_streams + next_st++) % ns);
         }
     }
 }
