@@ -244,6 +244,7 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     }
     a.go = p.go;
     a.ge = p.ge;
+    a.lbias = p.lbias;
     a.tbl_lo = p.tbl_lo[v];
     a.tbl_hi = p.tbl_hi[v];
     a.fc = p.fc; a.fr = p.fr; a.lc = p.lc; a.lr = p.lr;
@@ -468,12 +469,14 @@ pa_status learned_widths(pa_ctx *ctx, const pa_config &cfg, const pa::BatchView 
         PA_HIP(hipMemcpyAsync(y.data(), dy.ptr, y.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
     PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     const double smax = std::max(1, pa::max_substitution(cfg));
+    // the largest score the int16 lanes hold exactly, less a fixed margin
+    const double limit = double(pa::int16_score_limit(cfg)) - 256.0;
     forced.assign(v.n_pairs, 0);
     cost.assign(v.n_pairs, 0.0f);
     for (uint64_t p = 0; p < v.n_pairs; ++p) {
         const double mn = double(std::min(v.len1(p), v.off2[p + 1] - v.off2[p]));
         const double pred = std::max(0.0, double(y[p * pa::kModelOutputs])) * smax * mn;
-        forced[p] = (pred * (1.0 + double(cfg.model->margin)) + 256.0 < 32767.0) ? 16 : 32;
+        forced[p] = (pred * (1.0 + double(cfg.model->margin)) < limit) ? 16 : 32;
         cost[p] = float(std::exp2(std::min(40.0, std::max(0.0, double(y[p * pa::kModelOutputs + 1]) * 32.0))) - 1.0);
     }
     return PA_OK;

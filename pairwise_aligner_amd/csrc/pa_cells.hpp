@@ -36,6 +36,18 @@ template <> struct Lane<16> {
     __device__ static V pair(int a, int b) { return V{short(clamp(a)), short(clamp(b))}; }   // halves a, b
     // packed top-boundary word of an int value (<= 0): both halves, clamped at -32768
     __device__ static uint32_t top_word(int x) { return (uint32_t(x < -32768 ? -32768 : x) & 0xFFFFu) * 0x10001u; }
+    // Maximum of three values, per half, in one instruction: the packed fp16 three-input maximum
+    // (v_pk_maximum3_f16).  It is the integer maximum when one operand is in [0, kInt16Top] and
+    // each other one is in [0, kInt16Top] or at most -1024: the patterns 0..0x7C00 order as
+    // their integers, and 0x8000..0xFC00 are -0, the negative numbers and -infinity, below them
+    // (pa_plan.hpp, kInt16Top).
+    __device__ static V max3(V a, V b, V c)
+    {
+        typedef _Float16 H2 __attribute__((ext_vector_type(2)));
+        const H2 r = __builtin_elementwise_maximum(
+            __builtin_elementwise_maximum(__builtin_bit_cast(H2, a), __builtin_bit_cast(H2, b)), __builtin_bit_cast(H2, c));
+        return __builtin_bit_cast(V, r);
+    }
 };
 
 template <> struct Lane<32> {
@@ -51,6 +63,7 @@ template <> struct Lane<32> {
     __device__ static V lowest() { return INT32_MIN; }
     __device__ static V pair(int a, int) { return a; }
     __device__ static uint32_t top_word(int x) { return uint32_t(x); }
+    __device__ static V max3(V a, V b, V c) { return vmax(vmax(a, b), c); }
 };
 
 // DPP controls (gfx9).  Lanes without a source keep `old` (bound_ctrl off).
@@ -119,10 +132,12 @@ struct Rows {
 };
 
 // Loads the residue codes of rows row0+1 .. row0+R and sets H(i,0) and W(i,1) (drifted if DRIFT;
-// with the column drift CDRIFT, H carries `hcol` more and W enters one column later).
+// with the column drift CDRIFT, H carries `hcol` more and W enters one column later; values of the
+// max3 variants carry their bias `vbias`).
 template <int W, int R, int SCORE, bool ACC, bool DRIFT, bool CDRIFT>
 __device__ __forceinline__ void init_rows(Rows<W, R, ACC> &st, const uint32_t *codes, int row0, int fc, int go,
-                                          int ge, uint32_t lane_bank, int hcol)
+                                          int ge, uint32_t lane_bank, int hcol,
+                                          typename Lane<W>::V vbias = typename Lane<W>::V{})
 {
     using L = Lane<W>;
     const typename L::V vgogo = L::splat(CDRIFT ? go : go + ge);
@@ -150,7 +165,8 @@ __device__ __forceinline__ void init_rows(Rows<W, R, ACC> &st, const uint32_t *c
                 st.Ahi[r] = 0;
             }
         }
-        st.H[r] = L::splat(h0_value(row0 + r + 1, fc, go, ge) + drift_of<DRIFT>(ge, r) + (CDRIFT ? hcol : 0));
+        st.H[r] = L::add(L::splat(h0_value(row0 + r + 1, fc, go, ge) + drift_of<DRIFT>(ge, r) + (CDRIFT ? hcol : 0)),
+                         vbias);
         st.F[r] = L::add(st.H[r], vgogo);
         if constexpr (ACC) st.rowacc[r] = L::lowest();
     }
@@ -248,8 +264,10 @@ __device__ __forceinline__ void finish_scores(const Rows<W, R, ACC> &st, uint32_
 // rows are drifted (see drift_of) and the scores already shifted by |ge|; `Eup` enters as the
 // vertical gap state of row 0 and leaves as that of row R.  `diag` is the exact H of the row above
 // at column j-1.  RUN: keep the exact running maximum in `runmax` (local method); per-row
-// maxima go to rowacc (ACC).
-template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT, bool CDRIFT>
+// maxima go to rowacc (ACC).  MAX3 (int16 lanes with the column drift, values biased into the
+// three-way maximum's range): the local clamp and max(D, W) are one three-way maximum, the
+// running maximum takes two rows per instruction, and the global method's max(D, W, V) is one.
+template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT, bool CDRIFT, bool MAX3 = false>
 __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<W>::V &Eup,
                                              typename Lane<W>::V diag, typename Lane<W>::V &runmax,
                                              const typename Lane<W>::V (&sc)[R], uint32_t Bin, uint32_t tbl_lo,
@@ -259,6 +277,7 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
 {
     using L = Lane<W>;
     using V = typename L::V;
+    static_assert(!MAX3 || (W == 16 && CDRIFT), "three-way maxima: int16 lanes with the column drift");
     // Diagonal terms of all R rows first: they read only the previous column's H, so every
     // H[r] can then be overwritten in place (no register rotation from step to step).
     V D[R];
@@ -269,11 +288,14 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
         else s = sc[r];
         D[r] = L::add(r == 0 ? diag : st.H[r - 1], s);
         // the local clamp at 0; with the column drift, 0 of this step's frame (vfloor)
-        if constexpr (LOCAL) D[r] = L::vmax(D[r], CDRIFT ? vfloor : L::splat(0));
+        if constexpr (LOCAL && !MAX3) D[r] = L::vmax(D[r], CDRIFT ? vfloor : L::splat(0));
     }
     V DF[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) DF[r] = L::vmax(D[r], st.F[r]);
+    for (int r = 0; r < R; ++r) {
+        if constexpr (MAX3 && LOCAL) DF[r] = L::max3(D[r], vfloor, st.F[r]);   // the clamp included
+        else if constexpr (!MAX3) DF[r] = L::vmax(D[r], st.F[r]);
+    }
     if constexpr (CDRIFT && !DRIFT) {
         // Column drift only (local method): every value of a step carries the step's drift, so
         // the horizontal gap state loses its extension term, W'(j+1) = max(W'(j), H'(j) + go),
@@ -285,7 +307,12 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
             const V open = L::add(best, vgo);
             Eup = L::add(L::vmax(Eup, open), vge);
             st.H[r] = best;
-            if constexpr (LOCAL) runmax = L::vmax(runmax, best);
+            if constexpr (LOCAL && MAX3) {   // rows r-1 and r in one three-way maximum
+                if (r % 2 == 1) runmax = L::max3(runmax, st.H[r - 1], best);
+                else if (r == R - 1) runmax = L::vmax(runmax, best);
+            } else if constexpr (LOCAL) {
+                runmax = L::vmax(runmax, best);
+            }
             st.F[r] = L::vmax(st.F[r], open);
         }
     } else if constexpr (CDRIFT) {
@@ -294,7 +321,9 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
         // and share the opening term.
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const V best = L::vmax(DF[r], Eup);
+            V best;
+            if constexpr (MAX3) best = L::max3(D[r], st.F[r], Eup);
+            else best = L::vmax(DF[r], Eup);
             const V open = L::add(best, vgo);
             Eup = L::vmax(Eup, open);
             st.H[r] = best;

@@ -132,6 +132,14 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         // column drift: on top of the row drift for the global method, alone for the local method
         // (whose zero clamp and running maximum then work in the step's frame)
         constexpr bool CDRIFT = !ACC && !EXPORT;
+        // int16 lanes with the column drift take three-way maxima in one instruction and hold
+        // their values biased into its range (pa_plan.hpp, kInt16Top): the local method by one
+        // bias for the launch, the global method by one per half, from the task
+        constexpr bool MAX3 = W == 16 && CDRIFT;
+        const int lbias = MAX3 && LOCAL ? a.lbias : 0;
+        const int b0 = MAX3 && !LOCAL ? int(tk.bias & 0xFFFFu) : lbias;
+        const int b1 = MAX3 && !LOCAL ? int(tk.bias >> 16) : lbias;
+        const V vbias = L::pair(b0, b1);
 
         const int dge = DRIFT ? ge : 0;
         const int cg = CDRIFT ? -ge : 0;         // column drift per step
@@ -160,7 +168,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 
         // exact; for the local method with the column drift, 0 in the frame of step -1 (it moves to
         // the frame of each step as the step begins and back by G steps at each chunk end)
-        V runmax = LOCAL ? L::splat(CDRIFT ? -cg : 0) : L::lowest();
+        V runmax = LOCAL ? L::splat((CDRIFT ? -cg : 0) + lbias) : L::lowest();
         V satmax = L::lowest();                         // drifted (int16 saturation test)
         const V vcg = L::splat(cg);
 
@@ -170,8 +178,8 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             Rows<W, R, ACC> st;
             // lane t starts at step t: its initial column (j = 0) counts as computed at step t-1
             init_rows<W, R, SCORE, ACC, DRIFT, CDRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank,
-                                                       cg * (t - 1));
-            V Hdiag = L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1));
+                                                       cg * (t - 1), vbias);
+            V Hdiag = L::add(L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1)), vbias);
             Capture<W, R> cap;
             // the step at which this lane's next capture event comes (events are rare: one wave
             // test per step against it)
@@ -213,8 +221,8 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 } else {
                     // H(0, col+1) exact, the vertical gap state below it (drifted by |ge| for row 0)
                     const int hv = h0_value(col + 1, fr, go, ge);
-                    h = L::top_word(hv + cg * t);
-                    e = L::top_word(hv + go + ge - dge + cg * t);
+                    h = L::bits(L::add(L::from(L::top_word(hv + cg * t)), vbias));
+                    e = L::bits(L::add(L::from(L::top_word(hv + go + ge - dge + cg * t)), vbias));
                 }
             };
             uint32_t convH = 0, convE = 0, nextH = 0, nextE = 0;
@@ -241,10 +249,14 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             V sc[R];
             finish_scores<W, R, SCORE>(st, Bin, tbl_lo, tbl_hi, rd, sc);
 
-            // One step.  MASKED: lanes outside 0 <= k - t < N leave their state alone (needed in
-            // the first chunk, where lane t waits t steps, and by the ROWMAX/TRACK variants whose
-            // per-row maxima must not see pad columns).  Unmasked, lanes past column N-1 sweep pad
-            // columns: pad cells never exceed the real ones and never feed them.
+            // One step.  MASKED: lanes before their first column (k - t < 0; the first chunk,
+            // where lane t waits t steps) leave their state alone, and so do lanes past column N-1
+            // in the ROWMAX/TRACK/EXPORT variants, whose per-row maxima and exported rows must not
+            // see pad columns.  Elsewhere lanes past column N-1 sweep pad columns, as in every
+            // unmasked step: pad cells never exceed the real ones and never feed them, and a lane's
+            // state never falls behind the column drift (a state left alone while the chunk ends
+            // drop the drift would sink by G|ge| per chunk, for the max3 variants into patterns
+            // their three-way maximum does not order).
             // TOPSTEP (ONE only): lane 0's top boundary changes from step to step (top_step != 0)
             auto step = [&]<bool MASKED, bool TOPSTEP>(int k) {
                 const int c = k - t;   // DP column (0-based) of this lane at this step
@@ -272,10 +284,11 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                     convE = nextE_;
                 }
                 if constexpr (LOCAL && CDRIFT) runmax = L::add(runmax, vcg);   // into this step's frame
-                if (!MASKED || uint32_t(c) < uint32_t(N)) {
+                if (!MASKED || (ROWMAX || TRACK || EXPORT ? uint32_t(c) < uint32_t(N) : c >= 0)) {
                     V Eup = L::from(Ein);
-                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo,
-                                                                         tbl_hi, vgo, vgogo, vge, L::splat(cg * (k % G)));
+                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT, MAX3>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo,
+                                                                               tbl_hi, vgo, vgogo, vge,
+                                                                               L::splat(cg * (k % G) + lbias));
                     Hdiag = L::from(Hin);
                     Hout = L::bits(DRIFT || CDRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
                     Eout = L::bits(DRIFT || CDRIFT ? L::add(Eup, vout) : Eup);
@@ -300,7 +313,7 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 if constexpr (!LOCAL) {
                     // capture events are rare: test the whole wave first
                     if (__builtin_expect(__builtin_amdgcn_ballot_w64(k == kev) != 0, 0)) {
-                        cap.at_column(c, st, tk, a.scores, dge, -cg * (k % G));
+                        cap.at_column(c, st, tk, a.scores, dge, -cg * (k % G), -b0, -b1);
                         kev = cap.ev == INT32_MAX ? INT32_MAX : cap.ev + t;
                     }
                 }
@@ -391,11 +404,13 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 for (int q = 0; q < L::NH; ++q) {
                     if (tk.pair[q] == kNone) continue;
                     // the local maximum is in the frame of step -1 with the column drift
-                    const int v = L::half(mv, q) + (LOCAL && CDRIFT ? cg : 0);
+                    const int v = L::half(mv, q) + (LOCAL && CDRIFT ? cg : 0) - lbias;
                     if constexpr (LOCAL) a.scores[tk.pair[q]] = v;
-                    // with the local column drift a value clamped at kMax in the frame of step u is
-                    // kMax - u|ge| in exact terms and at least kMax - G|ge| in the frame of step -1
-                    if (W == 16 && L::half(sv, q) >= L::kMax - (LOCAL && CDRIFT ? G * cg : 0))
+                    // int16 overflow: a value clamped at kMax, or (max3 variants) above kInt16Top,
+                    // whose NaN pattern the running maximum keeps at 0x7E00 or more; in the frame
+                    // of step u, so with the local column drift at least G|ge| less in the frame
+                    // of step -1
+                    if (W == 16 && L::half(sv, q) >= (MAX3 ? kInt16Top + 1 : L::kMax) - (LOCAL && CDRIFT ? G * cg : 0))
                         atomicOr(a.flags + tk.pair[q], kFlagSaturated);
                     if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
                 }
@@ -506,6 +521,11 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
     // when that spread bound holds.  Values crossing to another wave carry their writer's
     // offsets (the LDS ring per chunk, the HBM row as exact int32).
     constexpr bool RC = W == 16 && CDRIFT && !LOCAL;
+    // int16 local lanes take three-way maxima in one instruction, their values biased by the
+    // launch's local bias into its range (pa_plan.hpp, kInt16Top); the re-centred lanes keep
+    // two-operand maxima
+    constexpr bool MAX3 = W == 16 && CDRIFT && LOCAL;
+    const int lbias = MAX3 ? a.lbias : 0;
     const int dge = DRIFT ? ge : 0;
     const int cg = CDRIFT ? -ge : 0;         // column drift per step
     const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
@@ -526,7 +546,7 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
 
     // exact; for the local method (column drift) 0 in the frame of step -1, moved into each
     // step's frame as the step begins and back by G steps at each chunk end
-    V runmax = LOCAL ? L::splat(-cg) : L::lowest();
+    V runmax = LOCAL ? L::splat(-cg + lbias) : L::lowest();
     V satmax = L::lowest();                         // drifted (int16 saturation test)
     const V vcg = L::splat(cg);
 
@@ -540,8 +560,8 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
         if constexpr (RC) off0 = off1 = h0_value(s * ROWS, fc, go, ge);
         // lane t starts at step t: its initial column (j = 0) counts as computed at step t-1
         init_rows<W, R, SCORE, ACC, DRIFT, CDRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank,
-                                                   cg * (t - 1) - off0);
-        V Hdiag = L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1) - off0);
+                                                   cg * (t - 1) - off0 + lbias);
+        V Hdiag = L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1) - off0 + lbias);
         Capture<W, R> cap;
         if constexpr (!LOCAL) cap.init(tk, row0, lc);
 
@@ -629,8 +649,8 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                     topE = L::bits(L::pair(e - off0, e - off1));
                     hb_top = fr ? 0 : hb_top + ge;
                 } else {
-                    topH = L::top_word(hb_top + cg * u);
-                    topE = L::top_word(hb_top + go + ge - dge + cg * u);   // drifted by |ge| for row 0
+                    topH = L::top_word(hb_top + cg * u + lbias);
+                    topE = L::top_word(hb_top + go + ge - dge + cg * u + lbias);   // drifted by |ge| for row 0
                     hb_top = fr ? 0 : hb_top + ge;
                 }
                 if (src != 0) {
@@ -642,8 +662,9 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
                 if constexpr (LOCAL) runmax = L::add(runmax, vcg);   // into this step's frame
                 if (uint32_t(c) < uint32_t(N)) {
                     V Eup = L::from(Ein);
-                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo,
-                                                                         tbl_hi, vgo, vgogo, vge, L::splat(cg * u));
+                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT, MAX3>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo,
+                                                                               tbl_hi, vgo, vgogo, vge,
+                                                                               L::splat(cg * u + lbias));
                     Hdiag = L::from(Hin);
                     Hout = L::bits(DRIFT || CDRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
                     Eout = L::bits(DRIFT || CDRIFT ? L::add(Eup, vout) : Eup);
@@ -757,11 +778,11 @@ __global__ __launch_bounds__(64 * kWfWaves) void wavefront_kernel(KernelArgs a)
             for (int q = 0; q < L::NH; ++q) {
                 if (tk.pair[q] == kNone) continue;
                 // the local maximum is in the frame of step -1 (column drift)
-                const int v = L::half(mv, q) + (LOCAL ? cg : 0);
+                const int v = L::half(mv, q) + (LOCAL ? cg : 0) - lbias;
                 if constexpr (LOCAL) atomicMax(a.scores + tk.pair[q], v);
-                // with the local column drift a value clamped at kMax in the frame of step u is
-                // kMax - u|ge| in exact terms and at least kMax - G|ge| in the frame of step -1
-                if (W == 16 && L::half(sv, q) >= L::kMax - (LOCAL && CDRIFT ? G * cg : 0))
+                // int16 overflow (a value clamped at kMax, or above kInt16Top in the max3 lanes),
+                // in the frame of step u: at least G|ge| less in the frame of step -1
+                if (W == 16 && L::half(sv, q) >= (MAX3 ? kInt16Top + 1 : L::kMax) - (LOCAL && CDRIFT ? G * cg : 0))
                     atomicOr(a.flags + tk.pair[q], kFlagSaturated);
                 if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
             }

@@ -25,6 +25,7 @@ struct KernelArgs {
     const int16_t *table;
     uint32_t table_entries;
     int32_t go, ge;
+    int32_t lbias;              // int16 max3 variants of the local method: the bias of their values
     uint32_t tbl_lo, tbl_hi;
     int32_t fc, fr, lc, lr;
     int32_t *dp_out;            // DP export variants: the final rows and columns

@@ -431,6 +431,38 @@ int32_t max_substitution(const pa_config &cfg)
     return m;
 }
 
+namespace {
+int32_t min_substitution(const pa_config &cfg)
+{
+    if (cfg.score_kind == PA_SCORE_UNITARY) return std::min(cfg.match, cfg.mismatch);
+    int32_t m = INT32_MAX;
+    if (cfg.matrix)
+        for (int32_t i = 0; i < cfg.matrix_dim * cfg.matrix_dim; ++i) m = std::min(m, cfg.matrix[i]);
+    return m;
+}
+} // namespace
+
+int32_t int16_local_bias(int32_t smin, int32_t go, int32_t ge)
+{
+    const int64_t g = std::min<int64_t>(go, 0), e = std::min<int64_t>(ge, 0);
+    const int64_t low = std::min<int64_t>({int64_t(smin), -1, g + e});   // -1: the pad's score
+    return int32_t(std::min<int64_t>(-low - e + 1, 32767));
+}
+
+int32_t int16_bias(const Plan &plan, bool local, uint32_t m, uint32_t n)
+{
+    if (local) return plan.lbias;
+    const RangeBound rb = range_bound(plan, false, m, n);
+    const int64_t low = rb.lower + std::min<int64_t>(plan.smin, -1);
+    return int32_t(std::min<int64_t>(-low - std::min<int64_t>(plan.ge, 0) + 1, 32767));
+}
+
+int32_t int16_score_limit(const pa_config &cfg)
+{
+    const int64_t e = std::min<int64_t>(cfg.gap_extension, 0);
+    return int32_t(kInt16Top - int16_local_bias(min_substitution(cfg), cfg.gap_open, cfg.gap_extension) + e * 81);
+}
+
 pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *forced_width,
                     const std::vector<uint32_t> *only, Plan &plan, std::string &err, bool track_all,
                     bool export_dp, const float *pair_cost)
@@ -466,6 +498,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
 
     Alphabet alpha;
     if (pa_status st = build_alphabet(cfg, b, plan, alpha, err); st != PA_OK) return st;
+    plan.lbias = int16_local_bias(plan.smin, plan.go, plan.ge);
     if (plan.score == kScoreMatrix && plan.alphabet <= kPairTableMaxAlphabet) {
         const uint32_t K = plan.alphabet;
         for (int v = 0; v < kScoreShifts; ++v) {
@@ -560,7 +593,8 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
             if (!local && recentre_ok && m > plan.tuning.wavefront_min_rows) return true;
             const RangeBound rb = range_bound(plan, local, m, n);
             const bool lower16 = rb.lower > -32700;
-            return cfg.lane_policy == PA_LANES_AUTO ? lower16 && rb.upper < 32767 : lower16;
+            const bool upper16 = local ? int16_fits(rb, plan.lbias) : rb.upper < 32767;
+            return cfg.lane_policy == PA_LANES_AUTO ? lower16 && upper16 : lower16;
         };
         parallel_for(nblk, [&](size_t k) {
             for (size_t p = k * kBlock; p < std::min<size_t>(N, (k + 1) * kBlock); ++p) {
@@ -598,7 +632,8 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         const RangeBound rb = range_bound(plan, local, pm.m, pm.n);
         if (rb.lower < INT32_MIN / 2 || rb.upper > INT32_MAX / 2) { out_of_range = true; return; }
         bool lower16 = rb.lower > -32700;
-        bool upper16 = rb.upper < 32767;
+        // every local int16 variant is a max3 variant (int16 values up to kInt16Top)
+        bool upper16 = local ? int16_fits(rb, plan.lbias) : rb.upper < 32767;
         // long global pairs without per-row maxima run re-centred in the wavefront kernel: int16
         // whatever the absolute scores when the spread bound holds; otherwise int16 only with
         // the absolute bounds and the saturation tracking (which keeps absolute values)
@@ -642,6 +677,9 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
         key.track = (!local && ((width == 16 && !upper16) || track_all ||
                                 (width == 16 && rc_candidate && !recentre_ok))) ? 1 : 0;
         key.rowmax = (!local && lc && lr) ? 1 : 0;
+        // a global pair whose range does not fit the max3 variants' window keeps absolute int16
+        // values with the running maximum (exact: its 16-bit bound is proven or tracked)
+        if (!local && max3_variant(key) && !int16_fits(rb, int16_bias(plan, false, pm.m, pm.n))) key.track = 1;
     });
     if (out_of_range) { err = "scores may leave the int32 range"; return PA_ERR_RANGE; }
     // Group the pairs by variant: groups in VariantKey order, pairs in pair order within a group
@@ -842,6 +880,8 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
                 t.pair[q] = h[q] ? h[q]->pair : kNone;
                 t.m[q] = h[q] ? h[q]->m : 0;
                 t.n[q] = h[q] ? h[q]->n : 0;
+                if (h[q] && !key.local && max3_variant(key))
+                    t.bias |= uint32_t(int16_bias(plan, false, h[q]->m, h[q]->n)) << (16 * q);
             }
             t.row_off = uint32_t(std::min<uint64_t>(rc, 0xFFFFFFFFull));
             t.col_off = uint32_t(std::min<uint64_t>(cc, 0xFFFFFFFFull));

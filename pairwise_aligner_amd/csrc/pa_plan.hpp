@@ -105,7 +105,8 @@ struct Task {
     uint32_t pair[2];   // output index per half, kNone if the half is unused
     uint32_t m[2];      // DP rows of each half (>= 1)
     uint32_t n[2];      // DP columns of each half (>= 1)
-    uint32_t pad_;
+    uint32_t bias;      // global int16 three-way-maximum variants (max3_variant): the bias of half 0
+                        // in the low and of half 1 in the high 16 bits (int16_bias); else 0
 };
 static_assert(sizeof(Task) == 48, "Task layout is shared with the device");
 
@@ -194,6 +195,7 @@ struct Plan {
     uint32_t profile_rows = 0;                // query rows covered (padded for every allowed R)
     uint32_t profile_ranks = 0;               // residue ranks including the pad
     int32_t smax = 0, smin = 0;               // extreme substitution scores
+    int32_t lbias = 0;                        // int16_local_bias of this configuration
 
     std::vector<PairMeta> pairs;
     FillVector<Task> tasks;
@@ -274,6 +276,34 @@ struct RangeBound {
     int64_t upper;   // every real H is <= upper
 };
 RangeBound range_bound(const Plan &plan, bool local, uint32_t m, uint32_t n);
+
+// int16 variants with the column drift take the maximum of three values in one instruction, the
+// packed fp16 three-input maximum (v_pk_maximum3_f16), whose order on the 16-bit patterns
+// 0x0000..0x7C00 (zero, the positive numbers, +infinity) is their integer order.  Patterns
+// 0x7C01..0x7FFF are NaNs (the instruction returns a NaN) and 0x8000..0xFFFF are negative (in
+// reverse order; -1023..-1 are NaNs too).  The kernels therefore hold the values of these
+// variants biased so that every operand of a three-way maximum is in [0, kInt16Top] -- or,
+// for values derived from the low boundary words of columns past a task's end, at most -1024,
+// below every operand in range -- and a value above kInt16Top is an int16 overflow: it turns
+// the pair's running maximum into a NaN pattern, which the saturation test flags.
+inline constexpr int32_t kInt16Top = 31744;
+inline bool max3_variant(const VariantKey &key)
+{
+    return key.width == 16 && !key.track && !key.rowmax && !key.export_dp && !recentred(key);
+}
+// Bias of the local method's max3 variants, the same for every pair of a plan: every operand of
+// a three-way maximum is >= min(smin, go + ge) in exact terms (H >= 0; the diagonal term adds a
+// substitution score, the pad's -1 included; gap states are >= H + go + ge), plus a margin of
+// |ge| + 1 that also keeps the running maximum's start (the bias in the frame of step -1) >= 0.
+int32_t int16_local_bias(int32_t smin, int32_t go, int32_t ge);
+// Bias of one pair of a max3 variant: the local bias, or for the global method one that lifts
+// the pair's lower bound (less one substitution score for the diagonal term) to >= 0.  The
+// variant fits the pair when upper + bias <= kInt16Top (int16_fits).
+int32_t int16_bias(const Plan &plan, bool local, uint32_t m, uint32_t n);
+inline bool int16_fits(const RangeBound &rb, int32_t bias) { return rb.upper + bias <= kInt16Top; }
+// Largest local score the max3 int16 lanes compute exactly for this configuration (the learned
+// dispatcher's limit): kInt16Top less the local bias and the largest drift, |ge| * 81.
+int32_t int16_score_limit(const pa_config &cfg);
 
 // Pairs with more rows than this go to the wavefront kernel (G = 64, a workgroup per task).
 inline constexpr uint32_t kWavefrontMinRows = 3072;

@@ -165,6 +165,24 @@ static void check_plan(const char *label, const pa_config &cfg, const Batch &b, 
                     CHECK(p.dp_base[2 * t] == (pm.swapped ? c2 : c1) && p.dp_base[2 * t + 1] == (pm.swapped ? c1 : c2),
                           "%s: export offsets of pair %u", label, pr);
                 }
+                // int16 three-way-maximum variants: every operand of a three-way maximum biased into
+                // [0, kInt16Top] -- the lower bound less one substitution score lifted to >= 0, the
+                // upper bound (drift included) at most kInt16Top; global halves carry their bias
+                const uint32_t hb = (tk.bias >> (16 * q)) & 0xFFFFu;
+                if (max3_variant(key)) {
+                    const RangeBound rb = range_bound(p, pm.local, pm.m, pm.n);
+                    const int64_t b = pm.local ? p.lbias : int64_t(hb);
+                    CHECK(pm.local ? hb == 0 : hb == uint32_t(int16_bias(p, false, pm.m, pm.n)),
+                          "%s: bias of pair %u", label, pr);
+                    // local: H >= 0, so an operand is >= min(smin, pad -1, go + ge); global: >= the
+                    // lower bound of H and the gap states, or one substitution score below it
+                    const int64_t low = pm.local ? std::min<int64_t>({p.smin, -1, int64_t(p.go) + p.ge})
+                                                 : rb.lower + std::min(p.smin, -1);
+                    CHECK(low + b >= 0, "%s: biased lower bound of pair %u", label, pr);
+                    if (!pm.local) CHECK(rb.upper + b <= kInt16Top, "%s: biased upper bound of pair %u", label, pr);
+                } else {
+                    CHECK(hb == 0, "%s: bias of a pair outside the max3 variants", label);
+                }
                 cells += uint64_t(pm.m) * pm.n;
                 if (key.width == 16) ++n16; else ++n32;
             }

@@ -5,6 +5,7 @@ test/unit/affine/*_scalar*_test.cpp through the configuration API, and seeded di
 tests (alignment_simd_test_template.hpp:195-227) over every method, end-gap combination, score
 model and lane width, with equal and variable lengths, empty sequences and saturating pairs.
 """
+import dataclasses
 import itertools
 import json
 import os
@@ -174,6 +175,99 @@ def test_saturation_recompute(ctx, length, match):
         assert np.all(flags[:6] & device.PAIR_RECOMPUTED), flags[:6]
         assert np.all(got[:6] == length * match)
         assert not np.any(flags[6:] & device.PAIR_RECOMPUTED)
+
+
+INT16_TOP = 31744   # the largest value the int16 three-way-maximum lanes hold exactly
+
+
+@pytest.mark.parametrize("kernel", ["striped", "wavefront"])
+def test_int16_top_local(ctx, kernel):
+    """Local scores on both sides of the int16 lanes' top value, 31,744 (pa_plan.hpp, kInt16Top),
+    in the striped and in the wavefront kernel: identical DNA sequences at +4 (31,000 to 32,760)
+    and identical protein sequences under BLOSUM62.  Under int16 speculation every score is exact,
+    every pair scoring above the top is flagged and recomputed in int32 (such scores were exact
+    in int16 lanes when the top was 32,767), and pairs far below it are not; under the length
+    bound nothing is recomputed and the pairs above the top run in int32."""
+    rng = np.random.default_rng(31744)
+    lengths = [7750, 7900, 7936, 7950, 8100, 8190]
+    dna = workloads.unpack(*workloads.random_packed(rng, np.array(lengths), workloads.DNA))
+    short1, short2 = workloads.dna_pairs(rng, 30, 100, 400, divergence=0.1)
+    tuning = device.Tuning(wavefront_min_rows=1 << 20) if kernel == "striped" else device.Tuning(wavefront_tail=-1.0)
+    t = tables.blosum62_standard()
+    prot = workloads.unpack(*workloads.random_packed(rng, np.array([5600, 6000, 6150, 6300]), workloads.AMINO,
+                                                     workloads.AMINO_FREQ))
+    cases = [("dna", device.DeviceConfig(method=device.METHOD_LOCAL, match=4, mismatch=-5, gap_open=-10,
+                                         gap_extension=-1), dna + short1, dna + short2),
+             ("protein", device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
+                                             symbols="".join(s for s, _ in t).encode(),
+                                             matrix=tuple(v for _, r in t for v in r), gap_open=-11,
+                                             gap_extension=-1), prot, list(prot))]
+    for name, base, s1, s2 in cases:
+        for policy in (device.LANES_INT16, device.LANES_AUTO):
+            dc = dataclasses.replace(base, lane_policy=policy, tuning=tuning)
+            got, flags = check(ctx, s1, s2, dc, label=f"int16 top {name} {kernel} policy={policy}")
+            long_ = np.arange(len(s1)) < (len(dna) if name == "dna" else len(prot))
+            log(f"  scores {got[long_].tolist()} flags {flags[long_].tolist()}")
+            if policy == device.LANES_INT16:
+                assert np.all(flags[got > INT16_TOP] & device.PAIR_RECOMPUTED), flags
+                assert not np.any(flags[got < 30000] & device.PAIR_RECOMPUTED), flags
+            else:   # the bound policy proves int16 or runs int32: nothing to recompute
+                assert not np.any(flags & device.PAIR_RECOMPUTED), flags
+                assert np.all(flags[got > INT16_TOP] & device.PAIR_INT32), flags
+        assert np.any(got > INT16_TOP) and np.any((got < INT16_TOP) & long_)
+
+
+def test_short_pair_alone_int16(ctx):
+    """A batch of one short pair is a task of fewer columns (N) than the 16-step first chunk of the
+    striped kernel: its lanes pass column N-1 inside that chunk and sweep pad columns from there
+    on, so that no lane's state falls behind the column drift -- a stale state would sink below
+    the range of the int16 lanes' three-way maximum, and an identical 3-residue pair then scored 0.
+    Identical and unrelated pairs of 1-20 residues, each alone in its batch, local and global,
+    DNA and BLOSUM62, int16 lanes."""
+    rng = np.random.default_rng(316)
+    t = tables.blosum62_standard()
+    blosum = dict(score_kind=device.SCORE_MATRIX, symbols="".join(s for s, _ in t).encode(),
+                  matrix=tuple(v for _, r in t for v in r), gap_open=-11, gap_extension=-1)
+    for method in (device.METHOD_LOCAL, device.METHOD_GLOBAL):
+        for alpha, extra in ((workloads.DNA, dict(match=2, mismatch=-3, gap_open=-5, gap_extension=-2)),
+                             (workloads.AMINO, blosum)):
+            dc = device.DeviceConfig(method=method, lane_policy=device.LANES_INT16, **extra)
+            got_all, want_all = [], []
+            for n in range(1, 21):
+                a = bytes(workloads.random_residues(rng, alpha, n))
+                b = bytes(workloads.random_residues(rng, alpha, n))
+                for s1, s2 in (([a], [a]), ([a], [b])):
+                    got, flags = ctx.score(s1, s2, dc)
+                    assert flags[0] & device.PAIR_INT16, flags
+                    got_all.append(int(got[0]))
+                    want_all.append(int(O.score_pairs(s1, s2, oracle_cfg(dc))[0]))
+            log(f"  short pairs alone method={method} alphabet={len(alpha)}: {got_all}")
+            assert got_all == want_all, (method, len(alpha), got_all, want_all)
+
+
+def test_int16_bias_window_global(ctx):
+    """Global pairs on both sides of the int16 window of the three-way-maximum lanes (their lower
+    bound lifted to 0 by a per-pair bias, their upper bound at most 31,744): with +4/-5 and gap
+    (-10, -1), m = n up to about 5,270 fit (biased lanes), 5,300 and 5,500 do not (absolute int16
+    lanes with the running maximum).  Identical sequences score near the top, unrelated ones near
+    the bottom of the window; every score is exact and none is recomputed, in the striped kernel,
+    for every end-gap combination."""
+    rng = np.random.default_rng(5270)
+    lengths = [4000, 5000, 5200, 5300, 5500]
+    a = workloads.unpack(*workloads.random_packed(rng, np.array(lengths * 2), workloads.DNA))
+    b = workloads.unpack(*workloads.random_packed(rng, np.array(lengths), workloads.DNA))
+    s1 = a
+    s2 = a[:len(lengths)] + b            # identical pairs, then unrelated pairs of the same lengths
+    tuning = device.Tuning(wavefront_min_rows=1 << 20)
+    for fc, fr, lc, lr in itertools.product([False, True], repeat=4):
+        if (fc, fr, lc, lr) not in ((False, False, False, False), (True, True, True, True), (True, False, False, True),
+                                    (False, True, True, False)):
+            continue
+        dc = device.DeviceConfig(method=device.METHOD_GLOBAL, match=4, mismatch=-5, gap_open=-10, gap_extension=-1,
+                                 first_column_free=fc, first_row_free=fr, last_column_free=lc, last_row_free=lr,
+                                 lane_policy=device.LANES_AUTO, tuning=tuning)
+        got, flags = check(ctx, s1, s2, dc, label=f"global bias window ends={(fc, fr, lc, lr)}")
+        assert np.all(flags & device.PAIR_INT16) and not np.any(flags & device.PAIR_RECOMPUTED), flags
 
 
 @pytest.mark.parametrize("ends", [(False, False), (True, False), (False, True), (True, True)],
