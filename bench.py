@@ -488,9 +488,13 @@ def bench_mixed(count, steps, warmup, side_steps=5, only=None):
 
 def bench_saturating(model, prot, steps, count=20_000):
     """Where the lane choice decides: 20,000 protein pairs of 4.8-7.8 kbp at 80-100% identity,
-    about one in six of which score beyond the int16 range.  The same four policies with the same trained
-    network; int16 speculation pays an int32 recomputation for every saturated pair (the batch's
-    first run plans it, the timed runs replay it)."""
+    about a quarter of which score beyond the int16 lanes' range.  The same four policies with the
+    same trained network; int16 speculation pays an int32 recomputation for every saturated pair
+    (the batch's first run plans it, the timed runs replay it).  Each policy is timed with the
+    default launch shape and with the shape the library's autotuner picks for it on this batch
+    (pa_tune: long pairs split between the striped and the wavefront kernel, streams, launch form),
+    and credited with the faster of the two: how a small batch of long pairs is best laid out
+    depends on the widths the policy mixes."""
     t0 = time.perf_counter()
     w = workloads.config_saturating(count=count)
     cells = cells_of(w["o1"], w["o2"])
@@ -502,16 +506,28 @@ def bench_saturating(model, prot, steps, count=20_000):
     for name, policy in (("learned", device.LANES_LEARNED), ("int32", device.LANES_INT32),
                          ("int16_speculative", device.LANES_INT16), ("bound", device.LANES_AUTO)):
         dc = dataclasses.replace(prot, lane_policy=policy, model=model if policy == device.LANES_LEARNED else None)
-        r = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, None, steps, 1)
+        r0 = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, None, steps, 1)
+        t1 = time.perf_counter()
+        tuning = device.tune(ctx, dc, w["b1"], w["o1"], w["b2"], w["o2"], None, iters=2)
+        tune_s = time.perf_counter() - t1
+        r1 = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dataclasses.replace(dc, tuning=tuning), None, steps, 1)
+        # the policy is credited with the faster of its two measured shapes (the tuner times two
+        # passes per candidate, which can mislead it where a recomputation follows the pass)
+        r = r1 if r1["ms_per_step"] <= r0["ms_per_step"] else r0
         mism = 0 if ref is None else int(np.count_nonzero(ref != r["scores"]))
+        mism += int(np.count_nonzero(r0["scores"] != r1["scores"]))
         ref = r["scores"] if ref is None else ref
         out["policies"][name] = {"gcups": round(cells / (r["ms_per_step"] / 1e3) / 1e9, 2),
                                  "ms_per_step": round(r["ms_per_step"], 3), "steps": steps,
+                                 "shape": "autotuned" if r is r1 else "default",
+                                 "default_shape_ms_per_step": round(r0["ms_per_step"], 3),
+                                 "tuned_shape_ms_per_step": round(r1["ms_per_step"], 3),
+                                 "tuning": dataclasses.asdict(tuning), "tune_seconds": round(tune_s, 1),
                                  "n_int16": r["stats"]["n_int16"], "n_int32": r["stats"]["n_int32"],
                                  "n_recomputed": r["stats"]["n_recomputed"], "score_mismatches_vs_learned": mism}
         log(f"[bench] saturating/{name}: {out['policies'][name]}")
     pol = out["policies"]
-    out["scores_at_int16_limit"] = int(np.count_nonzero(ref >= 32767))
+    out["scores_above_int16_top"] = int(np.count_nonzero(ref > 31744))
     for name in ("learned", "int16_speculative", "bound"):
         out[f"speedup_{name}_vs_int32"] = round(pol[name]["gcups"] / pol["int32"]["gcups"], 3)
     out["speedup_learned_vs_int16_speculative"] = round(pol["learned"]["gcups"] / pol["int16_speculative"]["gcups"], 3)

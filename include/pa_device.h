@@ -229,8 +229,9 @@ pa_status pa_model_forward(pa_ctx *ctx, const pa_model *model, const float *x, u
 pa_status pa_model_loss_grad(pa_ctx *ctx, const pa_model *model, const float *x, const float *target,
                              const float *weight, uint64_t n, float *loss, float *grad);
 /* The relative safety margin of the lane choice: a pair runs in int16 lanes when its predicted
- * maximum of H times (1 + margin), plus 256, stays below 32767 (else int32).  A new model has 0.10;
- * a trainer sets it from the network's error on held-out pairs. */
+ * maximum of H times (1 + margin), plus 256, stays below the largest score the int16 lanes hold
+ * exactly (31,744 less the configuration's value bias and the largest drift of its frames; else
+ * int32).  A new model has 0.10; a trainer sets it from the network's error on held-out pairs. */
 pa_status pa_model_set_margin(pa_model *model, float margin);
 /* One Adam step on a minibatch; returns the loss before the step. */
 pa_status pa_model_train_step(pa_ctx *ctx, pa_model *model, const float *x, const float *target,

@@ -16,7 +16,8 @@ Configurations (sizes as stated; names are used in bench.py):
   mixed      500,000 pairs, half DNA half protein, lengths uniform 100-5,000, identity uniform
              50-100%.
   saturating 20,000 protein pairs, lengths uniform 4,800-7,800, identity uniform 80-100%: about
-             one in six scores beyond the int16 range (local BLOSUM62, gap (-11,-1)).
+             a quarter score beyond the int16 lanes' exact range, 31,744 (one in six beyond
+             32,767; local BLOSUM62, gap (-11,-1)).
 """
 from __future__ import annotations
 
@@ -298,7 +299,7 @@ def config_mixed(seed: int = 5, count: int = 500_000, limit: int = None, chunk: 
 def config_saturating(seed: int = 8, count: int = 20_000, limit: int = None, chunk: int = 2_000):
     """Protein pairs whose local scores straddle the int16 limit: lengths uniform 4,800-7,800,
     identity uniform 80-100% (90% of events substitutions), background residue frequencies.
-    About one in six scores 32,767 or more under BLOSUM62 gap (-11,-1) -- the slice where running
+    About a quarter score above 31,744 (one in six 32,767 or more) under BLOSUM62 gap (-11,-1) -- the slice where running
     every pair in int16 lanes costs a recomputation.  Chunk c from stream (seed, c)."""
     n = count if limit is None else min(limit, count)
 
