@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PA_ABI_VERSION 6
+#define PA_ABI_VERSION 7
 
 typedef enum pa_status {
     PA_OK = 0,
@@ -127,13 +127,27 @@ pa_status pa_score_batch(pa_ctx *ctx, const pa_config *cfg,
  * dp_algorithm_template_standard.hpp:43-100; the vectors the global tracker reads,
  * tracker_global_scalar.hpp:36-61).  dp_column receives H(i, len2), i = 0..len1, of pair p at
  * (off1[p] - off1[0]) + p; dp_row receives H(len1, j), j = 0..len2, at (off2[p] - off2[0]) + p.
- * The gap-state halves of the reference's cells are not exported.  Runs every pair in int32
- * lanes of the striped kernel. */
+ * Runs every pair in int32 lanes of the striped kernel. */
 pa_status pa_score_batch_dp(pa_ctx *ctx, const pa_config *cfg,
                             const uint8_t *seq1, const uint64_t *off1,
                             const uint8_t *seq2, const uint64_t *off2,
                             uint64_t n_pairs, const uint8_t *pair_method,
                             int32_t *scores, uint8_t *flags, int32_t *dp_column, int32_t *dp_row);
+
+/* pa_score_batch_dp with whole cells: the reference's DP vectors hold affine cells (score, gap
+ * state) (affine/affine_cell.hpp:25-73), updated by compute_cell (affine_dp_algorithm.hpp:139-156).
+ * dp_column_gap and dp_row_gap (both or neither; the layout of dp_column / dp_row) receive the gap
+ * halves: W(i, len2 + 1) = max(W(i, len2) + ge, H(i, len2) + go + ge), the horizontal gap state
+ * right of the last column, for i >= 1, and V(len1 + 1, j), the vertical one below the last row,
+ * for j >= 1; entry 0 of each vector keeps its initial go + ge (the sweep never updates it,
+ * dp_algorithm_template_base.hpp:94-128).  Without a sequence, a vector keeps its initial cells
+ * (H, H + go + ge) (affine_initialisation_strategy.hpp:38-51). */
+pa_status pa_score_batch_dp_cells(pa_ctx *ctx, const pa_config *cfg,
+                                  const uint8_t *seq1, const uint64_t *off1,
+                                  const uint8_t *seq2, const uint64_t *off2,
+                                  uint64_t n_pairs, const uint8_t *pair_method,
+                                  int32_t *scores, uint8_t *flags, int32_t *dp_column, int32_t *dp_row,
+                                  int32_t *dp_column_gap, int32_t *dp_row_gap);
 
 /* One-shot scoring that allocates no device memory.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (tight serving loops, harnesses that forbid

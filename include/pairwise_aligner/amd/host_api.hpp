@@ -25,9 +25,9 @@
 //   * score() is int32_t and exact for every score model, including the *_simd ones whose
 //     reference lanes wrap on overflow (simd_score_type.hpp:164-192); saturated int16 lanes
 //     are recomputed in int32;
-//   * dp_column()/dp_row() hold the scores of the final DP column and row only after a
-//     four-argument compute (one pair, one-to-one bulk or one-to-many); they are empty otherwise;
-//     a bulk result's dp_column()/dp_row() is its own pair's, not the whole bulk's.
+//   * dp_column()/dp_row() hold the cells (score, gap state) of the final DP column and row only
+//     after a four-argument compute (one pair, one-to-one bulk or one-to-many); they are empty
+//     otherwise; a bulk result's dp_column()/dp_row() is its own pair's, not the whole bulk's.
 #pragma once
 
 #include <array>
@@ -287,6 +287,19 @@ void append(std::vector<uint8_t> &buf, std::vector<uint64_t> &off, seq_t const &
 
 } // namespace detail
 
+// One entry of the final DP column or row: the score H and the gap state, as the reference's
+// affine cell (affine/affine_cell.hpp:25-73: a pair with score() = first, read with get<0>/get<1>
+// or a structured binding).  The column's gap state is the horizontal one right of the last
+// column, the row's the vertical one below the last row; entry 0 of either holds go + ge.
+struct dp_cell : std::pair<int32_t, int32_t> {
+    using base_t = std::pair<int32_t, int32_t>;
+    using score_type = int32_t;
+    using base_t::base_t;
+
+    constexpr int32_t &score() noexcept { return this->first; }
+    constexpr int32_t const &score() const noexcept { return this->first; }
+};
+
 // aligner_result.hpp:33-105: the sequences (as passed: references stay references) and the score.
 template <typename sequence1_t, typename sequence2_t>
 struct aligner_result {
@@ -294,17 +307,17 @@ struct aligner_result {
     sequence2_t _sequence2;
     int32_t _score;
     uint8_t _flags;
-    // scores of the final DP column (H(i, n), i = 0..m) and row (H(m, j), j = 0..n); filled by
+    // cells of the final DP column (H(i, n), i = 0..m) and row (H(m, j), j = 0..n); filled by
     // compute(seq1, seq2, first_dp_column, first_dp_row), empty otherwise
-    std::vector<int32_t> _dp_column{};
-    std::vector<int32_t> _dp_row{};
+    std::vector<dp_cell> _dp_column{};
+    std::vector<dp_cell> _dp_row{};
 
     sequence1_t const &sequence1() const noexcept { return _sequence1; }
     sequence2_t const &sequence2() const noexcept { return _sequence2; }
     int32_t const &score() const noexcept { return _score; }
     uint8_t flags() const noexcept { return _flags; }   // PA_PAIR_* bits
-    std::vector<int32_t> const &dp_column() const noexcept { return _dp_column; }
-    std::vector<int32_t> const &dp_row() const noexcept { return _dp_row; }
+    std::vector<dp_cell> const &dp_column() const noexcept { return _dp_column; }
+    std::vector<dp_cell> const &dp_row() const noexcept { return _dp_row; }
 };
 
 // aligner_result_bulk.hpp:24-70: a view of one pair of a bulk result shared by all its views.
@@ -315,8 +328,8 @@ struct bulk_result {
     std::vector<int32_t> _scores;
     std::vector<uint8_t> _flags;
     // per pair: the final DP column and row (four-argument compute only)
-    std::vector<std::vector<int32_t>> _dp_columns{};
-    std::vector<std::vector<int32_t>> _dp_rows{};
+    std::vector<std::vector<dp_cell>> _dp_columns{};
+    std::vector<std::vector<dp_cell>> _dp_rows{};
 };
 
 template <typename bulk_result_t, bool one_to_many>
@@ -344,21 +357,21 @@ public:
     uint8_t flags() const noexcept { return _result->_flags[_index]; }
 
     // aligner_result_bulk.hpp:43-51.  The reference returns the bulk's whole DP column / row
-    // (one SIMD vector per position); here each pair's own final column H(i, n), i = 0..m, and
-    // row H(m, j), j = 0..n, filled by the four-argument compute and empty otherwise.
-    std::vector<int32_t> const &dp_column() const & noexcept
+    // (one SIMD vector per position); here each pair's own final column, cells (H(i, n), gap
+    // state), i = 0..m, and row, j = 0..n, filled by the four-argument compute and empty otherwise.
+    std::vector<dp_cell> const &dp_column() const & noexcept
     {
         return _result->_dp_columns.empty() ? none() : _result->_dp_columns[_index];
     }
-    std::vector<int32_t> const &dp_row() const & noexcept
+    std::vector<dp_cell> const &dp_row() const & noexcept
     {
         return _result->_dp_rows.empty() ? none() : _result->_dp_rows[_index];
     }
 
 private:
-    static std::vector<int32_t> const &none() noexcept
+    static std::vector<dp_cell> const &none() noexcept
     {
-        static const std::vector<int32_t> empty{};
+        static const std::vector<dp_cell> empty{};
         return empty;
     }
 };
@@ -401,26 +414,30 @@ class aligner {
                              b2.empty() ? &empty : b2.data(), o2.data(), n, nullptr, scores.data(), flags.data()));
     }
 
-    // scores plus every pair's final DP column and row: pa_score_batch_dp, sliced per pair
+    // scores plus every pair's final DP column and row, whole cells: pa_score_batch_dp_cells,
+    // sliced per pair
     void run_dp(std::vector<uint8_t> const &b1, std::vector<uint64_t> const &o1, std::vector<uint8_t> const &b2,
                 std::vector<uint64_t> const &o2, std::vector<int32_t> &scores, std::vector<uint8_t> &flags,
-                std::vector<std::vector<int32_t>> &columns, std::vector<std::vector<int32_t>> &rows)
+                std::vector<std::vector<dp_cell>> &columns, std::vector<std::vector<dp_cell>> &rows)
     {
         const uint64_t n = o1.size() - 1;
         scores.assign(n, 0);
         flags.assign(n, 0);
-        std::vector<int32_t> col(b1.size() + n + 1), row(b2.size() + n + 1);
+        std::vector<int32_t> col(b1.size() + n + 1), row(b2.size() + n + 1), col_gap(col.size()),
+            row_gap(row.size());
         const pa_config cfg = device_config();
         static const uint8_t empty = 0;
-        check(pa_score_batch_dp(_ctx.get(), &cfg, b1.empty() ? &empty : b1.data(), o1.data(),
-                                b2.empty() ? &empty : b2.data(), o2.data(), n, nullptr, scores.data(), flags.data(),
-                                col.data(), row.data()));
+        check(pa_score_batch_dp_cells(_ctx.get(), &cfg, b1.empty() ? &empty : b1.data(), o1.data(),
+                                      b2.empty() ? &empty : b2.data(), o2.data(), n, nullptr, scores.data(),
+                                      flags.data(), col.data(), row.data(), col_gap.data(), row_gap.data()));
         columns.resize(n);
         rows.resize(n);
         for (uint64_t p = 0; p < n; ++p) {
-            const int32_t *c = col.data() + o1[p] + p, *r = row.data() + o2[p] + p;
-            columns[p].assign(c, c + (o1[p + 1] - o1[p]) + 1);
-            rows[p].assign(r, r + (o2[p + 1] - o2[p]) + 1);
+            const uint64_t c0 = o1[p] + p, r0 = o2[p] + p;
+            columns[p].clear();
+            rows[p].clear();
+            for (uint64_t i = 0; i <= o1[p + 1] - o1[p]; ++i) columns[p].emplace_back(col[c0 + i], col_gap[c0 + i]);
+            for (uint64_t j = 0; j <= o2[p + 1] - o2[p]; ++j) rows[p].emplace_back(row[r0 + j], row_gap[r0 + j]);
         }
     }
 
@@ -467,8 +484,8 @@ public:
     // one pair with the initial DP column and row (interface_one_to_one_single.hpp:52-67).  Like the
     // reference, which re-initialises both from the gap model and the end-gap rules before the
     // recursion (dp_vector_single.hpp:67-80, affine_dp_algorithm.hpp:48-72), the device takes the
-    // boundary from the configuration; the result carries the scores of the final DP column and
-    // row (aligner_result.hpp:41-59) through pa_score_batch_dp.
+    // boundary from the configuration; the result carries the cells of the final DP column and
+    // row (aligner_result.hpp:41-59) through pa_score_batch_dp_cells.
     template <detail::sequence sequence1_t, detail::sequence sequence2_t, typename dp_column_t, typename dp_row_t>
     auto compute(sequence1_t &&sequence1, sequence2_t &&sequence2, dp_column_t &&, dp_row_t &&)
     {
@@ -476,17 +493,13 @@ public:
         std::vector<uint64_t> o1{0}, o2{0};
         detail::append(b1, o1, sequence1);
         detail::append(b2, o2, sequence2);
-        std::vector<int32_t> col(b1.size() + 1), row(b2.size() + 1);
-        int32_t score = 0;
-        uint8_t flags = 0;
-        const pa_config cfg = device_config();
-        static const uint8_t empty = 0;
-        check(pa_score_batch_dp(_ctx.get(), &cfg, b1.empty() ? &empty : b1.data(), o1.data(),
-                                b2.empty() ? &empty : b2.data(), o2.data(), 1, nullptr, &score, &flags, col.data(),
-                                row.data()));
+        std::vector<int32_t> scores;
+        std::vector<uint8_t> flags;
+        std::vector<std::vector<dp_cell>> cols, rows;
+        run_dp(b1, o1, b2, o2, scores, flags, cols, rows);
         return aligner_result<sequence1_t, sequence2_t>{std::forward<sequence1_t>(sequence1),
-                                                        std::forward<sequence2_t>(sequence2), score, flags,
-                                                        std::move(col), std::move(row)};
+                                                        std::forward<sequence2_t>(sequence2), scores[0], flags[0],
+                                                        std::move(cols[0]), std::move(rows[0])};
     }
 
     // one-to-one bulk (interface_one_to_one_bulk.hpp:47-100), any bulk size
@@ -666,3 +679,11 @@ inline constexpr configure_aligner_fn configure_aligner{};
 } // namespace cfg
 } // inline namespace v1
 } // namespace seqan::pairwise_aligner
+
+// structured bindings of a cell, as for the reference's affine cell (affine/affine_cell.hpp:86-99)
+template <>
+struct std::tuple_size<seqan::pairwise_aligner::dp_cell> : std::integral_constant<std::size_t, 2> {};
+template <std::size_t I>
+struct std::tuple_element<I, seqan::pairwise_aligner::dp_cell> {
+    using type = int32_t;
+};

@@ -33,6 +33,13 @@
  *   local score  = max of H over interior cells (i,j >= 1), starting from INT32_MIN,
  *                  so an empty sequence yields INT32_MIN (local_affine_scalar_test.cpp:100-105).
  *   local forces all four end gaps free (method_local.hpp:41-49).
+ *
+ * The final DP column and row hold affine cells (score, gap state) (affine/affine_cell.hpp:25-73),
+ * initialised as (H, H + go + ge) (affine_initialisation_strategy.hpp:38-51) and updated by
+ * compute_cell (affine_dp_algorithm.hpp:139-156) whenever the cell is on the path of the sweep
+ * (dp_algorithm_template_base.hpp:82-129, which never touches entry 0 of either vector):
+ *   column gap state, i >= 1:  W(i, n+1) = max(W(i,n) + ge, H(i,n) + go + ge);   i = 0: go + ge
+ *   row gap state,    j >= 1:  V(m+1, j) = max(V(m,j) + ge, H(m,j) + go + ge);   j = 0: go + ge
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -72,9 +79,11 @@ static void build_rank_map(const pao_config *cfg, int32_t map[256])
 /* Returns 0 on success, -1 if a symbol is outside the matrix alphabet.  last_row (n+1 entries,
  * nullable) receives H(m,j), j = 0..n, and last_col (m+1 entries, nullable) H(i,n), i = 0..m: the
  * scores of the reference's final DP row and column (aligner_result.hpp:41-59; the vectors the
- * global tracker reads, tracker_global_scalar.hpp:36-61). */
+ * global tracker reads, tracker_global_scalar.hpp:36-61).  last_row_gap / last_col_gap (nullable,
+ * same sizes) receive the gap-state halves of the same cells (see the header). */
 int pao_score_dp(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
-                 const pao_config *cfg, int64_t *out_score, int64_t *last_row, int64_t *last_col)
+                 const pao_config *cfg, int64_t *out_score, int64_t *last_row, int64_t *last_col,
+                 int64_t *last_row_gap, int64_t *last_col_gap)
 {
     const int local = cfg->mode == PAO_LOCAL;
     const int fc = local || cfg->first_column_free;
@@ -103,6 +112,7 @@ int pao_score_dp(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
     int64_t best_local = INT32_MIN;
     int64_t best_col = Hrow[n];          /* max over H(i,n), i = 0..m */
     if (last_col) last_col[0] = Hrow[n];
+    if (last_col_gap) last_col_gap[0] = go + ge;
 
     for (int64_t i = 1; i <= m; ++i) {
         int64_t hcol0 = fc ? 0 : go + ge * i;  /* H(i,0) */
@@ -128,8 +138,11 @@ int pao_score_dp(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
         }
         best_col = max64(best_col, Hrow[n]);
         if (last_col) last_col[i] = Hrow[n];
+        if (last_col_gap) last_col_gap[i] = w;   /* W(i,n+1) */
     }
     if (last_row) for (int64_t j = 0; j <= n; ++j) last_row[j] = Hrow[j];
+    /* V(m+1,j); entry 0 keeps its initial go + ge (Vrow[0] is never updated) */
+    if (last_row_gap) for (int64_t j = 0; j <= n; ++j) last_row_gap[j] = Vrow[j];
 
     int64_t score;
     if (local) {
@@ -147,7 +160,7 @@ int pao_score_dp(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
 int pao_score(const uint8_t *s1, int64_t m, const uint8_t *s2, int64_t n,
               const pao_config *cfg, int64_t *out_score)
 {
-    return pao_score_dp(s1, m, s2, n, cfg, out_score, NULL, NULL);
+    return pao_score_dp(s1, m, s2, n, cfg, out_score, NULL, NULL, NULL, NULL);
 }
 
 typedef struct batch_job {
@@ -157,6 +170,7 @@ typedef struct batch_job {
     const uint8_t *pair_mode;   /* optional per-pair PAO_GLOBAL/PAO_LOCAL override */
     int64_t *scores;
     int64_t *rows, *cols;       /* optional final DP rows / columns (pao_dp_batch) */
+    int64_t *row_gaps, *col_gaps;   /* optional gap-state halves of the same cells */
     int64_t begin, end;
     int status;
 } batch_job;
@@ -173,19 +187,23 @@ static void *batch_worker(void *arg)
         int64_t n = (int64_t)(job->off2[p + 1] - job->off2[p]);
         int64_t *row = job->rows ? job->rows + (job->off2[p] - job->off2[0]) + p : NULL;
         int64_t *col = job->cols ? job->cols + (job->off1[p] - job->off1[0]) + p : NULL;
-        if (pao_score_dp(a, m, b, n, &local_cfg, &job->scores[p], row, col) != 0) job->status = -1;
+        int64_t *rg = job->row_gaps ? job->row_gaps + (job->off2[p] - job->off2[0]) + p : NULL;
+        int64_t *cg = job->col_gaps ? job->col_gaps + (job->off1[p] - job->off1[0]) + p : NULL;
+        if (pao_score_dp(a, m, b, n, &local_cfg, &job->scores[p], row, col, rg, cg) != 0) job->status = -1;
     }
     return NULL;
 }
 
 /* Scores n_pairs pairs given as concatenated sequences with n_pairs+1 offsets and, when rows /
  * cols are given, their final DP rows (pair p at (off2[p]-off2[0]) + p, n_p+1 entries) and columns
- * (pair p at (off1[p]-off1[0]) + p, m_p+1 entries).  Splits the batch over n_threads POSIX
- * threads.  Returns 0 or -1 (invalid symbol). */
+ * (pair p at (off1[p]-off1[0]) + p, m_p+1 entries); row_gaps / col_gaps (nullable) receive the
+ * gap-state halves in the same layout.  Splits the batch over n_threads POSIX threads.  Returns 0
+ * or -1 (invalid symbol). */
 int pao_dp_batch(const uint8_t *seq1, const uint64_t *off1,
                  const uint8_t *seq2, const uint64_t *off2,
                  int64_t n_pairs, const pao_config *cfg, const uint8_t *pair_mode,
-                 int64_t *scores, int64_t *rows, int64_t *cols, int n_threads)
+                 int64_t *scores, int64_t *rows, int64_t *cols, int64_t *row_gaps, int64_t *col_gaps,
+                 int n_threads)
 {
     if (n_threads < 1) n_threads = 1;
     if (n_threads > 256) n_threads = 256;
@@ -197,7 +215,7 @@ int pao_dp_batch(const uint8_t *seq1, const uint64_t *off1,
         int64_t b = t * chunk, e = b + chunk;
         if (b >= n_pairs) break;
         if (e > n_pairs) e = n_pairs;
-        jobs[t] = (batch_job){seq1, off1, seq2, off2, cfg, pair_mode, scores, rows, cols, b, e, 0};
+        jobs[t] = (batch_job){seq1, off1, seq2, off2, cfg, pair_mode, scores, rows, cols, row_gaps, col_gaps, b, e, 0};
         if (n_threads == 1) { batch_worker(&jobs[t]); started = 1; break; }
         pthread_create(&tids[t], NULL, batch_worker, &jobs[t]);
         ++started;
@@ -215,5 +233,5 @@ int pao_score_batch(const uint8_t *seq1, const uint64_t *off1,
                     int64_t n_pairs, const pao_config *cfg, const uint8_t *pair_mode,
                     int64_t *scores, int n_threads)
 {
-    return pao_dp_batch(seq1, off1, seq2, off2, n_pairs, cfg, pair_mode, scores, NULL, NULL, n_threads);
+    return pao_dp_batch(seq1, off1, seq2, off2, n_pairs, cfg, pair_mode, scores, NULL, NULL, NULL, NULL, n_threads);
 }

@@ -78,7 +78,7 @@ def _lib():
     lib.pao_dp_batch.argtypes = [
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
         ctypes.c_int64, ctypes.POINTER(_CConfig), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-        ctypes.c_void_p, ctypes.c_int]
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     lib.pao_score_batch.restype = ctypes.c_int
     lib.pao_score_batch.argtypes = [
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -123,15 +123,20 @@ def score_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_method=None, th
     return scores[:n]
 
 
-def dp_vectors_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_method=None, threads: int = 1):
+def dp_vectors_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_method=None, threads: int = 1,
+                      gaps: bool = False):
     """Scores plus the final DP row and column of every pair (int64): rows[(off2[p]-off2[0]) + p + j]
-    = H(m_p, j), j = 0..n_p, and cols[(off1[p]-off1[0]) + p + i] = H(i, n_p), i = 0..m_p."""
+    = H(m_p, j), j = 0..n_p, and cols[(off1[p]-off1[0]) + p + i] = H(i, n_p), i = 0..m_p.  With
+    gaps, also the gap-state halves of the same cells in the same layout (row_gaps: V(m_p+1, j),
+    col_gaps: W(i, n_p+1), entry 0 of each go + ge): returns (scores, rows, cols, row_gaps, col_gaps)."""
     n = len(off1) - 1
     scores = np.zeros(max(n, 1), dtype=np.int64)
     off1 = np.ascontiguousarray(off1, dtype=np.uint64)
     off2 = np.ascontiguousarray(off2, dtype=np.uint64)
     rows = np.zeros(max(int(off2[-1] - off2[0]) + n, 1), dtype=np.int64)
     cols = np.zeros(max(int(off1[-1] - off1[0]) + n, 1), dtype=np.int64)
+    row_gaps = np.zeros_like(rows) if gaps else None
+    col_gaps = np.zeros_like(cols) if gaps else None
     c = _CConfig(cfg.mode, int(cfg.first_column_free), int(cfg.first_row_free),
                  int(cfg.last_column_free), int(cfg.last_row_free), cfg.gap_open, cfg.gap_extension,
                  cfg.score_kind, cfg.match, cfg.mismatch, len(cfg.symbols), None, None)
@@ -146,10 +151,15 @@ def dp_vectors_packed(buf1, off1, buf2, off2, cfg: OracleConfig, pair_method=Non
     buf2 = np.ascontiguousarray(buf2, dtype=np.uint8)
     rc = _lib().pao_dp_batch(buf1.ctypes.data, off1.ctypes.data, buf2.ctypes.data, off2.ctypes.data, n,
                              ctypes.byref(c), None if pm is None else pm.ctypes.data, scores.ctypes.data,
-                             rows.ctypes.data, cols.ctypes.data, threads)
+                             rows.ctypes.data, cols.ctypes.data,
+                             None if row_gaps is None else row_gaps.ctypes.data,
+                             None if col_gaps is None else col_gaps.ctypes.data, threads)
     if rc != 0:
         raise ValueError("symbol outside the substitution matrix alphabet")
-    return scores[:n], rows[:int(off2[-1] - off2[0]) + n], cols[:int(off1[-1] - off1[0]) + n]
+    nr, nc = int(off2[-1] - off2[0]) + n, int(off1[-1] - off1[0]) + n
+    if gaps:
+        return scores[:n], rows[:nr], cols[:nc], row_gaps[:nr], col_gaps[:nc]
+    return scores[:n], rows[:nr], cols[:nc]
 
 
 SIMD_LIB_PATH = os.path.join(_HERE, "libsimdport.so")

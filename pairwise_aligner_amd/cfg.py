@@ -144,9 +144,33 @@ def lane_policy(*args):
 INT32_MIN = -(2 ** 31)
 
 
+class dp_cell(tuple):
+    """One entry of the final DP column or row, (score, gap state), as the reference's affine cell
+    (affine/affine_cell.hpp:25-73: a pair whose score() is the first member).  The column's gap
+    state is the horizontal one right of the last column, the row's the vertical one below the last
+    row; entry 0 of either holds go + ge."""
+
+    __slots__ = ()
+
+    def __new__(cls, score, gap):
+        return tuple.__new__(cls, (int(score), int(gap)))
+
+    def score(self) -> int:
+        return self[0]
+
+    @property
+    def first(self) -> int:
+        return self[0]
+
+    @property
+    def second(self) -> int:
+        return self[1]
+
+
 class aligner_result:
     """aligner_result.hpp:33-105: the two sequences, the score and, after the four-argument
-    compute, the final DP column H(i, n), i = 0..m, and row H(m, j), j = 0..n (empty otherwise)."""
+    compute, the cells of the final DP column (H(i, n), i = 0..m) and row (H(m, j), j = 0..n),
+    empty otherwise."""
 
     __slots__ = ("_s1", "_s2", "_score", "_flags", "_dp_column", "_dp_row")
 
@@ -225,12 +249,14 @@ class Aligner:
             raise ValueError("bulk compute needs the same number of sequences on both sides")
         b1, o1 = _dev.pack_sequences(s1)
         b2, o2 = _dev.pack_sequences(s2)
-        scores, flags, col, row = self._ctx().score_packed_dp(b1, o1, b2, o2, self.config)
+        scores, flags, col, row, col_gap, row_gap = self._ctx().score_packed_dp(b1, o1, b2, o2, self.config,
+                                                                                gaps=True)
         out = []
         for p, (a, b) in enumerate(zip(firsts, seq2)):
-            c0, r0 = int(o1[p]) + p, int(o2[p]) + p
-            out.append(aligner_result(a, b, scores[p], flags[p], col[c0:c0 + len(s1[p]) + 1].tolist(),
-                                      row[r0:r0 + len(s2[p]) + 1].tolist()))
+            c = slice(int(o1[p]) + p, int(o1[p]) + p + len(s1[p]) + 1)
+            r = slice(int(o2[p]) + p, int(o2[p]) + p + len(s2[p]) + 1)
+            out.append(aligner_result(a, b, scores[p], flags[p], map(dp_cell, col[c], col_gap[c]),
+                                      map(dp_cell, row[r], row_gap[r])))
         return out
 
 

@@ -255,6 +255,7 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     if (L.key.export_dp) {
         a.dp_out = static_cast<int32_t *>(b->dp_out.ptr);
         a.dp_base = static_cast<const uint64_t *>(pd.dp_base.ptr) + 2ull * L.task_begin;
+        a.dp_gap = p.dp_entries;
     }
     return a;
 }
@@ -597,7 +598,7 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
 
     if ((st = upload_plan(ctx, b->main, b->residues, arena)) != PA_OK) return st;
     if (export_dp) {
-        b->dp_out.bytes = std::max<size_t>(p.dp_entries * 4, 16);
+        b->dp_out.bytes = std::max<size_t>(p.dp_entries * 8, 16);   // scores, then gap states
         PA_HIP(hipMalloc(&b->dp_out.ptr, b->dp_out.bytes), "hipMalloc");
     }
     if ((st = upload(ctx, b->pairs, p.pairs, arena)) != PA_OK) return st;
@@ -1090,8 +1091,19 @@ pa_status pa_score_batch_dp(pa_ctx *ctx, const pa_config *cfg, const uint8_t *se
                             const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, const uint8_t *pair_method,
                             int32_t *scores, uint8_t *flags, int32_t *dp_column, int32_t *dp_row)
 {
+    return pa_score_batch_dp_cells(ctx, cfg, seq1, off1, seq2, off2, n_pairs, pair_method, scores, flags, dp_column,
+                                   dp_row, nullptr, nullptr);
+}
+
+pa_status pa_score_batch_dp_cells(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
+                                  const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs,
+                                  const uint8_t *pair_method, int32_t *scores, uint8_t *flags, int32_t *dp_column,
+                                  int32_t *dp_row, int32_t *dp_column_gap, int32_t *dp_row_gap)
+{
     if (!ctx || !cfg || !off1 || !off2 || (n_pairs && (!seq1 || !seq2 || !dp_column || !dp_row)))
         return fail(ctx, PA_ERR_INVALID_ARGUMENT, "null argument");
+    if (!dp_column_gap != !dp_row_gap)
+        return fail(ctx, PA_ERR_INVALID_ARGUMENT, "dp_column_gap and dp_row_gap go together");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PA_ERR_DEVICE, "hipSetDevice");
     pa_batch *b = nullptr;
     pa_status st = create_batch(ctx, cfg, seq1, off1, false, seq2, off2, n_pairs, pair_method, &b, true);
@@ -1100,14 +1112,22 @@ pa_status pa_score_batch_dp(pa_ctx *ctx, const pa_config *cfg, const uint8_t *se
     if ((st = pa_batch_run(ctx, b)) != PA_OK) return st;
     if ((st = pa_batch_results(ctx, b, scores, flags)) != PA_OK) return st;
     const uint64_t n1 = off1[n_pairs] - off1[0] + n_pairs, n2 = off2[n_pairs] - off2[0] + n_pairs;
+    const bool gaps = dp_column_gap != nullptr;
     if (n_pairs) {
-        PA_HIP(hipMemcpy(dp_column, b->dp_out.ptr, n1 * 4, hipMemcpyDeviceToHost), "hipMemcpy");
-        PA_HIP(hipMemcpy(dp_row, static_cast<char *>(b->dp_out.ptr) + n1 * 4, n2 * 4, hipMemcpyDeviceToHost),
-               "hipMemcpy");
+        // the device buffer holds the columns, then the rows (n1 + n2 entries), then the gap halves
+        const char *dev = static_cast<const char *>(b->dp_out.ptr);
+        PA_HIP(hipMemcpy(dp_column, dev, n1 * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        PA_HIP(hipMemcpy(dp_row, dev + n1 * 4, n2 * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (gaps) {
+            PA_HIP(hipMemcpy(dp_column_gap, dev + (n1 + n2) * 4, n1 * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+            PA_HIP(hipMemcpy(dp_row_gap, dev + (2 * n1 + n2) * 4, n2 * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        }
     }
-    // pairs with an empty sequence have no task: their vectors are boundary values
-    // (affine_initialisation_strategy.hpp:38-51; local: every end free)
+    // pairs with an empty sequence have no task: the vector along the other sequence keeps its
+    // initial cells (H, H + go + ge) (affine_initialisation_strategy.hpp:38-51; local: every end
+    // free), and the vector of one cell holds (H(0, n) or H(m, 0), go + ge)
     const bool local_all = cfg->method == PA_METHOD_LOCAL;
+    const int32_t gogo = int32_t(cfg->gap_open + cfg->gap_extension);
     for (uint64_t p = 0; p < n_pairs; ++p) {
         const uint64_t m = off1[p + 1] - off1[p], n = off2[p + 1] - off2[p];
         if (m && n) continue;
@@ -1116,13 +1136,24 @@ pa_status pa_score_batch_dp(pa_ctx *ctx, const pa_config *cfg, const uint8_t *se
         auto h = [&](uint64_t k, bool free_end) -> int32_t {
             return (k == 0 || free_end) ? 0 : int32_t(cfg->gap_open + int64_t(cfg->gap_extension) * int64_t(k));
         };
-        int32_t *col = dp_column + (off1[p] - off1[0]) + p, *row = dp_row + (off2[p] - off2[0]) + p;
+        const uint64_t c0 = (off1[p] - off1[0]) + p, r0 = (off2[p] - off2[0]) + p;
+        int32_t *col = dp_column + c0, *row = dp_row + r0;
         if (m == 0) {
             for (uint64_t j = 0; j <= n; ++j) row[j] = h(j, fr);
             col[0] = h(n, fr);
         } else {
             for (uint64_t i = 0; i <= m; ++i) col[i] = h(i, fc);
             row[0] = h(m, fc);
+        }
+        if (gaps) {
+            int32_t *cg = dp_column_gap + c0, *rg = dp_row_gap + r0;
+            if (m == 0) {
+                for (uint64_t j = 0; j <= n; ++j) rg[j] = h(j, fr) + gogo;
+                cg[0] = gogo;
+            } else {
+                for (uint64_t i = 0; i <= m; ++i) cg[i] = h(i, fc) + gogo;
+                rg[0] = gogo;
+            }
         }
     }
     return PA_OK;

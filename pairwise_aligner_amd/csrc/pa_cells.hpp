@@ -267,17 +267,22 @@ __device__ __forceinline__ void finish_scores(const Rows<W, R, ACC> &st, uint32_
 // maxima go to rowacc (ACC).  MAX3 (int16 lanes with the column drift, values biased into the
 // three-way maximum's range): the local clamp and max(D, W) are one three-way maximum, the
 // running maximum takes two rows per instruction, and the global method's max(D, W, V) is one.
-template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT, bool CDRIFT, bool MAX3 = false>
+// CAPV (the DP export, no column drift): *vcap receives the vertical gap state below row vrow
+// of this lane (Eup after that row; unchanged when vrow is not one of the lane's rows).
+template <int W, int R, int SCORE, bool LOCAL, bool ACC, bool DRIFT, bool CDRIFT, bool MAX3 = false,
+          bool CAPV = false>
 __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<W>::V &Eup,
                                              typename Lane<W>::V diag, typename Lane<W>::V &runmax,
                                              const typename Lane<W>::V (&sc)[R], uint32_t Bin, uint32_t tbl_lo,
                                              uint32_t tbl_hi, typename Lane<W>::V vgo,
                                              typename Lane<W>::V vgogo, typename Lane<W>::V vge,
-                                             typename Lane<W>::V vfloor = typename Lane<W>::V{})
+                                             typename Lane<W>::V vfloor = typename Lane<W>::V{},
+                                             typename Lane<W>::V *vcap = nullptr, int vrow = -1)
 {
     using L = Lane<W>;
     using V = typename L::V;
     static_assert(!MAX3 || (W == 16 && CDRIFT), "three-way maxima: int16 lanes with the column drift");
+    static_assert(!CAPV || !CDRIFT, "the gap-state capture serves the DP export (no column drift)");
     // Diagonal terms of all R rows first: they read only the previous column's H, so every
     // H[r] can then be overwritten in place (no register rotation from step to step).
     V D[R];
@@ -343,6 +348,7 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
         for (int r = 0; r < R; ++r) {
             const V best = L::vmax(DF[r], Eup);
             Eup = L::vmax(Eup, DFo[r]);
+            if constexpr (CAPV) if (r == vrow) *vcap = Eup;
             st.H[r] = best;
             if constexpr (LOCAL) runmax = L::vmax(runmax, best);
             if constexpr (ACC) st.rowacc[r] = L::vmax(st.rowacc[r], best);
@@ -356,6 +362,7 @@ __device__ __forceinline__ void update_cells(Rows<W, R, ACC> &st, typename Lane<
             const V best = L::vmax(DF[r], Eup);
             const V open = L::add(best, vgogo);
             Eup = L::vmax(L::add(Eup, vge), open);
+            if constexpr (CAPV) if (r == vrow) *vcap = Eup;
             st.H[r] = best;
             if constexpr (LOCAL) runmax = L::vmax(runmax, best);
             if constexpr (ACC) st.rowacc[r] = L::vmax(st.rowacc[r], best);

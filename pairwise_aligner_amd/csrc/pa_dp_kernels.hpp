@@ -163,6 +163,10 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
             if (t == 0) {
                 dp_col[0] = h0_value(int(tk.n[0]), fr, go, ge);
                 dp_row[0] = h0_value(int(tk.m[0]), fc, go, ge);
+                // entry 0 of each vector keeps its initial gap state, go + ge: the sweep never
+                // updates it (dp_algorithm_template_base.hpp:94-128 reads entries from 1 on)
+                dp_col[a.dp_gap] = go + ge;
+                dp_row[a.dp_gap] = go + ge;
             }
         }
 
@@ -286,27 +290,38 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
                 if constexpr (LOCAL && CDRIFT) runmax = L::add(runmax, vcg);   // into this step's frame
                 if (!MASKED || (ROWMAX || TRACK || EXPORT ? uint32_t(c) < uint32_t(N) : c >= 0)) {
                     V Eup = L::from(Ein);
-                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT, MAX3>(st, Eup, Hdiag, runmax, sc, Bin, tbl_lo,
-                                                                               tbl_hi, vgo, vgogo, vge,
-                                                                               L::splat(cg * (k % G) + lbias));
+                    // EXPORT: the vertical gap state below the last row, from the lane that holds it
+                    V vlast{};
+                    const int er = EXPORT ? int(tk.m[0]) - 1 - row0 : -1;
+                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT, MAX3, EXPORT>(
+                        st, Eup, Hdiag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo, vge,
+                        L::splat(cg * (k % G) + lbias), &vlast, er);
                     Hdiag = L::from(Hin);
                     Hout = L::bits(DRIFT || CDRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
                     Eout = L::bits(DRIFT || CDRIFT ? L::add(Eup, vout) : Eup);
                     if constexpr (EXPORT) {
                         // the last row: H(m, c+1) from the lane that holds row m; the last column:
                         // H(i, n) of every row of this lane at column n
+                        // Gap halves (affine_dp_algorithm.hpp:139-156): below row m, V(m+1, c+1), and
+                        // right of column n, W(i, n+1), the gap states after the last update.  With
+                        // the row drift, row r of a lane holds H and W less ge*(r+1), and the vertical
+                        // state below it less ge*(r+2).
                         const int m = int(tk.m[0]), n = int(tk.n[0]);
-                        const int er = m - 1 - row0;
                         if (er >= 0 && er < R && c < n) {
                             int v = 0;
 #pragma unroll
                             for (int r = 0; r < R; ++r) if (r == er) v = L::half(st.H[r], 0);
                             dp_row[c + 1] = v + dge * (er + 1);
+                            dp_row[a.dp_gap + c + 1] = L::half(vlast, 0) + dge * (er + 2);
                         }
                         if (c == n - 1) {
 #pragma unroll
-                            for (int r = 0; r < R; ++r)
-                                if (row0 + r + 1 <= m) dp_col[row0 + r + 1] = L::half(st.H[r], 0) + dge * (r + 1);
+                            for (int r = 0; r < R; ++r) {
+                                if (row0 + r + 1 <= m) {
+                                    dp_col[row0 + r + 1] = L::half(st.H[r], 0) + dge * (r + 1);
+                                    dp_col[a.dp_gap + row0 + r + 1] = L::half(st.F[r], 0) + dge * (r + 1);
+                                }
+                            }
                         }
                     }
                 }

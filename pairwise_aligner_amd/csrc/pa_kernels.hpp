@@ -28,8 +28,10 @@ struct KernelArgs {
     int32_t lbias;              // int16 max3 variants of the local method: the bias of their values
     uint32_t tbl_lo, tbl_hi;
     int32_t fc, fr, lc, lr;
-    int32_t *dp_out;            // DP export variants: the final rows and columns
+    int32_t *dp_out;            // DP export variants: the final rows and columns (scores), then
+                                // the gap-state halves of the same cells, dp_gap entries further on
     const uint64_t *dp_base;    // 2 per task of this launch (pa_plan.hpp)
+    uint64_t dp_gap;
     uint32_t *queue;            // nullable: the launch's work queue (next task, finished blocks), zero
                                 // between passes; striped kernel only
 };

@@ -30,7 +30,7 @@ EXPORTS = ["pa_abi_version", "pa_ctx_create", "pa_ctx_destroy", "pa_ctx_last_err
            "pa_batch_destroy", "pa_model_create", "pa_model_destroy", "pa_model_param_count",
            "pa_model_set_params", "pa_model_get_params", "pa_features", "pa_model_forward", "pa_model_loss_grad",
            "pa_model_train_step", "pa_dp_labels", "pa_tune", "pa_score_one_to_many", "pa_batch_create_one_to_many",
-           "pa_score_batch_dp", "pa_score_batch_noalloc", "pa_model_set_margin"]
+           "pa_score_batch_dp", "pa_score_batch_dp_cells", "pa_score_batch_noalloc", "pa_model_set_margin"]
 N_FEATURES, N_OUTPUTS, N_HIDDEN = 8, 2, 64
 
 
@@ -144,6 +144,8 @@ def lib() -> ctypes.CDLL:
                                               ctypes.POINTER(vp)]
     L.pa_score_batch_noalloc.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp, vp]
     L.pa_score_batch_dp.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp, vp, vp, vp]
+    L.pa_score_batch_dp_cells.argtypes = [vp, ctypes.POINTER(CConfig), vp, vp, vp, vp, u64, vp, vp, vp, vp, vp, vp,
+                                          vp]
     L.pa_batch_run.argtypes = [vp, vp]
     L.pa_batch_results.argtypes = [vp, vp, vp, vp]
     L.pa_batch_stats_get.argtypes = [vp, ctypes.POINTER(CStats)]
@@ -166,7 +168,7 @@ def lib() -> ctypes.CDLL:
                  "pa_model_destroy", "pa_model_set_params", "pa_model_get_params", "pa_features",
                  "pa_model_forward", "pa_model_loss_grad", "pa_model_train_step", "pa_dp_labels", "pa_tune",
                  "pa_score_one_to_many", "pa_batch_create_one_to_many", "pa_score_batch_dp",
-                 "pa_score_batch_noalloc", "pa_model_set_margin"]:
+                 "pa_score_batch_dp_cells", "pa_score_batch_noalloc", "pa_model_set_margin"]:
         getattr(L, name).restype = i32
     return L
 
@@ -304,9 +306,11 @@ class Context:
         self.check(st)
         return scores[:n], flags[:n]
 
-    def score_packed_dp(self, b1, o1, b2, o2, cfg: DeviceConfig, pair_method=None):
+    def score_packed_dp(self, b1, o1, b2, o2, cfg: DeviceConfig, pair_method=None, gaps: bool = False):
         """Scores, flags and the final DP column and row of every pair (pa_score_batch_dp):
-        dp_column[(o1[p]-o1[0]) + p + i] = H(i, len2), dp_row[(o2[p]-o2[0]) + p + j] = H(len1, j)."""
+        dp_column[(o1[p]-o1[0]) + p + i] = H(i, len2), dp_row[(o2[p]-o2[0]) + p + j] = H(len1, j).
+        With gaps (pa_score_batch_dp_cells), also the gap-state halves of the same cells in the same
+        layout: returns (scores, flags, dp_column, dp_row, dp_column_gap, dp_row_gap)."""
         b1, o1, b2, o2, pm = checked_batch(b1, o1, b2, o2, pair_method)
         n = len(o1) - 1
         holder = _CConfigHolder(cfg)
@@ -316,12 +320,22 @@ class Context:
         n2 = int(o2[-1] - o2[0]) + n
         dp_column = np.zeros(max(n1, 1), dtype=np.int32)
         dp_row = np.zeros(max(n2, 1), dtype=np.int32)
-        st = self._lib.pa_score_batch_dp(self.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
-                                         b2.ctypes.data, o2.ctypes.data, n, None if pm is None else pm.ctypes.data,
-                                         scores.ctypes.data, flags.ctypes.data, dp_column.ctypes.data,
-                                         dp_row.ctypes.data)
+        if not gaps:
+            st = self._lib.pa_score_batch_dp(self.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
+                                             b2.ctypes.data, o2.ctypes.data, n, None if pm is None else pm.ctypes.data,
+                                             scores.ctypes.data, flags.ctypes.data, dp_column.ctypes.data,
+                                             dp_row.ctypes.data)
+            self.check(st)
+            return scores[:n], flags[:n], dp_column[:n1], dp_row[:n2]
+        column_gap = np.zeros_like(dp_column)
+        row_gap = np.zeros_like(dp_row)
+        st = self._lib.pa_score_batch_dp_cells(self.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
+                                               b2.ctypes.data, o2.ctypes.data, n,
+                                               None if pm is None else pm.ctypes.data, scores.ctypes.data,
+                                               flags.ctypes.data, dp_column.ctypes.data, dp_row.ctypes.data,
+                                               column_gap.ctypes.data, row_gap.ctypes.data)
         self.check(st)
-        return scores[:n], flags[:n], dp_column[:n1], dp_row[:n2]
+        return scores[:n], flags[:n], dp_column[:n1], dp_row[:n2], column_gap[:n1], row_gap[:n2]
 
     def score_one_to_many(self, query, b2, o2, cfg: DeviceConfig, pair_method=None):
         """Scores of (query, target p) for every packed target (pa_score_one_to_many)."""

@@ -209,20 +209,43 @@ static void bulk_calls()
     }
 }
 
+// the two halves of a vector of cells
+static std::vector<int32_t> scores_of(std::vector<aligner::dp_cell> const &v)
+{
+    std::vector<int32_t> out;
+    for (auto const &c : v) out.push_back(c.score());
+    return out;
+}
+static std::vector<int32_t> gaps_of(std::vector<aligner::dp_cell> const &v)
+{
+    std::vector<int32_t> out;
+    for (auto const &[score, gap] : v) out.push_back(gap);
+    return out;
+}
+
 // compute(seq1, seq2, first_dp_column, first_dp_row): the final DP column and row.  Expected
 // values worked by hand from the recurrence (+4/-5, gap (-10,-1), global): H(4, j) of ACGT x AGT is
-// -14 -9 -13 1 and H(i, 3) is -13 -8 -12 -6 1.
+// -14 -9 -13 1 and H(i, 3) is -13 -8 -12 -6 1.  The gap halves: below row 4, V(5, j) =
+// max over i of H(i, j) - 10 - (5 - i), e.g. V(5, 1) = H(1, 1) - 14 = -10; right of column 3,
+// W(i, 4) = max over j of H(i, j) - 10 - (4 - j), e.g. W(4, 4) = H(4, 3) - 11 = -10; entry 0 of
+// each vector -11.
 void dp_vectors()
 {
     auto a = aligner::cfg::configure_aligner(aligner::cfg::score_model_unitary(standard, 4, -5));
     std::vector<int> unused_column, unused_row;
     auto r = a.compute(std::string{"ACGT"}, std::string{"AGT"}, unused_column, unused_row);
     EXPECT_EQ(r.score(), 1);
-    EXPECT_EQ(r.dp_row() == (std::vector<int32_t>{-14, -9, -13, 1}), true);
-    EXPECT_EQ(r.dp_column() == (std::vector<int32_t>{-13, -8, -12, -6, 1}), true);
+    EXPECT_EQ(scores_of(r.dp_row()) == (std::vector<int32_t>{-14, -9, -13, 1}), true);
+    EXPECT_EQ(scores_of(r.dp_column()) == (std::vector<int32_t>{-13, -8, -12, -6, 1}), true);
+    EXPECT_EQ(gaps_of(r.dp_row()) == (std::vector<int32_t>{-11, -10, -14, -10}), true);
+    EXPECT_EQ(gaps_of(r.dp_column()) == (std::vector<int32_t>{-11, -9, -13, -15, -10}), true);
+    EXPECT_EQ(std::get<1>(r.dp_row()[1]), -10);
+    EXPECT_EQ(r.dp_column()[4].second, -10);
     auto e = a.compute(std::string{""}, std::string{"CC"}, unused_column, unused_row);
-    EXPECT_EQ(e.dp_row() == (std::vector<int32_t>{0, -11, -12}), true);
-    EXPECT_EQ(e.dp_column() == (std::vector<int32_t>{-12}), true);
+    EXPECT_EQ(scores_of(e.dp_row()) == (std::vector<int32_t>{0, -11, -12}), true);
+    EXPECT_EQ(scores_of(e.dp_column()) == (std::vector<int32_t>{-12}), true);
+    EXPECT_EQ(gaps_of(e.dp_row()) == (std::vector<int32_t>{-11, -22, -23}), true);
+    EXPECT_EQ(gaps_of(e.dp_column()) == (std::vector<int32_t>{-11}), true);
 
     // bulk and one-to-many forms (interface_one_to_one_bulk.hpp:73-100,
     // interface_one_to_many_bulk.hpp:67-77; aligner_result_bulk.hpp:43-51): each result carries its
@@ -249,8 +272,9 @@ void dp_vectors()
         }
     }
     auto bulk = a.compute(s1, s2, unused_column, unused_row);
-    EXPECT_EQ(bulk[0].dp_row() == (std::vector<int32_t>{-14, -9, -13, 1}), true);
-    EXPECT_EQ(bulk[0].dp_column() == (std::vector<int32_t>{-13, -8, -12, -6, 1}), true);
+    EXPECT_EQ(scores_of(bulk[0].dp_row()) == (std::vector<int32_t>{-14, -9, -13, 1}), true);
+    EXPECT_EQ(scores_of(bulk[0].dp_column()) == (std::vector<int32_t>{-13, -8, -12, -6, 1}), true);
+    EXPECT_EQ(gaps_of(bulk[0].dp_row()) == (std::vector<int32_t>{-11, -10, -14, -10}), true);
     auto plain = a.compute(s1, s2);
     EXPECT_EQ(plain[0].dp_column().empty() && plain[0].dp_row().empty(), true);
 }

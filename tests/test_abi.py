@@ -25,7 +25,7 @@ def test_library_exports_every_symbol():
     lib = ctypes.CDLL(device.LIB_PATH)
     for name in declared_symbols() + declared_symbols("pa_fastx.h"):
         assert hasattr(lib, name), name
-    assert lib.pa_abi_version() == 6
+    assert lib.pa_abi_version() == 7
 
 
 def test_configure_aligner_requires_score_and_gap_models():

@@ -1,5 +1,7 @@
-"""GPU: the final DP column and row of every pair (pa_score_batch_dp), the device counterpart of
-the reference's result.dp_column() / dp_row() (result/aligner_result.hpp:41-59).
+"""GPU: the final DP column and row of every pair, whole cells (score and gap state;
+pa_score_batch_dp_cells, and pa_score_batch_dp for the scores alone), the device counterpart of
+the reference's result.dp_column() / dp_row() (result/aligner_result.hpp:41-59,
+affine/affine_cell.hpp:25-73).
 
 Every exported entry is compared with the oracle's (oracle.dp_vectors_packed, pinned by
 enumeration in tests/test_oracle.py), across the end-gap combinations, the local method, per-pair
@@ -40,14 +42,20 @@ def check(ctx, s1, s2, dc, pair_method=None, label=""):
     log(f"dp export {label}: {len(s1)} pairs")
     b1, o1 = workloads.pack(s1)
     b2, o2 = workloads.pack(s2)
-    want_s, want_rows, want_cols = O.dp_vectors_packed(b1, o1, b2, o2, oracle_cfg(dc), pair_method=pair_method,
-                                                       threads=8)
-    got_s, flags, cols, rows = ctx.score_packed_dp(b1, o1, b2, o2, dc, pair_method)
+    want = O.dp_vectors_packed(b1, o1, b2, o2, oracle_cfg(dc), pair_method=pair_method, threads=8, gaps=True)
+    want_s = want[0]
+    got = ctx.score_packed_dp(b1, o1, b2, o2, dc, pair_method, gaps=True)
+    got_s = got[0]
     assert np.array_equal(got_s.astype(np.int64), want_s), f"{label}: scores"
-    bad_c = np.nonzero(cols.astype(np.int64) != want_cols)[0]
-    bad_r = np.nonzero(rows.astype(np.int64) != want_rows)[0]
-    assert len(bad_c) == 0, f"{label}: {len(bad_c)} column entries differ, first at {bad_c[:5].tolist()}"
-    assert len(bad_r) == 0, f"{label}: {len(bad_r)} row entries differ, first at {bad_r[:5].tolist()}"
+    # columns, rows, then their gap halves (pa_score_batch_dp_cells); the oracle returns rows first
+    _, want_rows, want_cols, want_row_gaps, want_col_gaps = want
+    for name, g, w in zip(("column", "row", "column gap", "row gap"), got[2:],
+                          (want_cols, want_rows, want_col_gaps, want_row_gaps)):
+        bad = np.nonzero(g.astype(np.int64) != w)[0]
+        assert len(bad) == 0, f"{label}: {len(bad)} {name} entries differ, first at {bad[:5].tolist()}"
+    # the scores-only entry point writes the same score halves
+    _, _, cols, rows = ctx.score_packed_dp(b1, o1, b2, o2, dc, pair_method)
+    assert np.array_equal(cols, got[2]) and np.array_equal(rows, got[3])
     plain, _ = ctx.score_packed(b1, o1, b2, o2, dc, pair_method)
     assert np.array_equal(plain, got_s)
 
@@ -118,15 +126,20 @@ def test_configured_aligner_bulk_and_one_to_many_vectors():
     for firsts, results in ((s1, a.compute(s1, s2, [], [])), ([s1[0]] * len(s2), a.compute(s1[0], s2, [], []))):
         b1, o1 = workloads.pack(firsts)
         b2, o2 = workloads.pack(s2)
-        want_s, want_rows, want_cols = O.dp_vectors_packed(b1, o1, b2, o2, oracle_cfg(dc), threads=8)
+        want_s, want_rows, want_cols, want_rg, want_cg = O.dp_vectors_packed(b1, o1, b2, o2, oracle_cfg(dc),
+                                                                             threads=8, gaps=True)
         for p, r in enumerate(results):
             assert r.score() == want_s[p]
-            c0, r0 = int(o1[p]) + p, int(o2[p]) + p
-            assert r.dp_column() == want_cols[c0:c0 + len(firsts[p]) + 1].tolist(), p
-            assert r.dp_row() == want_rows[r0:r0 + len(s2[p]) + 1].tolist(), p
+            c = slice(int(o1[p]) + p, int(o1[p]) + p + len(firsts[p]) + 1)
+            w = slice(int(o2[p]) + p, int(o2[p]) + p + len(s2[p]) + 1)
+            assert r.dp_column() == list(zip(want_cols[c].tolist(), want_cg[c].tolist())), p
+            assert r.dp_row() == list(zip(want_rows[w].tolist(), want_rg[w].tolist())), p
+            assert [x.score() for x in r.dp_row()] == want_rows[w].tolist()
     loc = cfg.configure_aligner(cfg.method_local(cfg.gap_model_affine(cfg.score_model_unitary(cfg.initial, 4, -5),
                                                                       -10, -1)))
     one = loc.compute(s1[3], s2[3], [], [])
-    want_s, want_rows, want_cols = O.dp_vectors_packed(*workloads.pack([s1[3]]), *workloads.pack([s2[3]]),
-                                                       oracle_cfg(loc.config))
-    assert one.score() == want_s[0] and one.dp_row() == want_rows.tolist() and one.dp_column() == want_cols.tolist()
+    want_s, want_rows, want_cols, want_rg, want_cg = O.dp_vectors_packed(
+        *workloads.pack([s1[3]]), *workloads.pack([s2[3]]), oracle_cfg(loc.config), gaps=True)
+    assert one.score() == want_s[0]
+    assert one.dp_row() == list(zip(want_rows.tolist(), want_rg.tolist()))
+    assert one.dp_column() == list(zip(want_cols.tolist(), want_cg.tolist()))

@@ -101,8 +101,11 @@ def test_random_configuration(ctx, seed):
         got_q, _ = ctx.score_one_to_many(q, b2, o2, dc, pm)
         assert np.array_equal(got_q.astype(np.int64), want_q), f"case {seed}: one-to-many"
     if seed % 4 == 1:
-        want_s, want_rows, want_cols = O.dp_vectors_packed(b1, o1, b2, o2, oracle_cfg(dc), pair_method=pm, threads=8)
-        got_s, _, cols, rows = ctx.score_packed_dp(b1, o1, b2, o2, dc, pm)
+        want_s, want_rows, want_cols, want_rg, want_cg = O.dp_vectors_packed(b1, o1, b2, o2, oracle_cfg(dc),
+                                                                             pair_method=pm, threads=8, gaps=True)
+        got_s, _, cols, rows, cg, rg = ctx.score_packed_dp(b1, o1, b2, o2, dc, pm, gaps=True)
         assert np.array_equal(got_s.astype(np.int64), want_s), f"case {seed}: DP export scores"
         assert np.array_equal(cols.astype(np.int64), want_cols), f"case {seed}: DP export columns"
         assert np.array_equal(rows.astype(np.int64), want_rows), f"case {seed}: DP export rows"
+        assert np.array_equal(cg.astype(np.int64), want_cg), f"case {seed}: DP export column gap states"
+        assert np.array_equal(rg.astype(np.int64), want_rg), f"case {seed}: DP export row gap states"

@@ -344,6 +344,78 @@ def test_dp_vectors_local_by_enumeration():
         assert cols == [_cell_local(a[:i], b, cfg) for i in range(len(a) + 1)], (a, b, cfg)
 
 
+# The gap-state halves of the same cells.  Unrolling V(i+1, j) = max(V(i, j) + ge, H(i, j) + go + ge)
+# from V(1, j) = H(0, j) + go + ge gives the closed form
+#   V(m+1, j) = max over i = 0..m of H(i, j) + go + ge * (m + 1 - i)
+# (the best vertical gap into row m+1, over the row it opens below), and the same along a row for
+# W(i, n+1); entry 0 of each vector keeps its initial go + ge.  H(i, j) comes from enumeration, so
+# the gap vectors are pinned by a formula that does not restate the recurrence.
+
+def _gap_vectors_closed_form(a, b, cfg, cell):
+    go, ge = cfg.gap_open, cfg.gap_extension
+    m, n = len(a), len(b)
+    H = [[cell(a[:i], b[:j], cfg) for j in range(n + 1)] for i in range(m + 1)]
+    row_gaps = [go + ge] + [max(H[i][j] + go + ge * (m + 1 - i) for i in range(m + 1)) for j in range(1, n + 1)]
+    col_gaps = [go + ge] + [max(H[i][j] + go + ge * (n + 1 - j) for j in range(n + 1)) for i in range(1, m + 1)]
+    return row_gaps, col_gaps
+
+
+def _gap_vectors(a, b, cfg):
+    b1, o1 = O.pack([a])
+    b2, o2 = O.pack([b])
+    _, _, _, rg, cg = O.dp_vectors_packed(b1, o1, b2, o2, cfg, gaps=True)
+    return rg.tolist(), cg.tolist()
+
+
+@pytest.mark.parametrize("flags", FLAG_COMBOS, ids=lambda f: "".join("F" if x else "p" for x in f))
+def test_dp_gap_vectors_global_by_closed_form(flags):
+    rng = np.random.default_rng(2000 + (hash(flags) & 0xFFF))
+    fc, fr, lc, lr = flags
+    for trial in range(6):
+        cfg = O.OracleConfig(mode=O.GLOBAL, first_column_free=fc, first_row_free=fr, last_column_free=lc,
+                             last_row_free=lr, gap_open=int(rng.integers(-6, 1)), gap_extension=int(rng.integers(-3, 0)),
+                             match=int(rng.integers(1, 5)), mismatch=int(rng.integers(-5, 0)))
+        a = _random_seq(rng, "ACG", 0, 4)
+        b = _random_seq(rng, "ACG", 0, 4)
+        assert _gap_vectors(a, b, cfg) == _gap_vectors_closed_form(a, b, cfg, _prefix_global), (a, b, cfg)
+
+
+def test_dp_gap_vectors_local_by_closed_form():
+    rng = np.random.default_rng(79)
+    for trial in range(10):
+        cfg = O.OracleConfig(mode=O.LOCAL, gap_open=int(rng.integers(-6, 0)), gap_extension=int(rng.integers(-3, 0)),
+                             match=int(rng.integers(1, 5)), mismatch=int(rng.integers(-5, 0)))
+        a = _random_seq(rng, "ACGT", 0, 4)
+        b = _random_seq(rng, "ACGT", 0, 4)
+        assert _gap_vectors(a, b, cfg) == _gap_vectors_closed_form(a, b, cfg, _cell_local), (a, b, cfg)
+
+
+@pytest.mark.parametrize("bug", ["state_before_last_row", "extension_dropped", "entry0_updated"])
+def test_dp_gap_vectors_planted_mistakes_are_caught(bug):
+    """Plausible mistakes in the gap halves (exporting the state one row early, dropping the
+    extension, updating entry 0 like the others) must fail the closed form."""
+    rng = np.random.default_rng(80)
+    caught = False
+    for trial in range(60):
+        cfg = O.OracleConfig(mode=O.GLOBAL, first_column_free=bool(rng.integers(0, 2)),
+                             first_row_free=bool(rng.integers(0, 2)), gap_open=int(rng.integers(-6, 0)),
+                             gap_extension=int(rng.integers(-3, 0)), match=2, mismatch=-3)
+        a = _random_seq(rng, "ACG", 1, 4)
+        b = _random_seq(rng, "ACG", 1, 4)
+        rg, cg = _gap_vectors(a, b, cfg)
+        if bug == "state_before_last_row":
+            rg = _gap_vectors(a[:-1], b, cfg)[0]
+        elif bug == "extension_dropped":
+            rg = [v - cfg.gap_extension if j else v for j, v in enumerate(rg)]
+        else:
+            h0 = 0 if cfg.first_row_free else cfg.gap_open + cfg.gap_extension * len(b)
+            cg = [h0 + cfg.gap_open + cfg.gap_extension] + cg[1:]
+        if (rg, cg) != _gap_vectors_closed_form(a, b, cfg, _prefix_global):
+            caught = True
+            break
+    assert caught
+
+
 def test_dp_vectors_transpose_and_batch_layout():
     rng = np.random.default_rng(78)
     seqs1 = [_random_seq(rng, "ACGT", 0, 30) for _ in range(40)]
@@ -351,9 +423,15 @@ def test_dp_vectors_transpose_and_batch_layout():
     cfg = O.OracleConfig(mode=O.GLOBAL, gap_open=-5, gap_extension=-2, match=2, mismatch=-3)
     b1, o1 = O.pack(seqs1)
     b2, o2 = O.pack(seqs2)
-    s, rows, cols = O.dp_vectors_packed(b1, o1, b2, o2, cfg, threads=4)
-    st, rows_t, cols_t = O.dp_vectors_packed(b2, o2, b1, o1, cfg, threads=4)
+    s, rows, cols, rg, cg = O.dp_vectors_packed(b1, o1, b2, o2, cfg, threads=4, gaps=True)
+    st, rows_t, cols_t, rg_t, cg_t = O.dp_vectors_packed(b2, o2, b1, o1, cfg, threads=4, gaps=True)
     assert np.array_equal(s, st) and np.array_equal(rows, cols_t) and np.array_equal(cols, rows_t)
+    assert np.array_equal(rg, cg_t) and np.array_equal(cg, rg_t)
+    # the batch writes each pair's gap halves where the pair alone puts them
+    for p in (0, 7, 39):
+        one_rg, one_cg = _gap_vectors(seqs1[p], seqs2[p], cfg)
+        r0, c0 = int(o2[p]) + p, int(o1[p]) + p
+        assert rg[r0:r0 + len(seqs2[p]) + 1].tolist() == one_rg and cg[c0:c0 + len(seqs1[p]) + 1].tolist() == one_cg
     for p in range(40):
         r0 = int(o2[p]) + p
         c0 = int(o1[p]) + p
