@@ -429,7 +429,7 @@ def bench_mixed(count, steps, warmup, side_steps=5, only=None):
            "policies": {}}
     ref_scores = {}
     side = min(steps, side_steps)
-    # the last leg orders each launch's tasks by the network's predicted cost instead of the
+    # the last leg orders each launch's waves by the network's predicted cost instead of the
     # geometric model (an A/B of the cost output; the planner's default is the geometric order)
     legs = (("learned", device.LANES_LEARNED, steps, warmup, None), ("int32", device.LANES_INT32, side, 1, None),
             ("int16_speculative", device.LANES_INT16, side, 1, None), ("bound", device.LANES_AUTO, side, 1, None),

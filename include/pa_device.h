@@ -56,8 +56,9 @@ typedef struct pa_tuning {
     uint32_t wavefront_min_rows;  /* pairs with more DP rows use the wavefront kernel [3072] */
     uint32_t streams;             /* concurrent launch streams, 1..4 [4] */
     uint32_t eager;               /* 1: launch kernels one by one; 0: replay a captured hipGraph [0] */
-    uint32_t cost_order;          /* PA_LANES_LEARNED: 1 orders the tasks of a launch by the network's predicted
-                                     cost instead of the geometric model's [0] */
+    uint32_t cost_order;          /* PA_LANES_LEARNED: 1 orders the waves of a launch (runs of tasks formed in
+                                     the geometric order) by the network's predicted cost instead of the
+                                     geometric model's [0] */
     uint32_t persistent;          /* 1: striped launches run as many blocks as the GPU keeps resident, whose
                                      waves take tasks from a work queue; 0: one block per 16 tasks [0] */
     float wavefront_tail;         /* a pair beyond wavefront_min_rows (and within 8,192 rows) runs in the wavefront

@@ -806,16 +806,41 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
             const Pack &pk = group_packs[g][i];
             uint32_t S = it[pk.first].S, n = it[pk.first].n;
             if (pk.second >= 0) { S = std::max(S, it[pk.second].S); n = std::max(n, it[pk.second].n); }
-            if (pair_cost) {   // the predicted cost of the pack's costlier pair, in tenths of a ns
-                float c = pair_cost[it[pk.first].pair];
-                if (pk.second >= 0) c = std::max(c, pair_cost[it[pk.second].pair]);
-                weighed[g][i] = Weighed{uint64_t(std::max(c, 0.0f) * 10.0f), pk};
-            } else {
-                weighed[g][i] = Weighed{uint64_t(S) * (n + G - 1), pk};
-            }
+            weighed[g][i] = Weighed{uint64_t(S) * (n + G - 1), pk};
         }
     }, 1);
     parallel_sort(weighed, [](const Weighed &x, const Weighed &y) { return x.work > y.work; }, true);
+    if (pair_cost) {
+        // Predicted-cost order.  The lane groups of one wave sweep consecutive tasks in lockstep,
+        // so a wave lasts as long as its longest task: the geometric order above keeps the tasks
+        // of a wave alike, and the network's cost then orders whole waves (64 / G consecutive
+        // tasks, keyed by the predicted cost of their costliest pair; one task per workgroup in
+        // the wavefront kernel).  A last, partial wave stays last, so that the waves after it keep
+        // their alignment.
+        parallel_for(ng, [&](size_t g) {
+            const std::vector<Item> &it = items[g];
+            const size_t per_wave = keys[g].G >= 64 ? 1 : 64 / keys[g].G;
+            std::vector<Weighed> &w = weighed[g];
+            std::vector<std::pair<float, size_t>> waves;   // (key, first task)
+            const size_t whole = w.size() / per_wave * per_wave;
+            for (size_t lo = 0; lo < whole; lo += per_wave) {
+                float c = 0.0f;
+                for (size_t i = lo; i < std::min(w.size(), lo + per_wave); ++i) {
+                    c = std::max(c, pair_cost[it[w[i].pk.first].pair]);
+                    if (w[i].pk.second >= 0) c = std::max(c, pair_cost[it[w[i].pk.second].pair]);
+                }
+                waves.push_back({c, lo});
+            }
+            std::stable_sort(waves.begin(), waves.end(),
+                             [](const auto &x, const auto &y) { return x.first > y.first; });
+            std::vector<Weighed> out;
+            out.reserve(w.size());
+            for (const auto &wv : waves)
+                for (size_t i = wv.second; i < wv.second + per_wave; ++i) out.push_back(w[i]);
+            for (size_t i = whole; i < w.size(); ++i) out.push_back(w[i]);
+            w.swap(out);
+        }, 1);
+    }
     parallel_for(ng, [&](size_t g) {
         for (size_t i = 0; i < weighed[g].size(); ++i) group_packs[g][i] = weighed[g][i].pk;
     }, 1);

@@ -63,7 +63,7 @@ class Tuning:
     wavefront_min_rows: int = 0
     streams: int = 0
     eager: int = 0
-    cost_order: int = 0   # learned policy: order tasks by the predicted cost (1) or the geometric model (0)
+    cost_order: int = 0   # learned policy: order waves by the predicted cost (1) or the geometric model (0)
     persistent: int = 0   # striped launches: resident blocks taking tasks from a work queue (1)
     # long pairs go to the wavefront kernel only where their striped sweep would exceed this
     # multiple of the batch's striped work per resident lane group (< 0: every long pair)

@@ -359,8 +359,67 @@ static void check_wavefront_tail()
     std::printf("  wavefront tail rule: checked\n");
 }
 
+// Predicted-cost task order (pa_tuning.cost_order): the lane groups of a wave sweep 64 / G
+// consecutive tasks in lockstep, so the order may move whole waves only.  Every aligned run of
+// 64 / G tasks of a striped launch holds the same pairs as an aligned run of the geometric order,
+// and the runs come in non-increasing order of their costliest pair's predicted cost.
+static void check_cost_order()
+{
+    std::mt19937_64 rng(29);
+    Batch b = random_batch(rng, 30000, "ACGT", 600, false, true, false);
+    pa_config cfg{};
+    cfg.match = 2;
+    cfg.mismatch = -3;
+    cfg.gap_open = -5;
+    cfg.gap_extension = -2;
+    const BatchView v = b.view();
+    const size_t n = b.o2.size() - 1;
+    std::vector<float> cost(n);
+    for (float &c : cost) c = float(rng() % 100000);
+    Plan geo, by_cost;
+    std::string err;
+    CHECK(make_plan(cfg, v, nullptr, nullptr, geo, err) == PA_OK, "geometric plan: %s", err.c_str());
+    CHECK(make_plan(cfg, v, nullptr, nullptr, by_cost, err, false, false, cost.data()) == PA_OK, "cost plan: %s",
+          err.c_str());
+    CHECK(geo.launches.size() == by_cost.launches.size(), "launch counts %zu %zu", geo.launches.size(),
+          by_cost.launches.size());
+    size_t runs = 0;
+    for (size_t l = 0; l < std::min(geo.launches.size(), by_cost.launches.size()); ++l) {
+        const Launch &A = geo.launches[l], &B = by_cost.launches[l];
+        CHECK(A.task_begin == B.task_begin && A.task_end == B.task_end, "launch %zu ranges", l);
+        const uint32_t per = A.key.G >= 64 ? 1 : 64 / A.key.G;
+        auto run_pairs = [&](const Plan &p, uint32_t k0) {
+            std::vector<uint32_t> s;
+            for (uint32_t k = k0; k < std::min(k0 + per, A.task_end); ++k)
+                for (int h = 0; h < 2; ++h) s.push_back(p.tasks[k].pair[h]);
+            std::sort(s.begin(), s.end());
+            return s;
+        };
+        auto run_key = [&](uint32_t k0) {
+            float c = 0.0f;
+            for (uint32_t k = k0; k < std::min(k0 + per, A.task_end); ++k)
+                for (int h = 0; h < 2; ++h)
+                    if (by_cost.tasks[k].pair[h] != kNone) c = std::max(c, cost[by_cost.tasks[k].pair[h]]);
+            return c;
+        };
+        std::set<std::vector<uint32_t>> geo_runs;
+        for (uint32_t k0 = A.task_begin; k0 < A.task_end; k0 += per) geo_runs.insert(run_pairs(geo, k0));
+        float prev = INFINITY;
+        for (uint32_t k0 = B.task_begin; k0 < B.task_end; k0 += per, ++runs) {
+            CHECK(geo_runs.count(run_pairs(by_cost, k0)) == 1, "launch %zu: run at task %u is not a geometric run", l,
+                  k0);
+            if (k0 + per > B.task_end) break;   // a partial last run stays last
+            const float c = run_key(k0);
+            CHECK(c <= prev, "launch %zu: run at task %u costs %g after %g", l, k0, double(c), double(prev));
+            prev = c;
+        }
+    }
+    std::printf("  cost order: %zu runs of whole waves checked\n", runs);
+}
+
 int main()
 {
+    check_cost_order();
     check_recentring();
     check_wavefront_tail();
     check_large_gap_extension();

@@ -138,7 +138,8 @@ def test_gpu_minimizer_containment_matches_numpy():
 def test_learned_dispatch_gains_over_bound_policy():
     """Train on DP-generated labels, then score held-out protein pairs whose length bound does
     not prove int16 safety: the learned policy runs more of them in int16 than the bound policy,
-    and every policy returns the same scores (all equal to the oracle on a sample)."""
+    and every policy -- the learned one also with its tasks ordered by the predicted cost --
+    returns the same scores (all equal to the oracle on a sample)."""
     ctx = device.default_context(0)
     t = tables.blosum62_standard()
     prot = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
@@ -153,16 +154,19 @@ def test_learned_dispatch_gains_over_bound_policy():
 
     b1, o1, b2, o2 = learned.training_pairs("protein", 202, 600, 3000, 5000)   # held out
     results = {}
-    for name, policy, mdl in (("int32", device.LANES_INT32, None), ("auto", device.LANES_AUTO, None),
-                              ("learned", device.LANES_LEARNED, model)):
-        dc = device.DeviceConfig(**{**prot.__dict__, "lane_policy": policy, "model": mdl})
+    for name, policy, mdl, tuning in (("int32", device.LANES_INT32, None, None), ("auto", device.LANES_AUTO, None, None),
+                                      ("learned", device.LANES_LEARNED, model, None),
+                                      ("learned_cost_order", device.LANES_LEARNED, model, device.Tuning(cost_order=1))):
+        dc = device.DeviceConfig(**{**prot.__dict__, "lane_policy": policy, "model": mdl, "tuning": tuning})
         bt = device.Batch(ctx, b1, o1, b2, o2, dc)
         bt.run()
         results[name] = bt.results() + (bt.stats(),)
     s32, _, st32 = results["int32"]
     sa, _, sta = results["auto"]
     sl, fl, stl = results["learned"]
-    assert np.array_equal(s32, sa) and np.array_equal(s32, sl)
+    sc, _, stc = results["learned_cost_order"]
+    assert np.array_equal(s32, sa) and np.array_equal(s32, sl) and np.array_equal(s32, sc)
+    assert stc["n_int16"] == stl["n_int16"]   # the cost output orders tasks; it does not pick widths
     assert stl["n_int16"] > sta["n_int16"], (stl, sta)
     want = O.score_packed(b1, o1[:41], b2, o2[:41], O.OracleConfig(mode=O.LOCAL, score_kind=O.MATRIX,
                           symbols=prot.symbols, matrix=prot.matrix, gap_open=-11, gap_extension=-1), threads=8)
