@@ -105,12 +105,14 @@ struct pa_batch {
     hipGraph_t graph = nullptr;            // captured main pass (see launch_main)
     hipGraphExec_t graph_exec = nullptr;
     pa_batch_stats stats{};
-    double main_ms_accum = 0;
-    uint64_t main_ms_count = 0;
+    // pa_batch_time: a pair of events around the main launches of every timed pass, read after
+    // the last pass, so that timed passes follow each other on the stream without a host wait
+    std::vector<hipEvent_t> pass_ev;
     ~pa_batch()
     {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t ev : pass_ev) (void)hipEventDestroy(ev);
     }
 };
 
@@ -340,19 +342,14 @@ pa_status launch_main(pa_ctx *ctx, pa_batch *b)
     return PA_OK;
 }
 
-pa_status run_pass(pa_ctx *ctx, pa_batch *b, bool timed)
+// One pass: the main launches, then, for pairs speculated in int16, the saturation read-back and
+// the int32 recomputation.  ev_main (nullable): two events recorded around the main launches.
+pa_status run_pass(pa_ctx *ctx, pa_batch *b, const hipEvent_t *ev_main = nullptr)
 {
     pa_status st;
-    if (timed) PA_HIP(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
+    if (ev_main) PA_HIP(hipEventRecord(ev_main[0], ctx->stream), "hipEventRecord");
     if ((st = launch_main(ctx, b)) != PA_OK) return st;
-    if (timed) {
-        PA_HIP(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
-        PA_HIP(hipEventSynchronize(ctx->ev[3]), "hipEventSynchronize");
-        float ms = 0;
-        PA_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
-        b->main_ms_accum += ms;
-        b->main_ms_count += 1;
-    }
+    if (ev_main) PA_HIP(hipEventRecord(ev_main[1], ctx->stream), "hipEventRecord");
     b->stats.n_recomputed = 0;
     if (!b->can_saturate) return PA_OK;
 
@@ -651,7 +648,7 @@ pa_status pa_batch_run(pa_ctx *ctx, pa_batch *b)
 {
     if (!ctx || !b) return PA_ERR_INVALID_ARGUMENT;
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PA_ERR_DEVICE, "hipSetDevice");
-    pa_status st = run_pass(ctx, b, false);
+    pa_status st = run_pass(ctx, b);
     if (st != PA_OK) return st;
     PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return check_error_slot(ctx, b);
@@ -686,12 +683,19 @@ pa_status pa_batch_time(pa_ctx *ctx, pa_batch *b, int iters, double *ms_total, d
 {
     if (!ctx || !b || iters < 1) return fail(ctx, PA_ERR_INVALID_ARGUMENT, "invalid argument");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, PA_ERR_DEVICE, "hipSetDevice");
-    b->main_ms_accum = 0;
-    b->main_ms_count = 0;
+    if (ms_main) {
+        while (b->pass_ev.size() < 2 * size_t(iters)) {
+            hipEvent_t ev;
+            PA_HIP(hipEventCreate(&ev), "hipEventCreate");
+            b->pass_ev.push_back(ev);
+        }
+    }
     PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     PA_HIP(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
+    // the passes follow each other on the stream (the host waits only where a pass reads its
+    // saturation flags back); the main launches' events are read once all passes have ended
     for (int i = 0; i < iters; ++i) {
-        pa_status st = run_pass(ctx, b, ms_main != nullptr);
+        pa_status st = run_pass(ctx, b, ms_main ? &b->pass_ev[2 * size_t(i)] : nullptr);
         if (st != PA_OK) return st;
     }
     PA_HIP(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
@@ -699,7 +703,16 @@ pa_status pa_batch_time(pa_ctx *ctx, pa_batch *b, int iters, double *ms_total, d
     float ms = 0;
     PA_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
     if (ms_total) *ms_total = ms;
-    if (ms_main) *ms_main = b->main_ms_count ? b->main_ms_accum / double(b->main_ms_count) : 0.0;
+    if (ms_main) {
+        double sum = 0;
+        for (int i = 0; i < iters; ++i) {
+            float m = 0;
+            PA_HIP(hipEventElapsedTime(&m, b->pass_ev[2 * size_t(i)], b->pass_ev[2 * size_t(i) + 1]),
+                   "hipEventElapsedTime");
+            sum += m;
+        }
+        *ms_main = sum / iters;
+    }
     return check_error_slot(ctx, b);
 }
 
@@ -1081,7 +1094,7 @@ pa_status pa_score_batch_noalloc(pa_ctx *ctx, const pa_config *cfg, const uint8_
     pa_status st = create_batch(ctx, &c, seq1, off1, false, seq2, off2, n_pairs, pair_method, &b, false, &arena);
     if (st != PA_OK) return too_big(st);
     std::unique_ptr<pa_batch> hold(b);
-    if ((st = run_pass(ctx, b, false)) != PA_OK) return too_big(st);
+    if ((st = run_pass(ctx, b)) != PA_OK) return too_big(st);
     PA_HIP(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if ((st = check_error_slot(ctx, b)) != PA_OK) return st;
     return pa_batch_results(ctx, b, scores, flags);
