@@ -358,15 +358,22 @@ def bench_config(name, steps, warmup, pairs=None, with_cpu=True, e2e_reps=0, fil
                dtype=dtype_of([r["stats"]]), roofline=rl, plan=plan_summary(r["stats"]), inputs_sha256=digest)
     if tuned:
         out["autotuned"] = tuned
-        # the launch form the tuner did not pick, on the same pairs: one block per 16 tasks against
-        # resident blocks taking tasks from a work queue
-        other = dataclasses.replace(dc.tuning, persistent=0 if dc.tuning.persistent else 1)
-        r2 = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dataclasses.replace(dc, tuning=other),
-                     w.get("pair_method"), steps, warmup)
-        key = lambda p: "work_queue" if p else "one_block_per_16_tasks"
-        out["launch_form_ab"] = {key(dc.tuning.persistent): round(r["ms_per_step"], 3),
-                                 key(other.persistent): round(r2["ms_per_step"], 3),
-                                 "scores_identical": bool(np.array_equal(r2["scores"], r["scores"]))}
+        # every launch form (pa_tuning.persistent) with the tuner's other parameters, on the same
+        # pairs: ms per pass
+        names = ["default_pair_table_fused", "work_queue_per_launch", "work_queue_local_fused", "one_block_per_16_tasks"]
+        ab, same = {}, True
+        for form, form_name in enumerate(names):
+            if form == dc.tuning.persistent:
+                ab[form_name] = round(r["ms_per_step"], 3)
+                continue
+            r2 = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"],
+                         dataclasses.replace(dc, tuning=dataclasses.replace(dc.tuning, persistent=form)),
+                         w.get("pair_method"), steps, warmup)
+            ab[form_name] = round(r2["ms_per_step"], 3)
+            same = same and bool(np.array_equal(r2["scores"], r["scores"]))
+        ab["tuned"] = names[dc.tuning.persistent]
+        ab["scores_identical"] = same
+        out["launch_form_ab"] = ab
         log(f"[bench] {name}: launch forms {out['launch_form_ab']}")
     if one_to_one is not None:
         one_to_one["speedup_of_profile"] = round(one_to_one["ms_per_step"] / r["ms_per_step"], 3)

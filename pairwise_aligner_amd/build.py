@@ -18,8 +18,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libpa_device.so")
-SOURCES = ["pa_kernels.hip", "pa_encode.hip", "pa_learned.hip", "pa_noalloc.hip", "pa_capi.cpp", "pa_plan.cpp",
-           "pa_fastx.cpp"]
+SOURCES = ["pa_kernels.hip", "pa_fused.hip", "pa_encode.hip", "pa_learned.hip", "pa_noalloc.hip", "pa_capi.cpp",
+           "pa_plan.cpp", "pa_fastx.cpp"]
 # (lane width, rows per lane) instantiations of the DP kernels
 INSTANCES = [(w, r) for w in (16, 32) for r in (4, 6, 8, 10, 12, 14, 16)]
 

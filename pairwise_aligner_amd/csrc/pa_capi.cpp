@@ -62,6 +62,7 @@ struct PlanDevice {
     pa::Plan plan;
     DeviceBuffer tasks, rowcodes, colcodes, bnd, subset, enc, codes, dp_base;
     DeviceBuffer queues;   // persistent launches: two counters per launch, zero between passes
+    DeviceBuffer fused_items;   // wave items of the fused launches (pa::Plan::fused_items)
     std::vector<uint32_t> queues_host;   // their initial zeros (kept until the upload has read them)
     std::vector<pa::CodeTables> code_host;   // per launch (kept until the encoder has read them)
     DeviceBuffer table[pa::kScoreShifts], ptable[pa::kScoreShifts], profile[pa::kScoreShifts];
@@ -215,6 +216,8 @@ pa_status upload_plan(pa_ctx *ctx, PlanDevice &pd, const DeviceBuffer &residues,
     PA_HIP(dev_alloc(pd.bnd, pd.plan.bnd_entries * 8, arena), "device allocation");
     pd.queues_host.assign(2 * p.launches.size(), 0u);
     if ((st = upload(ctx, pd.queues, pd.queues_host, arena)) != PA_OK) return st;
+    if (!p.fused_items.empty())
+        if ((st = upload(ctx, pd.fused_items, p.fused_items, arena)) != PA_OK) return st;
     if (!pd.subset_host.empty())
         if ((st = upload(ctx, pd.subset, pd.subset_host, arena)) != PA_OK) return st;
     return PA_OK;
@@ -250,7 +253,7 @@ pa::KernelArgs kernel_args(const PlanDevice &pd, pa_batch *b, const pa::Launch &
     a.tbl_lo = p.tbl_lo[v];
     a.tbl_hi = p.tbl_hi[v];
     a.fc = p.fc; a.fr = p.fr; a.lc = p.lc; a.lr = p.lr;
-    if (p.tuning.persistent && L.key.G == 16) {
+    if ((p.tuning.persistent == 1 || p.tuning.persistent == 2) && L.key.G == 16) {
         const size_t li = size_t(&L - p.launches.data());
         a.queue = static_cast<uint32_t *>(pd.queues.ptr) + 2 * li;
     }
@@ -282,6 +285,21 @@ pa_status launch_plan(pa_ctx *ctx, pa_batch *b, PlanDevice &pd, bool self_init)
         a.self_init = self_init ? 1u : 0u;
         const uint32_t lds = pa::lds_bytes(p, L.key);
         hipStream_t st = fork ? ctx->side[p.launch_stream[li]] : ctx->stream;
+        const int32_t f = li < p.fused_of.size() ? p.fused_of[li] : -1;
+        if (f >= 0) {
+            // a fused launch: its leader launches the wave items of every member (the members'
+            // keys differ only in R: same tables, same LDS); the other members are not launched
+            const pa::FusedLaunch &F = p.fused[size_t(f)];
+            if (F.leader != li) continue;
+            a.tasks = static_cast<const pa::Task *>(pd.tasks.ptr);
+            a.ntasks = 0;
+            a.items = static_cast<const uint32_t *>(pd.fused_items.ptr) + F.items_begin;
+            a.nitems = F.items_end - F.items_begin;
+            a.queue = static_cast<uint32_t *>(pd.queues.ptr) + 2 * size_t(li);
+            hipError_t e = pa::launch_fused(L.key, a, lds, st);
+            if (e != hipSuccess) return hip_fail(ctx, e, "fused striped DP launch");
+            continue;
+        }
         hipError_t e = pa::launch_stripe(L.key, a, lds, st);
         if (e != hipSuccess) return hip_fail(ctx, e, "striped DP launch");
     }
@@ -618,7 +636,10 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
     b->stats.padded_cells = p.padded_cells;
     b->stats.n_int16 = p.n16;
     b->stats.n_int32 = p.n32;
-    b->stats.n_launches = p.launches.size() + (p.n_empty > 0 ? 1 : 0);
+    size_t launched = 0;
+    for (size_t l = 0; l < p.launches.size(); ++l)
+        launched += l >= p.fused_of.size() || p.fused_of[l] < 0 || p.fused[size_t(p.fused_of[l])].leader == l;
+    b->stats.n_launches = launched + (p.n_empty > 0 ? 1 : 0);
     b->stats.n_profiled = p.n_profiled;
     b->stats.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = b.release();
@@ -969,7 +990,7 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
         c.streams = t.streams;
         c.eager = t.eager ? 1 : 0;
         c.cost_order = t.cost_order ? 1 : 0;
-        c.persistent = t.persistent ? 1 : 0;
+        c.persistent = t.persistent;
         c.wavefront_tail = t.wavefront_tail > 0 ? float(t.wavefront_tail) : -1.0f;
         return c;
     };
@@ -1027,10 +1048,10 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
     const uint32_t wf[] = {2048, 3072, 4096};
     const uint32_t streams[] = {2, 4};
     const uint32_t eager[] = {0, 1};
-    const uint32_t persistent[] = {0, 1};
+    const uint32_t persistent[] = {0, 1, 2, 3};
     const float tails[] = {-1.0f, 0.5f, 1.0f};
     for (int param = 0; param < 5; ++param) {
-        const int nv = param == 0 || param == 4 ? 3 : 2;
+        const int nv = param == 3 ? 4 : (param == 0 || param == 4 ? 3 : 2);
         for (int v = 0; v < nv; ++v) {
             pa_tuning t = full;
             if (param == 0) t.wavefront_min_rows = wf[v];

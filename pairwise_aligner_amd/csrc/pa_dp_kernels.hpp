@@ -96,21 +96,336 @@ __device__ __forceinline__ void init_task_pairs(const KernelArgs &a, const Task 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 }
 
+// One task of the striped kernel, swept by the lane group of G lanes that holds it (t = the lane's
+// index in its group): one stripe after the other, the boundary row between stripes through HBM.
+// The launch's substitution table is in LDS (lds_raw) for the score kinds that read one.
 // EXPORT: also write the task's final DP column and row (int32 lanes; pa_plan.hpp, dp_base).
 // A task of one stripe (m <= G*R; the group's choice, uniform over its 16 lanes) takes a sweep
 // without boundary rows through HBM, whose lane 0 keeps its top boundary (the initial DP row) in
 // two registers instead of conveyors.  Tasks are ordered by decreasing work, so the one- and
 // multi-stripe tasks of a launch share at most one wave.
-// With a.queue the grid holds only as many blocks as the GPU keeps resident, and every wave takes
-// the next 64 / G tasks of the launch from an atomic counter until none are left (the tasks are in
-// decreasing order of work); the last block to finish resets the counter for the next pass.
-template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false>
-__global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
+template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT>
+__device__ __forceinline__ void stripe_task(const KernelArgs &a, const uint32_t gid, const char *lds_raw)
 {
     using L = Lane<W>;
     using V = typename L::V;
-    extern __shared__ char lds_raw[];
+    const int t = int(threadIdx.x % G);
+    const uint32_t lane_bank = (threadIdx.x & 31u) * 4u;   // this lane's copy of the LDS table
+    const Task tk = a.tasks[gid];
+    const int go = a.go, ge = a.ge;
+    // method_local.hpp:41-49: a local alignment has every end gap free
+    const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
+    if (a.self_init && t == 0) init_task_pairs<W, LOCAL>(a, tk, fc, fr, lc, LOCAL ? 0 : a.lr);
+    constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
+    constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
+    // column drift: on top of the row drift for the global method, alone for the local method
+    // (whose zero clamp and running maximum then work in the step's frame)
+    constexpr bool CDRIFT = !ACC && !EXPORT;
+    // int16 lanes with the column drift take three-way maxima in one instruction and hold
+    // their values biased into its range (pa_plan.hpp, kInt16Top): the local method by one
+    // bias for the launch, the global method by one per half, from the task
+    constexpr bool MAX3 = W == 16 && CDRIFT;
+    const int lbias = MAX3 && LOCAL ? a.lbias : 0;
+    const int b0 = MAX3 && !LOCAL ? int(tk.bias & 0xFFFFu) : lbias;
+    const int b1 = MAX3 && !LOCAL ? int(tk.bias >> 16) : lbias;
+    const V vbias = L::pair(b0, b1);
 
+    const int dge = DRIFT ? ge : 0;
+    const int cg = CDRIFT ? -ge : 0;         // column drift per step
+    const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
+    // shifts the last row's drift back when H, V leave the lane (and, with the column drift,
+    // forward by one step: the next lane uses them one step later)
+    const V vout = L::splat((DRIFT ? ge * R : 0) + (CDRIFT ? -ge : 0));
+    const V vrebase = L::splat(ge * G);      // end of a chunk: the column drift drops by G|ge|
+    uint32_t tbl_lo = a.tbl_lo;
+    const uint32_t tbl_hi = a.tbl_hi;
+    // the permute's second table word must be a VGPR (one scalar operand per instruction): keep
+    // it in one for the whole kernel instead of moving it from its scalar register every step
+    if constexpr (SCORE == kScoreDna) asm volatile("" : "+v"(tbl_lo));
+    const int N = int(tk.N);
+    constexpr int ROWS = G * R;
+    static_assert(!EXPORT || W == 32, "the DP export runs in int32 lanes");
+    int32_t *dp_col = nullptr, *dp_row = nullptr;   // H(0, n) and H(m, 0) of this task's vectors
+    if constexpr (EXPORT) {
+        dp_col = a.dp_out + a.dp_base[2 * gid];
+        dp_row = a.dp_out + a.dp_base[2 * gid + 1];
+        if (t == 0) {
+            dp_col[0] = h0_value(int(tk.n[0]), fr, go, ge);
+            dp_row[0] = h0_value(int(tk.m[0]), fc, go, ge);
+            // entry 0 of each vector keeps its initial gap state, go + ge: the sweep never
+            // updates it (dp_algorithm_template_base.hpp:94-128 reads entries from 1 on)
+            dp_col[a.dp_gap] = go + ge;
+            dp_row[a.dp_gap] = go + ge;
+        }
+    }
+
+    // exact; for the local method with the column drift, 0 in the frame of step -1 (it moves to
+    // the frame of each step as the step begins and back by G steps at each chunk end)
+    V runmax = LOCAL ? L::splat((CDRIFT ? -cg : 0) + lbias) : L::lowest();
+    V satmax = L::lowest();                         // drifted (int16 saturation test)
+    const V vcg = L::splat(cg);
+
+    // one stripe of the task; ONE: the task has no other
+    auto stripe = [&]<bool ONE>(uint32_t s) {
+        const int row0 = int(s) * ROWS + t * R;   // DP row above this lane's block
+        Rows<W, R, ACC> st;
+        // lane t starts at step t: its initial column (j = 0) counts as computed at step t-1
+        init_rows<W, R, SCORE, ACC, DRIFT, CDRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank,
+                                                   cg * (t - 1), vbias);
+        V Hdiag = L::add(L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1)), vbias);
+        Capture<W, R> cap;
+        // the step at which this lane's next capture event comes (events are rare: one wave
+        // test per step against it)
+        int kev = INT32_MAX;
+        if constexpr (!LOCAL) {
+            cap.init(tk, row0, lc);
+            kev = cap.ev == INT32_MAX ? INT32_MAX : cap.ev + t;
+        }
+
+        const bool has_bnd_in = !ONE && s > 0;
+        const bool has_bnd_out = !ONE && s + 1 < tk.S;
+        const uint32_t *colp = a.colcodes + tk.col_off;
+        uint32_t *bndp = has_bnd_in || has_bnd_out ? a.bnd + 2ull * tk.bnd_off : nullptr;
+
+        // Column-code conveyor: chunk j holds the residues of columns jG+1 .. jG+G, one per
+        // lane, so at the start of step k lane 0 of `conv` holds the residue of column k+1;
+        // `nextB` is the chunk after it.  The residue of step k+1 is thus known one step early,
+        // and its substitution scores are read while the cells of step k are computed.
+        uint32_t Bin = colp[t];   // step 0: lane 0 computes column 0
+        uint32_t conv = colp[1 + t];
+        uint32_t nextB = colp[G + 1 + t];
+        // Top-boundary conveyor: chunk j holds H and the vertical gap state above this stripe
+        // for columns jG .. jG+G-1: the initial row H(0, c+1) for the first stripe, the stored
+        // bottom row of the stripe above otherwise.  Columns past N (computed, never read, by
+        // the unmasked steps) get a low value that no real cell can be raised by.
+        // Lane t of a chunk is consumed at the chunk's step t: column drift cg*t.
+        auto top_chunk = [&](int col, uint32_t &h, uint32_t &e) {
+            if (has_bnd_in) {
+                if (col < N) {
+                    h = bndp[2 * col];
+                    e = bndp[2 * col + 1];
+                    if constexpr (CDRIFT) {
+                        h = L::bits(L::add(L::from(h), L::splat(cg * t)));
+                        e = L::bits(L::add(L::from(e), L::splat(cg * t)));
+                    }
+                } else {
+                    h = e = kLowWord;
+                }
+            } else {
+                // H(0, col+1) exact, the vertical gap state below it (drifted by |ge| for row 0)
+                const int hv = h0_value(col + 1, fr, go, ge);
+                h = L::bits(L::add(L::from(L::top_word(hv + cg * t)), vbias));
+                e = L::bits(L::add(L::from(L::top_word(hv + go + ge - dge + cg * t)), vbias));
+            }
+        };
+        uint32_t convH = 0, convE = 0, nextH = 0, nextE = 0;
+        // ONE: lane 0's top boundary at the current step, H(0, c+1) and the vertical gap state
+        // below it; it changes by top_step per step (the free first row's column drift, or the
+        // penalised first row's extension without the column drift) and by the chunk rebase
+        uint32_t topH = 0, topE = 0;
+        const int top_step = CDRIFT ? (fr ? cg : 0) : (fr ? 0 : ge);
+        const V vtop_step = L::splat(top_step);
+        if constexpr (ONE) {
+            top_chunk(0, topH, topE);
+        } else {
+            top_chunk(t, convH, convE);
+            top_chunk(G + t, nextH, nextE);
+        }
+        // outputs of this lane from the previous step (read by lane t+1)
+        uint32_t Hout = 0, Eout = 0;
+        // bottom-row outputs of lane G-1 over one chunk of G steps, shifted down one lane per
+        // step: at the end of the chunk lane j holds the output of the chunk's step j
+        uint32_t qH = 0, qE = 0;
+
+        ScoreReads<W, R, SCORE> rd;
+        issue_scores<W, R, SCORE>(st, Bin, lds_raw, rd);
+        V sc[R];
+        finish_scores<W, R, SCORE>(st, Bin, tbl_lo, tbl_hi, rd, sc);
+
+        // One step.  MASKED: lanes before their first column (k - t < 0; the first chunk,
+        // where lane t waits t steps) leave their state alone, and so do lanes past column N-1
+        // in the ROWMAX/TRACK/EXPORT variants, whose per-row maxima and exported rows must not
+        // see pad columns.  Elsewhere lanes past column N-1 sweep pad columns, as in every
+        // unmasked step: pad cells never exceed the real ones and never feed them, and a lane's
+        // state never falls behind the column drift (a state left alone while the chunk ends
+        // drop the drift would sink by G|ge| per chunk, for the max3 variants into patterns
+        // their three-way maximum does not order).
+        // TOPSTEP (ONE only): lane 0's top boundary changes from step to step (top_step != 0)
+        auto step = [&]<bool MASKED, bool TOPSTEP>(int k) {
+            const int c = k - t;   // DP column (0-based) of this lane at this step
+            // the conveyor moves on first, so that the old register is free for lane 0's value
+            const uint32_t shifted = dpp_any<kRowShl1>(conv);
+            const uint32_t Bnext = dpp<kRowShr1>(conv, Bin);
+            conv = shifted;
+            issue_scores<W, R, SCORE>(st, Bnext, lds_raw, rd);
+            uint32_t Hin, Ein;
+            if constexpr (ONE) {
+                // lanes 1.. take lane t-1's outputs, lane 0 keeps its top boundary
+                topH = dpp<kRowShr1>(topH, Hout);
+                topE = dpp<kRowShr1>(topE, Eout);
+                Hin = topH;
+                Ein = topE;
+                if constexpr (TOPSTEP) {
+                    topH = L::bits(L::add(L::from(topH), vtop_step));
+                    topE = L::bits(L::add(L::from(topE), vtop_step));
+                }
+            } else {
+                const uint32_t nextH_ = dpp_any<kRowShl1>(convH), nextE_ = dpp_any<kRowShl1>(convE);
+                Hin = dpp<kRowShr1>(convH, Hout);
+                Ein = dpp<kRowShr1>(convE, Eout);
+                convH = nextH_;
+                convE = nextE_;
+            }
+            if constexpr (LOCAL && CDRIFT) runmax = L::add(runmax, vcg);   // into this step's frame
+            if (!MASKED || (ROWMAX || TRACK || EXPORT ? uint32_t(c) < uint32_t(N) : c >= 0)) {
+                V Eup = L::from(Ein);
+                // EXPORT: the vertical gap state below the last row, from the lane that holds it
+                V vlast{};
+                const int er = EXPORT ? int(tk.m[0]) - 1 - row0 : -1;
+                update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT, MAX3, EXPORT>(
+                    st, Eup, Hdiag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo, vge,
+                    L::splat(cg * (k % G) + lbias), &vlast, er);
+                Hdiag = L::from(Hin);
+                Hout = L::bits(DRIFT || CDRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
+                Eout = L::bits(DRIFT || CDRIFT ? L::add(Eup, vout) : Eup);
+                if constexpr (EXPORT) {
+                    // the last row: H(m, c+1) from the lane that holds row m; the last column:
+                    // H(i, n) of every row of this lane at column n
+                    // Gap halves (affine_dp_algorithm.hpp:139-156): below row m, V(m+1, c+1), and
+                    // right of column n, W(i, n+1), the gap states after the last update.  With
+                    // the row drift, row r of a lane holds H and W less ge*(r+1), and the vertical
+                    // state below it less ge*(r+2).
+                    const int m = int(tk.m[0]), n = int(tk.n[0]);
+                    if (er >= 0 && er < R && c < n) {
+                        int v = 0;
+#pragma unroll
+                        for (int r = 0; r < R; ++r) if (r == er) v = L::half(st.H[r], 0);
+                        dp_row[c + 1] = v + dge * (er + 1);
+                        dp_row[a.dp_gap + c + 1] = L::half(vlast, 0) + dge * (er + 2);
+                    }
+                    if (c == n - 1) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            if (row0 + r + 1 <= m) {
+                                dp_col[row0 + r + 1] = L::half(st.H[r], 0) + dge * (r + 1);
+                                dp_col[a.dp_gap + row0 + r + 1] = L::half(st.F[r], 0) + dge * (r + 1);
+                            }
+                        }
+                    }
+                }
+            }
+            if constexpr (!LOCAL) {
+                // capture events are rare: test the whole wave first
+                if (__builtin_expect(__builtin_amdgcn_ballot_w64(k == kev) != 0, 0)) {
+                    cap.at_column(c, st, tk, a.scores, dge, -cg * (k % G), -b0, -b1);
+                    kev = cap.ev == INT32_MAX ? INT32_MAX : cap.ev + t;
+                }
+            }
+            if constexpr (!ONE) {
+                qH = dpp<kRowShl1>(Hout, qH);
+                qE = dpp<kRowShl1>(Eout, qE);
+            }
+            finish_scores<W, R, SCORE>(st, Bnext, tbl_lo, tbl_hi, rd, sc);
+            Bin = Bnext;
+        };
+        auto chunk_end = [&](int k0) {
+            conv = nextB;
+            nextB = colp[k0 + 2 * G + 1 + t];
+            if constexpr (!ONE) {
+                convH = nextH;
+                convE = nextE;
+                top_chunk(k0 + 2 * G + t, nextH, nextE);
+            }
+            if (has_bnd_out) {
+                // lane G-1 computed column k0 + j - (G-1) at the chunk's step j; stored without
+                // the column drift it was sent with (cg*(j+1))
+                const int col = k0 + t - (G - 1);
+                if (col >= 0 && col < N) {
+                    uint32_t h = qH, e = qE;
+                    if constexpr (CDRIFT) {
+                        h = L::bits(L::add(L::from(h), L::splat(-cg * (t + 1))));
+                        e = L::bits(L::add(L::from(e), L::splat(-cg * (t + 1))));
+                    }
+                    bndp[2 * col] = h;
+                    bndp[2 * col + 1] = e;
+                }
+            }
+            if constexpr (CDRIFT) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    st.H[r] = L::add(st.H[r], vrebase);
+                    st.F[r] = L::add(st.F[r], vrebase);
+                }
+                Hdiag = L::add(Hdiag, vrebase);
+                Hout = L::bits(L::add(L::from(Hout), vrebase));
+                Eout = L::bits(L::add(L::from(Eout), vrebase));
+                if constexpr (LOCAL) runmax = L::add(runmax, vrebase);
+                if constexpr (ONE) {
+                    topH = L::bits(L::add(L::from(topH), vrebase));
+                    topE = L::bits(L::add(L::from(topE), vrebase));
+                }
+            }
+        };
+
+        // whole chunks of G steps; the first is masked (lanes start one step apart)
+        const int nsteps = ((N + 2 * G - 2) / G) * G;
+        constexpr bool kMaskAll = ROWMAX || TRACK || EXPORT;
+        auto sweep = [&]<bool TOPSTEP>() {
+#pragma unroll 1
+            for (int u = 0; u < G; ++u) step.template operator()<true, TOPSTEP>(u);
+            chunk_end(0);
+            for (int k0 = G; k0 < nsteps; k0 += G) {
+                // two steps per iteration: the register allocator then alternates the exchanged
+                // values between two sets of registers instead of copying them back every step
+#pragma unroll 2
+                for (int u = 0; u < G; ++u) step.template operator()<kMaskAll, TOPSTEP>(k0 + u);
+                chunk_end(k0);
+            }
+        };
+        if (ONE && top_step != 0) sweep.template operator()<ONE>();
+        else sweep.template operator()<false>();
+        // free last row only: the int16 saturation tracking (TRACK) keeps per-row maxima too, but
+        // they are not candidates for the score
+        if constexpr (ROWMAX) cap.row_maximum(st, tk, a.scores, dge);
+        fold_rows<W, R, ACC, DRIFT>(st, runmax, satmax, ge);
+        if (has_bnd_out) {
+            // the boundary row written by lane G-1 is read by lane 0 of the next stripe
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+    };
+    if (!EXPORT && tk.S == 1) {
+        stripe.template operator()<!EXPORT>(0);
+    } else {
+        for (uint32_t s = 0; s < tk.S; ++s) stripe.template operator()<false>(s);
+    }
+
+    if constexpr (LOCAL || TRACK) {
+        const V mv = row_reduce_max<W>(runmax);
+        const V sv = row_reduce_max<W>(LOCAL ? runmax : satmax);
+        if (t == 0) {
+#pragma unroll
+            for (int q = 0; q < L::NH; ++q) {
+                if (tk.pair[q] == kNone) continue;
+                // the local maximum is in the frame of step -1 with the column drift
+                const int v = L::half(mv, q) + (LOCAL && CDRIFT ? cg : 0) - lbias;
+                if constexpr (LOCAL) a.scores[tk.pair[q]] = v;
+                // int16 overflow: a value clamped at kMax, or (max3 variants) above kInt16Top,
+                // whose NaN pattern the running maximum keeps at 0x7E00 or more; in the frame
+                // of step u, so with the local column drift at least G|ge| less in the frame
+                // of step -1
+                if (W == 16 && L::half(sv, q) >= (MAX3 ? kInt16Top + 1 : L::kMax) - (LOCAL && CDRIFT ? G * cg : 0))
+                    atomicOr(a.flags + tk.pair[q], kFlagSaturated);
+                if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
+            }
+        }
+    }
+}
+
+// Copies the launch's substitution table into LDS for the score kinds that read one there.
+template <int SCORE>
+__device__ __forceinline__ void load_launch_table(char *lds_raw, const KernelArgs &a)
+{
     if constexpr (SCORE == kScoreMatrix) {
         load_lds_table(lds_raw, a.table, a.table_entries);
         __syncthreads();
@@ -118,324 +433,21 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         load_lds_pairs(lds_raw, reinterpret_cast<const uint32_t *>(a.table), a.table_entries);
         __syncthreads();
     }
+}
 
-    const int t = int(threadIdx.x % G);
-    const uint32_t lane_bank = (threadIdx.x & 31u) * 4u;   // this lane's copy of the LDS table
-    auto task_body = [&](const uint32_t gid) {
-        const Task tk = a.tasks[gid];
-        const int go = a.go, ge = a.ge;
-        // method_local.hpp:41-49: a local alignment has every end gap free
-        const int fc = LOCAL ? 1 : a.fc, fr = LOCAL ? 1 : a.fr, lc = LOCAL ? 0 : a.lc;
-        if (a.self_init && t == 0) init_task_pairs<W, LOCAL>(a, tk, fc, fr, lc, LOCAL ? 0 : a.lr);
-        constexpr bool DRIFT = !LOCAL;           // row drift (pa_cells.hpp) for the global method
-        constexpr bool ACC = ROWMAX || TRACK;    // per-row maxima
-        // column drift: on top of the row drift for the global method, alone for the local method
-        // (whose zero clamp and running maximum then work in the step's frame)
-        constexpr bool CDRIFT = !ACC && !EXPORT;
-        // int16 lanes with the column drift take three-way maxima in one instruction and hold
-        // their values biased into its range (pa_plan.hpp, kInt16Top): the local method by one
-        // bias for the launch, the global method by one per half, from the task
-        constexpr bool MAX3 = W == 16 && CDRIFT;
-        const int lbias = MAX3 && LOCAL ? a.lbias : 0;
-        const int b0 = MAX3 && !LOCAL ? int(tk.bias & 0xFFFFu) : lbias;
-        const int b1 = MAX3 && !LOCAL ? int(tk.bias >> 16) : lbias;
-        const V vbias = L::pair(b0, b1);
-
-        const int dge = DRIFT ? ge : 0;
-        const int cg = CDRIFT ? -ge : 0;         // column drift per step
-        const V vgo = L::splat(go), vgogo = L::splat(go + ge), vge = L::splat(ge);
-        // shifts the last row's drift back when H, V leave the lane (and, with the column drift,
-        // forward by one step: the next lane uses them one step later)
-        const V vout = L::splat((DRIFT ? ge * R : 0) + (CDRIFT ? -ge : 0));
-        const V vrebase = L::splat(ge * G);      // end of a chunk: the column drift drops by G|ge|
-        uint32_t tbl_lo = a.tbl_lo;
-        const uint32_t tbl_hi = a.tbl_hi;
-        // the permute's second table word must be a VGPR (one scalar operand per instruction): keep
-        // it in one for the whole kernel instead of moving it from its scalar register every step
-        if constexpr (SCORE == kScoreDna) asm volatile("" : "+v"(tbl_lo));
-        const int N = int(tk.N);
-        constexpr int ROWS = G * R;
-        static_assert(!EXPORT || W == 32, "the DP export runs in int32 lanes");
-        int32_t *dp_col = nullptr, *dp_row = nullptr;   // H(0, n) and H(m, 0) of this task's vectors
-        if constexpr (EXPORT) {
-            dp_col = a.dp_out + a.dp_base[2 * gid];
-            dp_row = a.dp_out + a.dp_base[2 * gid + 1];
-            if (t == 0) {
-                dp_col[0] = h0_value(int(tk.n[0]), fr, go, ge);
-                dp_row[0] = h0_value(int(tk.m[0]), fc, go, ge);
-                // entry 0 of each vector keeps its initial gap state, go + ge: the sweep never
-                // updates it (dp_algorithm_template_base.hpp:94-128 reads entries from 1 on)
-                dp_col[a.dp_gap] = go + ge;
-                dp_row[a.dp_gap] = go + ge;
-            }
-        }
-
-        // exact; for the local method with the column drift, 0 in the frame of step -1 (it moves to
-        // the frame of each step as the step begins and back by G steps at each chunk end)
-        V runmax = LOCAL ? L::splat((CDRIFT ? -cg : 0) + lbias) : L::lowest();
-        V satmax = L::lowest();                         // drifted (int16 saturation test)
-        const V vcg = L::splat(cg);
-
-        // one stripe of the task; ONE: the task has no other
-        auto stripe = [&]<bool ONE>(uint32_t s) {
-            const int row0 = int(s) * ROWS + t * R;   // DP row above this lane's block
-            Rows<W, R, ACC> st;
-            // lane t starts at step t: its initial column (j = 0) counts as computed at step t-1
-            init_rows<W, R, SCORE, ACC, DRIFT, CDRIFT>(st, a.rowcodes + tk.row_off, row0, fc, go, ge, lane_bank,
-                                                       cg * (t - 1), vbias);
-            V Hdiag = L::add(L::splat(h0_value(row0, fc, go, ge) + cg * (t - 1)), vbias);
-            Capture<W, R> cap;
-            // the step at which this lane's next capture event comes (events are rare: one wave
-            // test per step against it)
-            int kev = INT32_MAX;
-            if constexpr (!LOCAL) {
-                cap.init(tk, row0, lc);
-                kev = cap.ev == INT32_MAX ? INT32_MAX : cap.ev + t;
-            }
-
-            const bool has_bnd_in = !ONE && s > 0;
-            const bool has_bnd_out = !ONE && s + 1 < tk.S;
-            const uint32_t *colp = a.colcodes + tk.col_off;
-            uint32_t *bndp = has_bnd_in || has_bnd_out ? a.bnd + 2ull * tk.bnd_off : nullptr;
-
-            // Column-code conveyor: chunk j holds the residues of columns jG+1 .. jG+G, one per
-            // lane, so at the start of step k lane 0 of `conv` holds the residue of column k+1;
-            // `nextB` is the chunk after it.  The residue of step k+1 is thus known one step early,
-            // and its substitution scores are read while the cells of step k are computed.
-            uint32_t Bin = colp[t];   // step 0: lane 0 computes column 0
-            uint32_t conv = colp[1 + t];
-            uint32_t nextB = colp[G + 1 + t];
-            // Top-boundary conveyor: chunk j holds H and the vertical gap state above this stripe
-            // for columns jG .. jG+G-1: the initial row H(0, c+1) for the first stripe, the stored
-            // bottom row of the stripe above otherwise.  Columns past N (computed, never read, by
-            // the unmasked steps) get a low value that no real cell can be raised by.
-            // Lane t of a chunk is consumed at the chunk's step t: column drift cg*t.
-            auto top_chunk = [&](int col, uint32_t &h, uint32_t &e) {
-                if (has_bnd_in) {
-                    if (col < N) {
-                        h = bndp[2 * col];
-                        e = bndp[2 * col + 1];
-                        if constexpr (CDRIFT) {
-                            h = L::bits(L::add(L::from(h), L::splat(cg * t)));
-                            e = L::bits(L::add(L::from(e), L::splat(cg * t)));
-                        }
-                    } else {
-                        h = e = kLowWord;
-                    }
-                } else {
-                    // H(0, col+1) exact, the vertical gap state below it (drifted by |ge| for row 0)
-                    const int hv = h0_value(col + 1, fr, go, ge);
-                    h = L::bits(L::add(L::from(L::top_word(hv + cg * t)), vbias));
-                    e = L::bits(L::add(L::from(L::top_word(hv + go + ge - dge + cg * t)), vbias));
-                }
-            };
-            uint32_t convH = 0, convE = 0, nextH = 0, nextE = 0;
-            // ONE: lane 0's top boundary at the current step, H(0, c+1) and the vertical gap state
-            // below it; it changes by top_step per step (the free first row's column drift, or the
-            // penalised first row's extension without the column drift) and by the chunk rebase
-            uint32_t topH = 0, topE = 0;
-            const int top_step = CDRIFT ? (fr ? cg : 0) : (fr ? 0 : ge);
-            const V vtop_step = L::splat(top_step);
-            if constexpr (ONE) {
-                top_chunk(0, topH, topE);
-            } else {
-                top_chunk(t, convH, convE);
-                top_chunk(G + t, nextH, nextE);
-            }
-            // outputs of this lane from the previous step (read by lane t+1)
-            uint32_t Hout = 0, Eout = 0;
-            // bottom-row outputs of lane G-1 over one chunk of G steps, shifted down one lane per
-            // step: at the end of the chunk lane j holds the output of the chunk's step j
-            uint32_t qH = 0, qE = 0;
-
-            ScoreReads<W, R, SCORE> rd;
-            issue_scores<W, R, SCORE>(st, Bin, lds_raw, rd);
-            V sc[R];
-            finish_scores<W, R, SCORE>(st, Bin, tbl_lo, tbl_hi, rd, sc);
-
-            // One step.  MASKED: lanes before their first column (k - t < 0; the first chunk,
-            // where lane t waits t steps) leave their state alone, and so do lanes past column N-1
-            // in the ROWMAX/TRACK/EXPORT variants, whose per-row maxima and exported rows must not
-            // see pad columns.  Elsewhere lanes past column N-1 sweep pad columns, as in every
-            // unmasked step: pad cells never exceed the real ones and never feed them, and a lane's
-            // state never falls behind the column drift (a state left alone while the chunk ends
-            // drop the drift would sink by G|ge| per chunk, for the max3 variants into patterns
-            // their three-way maximum does not order).
-            // TOPSTEP (ONE only): lane 0's top boundary changes from step to step (top_step != 0)
-            auto step = [&]<bool MASKED, bool TOPSTEP>(int k) {
-                const int c = k - t;   // DP column (0-based) of this lane at this step
-                // the conveyor moves on first, so that the old register is free for lane 0's value
-                const uint32_t shifted = dpp_any<kRowShl1>(conv);
-                const uint32_t Bnext = dpp<kRowShr1>(conv, Bin);
-                conv = shifted;
-                issue_scores<W, R, SCORE>(st, Bnext, lds_raw, rd);
-                uint32_t Hin, Ein;
-                if constexpr (ONE) {
-                    // lanes 1.. take lane t-1's outputs, lane 0 keeps its top boundary
-                    topH = dpp<kRowShr1>(topH, Hout);
-                    topE = dpp<kRowShr1>(topE, Eout);
-                    Hin = topH;
-                    Ein = topE;
-                    if constexpr (TOPSTEP) {
-                        topH = L::bits(L::add(L::from(topH), vtop_step));
-                        topE = L::bits(L::add(L::from(topE), vtop_step));
-                    }
-                } else {
-                    const uint32_t nextH_ = dpp_any<kRowShl1>(convH), nextE_ = dpp_any<kRowShl1>(convE);
-                    Hin = dpp<kRowShr1>(convH, Hout);
-                    Ein = dpp<kRowShr1>(convE, Eout);
-                    convH = nextH_;
-                    convE = nextE_;
-                }
-                if constexpr (LOCAL && CDRIFT) runmax = L::add(runmax, vcg);   // into this step's frame
-                if (!MASKED || (ROWMAX || TRACK || EXPORT ? uint32_t(c) < uint32_t(N) : c >= 0)) {
-                    V Eup = L::from(Ein);
-                    // EXPORT: the vertical gap state below the last row, from the lane that holds it
-                    V vlast{};
-                    const int er = EXPORT ? int(tk.m[0]) - 1 - row0 : -1;
-                    update_cells<W, R, SCORE, LOCAL, ACC, DRIFT, CDRIFT, MAX3, EXPORT>(
-                        st, Eup, Hdiag, runmax, sc, Bin, tbl_lo, tbl_hi, vgo, vgogo, vge,
-                        L::splat(cg * (k % G) + lbias), &vlast, er);
-                    Hdiag = L::from(Hin);
-                    Hout = L::bits(DRIFT || CDRIFT ? L::add(st.H[R - 1], vout) : st.H[R - 1]);
-                    Eout = L::bits(DRIFT || CDRIFT ? L::add(Eup, vout) : Eup);
-                    if constexpr (EXPORT) {
-                        // the last row: H(m, c+1) from the lane that holds row m; the last column:
-                        // H(i, n) of every row of this lane at column n
-                        // Gap halves (affine_dp_algorithm.hpp:139-156): below row m, V(m+1, c+1), and
-                        // right of column n, W(i, n+1), the gap states after the last update.  With
-                        // the row drift, row r of a lane holds H and W less ge*(r+1), and the vertical
-                        // state below it less ge*(r+2).
-                        const int m = int(tk.m[0]), n = int(tk.n[0]);
-                        if (er >= 0 && er < R && c < n) {
-                            int v = 0;
-#pragma unroll
-                            for (int r = 0; r < R; ++r) if (r == er) v = L::half(st.H[r], 0);
-                            dp_row[c + 1] = v + dge * (er + 1);
-                            dp_row[a.dp_gap + c + 1] = L::half(vlast, 0) + dge * (er + 2);
-                        }
-                        if (c == n - 1) {
-#pragma unroll
-                            for (int r = 0; r < R; ++r) {
-                                if (row0 + r + 1 <= m) {
-                                    dp_col[row0 + r + 1] = L::half(st.H[r], 0) + dge * (r + 1);
-                                    dp_col[a.dp_gap + row0 + r + 1] = L::half(st.F[r], 0) + dge * (r + 1);
-                                }
-                            }
-                        }
-                    }
-                }
-                if constexpr (!LOCAL) {
-                    // capture events are rare: test the whole wave first
-                    if (__builtin_expect(__builtin_amdgcn_ballot_w64(k == kev) != 0, 0)) {
-                        cap.at_column(c, st, tk, a.scores, dge, -cg * (k % G), -b0, -b1);
-                        kev = cap.ev == INT32_MAX ? INT32_MAX : cap.ev + t;
-                    }
-                }
-                if constexpr (!ONE) {
-                    qH = dpp<kRowShl1>(Hout, qH);
-                    qE = dpp<kRowShl1>(Eout, qE);
-                }
-                finish_scores<W, R, SCORE>(st, Bnext, tbl_lo, tbl_hi, rd, sc);
-                Bin = Bnext;
-            };
-            auto chunk_end = [&](int k0) {
-                conv = nextB;
-                nextB = colp[k0 + 2 * G + 1 + t];
-                if constexpr (!ONE) {
-                    convH = nextH;
-                    convE = nextE;
-                    top_chunk(k0 + 2 * G + t, nextH, nextE);
-                }
-                if (has_bnd_out) {
-                    // lane G-1 computed column k0 + j - (G-1) at the chunk's step j; stored without
-                    // the column drift it was sent with (cg*(j+1))
-                    const int col = k0 + t - (G - 1);
-                    if (col >= 0 && col < N) {
-                        uint32_t h = qH, e = qE;
-                        if constexpr (CDRIFT) {
-                            h = L::bits(L::add(L::from(h), L::splat(-cg * (t + 1))));
-                            e = L::bits(L::add(L::from(e), L::splat(-cg * (t + 1))));
-                        }
-                        bndp[2 * col] = h;
-                        bndp[2 * col + 1] = e;
-                    }
-                }
-                if constexpr (CDRIFT) {
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        st.H[r] = L::add(st.H[r], vrebase);
-                        st.F[r] = L::add(st.F[r], vrebase);
-                    }
-                    Hdiag = L::add(Hdiag, vrebase);
-                    Hout = L::bits(L::add(L::from(Hout), vrebase));
-                    Eout = L::bits(L::add(L::from(Eout), vrebase));
-                    if constexpr (LOCAL) runmax = L::add(runmax, vrebase);
-                    if constexpr (ONE) {
-                        topH = L::bits(L::add(L::from(topH), vrebase));
-                        topE = L::bits(L::add(L::from(topE), vrebase));
-                    }
-                }
-            };
-
-            // whole chunks of G steps; the first is masked (lanes start one step apart)
-            const int nsteps = ((N + 2 * G - 2) / G) * G;
-            constexpr bool kMaskAll = ROWMAX || TRACK || EXPORT;
-            auto sweep = [&]<bool TOPSTEP>() {
-#pragma unroll 1
-                for (int u = 0; u < G; ++u) step.template operator()<true, TOPSTEP>(u);
-                chunk_end(0);
-                for (int k0 = G; k0 < nsteps; k0 += G) {
-                    // two steps per iteration: the register allocator then alternates the exchanged
-                    // values between two sets of registers instead of copying them back every step
-#pragma unroll 2
-                    for (int u = 0; u < G; ++u) step.template operator()<kMaskAll, TOPSTEP>(k0 + u);
-                    chunk_end(k0);
-                }
-            };
-            if (ONE && top_step != 0) sweep.template operator()<ONE>();
-            else sweep.template operator()<false>();
-            // free last row only: the int16 saturation tracking (TRACK) keeps per-row maxima too, but
-            // they are not candidates for the score
-            if constexpr (ROWMAX) cap.row_maximum(st, tk, a.scores, dge);
-            fold_rows<W, R, ACC, DRIFT>(st, runmax, satmax, ge);
-            if (has_bnd_out) {
-                // the boundary row written by lane G-1 is read by lane 0 of the next stripe
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            }
-        };
-        if (!EXPORT && tk.S == 1) {
-            stripe.template operator()<!EXPORT>(0);
-        } else {
-            for (uint32_t s = 0; s < tk.S; ++s) stripe.template operator()<false>(s);
-        }
-
-        if constexpr (LOCAL || TRACK) {
-            const V mv = row_reduce_max<W>(runmax);
-            const V sv = row_reduce_max<W>(LOCAL ? runmax : satmax);
-            if (t == 0) {
-#pragma unroll
-                for (int q = 0; q < L::NH; ++q) {
-                    if (tk.pair[q] == kNone) continue;
-                    // the local maximum is in the frame of step -1 with the column drift
-                    const int v = L::half(mv, q) + (LOCAL && CDRIFT ? cg : 0) - lbias;
-                    if constexpr (LOCAL) a.scores[tk.pair[q]] = v;
-                    // int16 overflow: a value clamped at kMax, or (max3 variants) above kInt16Top,
-                    // whose NaN pattern the running maximum keeps at 0x7E00 or more; in the frame
-                    // of step u, so with the local column drift at least G|ge| less in the frame
-                    // of step -1
-                    if (W == 16 && L::half(sv, q) >= (MAX3 ? kInt16Top + 1 : L::kMax) - (LOCAL && CDRIFT ? G * cg : 0))
-                        atomicOr(a.flags + tk.pair[q], kFlagSaturated);
-                    if (a.maxh) atomicMax(a.maxh + tk.pair[q], v);
-                }
-            }
-        }
-    };
+// The striped kernel: one task per lane group (one block per 256 / G tasks), or with a.queue only
+// as many blocks as the GPU keeps resident, every wave taking the next 64 / G tasks of the launch
+// from an atomic counter until none are left (the tasks are in decreasing order of work); the
+// last block to finish resets the counter for the next pass.
+template <int W, int G, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX, bool EXPORT = false>
+__global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
+{
+    extern __shared__ char lds_raw[];
+    load_launch_table<SCORE>(lds_raw, a);
 
     if (!a.queue) {
         const uint32_t gid = (blockIdx.x * blockDim.x + threadIdx.x) / G;
-        if (gid < a.ntasks) task_body(gid);
+        if (gid < a.ntasks) stripe_task<W, G, R, SCORE, LOCAL, TRACK, ROWMAX, EXPORT>(a, gid, lds_raw);
         return;
     }
     const uint32_t group = (threadIdx.x % 64) / G;   // of the wave's 64 / G groups
@@ -445,11 +457,52 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
         if (threadIdx.x % 64 == 0) base = atomicAdd(a.queue, 64u / G);
         base = __builtin_amdgcn_readfirstlane(base);
         if (base >= a.ntasks) break;
-        if (base + group < a.ntasks) task_body(base + group);
+        if (base + group < a.ntasks) stripe_task<W, G, R, SCORE, LOCAL, TRACK, ROWMAX, EXPORT>(a, base + group, lds_raw);
     }
     __syncthreads();   // every wave of the block has seen the queue empty
     if (threadIdx.x == 0 && atomicAdd(a.queue + 1, 1u) == gridDim.x - 1) {
         // the last block: no wave of this launch takes a task any more
+        __hip_atomic_store(a.queue, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(a.queue + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// One persistent launch for the striped launches of a plan that differ only in their rows per
+// lane (pa_plan.hpp, FusedLaunch): as many blocks as the GPU keeps resident, whose waves take wave
+// items from one atomic counter in the planner's order (decreasing modelled time across the
+// member launches).  An item names up to 64 / 16 consecutive tasks of one member and its R, which
+// selects the sweep; the wave's lane groups take one task each.  The last block to finish resets
+// the counter for the next pass.
+template <int W, int SCORE, bool LOCAL>
+__global__ __launch_bounds__(256) void fused_stripe_kernel(KernelArgs a)
+{
+    constexpr int G = 16;
+    extern __shared__ char lds_raw[];
+    load_launch_table<SCORE>(lds_raw, a);
+    const uint32_t group = (threadIdx.x % 64) / G;   // of the wave's 64 / G groups
+    for (;;) {
+        // the whole wave is here (converged): lane 0 takes the next item for it
+        uint32_t idx = 0;
+        if (threadIdx.x % 64 == 0) idx = atomicAdd(a.queue, 1u);
+        idx = __builtin_amdgcn_readfirstlane(idx);
+        if (idx >= a.nitems) break;
+        const uint32_t item = __builtin_amdgcn_readfirstlane(a.items[idx]);
+        if (group <= ((item >> 27) & 3u)) {
+            const uint32_t gid = (item & (kFusedMaxTasks - 1u)) + group;
+            switch (item >> 29) {   // (R - 4) / 2, uniform over the wave
+            case 0: stripe_task<W, G, 4, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
+            case 1: stripe_task<W, G, 6, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
+            case 2: stripe_task<W, G, 8, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
+            case 3: stripe_task<W, G, 10, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
+            case 4: stripe_task<W, G, 12, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
+            case 5: stripe_task<W, G, 14, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
+            default: stripe_task<W, G, 16, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
+            }
+        }
+    }
+    __syncthreads();   // every wave of the block has seen the queue empty
+    if (threadIdx.x == 0 && atomicAdd(a.queue + 1, 1u) == gridDim.x - 1) {
+        // the last block: no wave of this launch takes an item any more
         __hip_atomic_store(a.queue, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(a.queue + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -34,6 +34,10 @@ struct KernelArgs {
     uint64_t dp_gap;
     uint32_t *queue;            // nullable: the launch's work queue (next task, finished blocks), zero
                                 // between passes; striped kernel only
+    // fused launch (pa_plan.hpp, FusedLaunch): its wave items; tasks index the plan's whole task
+    // array (tasks points at its first task)
+    const uint32_t *items;
+    uint32_t nitems;
 };
 
 struct InitArgs {
@@ -64,6 +68,9 @@ struct EncodeArgs {
 };
 
 hipError_t launch_stripe(const VariantKey &key, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st);
+// A fused launch of the variants that differ from `key` only in R (pa_plan.hpp, fused_launchable):
+// a.items / a.nitems its wave items, a.queue its work queue.
+hipError_t launch_fused(const VariantKey &key, const KernelArgs &a, uint32_t lds_bytes, hipStream_t st);
 hipError_t launch_encode(const EncodeArgs &a, hipStream_t st);
 hipError_t launch_init(const InitArgs &a, hipStream_t st);
 

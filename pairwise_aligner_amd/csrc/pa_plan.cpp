@@ -41,7 +41,7 @@ Tuning resolve_tuning(const pa_tuning *t)
     if (t->streams) r.streams = std::min<uint32_t>(t->streams, 4);
     r.eager = t->eager != 0;
     r.cost_order = t->cost_order != 0;
-    r.persistent = t->persistent != 0;
+    r.persistent = std::min<uint32_t>(t->persistent, 3);
     return r;
 }
 
@@ -368,28 +368,93 @@ void code_tables(const Plan &plan, const VariantKey &key, CodeTables &out)
     }
 }
 
+// Fused launches: the striped int16 local launches whose keys differ only in R become one
+// persistent launch with one work queue, whose wave items (runs of up to four consecutive tasks of
+// one member launch, alike in shape) come in decreasing order of modelled time across the members
+// -- one longest-first order for the whole group instead of one per launch, and no launch of the
+// group waiting for a stream or forming the pass's tail alone.  The kernel that holds every R runs
+// the pair-row-table sweeps at the occupancy of their largest member (167 against 148 VGPRs,
+// three waves per SIMD either way), so those are fused by default; it holds the byte-permute
+// sweeps at three waves per SIMD instead of four, which pays only where the pass is short or made
+// of many small launches (Tuning::persistent 2, the autotuner's choice).
+void fuse_launches(Plan &plan)
+{
+    const size_t nl = plan.launches.size();
+    plan.fused.clear();
+    plan.fused_items.clear();
+    plan.fused_of.assign(nl, -1);
+    const uint32_t form = plan.tuning.persistent;
+    if (form == 1 || form == 3) return;
+    auto fuses = [&](const VariantKey &k) {
+        return fused_launchable(k) && (k.score == kScorePairs || form == 2);
+    };
+    auto group_key = [](VariantKey k) { k.R = 0; return k; };
+    std::vector<bool> taken(nl, false);
+    for (size_t i = 0; i < nl; ++i) {
+        if (taken[i] || !fuses(plan.launches[i].key)) continue;
+        const VariantKey gk = group_key(plan.launches[i].key);
+        std::vector<uint32_t> members;
+        for (size_t j = i; j < nl; ++j) {
+            const Launch &L = plan.launches[j];
+            if (taken[j] || !fuses(L.key) || !(group_key(L.key) == gk)) continue;
+            if (L.task_end > kFusedMaxTasks) continue;   // beyond the item's task field
+            members.push_back(uint32_t(j));
+            taken[j] = true;
+        }
+        if (members.size() < 2) continue;
+        struct Item { double time; uint32_t item; };
+        std::vector<Item> items;
+        for (uint32_t li : members) {
+            const Launch &L = plan.launches[li];
+            for (uint32_t k0 = L.task_begin; k0 < L.task_end; k0 += 4) {
+                const uint32_t n = std::min<uint32_t>(4, L.task_end - k0);
+                double t = 0;
+                for (uint32_t k = k0; k < k0 + n; ++k) t = std::max(t, task_cost(L.key, plan.tasks[k], plan.tuning));
+                items.push_back(Item{t, fused_item(L.key.R, n, k0)});
+            }
+        }
+        std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.time > y.time; });
+        FusedLaunch f;
+        f.leader = members[0];
+        f.items_begin = uint32_t(plan.fused_items.size());
+        for (const Item &it : items) plan.fused_items.push_back(it.item);
+        f.items_end = uint32_t(plan.fused_items.size());
+        for (uint32_t li : members) plan.fused_of[li] = int32_t(plan.fused.size());
+        plan.fused.push_back(f);
+    }
+}
+
 void order_launches(Plan &plan)
 {
     // Wavefront launches first: each of their workgroups is one long serial sweep, so they
     // set the critical path (longest task first).  Then the striped launches by the time each
     // would take on its own: its longest task, or its whole work spread over the groups the GPU
     // holds at once, whichever is larger -- so that neither a launch of a few long tasks nor a
-    // large launch of short ones ends up in the tail of the pass.  Streams round robin.
+    // large launch of short ones ends up in the tail of the pass.  Streams round robin.  A fused
+    // launch counts once, with the tasks of all its members, in its leader's place; the other
+    // members come last and share the leader's stream (they are not launched on their own).
     const size_t nl = plan.launches.size();
-    std::vector<double> longest(nl, 0.0), alone(nl, 0.0);
+    if (plan.fused_of.size() != nl) plan.fused_of.assign(nl, -1);
+    auto unit_of = [&](size_t i) {
+        return plan.fused_of[i] < 0 ? uint32_t(i) : plan.fused[size_t(plan.fused_of[i])].leader;
+    };
+    std::vector<double> longest(nl, 0.0), total(nl, 0.0), alone(nl, 0.0);
     for (size_t i = 0; i < nl; ++i) {
         const Launch &L = plan.launches[i];
-        double total = 0;
+        const uint32_t u = unit_of(i);
         for (uint32_t t = L.task_begin; t < L.task_end; ++t) {
             const double c = task_cost(L.key, plan.tasks[t], plan.tuning);
-            longest[i] = std::max(longest[i], c);
-            total += c;
+            longest[u] = std::max(longest[u], c);
+            total[u] += c;
         }
-        alone[i] = std::max(longest[i], total / kResidentGroups);
     }
+    for (size_t i = 0; i < nl; ++i) alone[i] = std::max(longest[i], total[i] / kResidentGroups);
     plan.launch_order.resize(nl);
     std::iota(plan.launch_order.begin(), plan.launch_order.end(), 0u);
     std::stable_sort(plan.launch_order.begin(), plan.launch_order.end(), [&](uint32_t x, uint32_t y) {
+        const bool mx = unit_of(x) != x, my = unit_of(y) != y;   // fused members other than the leader
+        if (mx != my) return my;
+        if (mx) return false;
         const Launch &a = plan.launches[x], &b = plan.launches[y];
         const bool wa = a.key.G == 64, wb = b.key.G == 64;
         if (wa != wb) return wa;
@@ -405,16 +470,20 @@ void order_launches(Plan &plan)
     // so that the later (smaller) ones queue behind the wavefront launches, which end well before
     // the large striped ones, instead of forming the pass's tail behind those (uneven 1M batch,
     // default tuning: 51 -> 37 ms per pass).
-    const uint32_t ns = std::max<uint32_t>(1, std::min<uint32_t>(plan.tuning.streams, uint32_t(std::max<size_t>(nl, 1))));
+    size_t n_units = 0;
+    for (size_t i = 0; i < nl; ++i) n_units += unit_of(i) == i;
+    const uint32_t ns = std::max<uint32_t>(1, std::min<uint32_t>(plan.tuning.streams, uint32_t(std::max<size_t>(n_units, 1))));
     uint32_t n_wf = 0;
     for (const Launch &L : plan.launches) n_wf += L.key.G == 64;
-    const uint32_t n_striped = uint32_t(nl) - n_wf;
+    const uint32_t n_striped = uint32_t(n_units) - n_wf;
     const uint32_t wf_streams = ns == 1 || n_wf == 0 ? 0 : std::min(n_wf, n_striped ? ns - 1 : ns);
     plan.launch_stream.assign(nl, 0);
     uint32_t next_wf = 0, next_st = 0;
     for (size_t i = 0; i < nl; ++i) {
         const uint32_t li = plan.launch_order[i];
-        if (plan.launches[li].key.G == 64 && wf_streams) {
+        if (unit_of(li) != li) {
+            plan.launch_stream[li] = plan.launch_stream[unit_of(li)];   // placed after its leader
+        } else if (plan.launches[li].key.G == 64 && wf_streams) {
             plan.launch_stream[li] = uint8_t(next_wf++ % wf_streams);
         } else {
             plan.launch_stream[li] = uint8_t((wf_streams + next_st++) % ns);
@@ -956,6 +1025,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
 
     plan.n_rowcodes = n_rowcodes;
     plan.n_colcodes = n_colcodes;
+    fuse_launches(plan);
     order_launches(plan);
     return PA_OK;
 }

@@ -152,6 +152,32 @@ struct Launch {
     uint64_t padded_cells;   // lane-cells swept, both int16 halves counted
 };
 
+// A fused launch: its leader launch and its wave items [items_begin, items_end) in
+// Plan::fused_items, in decreasing order of modelled time.
+struct FusedLaunch {
+    uint32_t leader;
+    uint32_t items_begin, items_end;
+};
+
+// Wave item of a fused launch: up to 64 / 16 consecutive tasks of one member launch, as
+// (R - 4) / 2 in bits 29-31, the task count less one in bits 27-28 and the first task's index in
+// the plan's task array in bits 0-26.
+inline constexpr uint32_t kFusedMaxTasks = 1u << 27;
+inline uint32_t fused_item(uint32_t R, uint32_t count, uint32_t first)
+{
+    return ((R - 4) / 2) << 29 | (count - 1) << 27 | first;
+}
+
+// Launches that a fused launch can take: the int16 striped variants of the local method (without
+// the DP export) whose substitution scores are the byte permute or the pair-row table.  (The
+// global method's variants at R = 16 hold 232 VGPRs on their own; a kernel holding every R would
+// run one wave per SIMD.)
+inline bool fused_launchable(const VariantKey &k)
+{
+    return k.width == 16 && k.G == 16 && k.local && !k.track && !k.rowmax && !k.export_dp &&
+           (k.score == kScoreDna || k.score == kScorePairs);
+}
+
 struct PairMeta {
     uint32_t m, n;        // after orientation: m rows (seq1 side), n columns
     uint8_t local;
@@ -172,7 +198,14 @@ struct Tuning {
     uint32_t streams = 4;
     bool eager = false;
     bool cost_order = false;
-    bool persistent = false;
+    // Launch forms of the striped launches.  0 (default): one block per 16 tasks, except that the
+    // int16 local launches with the pair-row table that differ only in R run as one fused launch
+    // (resident blocks, one work queue; fused_launchable), whose kernel holds no more registers
+    // per lane than its members' largest; 1: every striped launch runs resident blocks taking
+    // tasks from a work queue of its own; 2: as 1, and the fused_launchable launches that differ
+    // only in R -- byte-permute DNA scores too -- run fused; 3: one block per 16 tasks, nothing
+    // fused.
+    uint32_t persistent = 0;
 };
 Tuning resolve_tuning(const pa_tuning *t);
 
@@ -217,6 +250,12 @@ struct Plan {
     uint32_t pad_rank = 0;                    // matrix rank of the pad symbol
     std::vector<uint32_t> launch_order;       // launches in start order (order_launches)
     std::vector<uint8_t> launch_stream;       // stream of each launch (indexed like launches)
+    // Fused launches (Tuning::persistent 0 or 2): one persistent launch runs the tasks of several
+    // launches whose keys differ only in R.  It takes its leader's place in launch_order, stream,
+    // work queue and arguments; the other members are not launched on their own.
+    std::vector<FusedLaunch> fused;
+    std::vector<int32_t> fused_of;            // per launch: index into fused, or -1
+    std::vector<uint32_t> fused_items;        // wave items of all fused launches (fused_item)
 };
 
 struct BatchView {
@@ -337,9 +376,15 @@ struct CodeTables {
 };
 void code_tables(const Plan &plan, const VariantKey &key, CodeTables &out);
 
+// Groups the plan's fused_launchable launches that differ only in R into fused launches
+// (Plan::fused, fused_of, fused_items): those with the pair-row table unless Tuning::persistent is
+// 1 or 3, those with byte-permute DNA scores when it is 2.
+void fuse_launches(Plan &plan);
+
 // Order of the launches of a plan and the stream of each: wavefront launches first (longest
-// task first), then striped launches by decreasing work.  Wavefront launches share stream 0,
-// striped launches take the other streams round robin.
+// task first), then striped launches by the time each would take on its own (a fused launch
+// once, in its leader's place).  Wavefront launches take streams of their own, striped launches
+// the others round robin.
 void order_launches(Plan &plan);
 
 } // namespace pa

@@ -417,8 +417,122 @@ static void check_cost_order()
     std::printf("  cost order: %zu runs of whole waves checked\n", runs);
 }
 
+// Fused launches (pa_tuning.persistent = 2): the int16 local striped launches that differ only in R
+// form one fused launch; its wave items cover every task of every member exactly once, in runs of
+// at most four consecutive tasks of one member (its R in the item), in non-increasing order of
+// the modelled time of their costliest task; leaders come before the other members in the launch
+// order, which share their leader's stream; global and int32 launches stay unfused.
+static void check_fused_launches()
+{
+    std::mt19937_64 rng(31);
+    Batch b = random_batch(rng, 40000, "ACGT", 600, false, true, false);   // per-pair methods
+    pa_config cfg{};
+    cfg.match = 2;
+    cfg.mismatch = -3;
+    cfg.gap_open = -5;
+    cfg.gap_extension = -2;
+    cfg.lane_policy = PA_LANES_INT16;
+    pa_tuning t{};
+    t.persistent = 2;
+    cfg.tuning = &t;
+    Plan p;
+    std::string err;
+    CHECK(make_plan(cfg, b.view(), nullptr, nullptr, p, err) == PA_OK, "fused plan: %s", err.c_str());
+    const size_t nl = p.launches.size();
+    CHECK(p.fused_of.size() == nl, "fused_of size %zu of %zu", p.fused_of.size(), nl);
+    CHECK(!p.fused.empty(), "no fused launch among %zu launches", nl);
+    for (size_t l = 0; l < nl && l < p.fused_of.size(); ++l) {
+        const VariantKey &k = p.launches[l].key;
+        if (p.fused_of[l] < 0) {
+            // an unfused launchable launch has no other launch of its group
+            if (!fused_launchable(k)) continue;
+            for (size_t o = 0; o < nl; ++o) {
+                VariantKey a = k, c = p.launches[o].key;
+                a.R = c.R = 0;
+                CHECK(o == l || !(a == c) || p.fused_of[o] >= 0, "launches %zu and %zu differ only in R, unfused", l, o);
+            }
+            continue;
+        }
+        CHECK(fused_launchable(k) && k.local, "launch %zu (width %d, local %d) fused", l, k.width, k.local);
+        const FusedLaunch &F = p.fused[size_t(p.fused_of[l])];
+        VariantKey a = k, c = p.launches[F.leader].key;
+        a.R = c.R = 0;
+        CHECK(a == c, "launch %zu fused with a leader of another variant", l);
+    }
+    for (size_t f = 0; f < p.fused.size(); ++f) {
+        const FusedLaunch &F = p.fused[f];
+        std::vector<uint8_t> seen(p.tasks.size(), 0);
+        double prev = INFINITY;
+        for (uint32_t i = F.items_begin; i < F.items_end; ++i) {
+            const uint32_t it = p.fused_items[i];
+            const uint32_t R = 4 + 2 * (it >> 29), n = ((it >> 27) & 3u) + 1, first = it & (kFusedMaxTasks - 1);
+            size_t owner = nl;
+            for (size_t l = 0; l < nl; ++l)
+                if (p.fused_of[l] == int32_t(f) && first >= p.launches[l].task_begin && first < p.launches[l].task_end)
+                    owner = l;
+            CHECK(owner < nl, "item %u: task %u in no member launch", i, first);
+            if (owner == nl) continue;
+            const Launch &L = p.launches[owner];
+            CHECK(L.key.R == R, "item %u: R %u in a launch of R %d", i, R, L.key.R);
+            CHECK(first + n <= L.task_end && (first - L.task_begin) % 4 == 0, "item %u: tasks %u+%u", i, first, n);
+            double cost = 0;
+            for (uint32_t k = first; k < std::min(first + n, L.task_end); ++k) {
+                CHECK(seen[k] == 0, "task %u in two items", k);
+                seen[k] = 1;
+                cost = std::max(cost, task_cost(L.key, p.tasks[k], p.tuning));
+            }
+            CHECK(cost <= prev, "item %u: modelled time %g after %g", i, cost, prev);
+            prev = cost;
+        }
+        for (size_t l = 0; l < nl; ++l)
+            if (p.fused_of[l] == int32_t(f))
+                for (uint32_t k = p.launches[l].task_begin; k < p.launches[l].task_end; ++k)
+                    CHECK(seen[k] == 1, "fused launch %zu: task %u of launch %zu in no item", f, k, l);
+    }
+    std::vector<size_t> pos(nl);
+    for (size_t i = 0; i < nl; ++i) pos[p.launch_order[i]] = i;
+    for (size_t l = 0; l < nl; ++l) {
+        if (p.fused_of[l] < 0) continue;
+        const uint32_t leader = p.fused[size_t(p.fused_of[l])].leader;
+        CHECK(l == leader || pos[leader] < pos[l], "member %zu ordered before its leader %u", l, leader);
+        CHECK(p.launch_stream[l] == p.launch_stream[leader], "member %zu on another stream than its leader", l);
+    }
+    // the default form fuses the pair-row-table launches only; forms 1 and 3 fuse nothing
+    static const char *kAmino = "ARNDCQEGHILKMFPSTWYVX";
+    std::vector<int32_t> matrix(21 * 21);
+    for (int i = 0; i < 21; ++i)
+        for (int j = 0; j < 21; ++j) matrix[i * 21 + j] = i == j ? 5 : -1 - (i + j) % 3;
+    pa_config prot{};
+    prot.method = PA_METHOD_LOCAL;
+    prot.gap_open = -11;
+    prot.gap_extension = -1;
+    prot.score_kind = PA_SCORE_MATRIX;
+    prot.matrix_dim = 21;
+    prot.matrix = matrix.data();
+    prot.matrix_symbols = reinterpret_cast<const uint8_t *>(kAmino);
+    Batch pb = random_batch(rng, 20000, kAmino, 500, false, false, false);
+    for (uint32_t form : {0u, 1u, 3u}) {
+        for (pa_config *c : {&prot, &cfg}) {
+            pa_tuning tf{};
+            tf.persistent = form;
+            c->tuning = &tf;
+            Plan q;
+            const Batch &bb = c == &prot ? pb : b;
+            CHECK(make_plan(*c, bb.view(), nullptr, nullptr, q, err) == PA_OK, "form %u plan: %s", form, err.c_str());
+            const bool want = form == 0 && c == &prot;
+            CHECK(q.fused.empty() != want, "form %u, %s: %zu fused launches", form, c == &prot ? "protein" : "DNA",
+                  q.fused.size());
+            for (const FusedLaunch &F : q.fused) CHECK(q.launches[F.leader].key.score == kScorePairs, "form %u", form);
+        }
+    }
+    std::printf("  fused launches: %zu of %zu launches in %zu fused, %zu items\n",
+                size_t(std::count_if(p.fused_of.begin(), p.fused_of.end(), [](int32_t x) { return x >= 0; })), nl,
+                p.fused.size(), p.fused_items.size());
+}
+
 int main()
 {
+    check_fused_launches();
     check_cost_order();
     check_recentring();
     check_wavefront_tail();

@@ -73,16 +73,23 @@ def test_graph_replays_match_eager(ctx):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("form", [0, 1, 2, 3], ids=["default", "queue_per_launch", "fused", "unfused"])
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
-def test_persistent_work_queue_launches(ctx, graph):
-    """Striped launches as resident blocks taking tasks from a work queue (pa_tuning.persistent):
-    exact on every replay (the last block resets the queue for the next pass), with eager work in
+def test_persistent_work_queue_launches(ctx, graph, form):
+    """The launch forms of pa_tuning.persistent: the default (pair-row-table local launches fused
+    into one launch with one work queue), resident blocks taking tasks from a work queue per launch
+    (1), with every int16 local launch that differs only in R fused (2), and no fusion (3): exact
+    on every replay (the last block resets the queue for the next pass), with eager work in
     between, int16 saturation recomputed, a matrix score model and a batch larger than the grid."""
     b1, o1, b2, o2, pm = uneven_batch(47, 30000, hi=3000)
     base = dict(match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
-    dc = device.DeviceConfig(**base, tuning=device.Tuning(persistent=1, eager=0 if graph else 1))
+    dc = device.DeviceConfig(**base, tuning=device.Tuning(persistent=form, eager=0 if graph else 1))
     want = oracle_scores(b1, o1, b2, o2, dc, pm)
     bt = device.Batch(ctx, b1, o1, b2, o2, dc, pm)
+    if form == 2:   # the local launches (one per R) are one launch
+        unfused = device.Batch(ctx, b1, o1, b2, o2, device.DeviceConfig(**base, tuning=device.Tuning(persistent=1)), pm)
+        assert bt.stats()["n_launches"] < unfused.stats()["n_launches"], (bt.stats(), unfused.stats())
+        unfused.close()
     be = device.Batch(ctx, b1, o1, b2, o2, device.DeviceConfig(**base, tuning=device.Tuning(eager=1)), pm)
     for rep in range(4):
         log(f"persistent pass {rep}")
@@ -101,7 +108,7 @@ def test_persistent_work_queue_launches(ctx, graph):
     prot = device.DeviceConfig(method=device.METHOD_LOCAL, score_kind=device.SCORE_MATRIX,
                                symbols="".join(x for x, _ in t).encode(), matrix=tuple(v for _, r in t for v in r),
                                gap_open=-11, gap_extension=-1,
-                               tuning=device.Tuning(persistent=1, eager=0 if graph else 1))
+                               tuning=device.Tuning(persistent=form, eager=0 if graph else 1))
     want = O.score_pairs(s1, s2, oracle_cfg(prot), threads=8)
     for rep in range(2):
         got, _ = ctx.score(s1, s2, prot)
@@ -111,18 +118,20 @@ def test_persistent_work_queue_launches(ctx, graph):
     d1, d2 = workloads.dna_pairs(rng, 400, 50, 400, divergence=0.1)
     same = workloads.unpack(*workloads.random_packed(rng, np.full(3, 2500), workloads.DNA))
     d1, d2 = d1 + same, d2 + list(same)
-    dna = device.DeviceConfig(match=15, mismatch=-5, gap_open=-10, gap_extension=-1, lane_policy=device.LANES_INT16,
-                              tuning=device.Tuning(persistent=1, eager=0 if graph else 1))
-    want = O.score_pairs(d1, d2, oracle_cfg(dna), threads=8)
     p1, q1 = workloads.pack(d1)
     p2, q2 = workloads.pack(d2)
-    bt = device.Batch(ctx, p1, q1, p2, q2, dna)
-    for rep in range(3):
-        bt.run()
-        got, flags = bt.results()
-        assert np.array_equal(got.astype(np.int64), want), f"saturating pass {rep}"
-    assert np.all(flags[-3:] & device.PAIR_RECOMPUTED) and np.all(got[-3:] == 2500 * 15)
-    bt.close()
+    for method in (device.METHOD_GLOBAL, device.METHOD_LOCAL):
+        dna = device.DeviceConfig(method=method, match=15, mismatch=-5, gap_open=-10, gap_extension=-1,
+                                  lane_policy=device.LANES_INT16,
+                                  tuning=device.Tuning(persistent=form, eager=0 if graph else 1))
+        want = O.score_pairs(d1, d2, oracle_cfg(dna), threads=8)
+        bt = device.Batch(ctx, p1, q1, p2, q2, dna)
+        for rep in range(3):
+            bt.run()
+            got, flags = bt.results()
+            assert np.array_equal(got.astype(np.int64), want), f"saturating pass {rep}, method {method}"
+        assert np.all(flags[-3:] & device.PAIR_RECOMPUTED) and np.all(got[-3:] == 2500 * 15)
+        bt.close()
 
 
 def test_graph_with_saturation_recompute(ctx):
