@@ -84,7 +84,11 @@ struct pa_ctx {
 };
 
 struct pa_batch {
+    // The caller's configuration, with the pointers it holds beyond the call made the batch's own:
+    // the matrix and symbols copied below, the tuning copied into `tuning` (the int32 re-plan of
+    // saturated pairs reads it at a later run), the model cleared once it has been read.
     pa_config cfg{};
+    pa_tuning tuning{};
     std::vector<int32_t> matrix;
     std::vector<uint8_t> symbols;
     // What re-planning the saturated pairs needs of the caller's inputs: the offsets, per-pair
@@ -571,6 +575,10 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
     auto b = std::make_unique<pa_batch>();
     b->arena = arena;
     b->cfg = *cfg;
+    if (cfg->tuning) {
+        b->tuning = *cfg->tuning;
+        b->cfg.tuning = &b->tuning;
+    }
     if (cfg->score_kind == PA_SCORE_MATRIX && cfg->matrix && cfg->matrix_symbols && cfg->matrix_dim > 0) {
         b->matrix.assign(cfg->matrix, cfg->matrix + size_t(cfg->matrix_dim) * cfg->matrix_dim);
         b->symbols.assign(cfg->matrix_symbols, cfg->matrix_symbols + cfg->matrix_dim);
@@ -597,6 +605,7 @@ static pa_status create_batch(pa_ctx *ctx, const pa_config *cfg, const uint8_t *
         if ((st = learned_widths(ctx, b->cfg, view, forced, cost)) != PA_OK) return st;
         if (!pa::resolve_tuning(cfg->tuning).cost_order) cost.clear();
     }
+    b->cfg.model = nullptr;   // read above only; the re-plan of saturated pairs forces int32
     b->export_dp = export_dp;
     // The residue bytes go to the device while the host plans (the upload needs nothing from
     // the plan); the helper thread is joined before anything else uses the stream.

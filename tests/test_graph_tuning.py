@@ -5,6 +5,7 @@ The replay test follows the rule for captured graphs: it replays the graph sever
 eager work and synchronisations of the program between the replays, and checks every replay
 against the eagerly launched result.
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -132,6 +133,44 @@ def test_persistent_work_queue_launches(ctx, graph, form):
             assert np.array_equal(got.astype(np.int64), want), f"saturating pass {rep}, method {method}"
         assert np.all(flags[-3:] & device.PAIR_RECOMPUTED) and np.all(got[-3:] == 2500 * 15)
         bt.close()
+
+
+def test_batch_keeps_its_own_tuning(ctx):
+    """A batch copies the caller's pa_tuning at creation: the int32 re-plan of its saturated pairs,
+    made at its first run, follows the tuning the batch was created with even after the caller
+    has changed (or freed) its struct.  Three identical 6 kbp pairs at +15 saturate int16 and are
+    recomputed in int32 through the wavefront kernel (wavefront_tail < 0); planned with the
+    caller's changed struct instead (R = 4 only, no wavefront kernel) the recomputation would be
+    a 94-stripe striped sweep, about ten times slower."""
+    rng = np.random.default_rng(71)
+    same = workloads.unpack(*workloads.random_packed(rng, np.full(3, 6000), workloads.DNA))
+    d1, d2 = workloads.dna_pairs(rng, 300, 50, 400, divergence=0.1)
+    d1, d2 = d1 + same, d2 + list(same)
+    dc = device.DeviceConfig(method=device.METHOD_LOCAL, match=15, mismatch=-5, gap_open=-10, gap_extension=-1,
+                             lane_policy=device.LANES_INT16, tuning=device.Tuning(wavefront_tail=-1.0))
+    want = O.score_pairs(d1, d2, oracle_cfg(dc), threads=8)
+    b1, o1 = workloads.pack(d1)
+    b2, o2 = workloads.pack(d2)
+    b1, o1, b2, o2, _ = device.checked_batch(b1, o1, b2, o2, None)
+    holder = device._CConfigHolder(dc)
+    h = ctypes.c_void_p()
+    ctx.check(ctx._lib.pa_batch_create(ctx.handle, ctypes.byref(holder.c), b1.ctypes.data, o1.ctypes.data,
+                                       b2.ctypes.data, o2.ctypes.data, len(o1) - 1, None, ctypes.byref(h)))
+    bt = device.Batch.__new__(device.Batch)
+    bt.ctx, bt.handle, bt.n, bt._keep = ctx, h, len(o1) - 1, (b1, o1, b2, o2)
+    # the caller's struct changes after the batch exists
+    holder.tun.rows_mask = 1
+    holder.tun.wavefront_min_rows = 60000
+    holder.tun.wavefront_tail = 1e9
+    for rep in range(2):
+        bt.run()
+        got, flags = bt.results()
+        assert np.array_equal(got.astype(np.int64), want), f"pass {rep}"
+    assert np.all(flags[-3:] & device.PAIR_RECOMPUTED) and np.all(got[-3:] == 6000 * 15)
+    ms, _ = bt.time(2)
+    log(f"pass with the int32 recomputation: {ms / 2:.1f} ms")
+    assert ms / 2 < 60.0, ms / 2
+    bt.close()
 
 
 def test_graph_with_saturation_recompute(ctx):
