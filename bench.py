@@ -338,7 +338,7 @@ def bench_config(name, steps, warmup, pairs=None, with_cpu=True, e2e_reps=0, fil
         # the library's autotuner times candidate planner/launch parameters on a 5% sample
         t1 = time.perf_counter()
         tuning = device.tune(ctx, dc, w["b1"], w["o1"], w["b2"], w["o2"], w.get("pair_method"),
-                             sample_pairs=max(2000, n // 20), iters=2)
+                             sample_pairs=max(2000, n // 20), iters=4)
         dc = dataclasses.replace(dc, tuning=tuning)
         tuned = dict(tuning=dataclasses.asdict(tuning), seconds=round(time.perf_counter() - t1, 2))
         log(f"[bench] {name}: autotuned {tuned}")

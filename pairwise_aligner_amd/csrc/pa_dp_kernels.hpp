@@ -367,18 +367,36 @@ __device__ __forceinline__ void stripe_task(const KernelArgs &a, const uint32_t 
             }
         };
 
-        // whole chunks of G steps; the first is masked (lanes start one step apart)
+        // whole chunks of G steps; the first is masked (lanes start one step apart).  Lane G-1
+        // computes the last column at step N+G-2: with the byte-permute DNA scores, a task of one
+        // stripe ends there instead of sweeping pad columns to the end of its last chunk (the
+        // other score kinds keep whole chunks: the shorter loop costs their sweeps three VGPRs,
+        // which the pair-row table's fused kernel cannot spare at three waves per SIMD)
         const int nsteps = ((N + 2 * G - 2) / G) * G;
+        const int last_step = N + G - 2;
+        constexpr bool kCutLast = SCORE == kScoreDna;
         constexpr bool kMaskAll = ROWMAX || TRACK || EXPORT;
         auto sweep = [&]<bool TOPSTEP>() {
 #pragma unroll 1
             for (int u = 0; u < G; ++u) step.template operator()<true, TOPSTEP>(u);
             chunk_end(0);
             for (int k0 = G; k0 < nsteps; k0 += G) {
+                // the last chunk of a one-stripe task ends after the pair of steps that holds the
+                // last useful one (k0 <= last_step here)
+                const bool cut = kCutLast && ONE && k0 + G - 1 > last_step;
+                const int pairs = cut ? (last_step - k0 + 2) / 2 : G / 2;
                 // two steps per iteration: the register allocator then alternates the exchanged
                 // values between two sets of registers instead of copying them back every step
-#pragma unroll 2
-                for (int u = 0; u < G; ++u) step.template operator()<kMaskAll, TOPSTEP>(k0 + u);
+#pragma unroll 1
+                for (int p = 0; p < pairs; ++p) {
+                    step.template operator()<kMaskAll, TOPSTEP>(k0 + 2 * p);
+                    step.template operator()<kMaskAll, TOPSTEP>(k0 + 2 * p + 1);
+                }
+                if (cut) {
+                    // the local running maximum goes back to the frame of step -1 by the steps taken
+                    if constexpr (LOCAL && CDRIFT) runmax = L::add(runmax, L::splat(ge * 2 * pairs));
+                    break;
+                }
                 chunk_end(k0);
             }
         };
