@@ -368,10 +368,11 @@ __device__ __forceinline__ void stripe_task(const KernelArgs &a, const uint32_t 
         };
 
         // whole chunks of G steps; the first is masked (lanes start one step apart).  Lane G-1
-        // computes the last column at step N+G-2: with the byte-permute DNA scores, a task of one
-        // stripe ends there instead of sweeping pad columns to the end of its last chunk (the
-        // other score kinds keep whole chunks: the shorter loop costs their sweeps three VGPRs,
-        // which the pair-row table's fused kernel cannot spare at three waves per SIMD)
+        // computes the last column at step N+G-2: with the byte-permute DNA scores, a stripe
+        // with no bottom row to store (the task's last) ends there instead of sweeping pad
+        // columns to the end of its last chunk (the other score kinds keep whole chunks: the
+        // shorter loop costs their sweeps three VGPRs, which the pair-row table's fused kernel
+        // cannot spare at three waves per SIMD)
         const int nsteps = ((N + 2 * G - 2) / G) * G;
         const int last_step = N + G - 2;
         constexpr bool kCutLast = SCORE == kScoreDna;
@@ -381,9 +382,9 @@ __device__ __forceinline__ void stripe_task(const KernelArgs &a, const uint32_t 
             for (int u = 0; u < G; ++u) step.template operator()<true, TOPSTEP>(u);
             chunk_end(0);
             for (int k0 = G; k0 < nsteps; k0 += G) {
-                // the last chunk of a one-stripe task ends after the pair of steps that holds the
-                // last useful one (k0 <= last_step here)
-                const bool cut = kCutLast && ONE && k0 + G - 1 > last_step;
+                // the last chunk of the task's last stripe ends after the pair of steps that holds
+                // the last useful one (k0 <= last_step here)
+                const bool cut = kCutLast && !has_bnd_out && k0 + G - 1 > last_step;
                 const int pairs = cut ? (last_step - k0 + 2) / 2 : G / 2;
                 // two steps per iteration: the register allocator then alternates the exchanged
                 // values between two sets of registers instead of copying them back every step
