@@ -60,13 +60,12 @@ typedef struct pa_tuning {
                                      the geometric order) by the network's predicted cost instead of the
                                      geometric model's [0] */
     uint32_t persistent;          /* launch form of the striped launches.  0: one block per 16 tasks, except
-                                     that the int16 local launches with a pair-row substitution table that
-                                     differ only in their rows per lane run as one launch of resident blocks
-                                     taking tasks from one work queue; 1: every striped launch runs as many
-                                     blocks as the GPU keeps resident, whose waves take tasks from a work
-                                     queue of its own; 2: as 1, and the int16 local launches that differ
-                                     only in their rows per lane (DNA scores too) run as one such launch;
-                                     3: one block per 16 tasks, nothing fused [0] */
+                                     that the int16 local launches that differ only in their rows per lane
+                                     run as one launch of resident blocks taking tasks from one work queue;
+                                     1: every striped launch runs as many blocks as the GPU keeps resident,
+                                     whose waves take tasks from a work queue of its own; 2: as 1, with the
+                                     int16 local launches fused as in 0; 3: one block per 16 tasks,
+                                     nothing fused [0] */
     float wavefront_tail;         /* a pair beyond wavefront_min_rows (and within 8,192 rows) runs in the wavefront
                                      kernel only if its modelled striped sweep exceeds this multiple of the batch's
                                      striped work per resident lane group (where the batch has less other work than

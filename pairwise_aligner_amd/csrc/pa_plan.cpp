@@ -373,11 +373,11 @@ void code_tables(const Plan &plan, const VariantKey &key, CodeTables &out)
 // one member launch, alike in shape) come in decreasing order of modelled time across the members
 // -- one longest-first order for the whole group instead of one per launch, and no launch of the
 // group waiting for a stream or forming the pass's tail alone.  The kernel that holds every R runs
-// the pair-row-table sweeps at the occupancy of their largest member (167 against 148 VGPRs,
-// three waves per SIMD either way), so those are fused by default; it holds the byte-permute
-// sweeps at three waves per SIMD instead of four, which pays only where the pass is short or made
-// of many small launches (Tuning::persistent 2, the autotuner's choice).
-void fuse_launches(Plan &plan)
+// at three waves per SIMD (167 VGPRs with the pair-row table, 150 with the byte-permute scores),
+// the occupancy of the pair-row-table sweep at R = 16 alone; the byte-permute sweeps alone would
+// hold four, and fused they still run faster (the mixed stream's DNA half at 100,000 pairs: 84.2
+// -> 79.9 ms; the uneven 1M batch: 38.0 -> 35.1 ms).  Fused unless Tuning::persistent is 1 or 3.
+void fuse_launches(Plan &plan, const float *pair_cost)
 {
     const size_t nl = plan.launches.size();
     plan.fused.clear();
@@ -385,9 +385,7 @@ void fuse_launches(Plan &plan)
     plan.fused_of.assign(nl, -1);
     const uint32_t form = plan.tuning.persistent;
     if (form == 1 || form == 3) return;
-    auto fuses = [&](const VariantKey &k) {
-        return fused_launchable(k) && (k.score == kScorePairs || form == 2);
-    };
+    auto fuses = [&](const VariantKey &k) { return fused_launchable(k); };
     auto group_key = [](VariantKey k) { k.R = 0; return k; };
     std::vector<bool> taken(nl, false);
     for (size_t i = 0; i < nl; ++i) {
@@ -408,8 +406,15 @@ void fuse_launches(Plan &plan)
             const Launch &L = plan.launches[li];
             for (uint32_t k0 = L.task_begin; k0 < L.task_end; k0 += 4) {
                 const uint32_t n = std::min<uint32_t>(4, L.task_end - k0);
-                double t = 0;
-                for (uint32_t k = k0; k < k0 + n; ++k) t = std::max(t, task_cost(L.key, plan.tasks[k], plan.tuning));
+                double t = 0;   // modelled time, or with pair_cost the predicted cost of the costliest pair
+                for (uint32_t k = k0; k < k0 + n; ++k) {
+                    if (!pair_cost) {
+                        t = std::max(t, task_cost(L.key, plan.tasks[k], plan.tuning));
+                        continue;
+                    }
+                    for (int h = 0; h < 2; ++h)
+                        if (plan.tasks[k].pair[h] != kNone) t = std::max(t, double(pair_cost[plan.tasks[k].pair[h]]));
+                }
                 items.push_back(Item{t, fused_item(L.key.R, n, k0)});
             }
         }
@@ -1025,7 +1030,7 @@ pa_status make_plan(const pa_config &cfg, const BatchView &b, const uint8_t *for
 
     plan.n_rowcodes = n_rowcodes;
     plan.n_colcodes = n_colcodes;
-    fuse_launches(plan);
+    fuse_launches(plan, pair_cost);
     order_launches(plan);
     return PA_OK;
 }

@@ -199,12 +199,10 @@ struct Tuning {
     bool eager = false;
     bool cost_order = false;
     // Launch forms of the striped launches.  0 (default): one block per 16 tasks, except that the
-    // int16 local launches with the pair-row table that differ only in R run as one fused launch
-    // (resident blocks, one work queue; fused_launchable), whose kernel holds no more registers
-    // per lane than its members' largest; 1: every striped launch runs resident blocks taking
-    // tasks from a work queue of its own; 2: as 1, and the fused_launchable launches that differ
-    // only in R -- byte-permute DNA scores too -- run fused; 3: one block per 16 tasks, nothing
-    // fused.
+    // int16 local launches that differ only in R run as one fused launch (resident blocks, one
+    // work queue; fused_launchable); 1: every striped launch runs resident blocks taking tasks
+    // from a work queue of its own; 2: as 1, with the fused_launchable launches fused as in 0;
+    // 3: one block per 16 tasks, nothing fused.
     uint32_t persistent = 0;
 };
 Tuning resolve_tuning(const pa_tuning *t);
@@ -377,9 +375,9 @@ struct CodeTables {
 void code_tables(const Plan &plan, const VariantKey &key, CodeTables &out);
 
 // Groups the plan's fused_launchable launches that differ only in R into fused launches
-// (Plan::fused, fused_of, fused_items): those with the pair-row table unless Tuning::persistent is
-// 1 or 3, those with byte-permute DNA scores when it is 2.
-void fuse_launches(Plan &plan);
+// (Plan::fused, fused_of, fused_items), unless Tuning::persistent is 1 or 3.  Wave items come in
+// decreasing order of modelled time, or with pair_cost (nullable) of predicted cost.
+void fuse_launches(Plan &plan, const float *pair_cost = nullptr);
 
 // Order of the launches of a plan and the stream of each: wavefront launches first (longest
 // task first), then striped launches by the time each would take on its own (a fused launch

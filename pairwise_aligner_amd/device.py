@@ -64,9 +64,9 @@ class Tuning:
     streams: int = 0
     eager: int = 0
     cost_order: int = 0   # learned policy: order waves by the predicted cost (1) or the geometric model (0)
-    # striped launch form: 0 one block per 16 tasks, the int16 local pair-row-table launches that
-    # differ only in R fused into one launch with one work queue; 1 resident blocks taking tasks from
-    # a work queue per launch; 2 as 1 with every int16 local launch fused (DNA too); 3 no fusion
+    # striped launch form: 0 one block per 16 tasks, the int16 local launches that differ only in R
+    # fused into one launch with one work queue; 1 resident blocks taking tasks from a work queue
+    # per launch; 2 as 1 with the int16 local launches fused; 3 no fusion
     persistent: int = 0
     # long pairs go to the wavefront kernel only where their striped sweep would exceed this
     # multiple of the batch's striped work per resident lane group (< 0: every long pair)

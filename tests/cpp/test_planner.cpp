@@ -414,7 +414,23 @@ static void check_cost_order()
             prev = c;
         }
     }
-    std::printf("  cost order: %zu runs of whole waves checked\n", runs);
+    // the wave items of fused launches follow the predicted cost too
+    size_t items = 0;
+    for (const FusedLaunch &F : by_cost.fused) {
+        float prev = INFINITY;
+        for (uint32_t i = F.items_begin; i < F.items_end; ++i, ++items) {
+            const uint32_t it = by_cost.fused_items[i];
+            const uint32_t n = ((it >> 27) & 3u) + 1, first = it & (kFusedMaxTasks - 1);
+            float c = 0.0f;
+            for (uint32_t k = first; k < first + n; ++k)
+                for (int h = 0; h < 2; ++h)
+                    if (by_cost.tasks[k].pair[h] != kNone) c = std::max(c, cost[by_cost.tasks[k].pair[h]]);
+            CHECK(c <= prev, "fused item %u: predicted cost %g after %g", i, double(c), double(prev));
+            prev = c;
+        }
+    }
+    CHECK(items > 0, "no fused wave items in the cost-ordered plan");
+    std::printf("  cost order: %zu runs of whole waves and %zu fused wave items checked\n", runs, items);
 }
 
 // Fused launches (pa_tuning.persistent = 2): the int16 local striped launches that differ only in R
@@ -497,7 +513,7 @@ static void check_fused_launches()
         CHECK(l == leader || pos[leader] < pos[l], "member %zu ordered before its leader %u", l, leader);
         CHECK(p.launch_stream[l] == p.launch_stream[leader], "member %zu on another stream than its leader", l);
     }
-    // the default form fuses the pair-row-table launches only; forms 1 and 3 fuse nothing
+    // the default form fuses the local launches of both score kinds; forms 1 and 3 fuse nothing
     static const char *kAmino = "ARNDCQEGHILKMFPSTWYVX";
     std::vector<int32_t> matrix(21 * 21);
     for (int i = 0; i < 21; ++i)
@@ -519,10 +535,11 @@ static void check_fused_launches()
             Plan q;
             const Batch &bb = c == &prot ? pb : b;
             CHECK(make_plan(*c, bb.view(), nullptr, nullptr, q, err) == PA_OK, "form %u plan: %s", form, err.c_str());
-            const bool want = form == 0 && c == &prot;
+            const bool want = form == 0;
             CHECK(q.fused.empty() != want, "form %u, %s: %zu fused launches", form, c == &prot ? "protein" : "DNA",
                   q.fused.size());
-            for (const FusedLaunch &F : q.fused) CHECK(q.launches[F.leader].key.score == kScorePairs, "form %u", form);
+            for (const FusedLaunch &F : q.fused)
+                CHECK(q.launches[F.leader].key.score == (c == &prot ? kScorePairs : kScoreDna), "form %u", form);
         }
     }
     std::printf("  fused launches: %zu of %zu launches in %zu fused, %zu items\n",
