@@ -300,6 +300,7 @@ pa_status launch_plan(pa_ctx *ctx, pa_batch *b, PlanDevice &pd, bool self_init)
             a.items = static_cast<const uint32_t *>(pd.fused_items.ptr) + F.items_begin;
             a.nitems = F.items_end - F.items_begin;
             a.queue = static_cast<uint32_t *>(pd.queues.ptr) + 2 * size_t(li);
+            a.share = float(F.share);
             hipError_t e = pa::launch_fused(L.key, a, lds, st);
             if (e != hipSuccess) return hip_fail(ctx, e, "fused striped DP launch");
             continue;

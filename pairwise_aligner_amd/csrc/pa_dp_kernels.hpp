@@ -892,7 +892,11 @@ static dim3 striped_grid(K kernel, const KernelArgs &a, uint32_t blocks, uint32_
     if (hipGetDevice(&device) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
         cus = 1;
-    return dim3(std::min<uint32_t>(blocks, uint32_t(per_cu) * uint32_t(cus)));
+    uint32_t resident = uint32_t(per_cu) * uint32_t(cus);
+    // a fused launch that shares its pass with other launches takes its share of the GPU
+    if (a.items && a.share > 0.0f && a.share < 1.0f)
+        resident = std::max<uint32_t>(uint32_t(cus), uint32_t(double(resident) * a.share + 0.5));
+    return dim3(std::min<uint32_t>(blocks, resident));
 }
 
 template <int W, int R, int SCORE, bool LOCAL, bool TRACK, bool ROWMAX>

@@ -38,6 +38,7 @@ struct KernelArgs {
     // array (tasks points at its first task)
     const uint32_t *items;
     uint32_t nitems;
+    float share;                // fused launch: the share of the GPU's resident blocks it takes
 };
 
 struct InitArgs {

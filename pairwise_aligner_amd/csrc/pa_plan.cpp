@@ -387,6 +387,15 @@ void fuse_launches(Plan &plan, const float *pair_cost)
     if (form == 1 || form == 3) return;
     auto fuses = [&](const VariantKey &k) { return fused_launchable(k); };
     auto group_key = [](VariantKey k) { k.R = 0; return k; };
+    // modelled work of every launch, local cells weighed 10/8 (ops per cell)
+    std::vector<double> work(nl, 0.0);
+    double total = 0;
+    for (size_t i = 0; i < nl; ++i) {
+        const Launch &L = plan.launches[i];
+        for (uint32_t t = L.task_begin; t < L.task_end; ++t) work[i] += task_cost(L.key, plan.tasks[t], plan.tuning);
+        if (L.key.local) work[i] *= 1.25;
+        total += work[i];
+    }
     std::vector<bool> taken(nl, false);
     for (size_t i = 0; i < nl; ++i) {
         if (taken[i] || !fuses(plan.launches[i].key)) continue;
@@ -421,6 +430,9 @@ void fuse_launches(Plan &plan, const float *pair_cost)
         std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.time > y.time; });
         FusedLaunch f;
         f.leader = members[0];
+        double w = 0;
+        for (uint32_t li : members) w += work[li];
+        f.share = total > 0 ? w / total : 1.0;
         f.items_begin = uint32_t(plan.fused_items.size());
         for (const Item &it : items) plan.fused_items.push_back(it.item);
         f.items_end = uint32_t(plan.fused_items.size());

@@ -157,6 +157,7 @@ struct Launch {
 struct FusedLaunch {
     uint32_t leader;
     uint32_t items_begin, items_end;
+    double share;   // its members' modelled work over the pass's (local cells weighed 10/8)
 };
 
 // Wave item of a fused launch: up to 64 / 16 consecutive tasks of one member launch, as
