@@ -515,7 +515,7 @@ def bench_saturating(model, prot, steps, count=20_000):
         dc = dataclasses.replace(prot, lane_policy=policy, model=model if policy == device.LANES_LEARNED else None)
         r0 = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dc, None, steps, 1)
         t1 = time.perf_counter()
-        tuning = device.tune(ctx, dc, w["b1"], w["o1"], w["b2"], w["o2"], None, iters=2)
+        tuning = device.tune(ctx, dc, w["b1"], w["o1"], w["b2"], w["o2"], None, iters=4)
         tune_s = time.perf_counter() - t1
         r1 = gpu_run(ctx, w["b1"], w["o1"], w["b2"], w["o2"], dataclasses.replace(dc, tuning=tuning), None, steps, 1)
         # the policy is credited with the faster of its two measured shapes (the tuner times two
