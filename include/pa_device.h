@@ -212,8 +212,9 @@ pa_status pa_batch_destroy(pa_ctx *ctx, pa_batch *batch);
  * the fastest to *out, in two stages of coordinate descent: the shape-model fields (step_cost,
  * rows_mask) on a sample of the batch (every k-th pair, about sample_pairs pairs; 0 = 5% of the
  * batch, at least 2,000), then the launch-level fields (wavefront_min_rows, streams, eager,
- * persistent, wavefront_tail) and the stage-1 result against the defaults on the whole batch.
- * `iters` passes are timed per candidate (0 = 3). */
+ * persistent, wavefront_tail) and the stage-1 result against the defaults on the whole batch;
+ * a final pick that differs from cfg's tuning is kept only if it is also faster in two more
+ * alternating timings against it.  `iters` passes are timed per candidate (0 = 3). */
 pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const uint64_t *off1,
                   const uint8_t *seq2, const uint64_t *off2, uint64_t n_pairs, const uint8_t *pair_method,
                   uint64_t sample_pairs, int iters, pa_tuning *out);
