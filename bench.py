@@ -360,7 +360,8 @@ def bench_config(name, steps, warmup, pairs=None, with_cpu=True, e2e_reps=0, fil
         out["autotuned"] = tuned
         # every launch form (pa_tuning.persistent) with the tuner's other parameters, on the same
         # pairs: ms per pass
-        names = ["default_local_fused", "work_queue_per_launch", "work_queue_local_fused", "one_block_per_16_tasks"]
+        names = ["default_local_fused", "work_queue_per_launch", "work_queue_local_fused", "one_block_per_16_tasks",
+                 "local_and_global_fused"]
         ab, same = {}, True
         for form, form_name in enumerate(names):
             if form == dc.tuning.persistent:

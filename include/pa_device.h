@@ -65,7 +65,9 @@ typedef struct pa_tuning {
                                      1: every striped launch runs as many blocks as the GPU keeps resident,
                                      whose waves take tasks from a work queue of its own; 2: as 1, with the
                                      int16 local launches fused as in 0; 3: one block per 16 tasks,
-                                     nothing fused [0] */
+                                     nothing fused; 4: as 0, with the global method's int16 launches
+                                     of the unitary (DNA) scores also fused, in two launches by rows
+                                     per lane [0] */
     float wavefront_tail;         /* a pair beyond wavefront_min_rows (and within 8,192 rows) runs in the wavefront
                                      kernel only if its modelled striped sweep exceeds this multiple of the batch's
                                      striped work per resident lane group (where the batch has less other work than

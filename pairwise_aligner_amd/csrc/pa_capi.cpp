@@ -1058,10 +1058,10 @@ pa_status pa_tune(pa_ctx *ctx, const pa_config *cfg, const uint8_t *seq1, const 
     const uint32_t wf[] = {2048, 3072, 4096};
     const uint32_t streams[] = {2, 4};
     const uint32_t eager[] = {0, 1};
-    const uint32_t persistent[] = {0, 1, 2, 3};
+    const uint32_t persistent[] = {0, 1, 2, 3, 4};
     const float tails[] = {-1.0f, 0.5f, 1.0f};
     for (int param = 0; param < 5; ++param) {
-        const int nv = param == 3 ? 4 : (param == 0 || param == 4 ? 3 : 2);
+        const int nv = param == 3 ? 5 : (param == 0 || param == 4 ? 3 : 2);
         for (int v = 0; v < nv; ++v) {
             pa_tuning t = full;
             if (param == 0) t.wavefront_min_rows = wf[v];

@@ -487,18 +487,22 @@ __global__ __launch_bounds__(256) void stripe_kernel(KernelArgs a)
 }
 
 // One persistent launch for the striped launches of a plan that differ only in their rows per
-// lane (pa_plan.hpp, FusedLaunch): as many blocks as the GPU keeps resident, whose waves take wave
-// items from one atomic counter in the planner's order (decreasing modelled time across the
-// member launches).  An item names up to 64 / 16 consecutive tasks of one member and its R, which
-// selects the sweep; the wave's lane groups take one task each.  The last block to finish resets
-// the counter for the next pass.
-template <int W, int SCORE, bool LOCAL>
+// lane R, within [RLO, RHI] (pa_plan.hpp, FusedLaunch, fused_rows): as many blocks as the GPU
+// keeps resident, whose waves take wave items from one atomic counter in the planner's order
+// (decreasing modelled time across the member launches).  An item names up to 64 / 16
+// consecutive tasks of one member and its R, which selects the sweep; the wave's lane groups take
+// one task each.  The last block to finish resets the counter for the next pass.
+template <int W, int SCORE, bool LOCAL, int RLO, int RHI>
 __global__ __launch_bounds__(256) void fused_stripe_kernel(KernelArgs a)
 {
     constexpr int G = 16;
     extern __shared__ char lds_raw[];
     load_launch_table<SCORE>(lds_raw, a);
     const uint32_t group = (threadIdx.x % 64) / G;   // of the wave's 64 / G groups
+    // the sweep of rows per lane RR, compiled only inside the kernel's range
+    auto sweep = [&]<int RR>(uint32_t gid) {
+        if constexpr (RR >= RLO && RR <= RHI) stripe_task<W, G, RR, SCORE, LOCAL, false, false, false>(a, gid, lds_raw);
+    };
     for (;;) {
         // the whole wave is here (converged): lane 0 takes the next item for it
         uint32_t idx = 0;
@@ -508,14 +512,14 @@ __global__ __launch_bounds__(256) void fused_stripe_kernel(KernelArgs a)
         const uint32_t item = __builtin_amdgcn_readfirstlane(a.items[idx]);
         if (group <= ((item >> 27) & 3u)) {
             const uint32_t gid = (item & (kFusedMaxTasks - 1u)) + group;
-            switch (item >> 29) {   // (R - 4) / 2, uniform over the wave
-            case 0: stripe_task<W, G, 4, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
-            case 1: stripe_task<W, G, 6, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
-            case 2: stripe_task<W, G, 8, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
-            case 3: stripe_task<W, G, 10, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
-            case 4: stripe_task<W, G, 12, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
-            case 5: stripe_task<W, G, 14, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
-            default: stripe_task<W, G, 16, SCORE, LOCAL, false, false, false>(a, gid, lds_raw); break;
+            switch (item >> 29) {   // (R - 4) / 2, uniform over the wave; the planner keeps R in range
+            case 0: sweep.template operator()<4>(gid); break;
+            case 1: sweep.template operator()<6>(gid); break;
+            case 2: sweep.template operator()<8>(gid); break;
+            case 3: sweep.template operator()<10>(gid); break;
+            case 4: sweep.template operator()<12>(gid); break;
+            case 5: sweep.template operator()<14>(gid); break;
+            default: sweep.template operator()<16>(gid); break;
             }
         }
     }

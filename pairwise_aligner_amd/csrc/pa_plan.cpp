@@ -41,7 +41,7 @@ Tuning resolve_tuning(const pa_tuning *t)
     if (t->streams) r.streams = std::min<uint32_t>(t->streams, 4);
     r.eager = t->eager != 0;
     r.cost_order = t->cost_order != 0;
-    r.persistent = std::min<uint32_t>(t->persistent, 3);
+    r.persistent = std::min<uint32_t>(t->persistent, 4);
     return r;
 }
 
@@ -385,8 +385,10 @@ void fuse_launches(Plan &plan, const float *pair_cost)
     plan.fused_of.assign(nl, -1);
     const uint32_t form = plan.tuning.persistent;
     if (form == 1 || form == 3) return;
-    auto fuses = [&](const VariantKey &k) { return fused_launchable(k); };
-    auto group_key = [](VariantKey k) { k.R = 0; return k; };
+    // the global method's launches are fused in form 4 only
+    auto fuses = [&](const VariantKey &k) { return fused_launchable(k) && (k.local || form == 4); };
+    // launches that differ only in R, within one range of the fused kernel (fused_rows)
+    auto group_key = [](VariantKey k) { k.R = uint8_t(fused_rows(k).lo); return k; };
     // modelled work of every launch, local cells weighed 10/8 (ops per cell)
     std::vector<double> work(nl, 0.0);
     double total = 0;

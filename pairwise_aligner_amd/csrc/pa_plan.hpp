@@ -169,14 +169,24 @@ inline uint32_t fused_item(uint32_t R, uint32_t count, uint32_t first)
     return ((R - 4) / 2) << 29 | (count - 1) << 27 | first;
 }
 
-// Launches that a fused launch can take: the int16 striped variants of the local method (without
-// the DP export) whose substitution scores are the byte permute or the pair-row table.  (The
-// global method's variants at R = 16 hold 232 VGPRs on their own; a kernel holding every R would
-// run one wave per SIMD.)
+// Launches that a fused launch can take: the int16 striped variants without per-row maxima or the
+// DP export, of the local method with the byte-permute scores or the pair-row table, and of the
+// global method with the byte-permute scores (fused in launch form 4 only).
 inline bool fused_launchable(const VariantKey &k)
 {
-    return k.width == 16 && k.G == 16 && k.local && !k.track && !k.rowmax && !k.export_dp &&
-           (k.score == kScoreDna || k.score == kScorePairs);
+    if (k.width != 16 || k.G != 16 || k.track || k.rowmax || k.export_dp) return false;
+    return k.local ? (k.score == kScoreDna || k.score == kScorePairs) : k.score == kScoreDna;
+}
+
+// The rows-per-lane range of the fused kernel that takes a fused_launchable launch: every R for
+// the local method; for the global method R <= 12 and R >= 14 apart, two kernels of 254 and 256
+// VGPRs, two waves per SIMD like its sweeps from R = 10 up alone (R = 16 holds 234 alone; one
+// kernel holding R = 12 to 16 needs 274, every R 331 -- one wave per SIMD).
+struct RowsRange { int lo, hi; };
+inline RowsRange fused_rows(const VariantKey &k)
+{
+    if (k.local) return {4, 16};
+    return k.R <= 12 ? RowsRange{4, 12} : RowsRange{14, 16};
 }
 
 struct PairMeta {
@@ -203,7 +213,8 @@ struct Tuning {
     // int16 local launches that differ only in R run as one fused launch (resident blocks, one
     // work queue; fused_launchable); 1: every striped launch runs resident blocks taking tasks
     // from a work queue of its own; 2: as 1, with the fused_launchable launches fused as in 0;
-    // 3: one block per 16 tasks, nothing fused.
+    // 3: one block per 16 tasks, nothing fused; 4: as 0, with the global method's int16
+    // byte-permute launches also fused, in two ranges of R (fused_rows).
     uint32_t persistent = 0;
 };
 Tuning resolve_tuning(const pa_tuning *t);

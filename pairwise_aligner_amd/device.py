@@ -66,7 +66,8 @@ class Tuning:
     cost_order: int = 0   # learned policy: order waves by the predicted cost (1) or the geometric model (0)
     # striped launch form: 0 one block per 16 tasks, the int16 local launches that differ only in R
     # fused into one launch with one work queue; 1 resident blocks taking tasks from a work queue
-    # per launch; 2 as 1 with the int16 local launches fused; 3 no fusion
+    # per launch; 2 as 1 with the int16 local launches fused; 3 no fusion; 4 as 0 with the global
+    # method's int16 unitary-score launches also fused (two launches by rows per lane)
     persistent: int = 0
     # long pairs go to the wavefront kernel only where their striped sweep would exceed this
     # multiple of the batch's striped work per resident lane group (< 0: every long pair)

@@ -460,8 +460,9 @@ static void check_fused_launches()
     for (size_t l = 0; l < nl && l < p.fused_of.size(); ++l) {
         const VariantKey &k = p.launches[l].key;
         if (p.fused_of[l] < 0) {
-            // an unfused launchable launch has no other launch of its group
-            if (!fused_launchable(k)) continue;
+            // an unfused launchable launch of the local method (form 2 fuses no global launch)
+            // has no other launch of its group
+            if (!fused_launchable(k) || !k.local) continue;
             for (size_t o = 0; o < nl; ++o) {
                 VariantKey a = k, c = p.launches[o].key;
                 a.R = c.R = 0;
@@ -541,6 +542,39 @@ static void check_fused_launches()
             for (const FusedLaunch &F : q.fused)
                 CHECK(q.launches[F.leader].key.score == (c == &prot ? kScorePairs : kScoreDna), "form %u", form);
         }
+    }
+    // form 4 also fuses the global method's launches, in the two ranges of rows per lane
+    {
+        pa_tuning t4{};
+        t4.persistent = 4;
+        cfg.tuning = &t4;
+        Plan q;
+        CHECK(make_plan(cfg, b.view(), nullptr, nullptr, q, err) == PA_OK, "form 4 plan: %s", err.c_str());
+        size_t global_groups = 0, global_members = 0;
+        for (size_t l = 0; l < q.launches.size(); ++l) {
+            const VariantKey &k = q.launches[l].key;
+            if (q.fused_of[l] < 0) {
+                // no launchable launch is left alone while another of its range is there
+                if (!fused_launchable(k)) continue;
+                for (size_t o = 0; o < q.launches.size(); ++o) {
+                    VariantKey a = k, c = q.launches[o].key;
+                    const bool same_range = fused_rows(a).lo == fused_rows(c).lo;
+                    a.R = c.R = 0;
+                    CHECK(o == l || !(a == c) || !same_range, "form 4: launches %zu and %zu unfused", l, o);
+                }
+                continue;
+            }
+            const FusedLaunch &F = q.fused[size_t(q.fused_of[l])];
+            const RowsRange rr = fused_rows(q.launches[F.leader].key);
+            CHECK(k.R >= rr.lo && k.R <= rr.hi, "form 4: launch %zu of R %d in a fused launch of R %d-%d", l, k.R,
+                  rr.lo, rr.hi);
+            if (!k.local) ++global_members;
+        }
+        for (const FusedLaunch &F : q.fused) global_groups += q.launches[F.leader].key.local ? 0 : 1;
+        // the batch's global pairs span R = 4 to 16: both ranges
+        CHECK(global_groups == 2 && global_members >= 4, "form 4: %zu global fused launches of %zu members",
+              global_groups, global_members);
+        cfg.tuning = &t;
     }
     std::printf("  fused launches: %zu of %zu launches in %zu fused, %zu items\n",
                 size_t(std::count_if(p.fused_of.begin(), p.fused_of.end(), [](int32_t x) { return x >= 0; })), nl,

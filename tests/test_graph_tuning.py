@@ -74,14 +74,15 @@ def test_graph_replays_match_eager(ctx):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("form", [0, 1, 2, 3], ids=["default", "queue_per_launch", "fused", "unfused"])
+@pytest.mark.parametrize("form", [0, 1, 2, 3, 4], ids=["default", "queue_per_launch", "fused", "unfused", "global_fused"])
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
 def test_persistent_work_queue_launches(ctx, graph, form):
     """The launch forms of pa_tuning.persistent: the default (pair-row-table local launches fused
     into one launch with one work queue), resident blocks taking tasks from a work queue per launch
-    (1), with every int16 local launch that differs only in R fused (2), and no fusion (3): exact
-    on every replay (the last block resets the queue for the next pass), with eager work in
-    between, int16 saturation recomputed, a matrix score model and a batch larger than the grid."""
+    (1), with every int16 local launch that differs only in R fused (2), no fusion (3), and the
+    global method's launches fused too (4): exact on every replay (the last block resets the
+    queue for the next pass), with eager work in between, int16 saturation recomputed, a matrix
+    score model and a batch larger than the grid."""
     b1, o1, b2, o2, pm = uneven_batch(47, 30000, hi=3000)
     base = dict(match=2, mismatch=-3, gap_open=-5, gap_extension=-2)
     dc = device.DeviceConfig(**base, tuning=device.Tuning(persistent=form, eager=0 if graph else 1))
@@ -91,6 +92,10 @@ def test_persistent_work_queue_launches(ctx, graph, form):
         unfused = device.Batch(ctx, b1, o1, b2, o2, device.DeviceConfig(**base, tuning=device.Tuning(persistent=1)), pm)
         assert bt.stats()["n_launches"] < unfused.stats()["n_launches"], (bt.stats(), unfused.stats())
         unfused.close()
+    if form == 4:   # the global launches too, in two launches by rows per lane
+        local_only = device.Batch(ctx, b1, o1, b2, o2, device.DeviceConfig(**base, tuning=device.Tuning(persistent=0)), pm)
+        assert bt.stats()["n_launches"] < local_only.stats()["n_launches"], (bt.stats(), local_only.stats())
+        local_only.close()
     be = device.Batch(ctx, b1, o1, b2, o2, device.DeviceConfig(**base, tuning=device.Tuning(eager=1)), pm)
     for rep in range(4):
         log(f"persistent pass {rep}")
