@@ -508,6 +508,18 @@ void order_launches(Plan &plan)
             plan.launch_stream[li] = uint8_t((wf_streams + next_st++) % ns);
         }
     }
+    // A fused launch queued behind another launch on its stream starts when that one ends and
+    // takes the slots it leaves: it runs as many blocks as the GPU keeps resident instead of its
+    // share of the pass (uneven 1M batch, launch form 4: the global R 4-12 launch behind the
+    // local wavefront launch formed the pass's tail at one block per CU).
+    std::vector<bool> stream_busy(ns, false);
+    for (size_t i = 0; i < nl; ++i) {
+        const uint32_t li = plan.launch_order[i];
+        if (unit_of(li) != li) continue;
+        const uint8_t s = plan.launch_stream[li];
+        if (stream_busy[s] && plan.fused_of[li] >= 0) plan.fused[size_t(plan.fused_of[li])].share = 1.0;
+        stream_busy[s] = true;
+    }
 }
 
 int32_t max_substitution(const pa_config &cfg)
